@@ -1,0 +1,922 @@
+/*
+ * grs_capi.hip -- C ABI of the scoring engine (declared in include/grscore.h).
+ *
+ * The context keeps the geometry, the reflection list, cos/sin of the
+ * rotation angles and the distance stack resident in HBM; per-call buffers
+ * grow on demand and are reused.  All work runs on the context's own stream.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/grscore.h"
+#include "grs_internal.h"
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct grs_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  char err[512] = {0};
+  bool geo_set = false, stack_set = false;
+  grs_dgeo hgeo;
+  std::vector<double> hkl;
+  int32_t H = 0, nrot = 0;
+  int64_t snz = 0, sny = 0, snrot = 0;
+  DevBuf d_geo, d_hkl, d_rot, d_cs, d_keys;
+  DevBuf d_q, d_sub, d_valid, d_pos, d_vw, d_scratch, d_ns, d_ni, d_C, d_med;
+  DevBuf d_rows, d_p1, d_U, d_eu, d_rowstart, d_best, d_evox, d_eu_idx, d_bad, d_x, d_y;
+  DevBuf d_mask, d_g1, d_edt_v, d_edt_z, d_grain;
+  int64_t mnz = 0, mny = 0, mnrot = 0; /* shape of d_mask */
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double ms_total = 0.0;
+  int64_t launches = 0;
+};
+
+static grs_status fail(grs_ctx *c, grs_status code, const char *fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                                        \
+  do {                                                                                                         \
+    hipError_t e_ = (call);                                                                                    \
+    if (e_ != hipSuccess) return fail((c), GRS_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));    \
+  } while (0)
+
+static grs_status ensure(grs_ctx *c, DevBuf &b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.cap >= bytes) return GRS_OK;
+  if (b.p) {
+    hipStreamSynchronize(c->st);
+    hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, GRS_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  b.cap = bytes;
+  return GRS_OK;
+}
+
+#define ENSURE(c, buf, bytes)                      \
+  do {                                             \
+    grs_status s_ = ensure((c), (buf), (bytes));   \
+    if (s_ != GRS_OK) return s_;                   \
+  } while (0)
+
+template <typename T>
+static T *P(DevBuf &b) {
+  return static_cast<T *>(b.p);
+}
+
+static void free_buf(DevBuf &b) {
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+/* largest non-negative float key k with (double)as_float(k) <= thr */
+static void hit_key_for(double thr, uint32_t *key, int32_t *has) {
+  if (!(thr >= 0.0)) {
+    *has = 0;
+    *key = 0;
+    return;
+  }
+  float f = (float)thr;
+  if ((double)f > thr) f = std::nextafter(f, -INFINITY);
+  if (f < 0.0f || std::signbit(f)) f = 0.0f;
+  uint32_t k;
+  std::memcpy(&k, &f, 4);
+  *has = 1;
+  *key = k;
+}
+
+static double nanmin2(double a, double b) {
+  if (a != a) return b;
+  if (b != b) return a;
+  return b < a ? b : a;
+}
+
+extern "C" {
+
+const char *grs_version(void) { return "grscore 0.1 (gfx950)"; }
+
+static char g_legacy_err[512];
+
+const char *grs_last_error(const grs_ctx *ctx) { return ctx ? ctx->err : g_legacy_err; }
+
+grs_status grs_create(int32_t device, grs_ctx **out) {
+  if (!out) return GRS_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GRS_ERR_DEVICE;
+  if (device < 0 || device >= n) return GRS_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return GRS_ERR_DEVICE;
+  grs_ctx *c = new grs_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GRS_ERR_DEVICE;
+  }
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    hipStreamDestroy(c->st);
+    delete c;
+    return GRS_ERR_DEVICE;
+  }
+  *out = c;
+  return GRS_OK;
+}
+
+void grs_destroy(grs_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->st);
+  DevBuf *bufs[] = {&c->d_geo, &c->d_hkl, &c->d_rot, &c->d_cs, &c->d_keys, &c->d_q, &c->d_sub, &c->d_valid,
+                    &c->d_pos, &c->d_vw, &c->d_scratch, &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_rows,
+                    &c->d_p1, &c->d_U, &c->d_eu, &c->d_rowstart, &c->d_best, &c->d_evox, &c->d_eu_idx,
+                    &c->d_bad, &c->d_x, &c->d_y, &c->d_mask, &c->d_g1, &c->d_edt_v, &c->d_edt_z,
+                    &c->d_grain};
+  for (DevBuf *b : bufs) free_buf(*b);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+static grs_status upload_geo(grs_ctx *c) {
+  c->hgeo.H = c->H;
+  c->hgeo.nrot = c->nrot;
+  c->hgeo.nz = c->snz;
+  c->hgeo.ny = c->sny;
+  ENSURE(c, c->d_geo, sizeof(grs_dgeo));
+  HIPCHK(c, hipMemcpyAsync(c->d_geo.p, &c->hgeo, sizeof(grs_dgeo), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_set_geometry(grs_ctx *c, const grs_geometry *geo, const double *hkl, int64_t nrhkl,
+                            const double *rot, int64_t nrot) {
+  if (!c) return GRS_ERR_ARG;
+  if (!geo || !hkl || !rot) return fail(c, GRS_ERR_ARG, "null argument");
+  if (nrhkl < 1 || nrot < 1) return fail(c, GRS_ERR_ARG, "need at least one reflection and one rotation angle");
+  if (nrhkl * nrot >= (int64_t)1 << 30) return fail(c, GRS_ERR_ARG, "nrhkl*nrot = %lld events is too large",
+                                                   (long long)(nrhkl * nrot));
+  if (!(geo->pixelysize > 0.0 && geo->pixelzsize > 0.0)) return fail(c, GRS_ERR_ARG, "pixel sizes must be > 0");
+  if (!(geo->detysize >= 1.0 && geo->detzsize >= 1.0 && geo->detysize < 2147483647.0 &&
+        geo->detzsize < 2147483647.0))
+    return fail(c, GRS_ERR_ARG, "detector size out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_dgeo &g = c->hgeo;
+  std::memset(&g, 0, sizeof(g));
+  std::memcpy(g.S, geo->S, sizeof(g.S));
+  std::memcpy(g.B, geo->B, sizeof(g.B));
+  std::memcpy(g.R, geo->RotDet, sizeof(g.R));
+  g.Lss = geo->Lsam2sou;
+  g.Lsd = geo->Lsam2det;
+  g.L = geo->Lsam2sou + geo->Lsam2det;
+  g.P0y = geo->P0y;
+  g.P0z = geo->P0z;
+  g.RAO = geo->RotAxisOffset;
+  g.dety00 = geo->dety00;
+  g.detz00 = geo->detz00;
+  g.py = geo->pixelysize;
+  g.pz = geo->pixelzsize;
+  g.dety0 = geo->dety0;
+  g.detz0 = geo->detz0;
+  g.detysize = geo->detysize;
+  g.detzsize = geo->detzsize;
+  g.bsy0 = geo->BeamStopY[0];
+  g.bsy1 = geo->BeamStopY[1];
+  g.bsz0 = geo->BeamStopZ[0];
+  g.bsz1 = geo->BeamStopZ[1];
+  g.beta_max = ((90.0 + geo->thetamax * 4.0) / 180.0) * GRS_PI;
+  g.lambda_min = geo->lambda_min;
+  g.lambda_max = geo->lambda_max;
+  g.pio2 = GRS_PI / 2.0;
+  g.thr = geo->minEucDis / ((geo->pixelysize + geo->pixelzsize) / 2.0);
+  g.cos_beta_max = std::cos(g.beta_max);
+  hit_key_for(g.thr, &g.hit_key, &g.has_hit);
+  c->H = (int32_t)nrhkl;
+  c->nrot = (int32_t)nrot;
+  c->hkl.assign(hkl, hkl + 3 * nrhkl);
+  ENSURE(c, c->d_hkl, sizeof(double) * 3 * nrhkl);
+  ENSURE(c, c->d_rot, sizeof(double) * nrot);
+  ENSURE(c, c->d_cs, sizeof(double) * 2 * nrot);
+  HIPCHK(c, hipMemcpyAsync(c->d_hkl.p, hkl, sizeof(double) * 3 * nrhkl, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_rot.p, rot, sizeof(double) * nrot, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_omega_cs(P<double>(c->d_rot), (int32_t)nrot, P<double>(c->d_cs), c->st));
+  grs_status s = upload_geo(c);
+  if (s != GRS_OK) return s;
+  c->geo_set = true;
+  return GRS_OK;
+}
+
+grs_status grs_set_distance_stack(grs_ctx *c, const float *proj, int64_t nz, int64_t ny, int64_t nrot,
+                                  int32_t on_device) {
+  if (!c) return GRS_ERR_ARG;
+  if (!proj || nz < 1 || ny < 1 || nrot < 1) return fail(c, GRS_ERR_ARG, "invalid distance stack shape");
+  HIPCHK(c, hipSetDevice(c->device));
+  int64_t n = nz * ny * nrot;
+  ENSURE(c, c->d_keys, sizeof(uint32_t) * n);
+  HIPCHK(c, hipMemcpyAsync(c->d_keys.p, proj, sizeof(float) * n,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+  ENSURE(c, c->d_bad, sizeof(uint32_t));
+  HIPCHK(c, hipMemsetAsync(c->d_bad.p, 0, sizeof(uint32_t), c->st));
+  HIPCHK(c, grs_launch_check_keys(P<uint32_t>(c->d_keys), n, P<uint32_t>(c->d_bad), c->st));
+  uint32_t bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (bad) {
+    c->stack_set = false;
+    return fail(c, GRS_ERR_ARG, "distance stack holds negative or NaN values");
+  }
+  c->snz = nz;
+  c->sny = ny;
+  c->snrot = nrot;
+  c->stack_set = true;
+  if (c->geo_set) return upload_geo(c);
+  return GRS_OK;
+}
+
+static grs_status check_ready(grs_ctx *c) {
+  if (!c->geo_set) return fail(c, GRS_ERR_STATE, "geometry not set");
+  if (!c->stack_set) return fail(c, GRS_ERR_STATE, "distance stack not set");
+  if ((double)c->snz < c->hgeo.detzsize || (double)c->sny < c->hgeo.detysize || c->snrot < c->nrot)
+    return fail(c, GRS_ERR_ARG,
+                "distance stack %lld x %lld x %lld is smaller than detzsize x detysize x nrot = %g x %g x %d",
+                (long long)c->snz, (long long)c->sny, (long long)c->snrot, c->hgeo.detzsize, c->hgeo.detysize,
+                c->nrot);
+  if (c->hgeo.nz != c->snz || c->hgeo.ny != c->sny) return upload_geo(c);
+  return GRS_OK;
+}
+
+/* Runs the scoring kernel on evaluations already described on the device.
+ * d_pos holds V voxel positions; d_sub/d_valid the orientations. */
+static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, const int32_t *d_evox,
+                           const int32_t *d_eu, int32_t method, int32_t *d_ns, int32_t *d_ni, double *d_C,
+                           float *d_med) {
+  if (n_eval <= 0) return GRS_OK;
+  const int64_t stride = (int64_t)c->nrot * c->H;
+  grs_eval_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = P<grs_dgeo>(c->d_geo);
+  a.hkl = P<double>(c->d_hkl);
+  a.cs = P<double>(c->d_cs);
+  a.pos = P<double>(c->d_pos);
+  a.SUB = P<double>(c->d_sub);
+  a.valid = P<int32_t>(c->d_valid);
+  a.keys = P<uint32_t>(c->d_keys);
+  a.n_eval = n_eval;
+  a.map = map;
+  a.eval_vox = d_evox;
+  a.eval_u = d_eu;
+  a.scratch_stride = stride;
+  a.out_ns = d_ns;
+  a.out_ni = d_ni;
+  a.out_C = d_C;
+  a.out_med = d_med;
+  if (method == GRS_METHOD_FAST) {
+    if (c->H > 240) return fail(c, GRS_ERR_ARG, "the fast method supports at most 240 reflections (got %d)", c->H);
+    ENSURE(c, c->d_vw, sizeof(double) * GRS_VW_DOUBLES * (size_t)V * c->nrot);
+    HIPCHK(c, grs_launch_vw_table(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
+                                  P<double>(c->d_vw), c->st));
+    int64_t waves = grs_fast_wave_slots(c->device);
+    if (waves > n_eval) waves = ((n_eval + 3) / 4) * 4;
+    ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)waves * stride);
+    a.vw = P<double>(c->d_vw);
+    a.scratch = P<uint32_t>(c->d_scratch);
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, grs_launch_fast(&a, c->H, waves, c->st));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  } else if (method == GRS_METHOD_PLAIN) {
+    int64_t T = n_eval < 32768 ? n_eval : 32768;
+    ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)T * stride);
+    a.scratch = P<uint32_t>(c->d_scratch);
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, grs_launch_plain(&a, T, c->st));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  } else {
+    return fail(c, GRS_ERR_ARG, "unknown method %d", method);
+  }
+  if (c->timing) {
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->ms_total += ms;
+    c->launches += 1;
+  }
+  return GRS_OK;
+}
+
+static grs_status forward_comp_impl(grs_ctx *c, const double *pos, const double *q, int64_t N, bool q_dev,
+                                    int32_t *ns, int32_t *ni, double *C, float *med, bool out_dev,
+                                    int32_t method) {
+  if (!c) return GRS_ERR_ARG;
+  if (N < 0 || !pos || (N > 0 && (!q || !ns || !ni || !C || !med))) return fail(c, GRS_ERR_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (N == 0) return GRS_OK;
+  const double *d_q = q;
+  if (!q_dev) {
+    ENSURE(c, c->d_q, sizeof(double) * 4 * N);
+    HIPCHK(c, hipMemcpyAsync(c->d_q.p, q, sizeof(double) * 4 * N, hipMemcpyHostToDevice, c->st));
+    d_q = P<double>(c->d_q);
+  }
+  ENSURE(c, c->d_sub, sizeof(double) * 9 * N);
+  ENSURE(c, c->d_valid, sizeof(int32_t) * N);
+  ENSURE(c, c->d_pos, sizeof(double) * 3);
+  HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), d_q, N, P<double>(c->d_sub), P<int32_t>(c->d_valid),
+                                  c->st));
+  int32_t *d_ns = ns, *d_ni = ni;
+  double *d_C = C;
+  float *d_med = med;
+  if (!out_dev) {
+    ENSURE(c, c->d_ns, sizeof(int32_t) * N);
+    ENSURE(c, c->d_ni, sizeof(int32_t) * N);
+    ENSURE(c, c->d_C, sizeof(double) * N);
+    ENSURE(c, c->d_med, sizeof(float) * N);
+    d_ns = P<int32_t>(c->d_ns);
+    d_ni = P<int32_t>(c->d_ni);
+    d_C = P<double>(c->d_C);
+    d_med = P<float>(c->d_med);
+  }
+  s = run_eval(c, GRS_MAP_ONE_VOXEL, N, 1, nullptr, nullptr, method, d_ns, d_ni, d_C, d_med);
+  if (s != GRS_OK) return s;
+  if (!out_dev) {
+    HIPCHK(c, hipMemcpyAsync(ns, d_ns, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ni, d_ni, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(C, d_C, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(med, d_med, sizeof(float) * N, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  return GRS_OK;
+}
+
+grs_status grs_forward_comp(grs_ctx *c, const double *pos, const double *q, int64_t N, int32_t *ns, int32_t *ni,
+                            double *C, float *med, int32_t method) {
+  return forward_comp_impl(c, pos, q, N, false, ns, ni, C, med, false, method);
+}
+
+grs_status grs_forward_comp_device(grs_ctx *c, const double *pos, const double *d_q, int64_t N, int32_t *d_ns,
+                                   int32_t *d_ni, double *d_C, float *d_med, int32_t method) {
+  return forward_comp_impl(c, pos, d_q, N, true, d_ns, d_ni, d_C, d_med, true, method);
+}
+
+/* voxel rows per device chunk (bounds the per-(voxel, omega) table) */
+static int64_t voxel_chunk(const grs_ctx *c) {
+  int64_t per = (int64_t)c->nrot * GRS_VW_DOUBLES * 8;
+  int64_t m = ((int64_t)1 << 30) / (per > 0 ? per : 1);
+  if (m < 256) m = 256;
+  if (m > 32768) m = 32768;
+  return m;
+}
+
+grs_status grs_grow_indexed_region(grs_ctx *c, const double *vox, int64_t M, const double *U, const double *p1,
+                                   double *out, int32_t method) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || (M > 0 && (!vox || !U || !p1 || !out))) return fail(c, GRS_ERR_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (M == 0) return GRS_OK;
+  /* only voxels inside the mask are scored */
+  std::vector<int64_t> idx;
+  idx.reserve(M);
+  for (int64_t i = 0; i < M; i++) {
+    double *o = out + 7 * i;
+    o[0] = vox[6 * i];
+    o[1] = vox[6 * i + 1];
+    o[2] = vox[6 * i + 2];
+    o[3] = o[4] = o[5] = o[6] = 0.0;
+    if (vox[6 * i + 4] > 0.0) idx.push_back(i);
+  }
+  ENSURE(c, c->d_U, sizeof(double) * 9);
+  ENSURE(c, c->d_sub, sizeof(double) * 9);
+  ENSURE(c, c->d_valid, sizeof(int32_t));
+  ENSURE(c, c->d_p1, sizeof(double) * 14);
+  HIPCHK(c, hipMemcpyAsync(c->d_U.p, U, sizeof(double) * 9, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_p1.p, p1, sizeof(double) * 14, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_sub_from_U(P<grs_dgeo>(c->d_geo), P<double>(c->d_U), 1, P<double>(c->d_sub),
+                                  P<int32_t>(c->d_valid), c->st));
+  const int64_t chunk = voxel_chunk(c);
+  std::vector<double> rows;
+  std::vector<int32_t> hns, hni;
+  std::vector<double> hC;
+  std::vector<float> hmed;
+  for (int64_t b = 0; b < (int64_t)idx.size(); b += chunk) {
+    int64_t m = (int64_t)idx.size() - b;
+    if (m > chunk) m = chunk;
+    rows.resize(3 * m);
+    for (int64_t k = 0; k < m; k++)
+      for (int j = 0; j < 3; j++) rows[3 * k + j] = vox[6 * idx[b + k] + j];
+    ENSURE(c, c->d_rows, sizeof(double) * 3 * m);
+    ENSURE(c, c->d_pos, sizeof(double) * 3 * m);
+    ENSURE(c, c->d_ns, sizeof(int32_t) * m);
+    ENSURE(c, c->d_ni, sizeof(int32_t) * m);
+    ENSURE(c, c->d_C, sizeof(double) * m);
+    ENSURE(c, c->d_med, sizeof(float) * m);
+    HIPCHK(c, hipMemcpyAsync(c->d_rows.p, rows.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_voxel_pos(P<double>(c->d_rows), m, 3, P<double>(c->d_p1), P<double>(c->d_pos), c->st));
+    s = run_eval(c, GRS_MAP_ONE_U, m, m, nullptr, nullptr, method, P<int32_t>(c->d_ns), P<int32_t>(c->d_ni),
+                 P<double>(c->d_C), P<float>(c->d_med));
+    if (s != GRS_OK) return s;
+    hns.resize(m);
+    hni.resize(m);
+    hC.resize(m);
+    hmed.resize(m);
+    HIPCHK(c, hipMemcpyAsync(hns.data(), c->d_ns.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hni.data(), c->d_ni.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hC.data(), c->d_C.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hmed.data(), c->d_med.p, sizeof(float) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int64_t k = 0; k < m; k++) {
+      const double *v = vox + 6 * idx[b + k];
+      double *o = out + 7 * idx[b + k];
+      double Cn = hC[k], dm = (double)hmed[k];
+      double lim = nanmin2(v[5], p1[13]);
+      bool flag = (Cn > v[3] && dm <= lim) && (Cn >= p1[11] * (1.0 - p1[12]));
+      o[3] = (double)hns[k];
+      o[4] = (double)hni[k];
+      o[5] = dm;
+      o[6] = flag ? 1.0 : 0.0;
+    }
+  }
+  return GRS_OK;
+}
+
+grs_status grs_forward_comp_compete(grs_ctx *c, const double *vox, int64_t M, const double *id_all, int64_t n_id,
+                                    const double *euler, int64_t G, const double *p1, double *out,
+                                    int32_t method) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || n_id < 0 || G < 0 || (M > 0 && (!vox || !p1 || !out)) || (n_id > 0 && !id_all) ||
+      (G > 0 && !euler))
+    return fail(c, GRS_ERR_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (M == 0) return GRS_OK;
+  for (int64_t j = 0; j < n_id; j++) {
+    double g = id_all[j];
+    if (!(g >= 1.0 && g <= (double)G && g == std::floor(g)))
+      return fail(c, GRS_ERR_ARG, "id_all[%lld] = %g is not a grain id in 1..%lld", (long long)j, g, (long long)G);
+  }
+  for (int64_t i = 0; i < M; i++) {
+    double a = vox[5 * i + 3], b = vox[5 * i + 4];
+    if (!(a == std::floor(a) && b == std::floor(b) && a >= 1.0 && b <= (double)n_id && b >= a - 1.0))
+      return fail(c, GRS_ERR_ARG, "voxel row %lld has candidate range [%g, %g] outside 1..%lld", (long long)i, a, b,
+                  (long long)n_id);
+  }
+  ENSURE(c, c->d_eu, sizeof(double) * 3 * (G > 0 ? G : 1));
+  ENSURE(c, c->d_sub, sizeof(double) * 9 * (G > 0 ? G : 1));
+  ENSURE(c, c->d_valid, sizeof(int32_t) * (G > 0 ? G : 1));
+  ENSURE(c, c->d_p1, sizeof(double) * 11);
+  if (G > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->d_eu.p, euler, sizeof(double) * 3 * G, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_sub_from_euler(P<grs_dgeo>(c->d_geo), P<double>(c->d_eu), G, P<double>(c->d_sub),
+                                        P<int32_t>(c->d_valid), c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_p1.p, p1, sizeof(double) * 11, hipMemcpyHostToDevice, c->st));
+  const int64_t chunk = voxel_chunk(c);
+  std::vector<double> rows;
+  std::vector<int32_t> evox, eu;
+  std::vector<int64_t> rs;
+  std::vector<int64_t> best;
+  std::vector<int32_t> hni;
+  std::vector<double> hC;
+  std::vector<float> hmed;
+  for (int64_t b = 0; b < M; b += chunk) {
+    int64_t m = M - b;
+    if (m > chunk) m = chunk;
+    rows.resize(3 * m);
+    rs.assign(m + 1, 0);
+    evox.clear();
+    eu.clear();
+    for (int64_t k = 0; k < m; k++) {
+      const double *v = vox + 5 * (b + k);
+      rows[3 * k] = v[0];
+      rows[3 * k + 1] = v[1];
+      rows[3 * k + 2] = v[2];
+      int64_t lo = (int64_t)v[3], hi = (int64_t)v[4];
+      for (int64_t j = lo; j <= hi; j++) {
+        evox.push_back((int32_t)k);
+        eu.push_back((int32_t)id_all[j - 1] - 1);
+      }
+      rs[k + 1] = (int64_t)evox.size();
+    }
+    int64_t E = (int64_t)evox.size();
+    ENSURE(c, c->d_rows, sizeof(double) * 3 * m);
+    ENSURE(c, c->d_pos, sizeof(double) * 3 * m);
+    ENSURE(c, c->d_rowstart, sizeof(int64_t) * (m + 1));
+    ENSURE(c, c->d_best, sizeof(int64_t) * m);
+    ENSURE(c, c->d_evox, sizeof(int32_t) * (E > 0 ? E : 1));
+    ENSURE(c, c->d_eu_idx, sizeof(int32_t) * (E > 0 ? E : 1));
+    ENSURE(c, c->d_ns, sizeof(int32_t) * (E > 0 ? E : 1));
+    ENSURE(c, c->d_ni, sizeof(int32_t) * (E > 0 ? E : 1));
+    ENSURE(c, c->d_C, sizeof(double) * (E > 0 ? E : 1));
+    ENSURE(c, c->d_med, sizeof(float) * (E > 0 ? E : 1));
+    HIPCHK(c, hipMemcpyAsync(c->d_rows.p, rows.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_rowstart.p, rs.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, c->st));
+    if (E > 0) {
+      HIPCHK(c, hipMemcpyAsync(c->d_evox.p, evox.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(c->d_eu_idx.p, eu.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->st));
+    }
+    HIPCHK(c, grs_launch_voxel_pos(P<double>(c->d_rows), m, 3, P<double>(c->d_p1), P<double>(c->d_pos), c->st));
+    s = run_eval(c, GRS_MAP_PAIRS, E, m, P<int32_t>(c->d_evox), P<int32_t>(c->d_eu_idx), method,
+                 P<int32_t>(c->d_ns), P<int32_t>(c->d_ni), P<double>(c->d_C), P<float>(c->d_med));
+    if (s != GRS_OK) return s;
+    HIPCHK(c, grs_launch_compete_reduce(P<int64_t>(c->d_rowstart), m, P<int32_t>(c->d_ns), P<int32_t>(c->d_ni),
+                                        P<double>(c->d_C), P<float>(c->d_med), P<int64_t>(c->d_best), c->st));
+    best.resize(m);
+    hni.resize(E > 0 ? E : 1);
+    hC.resize(E > 0 ? E : 1);
+    hmed.resize(E > 0 ? E : 1);
+    HIPCHK(c, hipMemcpyAsync(best.data(), c->d_best.p, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->st));
+    if (E > 0) {
+      HIPCHK(c, hipMemcpyAsync(hni.data(), c->d_ni.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipMemcpyAsync(hC.data(), c->d_C.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipMemcpyAsync(hmed.data(), c->d_med.p, sizeof(float) * E, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int64_t k = 0; k < m; k++) {
+      double *o = out + 8 * (b + k);
+      o[0] = (double)(b + k + 1);
+      int64_t j = best[k];
+      if (j < 0) {
+        for (int t = 1; t < 8; t++) o[t] = 0.0;
+        continue;
+      }
+      int32_t gi = eu[j];
+      o[1] = euler[3 * gi];
+      o[2] = euler[3 * gi + 1];
+      o[3] = euler[3 * gi + 2];
+      o[4] = (double)(gi + 1);
+      o[5] = hC[j];
+      o[6] = (double)hni[j];
+      o[7] = (double)hmed[j];
+    }
+  }
+  return GRS_OK;
+}
+
+/* ---- legacy single-precision operators ---------------------------------- */
+
+static std::mutex g_legacy_mu;
+static grs_ctx *g_legacy = nullptr;
+
+static grs_ctx *legacy_ctx() {
+  if (!g_legacy) {
+    if (grs_create(0, &g_legacy) != GRS_OK) {
+      std::snprintf(g_legacy_err, sizeof(g_legacy_err), "no HIP device available");
+      g_legacy = nullptr;
+    }
+  }
+  return g_legacy;
+}
+
+static grs_status legacy_setup(grs_ctx *c, const float *S, const float *B, const float *hkl, int64_t n_hkl,
+                               const float *proj, int64_t n_proj, const float *rot, int64_t n_rot,
+                               const float *RotDet, const float *param, int64_t n_param) {
+  if (!S || !B || !hkl || !proj || !rot || !RotDet || !param) return fail(c, GRS_ERR_ARG, "null argument");
+  if (n_param != 20) return fail(c, GRS_ERR_ARG, "param must hold 20 values (got %lld)", (long long)n_param);
+  int64_t H = (int64_t)param[0], nrot = (int64_t)param[1];
+  if ((double)H != (double)param[0] || (double)nrot != (double)param[1] || H < 1 || nrot < 1)
+    return fail(c, GRS_ERR_ARG, "param(1:2) must be positive integers");
+  if (n_hkl != 3 * H) return fail(c, GRS_ERR_ARG, "hkl holds %lld values, expected 3*nr_hkl = %lld",
+                                  (long long)n_hkl, (long long)(3 * H));
+  if (n_rot != nrot) return fail(c, GRS_ERR_ARG, "rot_angles holds %lld values, expected %lld", (long long)n_rot,
+                                 (long long)nrot);
+  double ysz = param[10], zsz = param[11];
+  int64_t ny = (int64_t)ysz, nz = (int64_t)zsz;
+  if ((double)ny != ysz || (double)nz != zsz || ny < 1 || nz < 1)
+    return fail(c, GRS_ERR_ARG, "detector size must be positive integers");
+  if (n_proj != ny * nz * nrot)
+    return fail(c, GRS_ERR_ARG, "projection stack holds %lld values, expected detzsize*detysize*nr_rot = %lld",
+                (long long)n_proj, (long long)(ny * nz * nrot));
+  grs_geometry g;
+  std::memset(&g, 0, sizeof(g));
+  for (int k = 0; k < 9; k++) {
+    g.S[k] = S[k];
+    g.B[k] = B[k];
+    g.RotDet[k] = RotDet[k];
+  }
+  g.Lsam2sou = param[2];
+  g.Lsam2det = param[3];
+  g.P0y = param[4];
+  g.P0z = param[5];
+  g.dety00 = param[6];
+  g.detz00 = param[7];
+  g.pixelysize = param[8];
+  g.pixelzsize = param[9];
+  g.detysize = ysz;
+  g.detzsize = zsz;
+  g.BeamStopY[0] = param[12];
+  g.BeamStopY[1] = param[13];
+  g.BeamStopZ[0] = param[14];
+  g.BeamStopZ[1] = param[15];
+  g.thetamax = param[16];
+  g.lambda_min = param[17];
+  g.lambda_max = param[18];
+  g.RotAxisOffset = param[19];
+  g.dety0 = ysz / 2.0 + 0.5;
+  g.detz0 = zsz / 2.0 + 0.5;
+  g.minEucDis = 0.5 * ((g.pixelysize + g.pixelzsize) / 2.0);
+  std::vector<double> h(3 * H), r(nrot);
+  for (int64_t k = 0; k < 3 * H; k++) h[k] = hkl[k];
+  for (int64_t k = 0; k < nrot; k++) r[k] = rot[k];
+  grs_status s = grs_set_geometry(c, &g, h.data(), H, r.data(), nrot);
+  if (s != GRS_OK) return s;
+  return grs_set_distance_stack(c, proj, nz, ny, nrot, 0);
+}
+
+static grs_status legacy_ret(grs_ctx *c, grs_status s) {
+  if (s != GRS_OK && c) std::snprintf(g_legacy_err, sizeof(g_legacy_err), "%s", c->err);
+  return s;
+}
+
+grs_status grs_cuda_forward_comp(const float *pos, const float *q, int64_t n_q, const float *S, const float *B,
+                                 const float *hkl, int64_t n_hkl, const float *proj, int64_t n_proj,
+                                 const float *rot, int64_t n_rot, const float *RotDet, const float *param,
+                                 int64_t n_param, float *out) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  grs_ctx *c = legacy_ctx();
+  if (!c) return GRS_ERR_DEVICE;
+  if (!pos || !q || !out || n_q < 0 || n_q % 4) return legacy_ret(c, fail(c, GRS_ERR_ARG, "q must hold 4*N values"));
+  grs_status s = legacy_setup(c, S, B, hkl, n_hkl, proj, n_proj, rot, n_rot, RotDet, param, n_param);
+  if (s != GRS_OK) return legacy_ret(c, s);
+  int64_t N = n_q / 4;
+  std::vector<double> qd(n_q), pd = {pos[0], pos[1], pos[2]};
+  for (int64_t k = 0; k < n_q; k++) qd[k] = q[k];
+  std::vector<int32_t> ns(N), ni(N);
+  std::vector<double> C(N);
+  std::vector<float> med(N);
+  s = grs_forward_comp(c, pd.data(), qd.data(), N, ns.data(), ni.data(), C.data(), med.data(), GRS_METHOD_FAST);
+  if (s != GRS_OK) return legacy_ret(c, s);
+  for (int64_t i = 0; i < N; i++) {
+    out[4 * i] = (float)ns[i];
+    out[4 * i + 1] = (float)ni[i];
+    out[4 * i + 2] = (float)C[i];
+    out[4 * i + 3] = med[i];
+  }
+  return GRS_OK;
+}
+
+grs_status grs_cuda_grow_indexed_region(const float *vox, int64_t n_vox, const float *U, const float *S,
+                                        const float *B, const float *hkl, int64_t n_hkl, const float *proj,
+                                        int64_t n_proj, const float *rot, int64_t n_rot, const float *RotDet,
+                                        const float *param, int64_t n_param, const float *param1,
+                                        int64_t n_param1, float *out) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  grs_ctx *c = legacy_ctx();
+  if (!c) return GRS_ERR_DEVICE;
+  if (!vox || !U || !param1 || !out || n_vox < 0 || n_vox % 6)
+    return legacy_ret(c, fail(c, GRS_ERR_ARG, "voxel list must hold 6*M values"));
+  if (n_param1 != 14) return legacy_ret(c, fail(c, GRS_ERR_ARG, "param1 must hold 14 values"));
+  grs_status s = legacy_setup(c, S, B, hkl, n_hkl, proj, n_proj, rot, n_rot, RotDet, param, n_param);
+  if (s != GRS_OK) return legacy_ret(c, s);
+  int64_t M = n_vox / 6;
+  std::vector<double> v(n_vox), Ud(9), p1(14), o(7 * M);
+  for (int64_t k = 0; k < n_vox; k++) v[k] = vox[k];
+  for (int k = 0; k < 9; k++) Ud[k] = U[k];
+  for (int k = 0; k < 14; k++) p1[k] = param1[k];
+  s = grs_grow_indexed_region(c, v.data(), M, Ud.data(), p1.data(), o.data(), GRS_METHOD_FAST);
+  if (s != GRS_OK) return legacy_ret(c, s);
+  for (int64_t k = 0; k < 7 * M; k++) out[k] = (float)o[k];
+  return GRS_OK;
+}
+
+grs_status grs_cuda_forward_comp_compete(const float *vox, int64_t n_vox, const float *id_all, int64_t n_id,
+                                         const float *euler, int64_t n_euler, const float *S, const float *B,
+                                         const float *hkl, int64_t n_hkl, const float *proj, int64_t n_proj,
+                                         const float *rot, int64_t n_rot, const float *RotDet,
+                                         const float *param, int64_t n_param, const float *param1,
+                                         int64_t n_param1, float *out) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  grs_ctx *c = legacy_ctx();
+  if (!c) return GRS_ERR_DEVICE;
+  if (!vox || !param1 || !out || n_vox < 0 || n_vox % 5 || n_euler < 0 || n_euler % 3 || n_id < 0)
+    return legacy_ret(c, fail(c, GRS_ERR_ARG, "voxel list must hold 5*M values and Euler 3*G values"));
+  if (n_param1 != 11) return legacy_ret(c, fail(c, GRS_ERR_ARG, "param1 must hold 11 values"));
+  grs_status s = legacy_setup(c, S, B, hkl, n_hkl, proj, n_proj, rot, n_rot, RotDet, param, n_param);
+  if (s != GRS_OK) return legacy_ret(c, s);
+  int64_t M = n_vox / 5, G = n_euler / 3;
+  std::vector<double> v(n_vox), ids(n_id > 0 ? n_id : 1), eu(n_euler > 0 ? n_euler : 1), p1(11), o(8 * M);
+  for (int64_t k = 0; k < n_vox; k++) v[k] = vox[k];
+  for (int64_t k = 0; k < n_id; k++) ids[k] = id_all[k];
+  for (int64_t k = 0; k < n_euler; k++) eu[k] = euler[k];
+  for (int k = 0; k < 11; k++) p1[k] = param1[k];
+  s = grs_forward_comp_compete(c, v.data(), M, ids.data(), n_id, eu.data(), G, p1.data(), o.data(),
+                               GRS_METHOD_FAST);
+  if (s != GRS_OK) return legacy_ret(c, s);
+  for (int64_t k = 0; k < 8 * M; k++) out[k] = (float)o[k];
+  return GRS_OK;
+}
+
+/* ---- input preparation ---------------------------------------------------- */
+
+grs_status grs_render_spots(grs_ctx *c, const double *pos, const int32_t *grain, int64_t M, const double *U,
+                            int64_t G, int32_t clear) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || G < 0 || (M > 0 && (!pos || !grain || !U || G == 0))) return fail(c, GRS_ERR_ARG, "null argument");
+  if (!c->geo_set) return fail(c, GRS_ERR_STATE, "geometry not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  for (int64_t i = 0; i < M; i++)
+    if (grain[i] < 0 || grain[i] >= G) return fail(c, GRS_ERR_ARG, "grain[%lld] = %d outside 0..%lld",
+                                                   (long long)i, grain[i], (long long)(G - 1));
+  const int64_t nz = (int64_t)c->hgeo.detzsize, ny = (int64_t)c->hgeo.detysize, nr = c->nrot;
+  if ((double)nz != c->hgeo.detzsize || (double)ny != c->hgeo.detysize)
+    return fail(c, GRS_ERR_ARG, "rendering needs an integer detector size");
+  const int64_t n = nz * ny * nr;
+  if (c->mnz != nz || c->mny != ny || c->mnrot != nr) clear = 1;
+  ENSURE(c, c->d_mask, (size_t)n);
+  c->mnz = nz;
+  c->mny = ny;
+  c->mnrot = nr;
+  if (clear) HIPCHK(c, hipMemsetAsync(c->d_mask.p, 0, (size_t)n, c->st));
+  /* the renderer indexes the mask with the detector size, so point the geometry at it */
+  c->hgeo.nz = nz;
+  c->hgeo.ny = ny;
+  ENSURE(c, c->d_geo, sizeof(grs_dgeo));
+  HIPCHK(c, hipMemcpyAsync(c->d_geo.p, &c->hgeo, sizeof(grs_dgeo), hipMemcpyHostToDevice, c->st));
+  ENSURE(c, c->d_U, sizeof(double) * 9 * G);
+  ENSURE(c, c->d_sub, sizeof(double) * 9 * G);
+  ENSURE(c, c->d_valid, sizeof(int32_t) * G);
+  HIPCHK(c, hipMemcpyAsync(c->d_U.p, U, sizeof(double) * 9 * G, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_sub_from_U(P<grs_dgeo>(c->d_geo), P<double>(c->d_U), G, P<double>(c->d_sub),
+                                  P<int32_t>(c->d_valid), c->st));
+  const int64_t chunk = (int64_t)1 << 22;
+  for (int64_t b = 0; b < M; b += chunk) {
+    int64_t m = M - b < chunk ? M - b : chunk;
+    ENSURE(c, c->d_pos, sizeof(double) * 3 * m);
+    ENSURE(c, c->d_grain, sizeof(int32_t) * m);
+    HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos + 3 * b, sizeof(double) * 3 * m, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_grain.p, grain + b, sizeof(int32_t) * m, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_render(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), P<int32_t>(c->d_grain), m, c->nrot,
+                                P<double>(c->d_sub), P<double>(c->d_hkl), P<double>(c->d_cs), P<uint8_t>(c->d_mask),
+                                c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  /* restore the stack shape the scoring kernels index with */
+  return upload_geo(c);
+}
+
+grs_status grs_set_binary_stack(grs_ctx *c, const uint8_t *mask, int64_t nz, int64_t ny, int64_t nrot,
+                                int32_t on_device) {
+  if (!c) return GRS_ERR_ARG;
+  if (!mask || nz < 1 || ny < 1 || nrot < 1) return fail(c, GRS_ERR_ARG, "invalid mask shape");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t n = nz * ny * nrot;
+  ENSURE(c, c->d_mask, (size_t)n);
+  HIPCHK(c, hipMemcpyAsync(c->d_mask.p, mask, (size_t)n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->mnz = nz;
+  c->mny = ny;
+  c->mnrot = nrot;
+  return GRS_OK;
+}
+
+grs_status grs_distance_from_mask(grs_ctx *c) {
+  if (!c) return GRS_ERR_ARG;
+  if (c->mnz < 1) return fail(c, GRS_ERR_STATE, "no mask: render or set one first");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t nz = c->mnz, ny = c->mny, nr = c->mnrot, n = nz * ny * nr;
+  ENSURE(c, c->d_g1, sizeof(int32_t) * n);
+  ENSURE(c, c->d_keys, sizeof(float) * n);
+  HIPCHK(c, grs_launch_edt_cols(P<uint8_t>(c->d_mask), nz, ny * nr, P<int32_t>(c->d_g1), c->st));
+  int64_t per_slice = nz * ny * (int64_t)(sizeof(int32_t) + sizeof(double));
+  int64_t ns = ((int64_t)1536 << 20) / (per_slice > 0 ? per_slice : 1);
+  if (ns < 1) ns = 1;
+  if (ns > nr) ns = nr;
+  ENSURE(c, c->d_edt_v, sizeof(int32_t) * (size_t)(nz * ny * ns));
+  ENSURE(c, c->d_edt_z, sizeof(double) * (size_t)(nz * ny * ns));
+  for (int64_t s0 = 0; s0 < nr; s0 += ns) {
+    int64_t k = nr - s0 < ns ? nr - s0 : ns;
+    HIPCHK(c, grs_launch_edt_rows(P<int32_t>(c->d_g1), nz, ny, s0, k, P<int32_t>(c->d_edt_v),
+                                  P<double>(c->d_edt_z), P<float>(c->d_keys), c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->snz = nz;
+  c->sny = ny;
+  c->snrot = nr;
+  c->stack_set = true;
+  if (c->geo_set) return upload_geo(c);
+  return GRS_OK;
+}
+
+grs_status grs_get_mask(grs_ctx *c, uint8_t *out, int64_t n) {
+  if (!c) return GRS_ERR_ARG;
+  if (!out || n != c->mnz * c->mny * c->mnrot) return fail(c, GRS_ERR_ARG, "output size does not match the mask");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_mask.p, (size_t)n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_get_distance_stack(grs_ctx *c, float *out, int64_t n) {
+  if (!c) return GRS_ERR_ARG;
+  if (!c->stack_set) return fail(c, GRS_ERR_STATE, "distance stack not set");
+  if (!out || n != c->snz * c->sny * c->snrot) return fail(c, GRS_ERR_ARG, "output size does not match the stack");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_keys.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_device_alloc(grs_ctx *c, int64_t bytes, void **d_out) {
+  if (!c || !d_out || bytes < 0) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  *d_out = nullptr;
+  HIPCHK(c, hipMalloc(d_out, bytes > 0 ? (size_t)bytes : 8));
+  return GRS_OK;
+}
+
+grs_status grs_device_free(grs_ctx *c, void *d) {
+  if (!c) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (d) HIPCHK(c, hipFree(d));
+  return GRS_OK;
+}
+
+grs_status grs_copy_to_device(grs_ctx *c, void *d_dst, const void *src, int64_t bytes) {
+  if (!c || !d_dst || !src || bytes < 0) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(d_dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_copy_to_host(grs_ctx *c, void *dst, const void *d_src, int64_t bytes) {
+  if (!c || !dst || !d_src || bytes < 0) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(dst, d_src, (size_t)bytes, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+/* ---- measurement and diagnostics ---------------------------------------- */
+
+grs_status grs_set_timing(grs_ctx *c, int32_t enable) {
+  if (!c) return GRS_ERR_ARG;
+  c->timing = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_kernel_time(grs_ctx *c, double *ms, int64_t *launches, int32_t reset) {
+  if (!c) return GRS_ERR_ARG;
+  if (ms) *ms = c->ms_total;
+  if (launches) *launches = c->launches;
+  if (reset) {
+    c->ms_total = 0.0;
+    c->launches = 0;
+  }
+  return GRS_OK;
+}
+
+grs_status grs_math_device(grs_ctx *c, int32_t fn, const double *x, double *y, int64_t n) {
+  if (!c) return GRS_ERR_ARG;
+  if (n < 0 || (n > 0 && (!x || !y))) return fail(c, GRS_ERR_ARG, "null argument");
+  if (n == 0) return GRS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  ENSURE(c, c->d_x, sizeof(double) * n);
+  ENSURE(c, c->d_y, sizeof(double) * n);
+  HIPCHK(c, hipMemcpyAsync(c->d_x.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_math_eval(fn, P<double>(c->d_x), P<double>(c->d_y), n, c->st));
+  HIPCHK(c, hipMemcpyAsync(y, c->d_y.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_synchronize(grs_ctx *c) {
+  if (!c) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+}  // extern "C"

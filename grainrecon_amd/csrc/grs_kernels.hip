@@ -1,0 +1,556 @@
+/*
+ * grs_kernels.hip -- gfx950 kernels of the completeness-scoring engine.
+ *
+ * Hot path: for every evaluation (voxel, orientation) and every event
+ * (omega, hkl) decide whether the reflection lands on the detector
+ * (forward_comp.m:28-65), look the pixel up in the distance stack
+ * (forward_comp.m:69-72) and reduce the integer counts and the median
+ * distance (forward_comp.m:67, 86-90).
+ *
+ * k_fast  -- one 64-wide wavefront per evaluation.
+ *   phase A: lanes walk the flattened (omega, hkl) event space 64 events at a
+ *            time and apply a conservative Bragg/wavelength filter that only
+ *            needs 4 multiplies and 3 FMAs per event; it rejects an event only
+ *            when the exact test is proven to fail (margins 1e-12 on cos(beta),
+ *            1e-9 relative on lambda, far above the FP64 rounding of the exact
+ *            path).  Survivors are compacted into a 128-slot LDS ring with
+ *            ballot + mbcnt.
+ *   phase B: whenever 64 survivors are queued the wave runs the exact FP64
+ *            event pipeline on them with every lane busy, gathers the
+ *            distance keys and counts hits with ballot/popcount.
+ *   median : selected keys go to a per-wave list in HBM; the median is found
+ *            by a 32-step radix bisection over the keys held in registers.
+ *   Decisions are those of the exact pipeline, so the counts are bit-exact.
+ *
+ * k_plain -- the straightforward port: one thread per evaluation, the whole
+ *   event pipeline for every event, per-thread key list and quickselect.
+ *   It is the denominator of the batched-vs-plain speedup.
+ */
+#include <hip/hip_runtime.h>
+
+#include "grs_internal.h"
+
+#define GRS_WPB 4      /* waves per block of k_fast */
+#define GRS_QCAP 128   /* LDS ring slots per wave */
+#define GRS_GWT 8      /* doubles per hkl row in LDS */
+#define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_prefix(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float key_to_float(uint32_t k) { return __uint_as_float(k); }
+
+/* ---------------------------------------------------------------------- */
+/* small set-up kernels                                                    */
+
+__global__ void k_omega_cs(const double *rot, int32_t nrot, double *cs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrot) return;
+  double c, s;
+  grs_omega_cs(rot[i], &c, &s);
+  cs[2 * i] = c;
+  cs[2 * i + 1] = s;
+}
+
+__global__ void k_voxel_pos(const double *rows, int64_t M, int32_t stride, const double *p1, double *pos) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const double *r = rows + i * stride;
+  grs_voxel_pos(r[0], r[1], r[2], p1, pos + 3 * i);
+}
+
+__global__ void k_vw_table(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
+                           double *vw) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V * nrot) return;
+  int64_t v = i / nrot;
+  int32_t om = (int32_t)(i - v * nrot);
+  grs_vw t;
+  grs_vw_terms(g, pos + 3 * v, cs[2 * om], cs[2 * om + 1], &t);
+  double *o = vw + i * GRS_VW_DOUBLES;
+  const double *src = (const double *)&t;
+#pragma unroll
+  for (int k = 0; k < GRS_VW_DOUBLES; k++) o[k] = src[k];
+}
+
+__global__ void k_sub_from_q(const grs_dgeo *g, const double *q, int64_t N, double *SUB, int32_t *valid) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double U[9];
+  int ok = grs_U_from_q(q[4 * i + 1], q[4 * i + 2], q[4 * i + 3], U);
+  valid[i] = ok;
+  if (ok) grs_SUB(g, U, SUB + 9 * i);
+  else
+    for (int k = 0; k < 9; k++) SUB[9 * i + k] = 0.0;
+}
+
+__global__ void k_sub_from_U(const grs_dgeo *g, const double *U, int64_t N, double *SUB, int32_t *valid) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  grs_SUB(g, U + 9 * i, SUB + 9 * i);
+  valid[i] = 1;
+}
+
+__global__ void k_sub_from_euler(const grs_dgeo *g, const double *eu, int64_t N, double *SUB, int32_t *valid) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double U[9];
+  grs_U_from_euler_deg(eu[3 * i], eu[3 * i + 1], eu[3 * i + 2], U);
+  grs_SUB(g, U, SUB + 9 * i);
+  valid[i] = 1;
+}
+
+__global__ void k_math_eval(int fn, const double *x, double *y, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = x[i], r;
+  switch (fn) {
+    case 0: r = grs_sin(v); break;
+    case 1: r = grs_cos(v); break;
+    case 2: r = grs_acos(v); break;
+    case 3: r = grs_atan(v); break;
+    case 4: r = grs_round(v); break;
+    case 5: r = grs_sqrt(v); break;
+    default: r = 1.0 / v; break;
+  }
+  y[i] = r;
+}
+
+/* Flags keys that are not non-negative, non-NaN floats. */
+__global__ void k_check_keys(const uint32_t *keys, int64_t n, uint32_t *bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t b = 0;
+  for (; i < n; i += stride) b |= (keys[i] > 0x7F800000u) ? 1u : 0u;
+  if (b) atomicOr(bad, 1u);
+}
+
+/* ---------------------------------------------------------------------- */
+/* mapping of evaluations                                                   */
+
+__device__ __forceinline__ void eval_slots(const grs_eval_args &a, int64_t e, int64_t *vi, int64_t *ui) {
+  if (a.map == GRS_MAP_ONE_VOXEL) {
+    *vi = 0;
+    *ui = e;
+  } else if (a.map == GRS_MAP_ONE_U) {
+    *vi = e;
+    *ui = 0;
+  } else {
+    *vi = a.eval_vox[e];
+    *ui = a.eval_u[e];
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* wave-level median over a key list in global memory                      */
+
+/* number of keys < x among list[0..n) */
+__device__ __forceinline__ int32_t wave_count_lt_global(const uint32_t *list, int32_t n, uint32_t x, int lane) {
+  int32_t c = 0;
+  for (int32_t b = 0; b < n; b += 64) {
+    int32_t j = b + lane;
+    bool p = j < n && list[j] < x;
+    c += __popcll(__ballot(p));
+  }
+  return c;
+}
+
+__device__ uint32_t wave_select_global(const uint32_t *list, int32_t n, int32_t k, int lane) {
+  uint32_t res = 0;
+  for (int bit = 31; bit >= 0; --bit) {
+    uint32_t cand = res | (1u << bit);
+    if (wave_count_lt_global(list, n, cand, lane) <= k) res = cand;
+  }
+  return res;
+}
+
+__device__ float wave_median(const uint32_t *list, int32_t n, int lane) {
+  const int32_t k1 = (n & 1) ? (n >> 1) : (n >> 1) - 1;
+  uint32_t v1, v2;
+  if (n <= 64 * GRS_REG_KEYS) {
+    uint32_t r[GRS_REG_KEYS];
+#pragma unroll
+    for (int j = 0; j < GRS_REG_KEYS; j++) {
+      int32_t idx = j * 64 + lane;
+      r[j] = idx < n ? list[idx] : 0xFFFFFFFFu;
+    }
+    uint32_t res = 0;
+    for (int bit = 31; bit >= 0; --bit) {
+      uint32_t cand = res | (1u << bit);
+      int32_t c = 0;
+#pragma unroll
+      for (int j = 0; j < GRS_REG_KEYS; j++) c += __popcll(__ballot(r[j] < cand));
+      if (c <= k1) res = cand;
+    }
+    v1 = res;
+    if (n & 1) return key_to_float(v1);
+    int32_t cle = 0;
+    uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < GRS_REG_KEYS; j++) {
+      cle += __popcll(__ballot(r[j] <= v1));
+      if (r[j] > v1 && r[j] < mn) mn = r[j];
+    }
+    v2 = cle > k1 + 1 ? v1 : wave_min_u32(mn);
+  } else {
+    v1 = wave_select_global(list, n, k1, lane);
+    if (n & 1) return key_to_float(v1);
+    int32_t cle = 0;
+    uint32_t mn = 0xFFFFFFFFu;
+    for (int32_t b = 0; b < n; b += 64) {
+      int32_t j = b + lane;
+      uint32_t kv = j < n ? list[j] : 0xFFFFFFFFu;
+      cle += __popcll(__ballot(j < n && kv <= v1));
+      if (j < n && kv > v1 && kv < mn) mn = kv;
+    }
+    v2 = cle > k1 + 1 ? v1 : wave_min_u32(mn);
+  }
+  float a = key_to_float(v1), b = key_to_float(v2);
+  return a + (b - a) / 2.0f;
+}
+
+/* ---------------------------------------------------------------------- */
+/* k_fast                                                                  */
+
+__global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t H) {
+  extern __shared__ double smem[];
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_QCAP / 2;
+  double *gwt = smem + wib * per_wave;
+  uint32_t *queue = (uint32_t *)(gwt + (int64_t)H * GRS_GWT);
+  const grs_dgeo *g = a.g;
+  const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
+  const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
+  uint32_t *wlist = a.scratch + gwave * a.scratch_stride;
+  const int32_t nrot = g->nrot;
+  const int32_t total = nrot * H;
+  const int64_t nz = g->nz, nzny = g->nz * g->ny;
+  const double m4pi = -4.0 * GRS_PI;
+  /* filter margins: see the header comment */
+  const double dcb = 1e-12;
+  const double cbm = g->cos_beta_max - dcb;
+  const double lmin = g->lambda_min, lmax = g->lambda_max;
+  const double lrel = 1e-9 * lmax;
+
+  for (int64_t e = gwave; e < a.n_eval; e += nwaves) {
+    int64_t vi, ui;
+    eval_slots(a, e, &vi, &ui);
+    if (!a.valid[ui]) {
+      if (lane == 0) {
+        a.out_ns[e] = 0;
+        a.out_ni[e] = 0;
+        a.out_C[e] = 0.0;
+        a.out_med[e] = 0.0f;
+      }
+      continue;
+    }
+    double sub[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) sub[k] = a.SUB[ui * 9 + k];
+    for (int h = lane; h < H; h += 64) {
+      double gw[3];
+      grs_Gw(sub, a.hkl + 3 * h, gw);
+      double n2 = (gw[0] * gw[0] + gw[1] * gw[1]) + gw[2] * gw[2];
+      double gn = __builtin_sqrt(n2);
+      double *t = gwt + h * GRS_GWT;
+      t[0] = gw[0];
+      t[1] = gw[1];
+      t[2] = gw[2];
+      t[3] = dcb * gn;                              /* keep if dot <  tb0 */
+      t[4] = cbm * gn;                              /* keep if dot >  tb1 */
+      double ml = lrel + 1e-12 * (4.0 * GRS_PI) / gn; /* absolute lambda margin */
+      t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
+      t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
+    }
+    wave_sync();
+
+    const double *vwv = a.vw + vi * (int64_t)nrot * GRS_VW_DOUBLES;
+    uint32_t head = 0, tail = 0;
+    int32_t ns = 0, ni = 0;
+
+    /* phase B on n queued events starting at ring position head */
+    auto drain = [&](uint32_t n) {
+      bool sel = false, hit = false;
+      uint32_t key = 0;
+      if ((uint32_t)lane < n) {
+        uint32_t ev = queue[(head + lane) & (GRS_QCAP - 1)];
+        int32_t om = (int32_t)(ev / (uint32_t)H);
+        int32_t h = (int32_t)ev - om * H;
+        grs_vw v;
+        const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
+        double *dst = (double *)&v;
+#pragma unroll
+        for (int k = 0; k < GRS_VW_DOUBLES; k++) dst[k] = src[k];
+        const double *t = gwt + h * GRS_GWT;
+        double Gt[3];
+        grs_Gt(v.c, v.s, t[0], t[1], t[2], Gt);
+        double Glen;
+        double beta = grs_event_beta(&v, Gt, &Glen);
+        int32_t dy = 0, dz = 0;
+        sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
+        if (sel) {
+          key = a.keys[(int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny];
+          hit = g->has_hit && key <= g->hit_key;
+        }
+      }
+      uint64_t ms = __ballot(sel);
+      if (sel) wlist[ns + (int32_t)lane_prefix(ms)] = key;
+      ns += (int32_t)__popcll(ms);
+      ni += (int32_t)__popcll(__ballot(hit));
+      head += n;
+    };
+
+    for (int32_t base = 0; base < total; base += 64) {
+      int32_t ev = base + lane;
+      bool pass = false;
+      if (ev < total) {
+        int32_t om = ev / H;
+        int32_t h = ev - om * H;
+        const double *vv = vwv + (int64_t)om * GRS_VW_DOUBLES;
+        double c = vv[0], s = vv[1], u0 = vv[2], u1 = vv[3], u2 = vv[4];
+        const double *t = gwt + h * GRS_GWT;
+        double Gt0 = c * t[0] - s * t[1];
+        double Gt1 = s * t[0] + c * t[1];
+        double dot = __builtin_fma(u2, t[2], __builtin_fma(u1, Gt1, u0 * Gt0));
+        double l4 = m4pi * dot;
+        pass = (dot < t[3]) && (dot > t[4]) && (l4 > t[5]) && (l4 < t[6]);
+      }
+      uint64_t m = __ballot(pass);
+      if (pass) queue[(tail + lane_prefix(m)) & (GRS_QCAP - 1)] = (uint32_t)ev;
+      tail += (uint32_t)__popcll(m);
+      if (tail - head >= 64) {
+        wave_sync();
+        drain(64);
+        wave_sync();
+      }
+    }
+    while (tail != head) {
+      uint32_t n = tail - head;
+      if (n > 64) n = 64;
+      wave_sync();
+      drain(n);
+      wave_sync();
+    }
+    __threadfence_block();
+    float med = 2000.0f;
+    if (ns > 0) med = wave_median(wlist, ns, lane);
+    if (lane == 0) {
+      a.out_ns[e] = ns;
+      a.out_ni[e] = ni;
+      a.out_C[e] = (double)ni / (double)ns;
+      a.out_med[e] = med;
+    }
+    wave_sync();
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* k_plain                                                                 */
+
+__device__ uint32_t plain_select(uint32_t *A, int64_t T, int32_t n, int32_t k) {
+  int32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t pivot = A[(int64_t)((lo + hi) >> 1) * T];
+    int32_t i = lo, j = hi;
+    while (i <= j) {
+      while (A[(int64_t)i * T] < pivot) i++;
+      while (A[(int64_t)j * T] > pivot) j--;
+      if (i <= j) {
+        uint32_t tmp = A[(int64_t)i * T];
+        A[(int64_t)i * T] = A[(int64_t)j * T];
+        A[(int64_t)j * T] = tmp;
+        i++;
+        j--;
+      }
+    }
+    if (k <= j) hi = j;
+    else if (k >= i) lo = i;
+    else break;
+  }
+  return A[(int64_t)k * T];
+}
+
+__global__ __launch_bounds__(256) void k_plain(grs_eval_args a, int64_t T) {
+  int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= T) return;
+  const grs_dgeo *g = a.g;
+  const int32_t nrot = g->nrot, H = g->H;
+  uint32_t *buf = a.scratch + tid; /* element j at buf[j*T] */
+  for (int64_t e = tid; e < a.n_eval; e += T) {
+    int64_t vi, ui;
+    eval_slots(a, e, &vi, &ui);
+    if (!a.valid[ui]) {
+      a.out_ns[e] = 0;
+      a.out_ni[e] = 0;
+      a.out_C[e] = 0.0;
+      a.out_med[e] = 0.0f;
+      continue;
+    }
+    const double *SUB = a.SUB + ui * 9;
+    const double *pos = a.pos + vi * 3;
+    int32_t ns = 0, ni = 0;
+    for (int32_t om = 0; om < nrot; om++) {
+      grs_vw v;
+      grs_vw_terms(g, pos, a.cs[2 * om], a.cs[2 * om + 1], &v);
+      for (int32_t h = 0; h < H; h++) {
+        double gw[3], Gt[3], Glen;
+        grs_Gw(SUB, a.hkl + 3 * h, gw);
+        grs_Gt(v.c, v.s, gw[0], gw[1], gw[2], Gt);
+        double beta = grs_event_beta(&v, Gt, &Glen);
+        int32_t dy, dz;
+        if (grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz)) {
+          uint32_t key = a.keys[(int64_t)(dz - 1) + (int64_t)(dy - 1) * g->nz + (int64_t)om * g->nz * g->ny];
+          buf[(int64_t)ns * T] = key;
+          ns++;
+          if (g->has_hit && key <= g->hit_key) ni++;
+        }
+      }
+    }
+    float med = 2000.0f;
+    if (ns > 0) {
+      int32_t k1 = (ns & 1) ? (ns >> 1) : (ns >> 1) - 1;
+      uint32_t v1 = plain_select(buf, T, ns, k1);
+      if (ns & 1) {
+        med = key_to_float(v1);
+      } else {
+        uint32_t v2 = 0xFFFFFFFFu;
+        for (int32_t j = k1 + 1; j < ns; j++) {
+          uint32_t x = buf[(int64_t)j * T];
+          v2 = x < v2 ? x : v2;
+        }
+        float fa = key_to_float(v1), fb = key_to_float(v2);
+        med = fa + (fb - fa) / 2.0f;
+      }
+    }
+    a.out_ns[e] = ns;
+    a.out_ni[e] = ni;
+    a.out_C[e] = (double)ni / (double)ns;
+    a.out_med[e] = med;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* compete: first maximum of C per voxel row, NaN never wins              */
+
+__global__ void k_compete_reduce(const int64_t *row_start, int64_t M, const int32_t *ns, const int32_t *ni,
+                                 const double *C, const float *med, int64_t *best) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= M) return;
+  int64_t a = row_start[r], b = row_start[r + 1];
+  int64_t bi = -1;
+  double bc = 0.0;
+  for (int64_t j = a; j < b; j++) {
+    double c = C[j];
+    if (c == c && (bi < 0 || c > bc)) {
+      bi = j;
+      bc = c;
+    }
+  }
+  if (bi < 0 && b > a) bi = a;
+  best[r] = bi;
+}
+
+/* ---------------------------------------------------------------------- */
+/* launchers                                                               */
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t grs_launch_omega_cs(const double *d_rot, int32_t nrot, double *d_cs, hipStream_t st) {
+  if (nrot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_omega_cs, dim3(blocks_for(nrot, 256)), dim3(256), 0, st, d_rot, nrot, d_cs);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t stride, const double *d_p1, double *d_pos,
+                                hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_voxel_pos, dim3(blocks_for(M, 256)), dim3(256), 0, st, d_rows, M, stride, d_p1, d_pos);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_vw_table(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs, int32_t nrot,
+                               double *d_vw, hipStream_t st) {
+  int64_t n = V * nrot;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vw_table, dim3(blocks_for(n, 128)), dim3(128), 0, st, d_g, d_pos, V, d_cs, nrot, d_vw);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_sub_from_q(const grs_dgeo *d_g, const double *d_q, int64_t N, double *d_SUB, int32_t *d_valid,
+                                 hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sub_from_q, dim3(blocks_for(N, 256)), dim3(256), 0, st, d_g, d_q, N, d_SUB, d_valid);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_sub_from_U(const grs_dgeo *d_g, const double *d_U, int64_t N, double *d_SUB, int32_t *d_valid,
+                                 hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sub_from_U, dim3(blocks_for(N, 256)), dim3(256), 0, st, d_g, d_U, N, d_SUB, d_valid);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, int64_t N, double *d_SUB,
+                                     int32_t *d_valid, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sub_from_euler, dim3(blocks_for(N, 256)), dim3(256), 0, st, d_g, d_eu, N, d_SUB, d_valid);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
+  if (a->n_eval <= 0) return hipSuccess;
+  int64_t blocks = (n_waves + GRS_WPB - 1) / GRS_WPB;
+  size_t lds = (size_t)GRS_WPB * ((size_t)H * GRS_GWT * sizeof(double) + GRS_QCAP * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_fast, dim3((unsigned)blocks), dim3(64 * GRS_WPB), lds, st, *a, H);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_plain(const grs_eval_args *a, int64_t T, hipStream_t st) {
+  if (a->n_eval <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_plain, dim3(blocks_for(T, 256)), dim3(256), 0, st, *a, T);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_compete_reduce(const int64_t *d_row_start, int64_t M, const int32_t *d_ns, const int32_t *d_ni,
+                                     const double *d_C, const float *d_med, int64_t *d_best, hipStream_t st) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compete_reduce, dim3(blocks_for(M, 256)), dim3(256), 0, st, d_row_start, M, d_ns, d_ni, d_C,
+                     d_med, d_best);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_math_eval(int fn, const double *d_x, double *d_y, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_math_eval, dim3(blocks_for(n, 256)), dim3(256), 0, st, fn, d_x, d_y, n);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_check_keys(const uint32_t *d_keys, int64_t n, uint32_t *d_bad, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t b = blocks_for(n, 256);
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_check_keys, dim3((unsigned)b), dim3(256), 0, st, d_keys, n, d_bad);
+  return hipGetLastError();
+}
+
+int64_t grs_fast_wave_slots(int device) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+  return (int64_t)cus * 8 * GRS_WPB;
+}

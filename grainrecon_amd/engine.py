@@ -1,0 +1,252 @@
+"""Host-side mirror of GrainRecon's native scoring operators on the gfx950 engine.
+
+Two layers:
+
+* ``Engine``: a device context with the geometry, reflections, rotation angles
+  and the distance stack resident in HBM (FP64 native entries).
+* ``cuda_forward_comp``, ``cuda_grow_indexed_region``,
+  ``cuda_forward_comp_compete``: the three MEX operators with their exact
+  argument lists and single-precision outputs (call sites
+  ForwardCalc_OR_space_comp_cuda.m:32, grow_indexed_region_cuda.m:105-107,
+  grow_unindexed_compete_comp_cuda.m:39-41), so the MATLAB wrappers' logic can
+  be driven unchanged from Python.
+
+Everything runs on the GPU through libgrscore.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .geometry import Geometry
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def _d(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def stack_to_matlab_order(proj: np.ndarray) -> np.ndarray:
+    """A distance stack indexed like proj_bin_bw(detz, dety, i), i.e. a numpy
+    array of shape (nz, ny, nrot), flattened in MATLAB column-major order."""
+    proj = np.asarray(proj, dtype=np.float32)
+    if proj.ndim != 3:
+        raise ValueError("distance stack must be 3-D (detz, dety, omega)")
+    return proj.reshape(-1, order="F")
+
+
+class Engine:
+    """FP64 scoring context on one GPU (grs_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = ctypes.c_void_p()
+        L.check(self._lib.grs_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.H = 0
+        self.nrot = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.grs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, code):
+        L.check(code, self._h)
+
+    # -- state ------------------------------------------------------------
+    def set_geometry(self, geo: Geometry, hkl: np.ndarray, rot_angles: np.ndarray) -> None:
+        hkl = _d(np.asarray(hkl).reshape(-1, 3))
+        rot = _d(np.asarray(rot_angles).reshape(-1))
+        g = geo.to_c()
+        self._chk(self._lib.grs_set_geometry(self._h, ctypes.byref(g), _ptr(hkl, _dp), hkl.shape[0],
+                                             _ptr(rot, _dp), rot.shape[0]))
+        self.H, self.nrot = hkl.shape[0], rot.shape[0]
+
+    def set_distance_stack(self, proj: np.ndarray) -> None:
+        """proj: (nz, ny, nrot) array indexed like proj_bin_bw(detz, dety, i)."""
+        proj = np.asarray(proj, dtype=np.float32)
+        nz, ny, nr = proj.shape
+        flat = np.ascontiguousarray(proj.reshape(-1, order="F"))
+        self._chk(self._lib.grs_set_distance_stack(self._h, flat.ctypes.data, nz, ny, nr, 0))
+
+    def set_distance_stack_device(self, dptr: int, nz: int, ny: int, nrot: int) -> None:
+        """Copies a device-resident MATLAB-order float stack (e.g. a torch tensor's data_ptr())."""
+        self._chk(self._lib.grs_set_distance_stack(self._h, ctypes.c_void_p(int(dptr)), nz, ny, nrot, 1))
+
+    # -- operators ----------------------------------------------------------
+    def forward_comp(self, pos, q, method: int = L.METHOD_FAST):
+        """One voxel (pos, mm) x N orientations [q0 q1 q2 q3] ->
+        (Nr_simu, Nr_intersect, C, dis_median)."""
+        pos = _d(np.asarray(pos).reshape(3))
+        q = _d(np.asarray(q).reshape(-1, 4))
+        N = q.shape[0]
+        ns = np.zeros(N, np.int32)
+        ni = np.zeros(N, np.int32)
+        C = np.zeros(N, np.float64)
+        med = np.zeros(N, np.float32)
+        self._chk(self._lib.grs_forward_comp(self._h, _ptr(pos, _dp), _ptr(q, _dp), N, _ptr(ns, _ip),
+                                             _ptr(ni, _ip), _ptr(C, _dp), _ptr(med, _fp), int(method)))
+        return ns, ni, C, med
+
+    def forward_comp_device(self, pos, d_q: int, N: int, d_ns: int, d_ni: int, d_C: int, d_med: int,
+                            method: int = L.METHOD_FAST) -> None:
+        pos = _d(np.asarray(pos).reshape(3))
+        self._chk(self._lib.grs_forward_comp_device(self._h, _ptr(pos, _dp), ctypes.c_void_p(d_q), int(N),
+                                                    ctypes.c_void_p(d_ns), ctypes.c_void_p(d_ni),
+                                                    ctypes.c_void_p(d_C), ctypes.c_void_p(d_med), int(method)))
+
+    def grow_indexed_region(self, vox6, U, param1, method: int = L.METHOD_FAST) -> np.ndarray:
+        vox = _d(np.asarray(vox6).reshape(-1, 6))
+        U = _d(np.asarray(U).reshape(9))
+        p1 = _d(np.asarray(param1).reshape(14))
+        out = np.zeros((vox.shape[0], 7))
+        self._chk(self._lib.grs_grow_indexed_region(self._h, _ptr(vox, _dp), vox.shape[0], _ptr(U, _dp),
+                                                    _ptr(p1, _dp), _ptr(out, _dp), int(method)))
+        return out
+
+    def forward_comp_compete(self, vox5, id_all, euler, param1, method: int = L.METHOD_FAST) -> np.ndarray:
+        vox = _d(np.asarray(vox5).reshape(-1, 5))
+        ids = _d(np.asarray(id_all).reshape(-1))
+        eu = _d(np.asarray(euler).reshape(-1, 3))
+        p1 = _d(np.asarray(param1).reshape(-1)[:11])
+        out = np.zeros((vox.shape[0], 8))
+        self._chk(self._lib.grs_forward_comp_compete(self._h, _ptr(vox, _dp), vox.shape[0], _ptr(ids, _dp),
+                                                     ids.shape[0], _ptr(eu, _dp), eu.shape[0], _ptr(p1, _dp),
+                                                     _ptr(out, _dp), int(method)))
+        return out
+
+    # -- input preparation ------------------------------------------------------
+    def render_spots(self, pos, grain, U, clear: bool = True) -> None:
+        """Forward-simulates spot masks: voxel i at pos[i] (mm) with orientation U[grain[i]]."""
+        pos = _d(np.asarray(pos).reshape(-1, 3))
+        grain = np.ascontiguousarray(np.asarray(grain).reshape(-1), dtype=np.int32)
+        U = _d(np.asarray(U).reshape(-1, 9))
+        self._chk(self._lib.grs_render_spots(self._h, _ptr(pos, _dp), _ptr(grain, _ip), pos.shape[0],
+                                             _ptr(U, _dp), U.shape[0], 1 if clear else 0))
+
+    def set_binary_stack(self, mask: np.ndarray) -> None:
+        """mask: (nz, ny, nrot) boolean/uint8 indexed like proj_bin(detz, dety, i)."""
+        m = np.asarray(mask).astype(np.uint8)
+        nz, ny, nr = m.shape
+        flat = np.ascontiguousarray(m.reshape(-1, order="F"))
+        self._chk(self._lib.grs_set_binary_stack(self._h, flat.ctypes.data, nz, ny, nr, 0))
+
+    def distance_from_mask(self) -> None:
+        self._chk(self._lib.grs_distance_from_mask(self._h))
+
+    def get_mask(self, shape) -> np.ndarray:
+        nz, ny, nr = shape
+        out = np.zeros(nz * ny * nr, np.uint8)
+        self._chk(self._lib.grs_get_mask(self._h, out.ctypes.data, out.size))
+        return out.reshape((nz, ny, nr), order="F")
+
+    def get_distance_stack(self, shape) -> np.ndarray:
+        nz, ny, nr = shape
+        out = np.zeros(nz * ny * nr, np.float32)
+        self._chk(self._lib.grs_get_distance_stack(self._h, _ptr(out, _fp), out.size))
+        return out.reshape((nz, ny, nr), order="F")
+
+    # -- device buffers (HBM-resident inputs/outputs) ---------------------------
+    def device_alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        self._chk(self._lib.grs_device_alloc(self._h, int(nbytes), ctypes.byref(p)))
+        return int(p.value)
+
+    def device_free(self, dptr: int) -> None:
+        self._chk(self._lib.grs_device_free(self._h, ctypes.c_void_p(dptr)))
+
+    def to_device(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr)
+        d = self.device_alloc(arr.nbytes)
+        self._chk(self._lib.grs_copy_to_device(self._h, ctypes.c_void_p(d), arr.ctypes.data, arr.nbytes))
+        return d
+
+    def to_host(self, dptr: int, shape, dtype) -> np.ndarray:
+        out = np.zeros(shape, dtype)
+        self._chk(self._lib.grs_copy_to_host(self._h, out.ctypes.data, ctypes.c_void_p(dptr), out.nbytes))
+        return out
+
+    # -- measurement ----------------------------------------------------------
+    def set_timing(self, on: bool) -> None:
+        self._chk(self._lib.grs_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self, reset: bool = False):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(self._lib.grs_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
+    def math_device(self, fn: int, x) -> np.ndarray:
+        x = _d(np.asarray(x).reshape(-1))
+        y = np.zeros_like(x)
+        self._chk(self._lib.grs_math_device(self._h, int(fn), _ptr(x, _dp), _ptr(y, _dp), x.shape[0]))
+        return y
+
+    def synchronize(self) -> None:
+        self._chk(self._lib.grs_synchronize(self._h))
+
+
+# ---- the MEX operators ------------------------------------------------------
+
+def _legacy_check(code):
+    L.check(code, None)
+
+
+def cuda_forward_comp(pos, q, S, B, hkl, proj, rot_angles, RotDet, param) -> np.ndarray:
+    """cuda_forward_comp(pos, q, S, B, hkl, proj, rot, RotDet, param): single
+    row vectors as built by ForwardCalc_OR_space_comp_cuda.m:17-27; returns the
+    1 x 4N single vector [Nr_simu Nr_intersect C dis_median] per orientation."""
+    pos, q, S, B, hkl, proj, rot, R, param = (_f(x) for x in (pos, q, S, B, hkl, proj, rot_angles, RotDet, param))
+    out = np.zeros(q.size, np.float32)
+    _legacy_check(L.lib().grs_cuda_forward_comp(
+        _ptr(pos, _fp), _ptr(q, _fp), q.size, _ptr(S, _fp), _ptr(B, _fp), _ptr(hkl, _fp), hkl.size,
+        _ptr(proj, _fp), proj.size, _ptr(rot, _fp), rot.size, _ptr(R, _fp), _ptr(param, _fp), param.size,
+        _ptr(out, _fp)))
+    return out
+
+
+def cuda_grow_indexed_region(vox, U, S, B, hkl, proj, rot_angles, RotDet, param, param1) -> np.ndarray:
+    """cuda_grow_indexed_region (grow_indexed_region_cuda.m:83-107): returns the
+    1 x 7M single vector [x y z Nr_simu Nr_intersect dis_median growth_flag]."""
+    vox, U, S, B, hkl, proj, rot, R, param, p1 = (_f(x) for x in (vox, U, S, B, hkl, proj, rot_angles, RotDet,
+                                                                     param, param1))
+    out = np.zeros(vox.size // 6 * 7, np.float32)
+    _legacy_check(L.lib().grs_cuda_grow_indexed_region(
+        _ptr(vox, _fp), vox.size, _ptr(U, _fp), _ptr(S, _fp), _ptr(B, _fp), _ptr(hkl, _fp), hkl.size,
+        _ptr(proj, _fp), proj.size, _ptr(rot, _fp), rot.size, _ptr(R, _fp), _ptr(param, _fp), param.size,
+        _ptr(p1, _fp), p1.size, _ptr(out, _fp)))
+    return out
+
+
+def cuda_forward_comp_compete(vox, id_all, Euler, S, B, hkl, proj, rot_angles, RotDet, param,
+                              param1) -> np.ndarray:
+    """cuda_forward_comp_compete (grow_unindexed_compete_comp_cuda.m:17-41):
+    returns the 1 x 8M single vector [i phi1 PHI phi2 GrainId C Nr_intersect dis_median]."""
+    vox, ids, eu, S, B, hkl, proj, rot, R, param, p1 = (
+        _f(x) for x in (vox, id_all, Euler, S, B, hkl, proj, rot_angles, RotDet, param, param1))
+    out = np.zeros(vox.size // 5 * 8, np.float32)
+    _legacy_check(L.lib().grs_cuda_forward_comp_compete(
+        _ptr(vox, _fp), vox.size, _ptr(ids, _fp), ids.size, _ptr(eu, _fp), eu.size, _ptr(S, _fp), _ptr(B, _fp),
+        _ptr(hkl, _fp), hkl.size, _ptr(proj, _fp), proj.size, _ptr(rot, _fp), rot.size, _ptr(R, _fp),
+        _ptr(param, _fp), param.size, _ptr(p1, _fp), p1.size, _ptr(out, _fp)))
+    return out

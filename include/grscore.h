@@ -1,0 +1,215 @@
+/*
+ * grscore.h -- C ABI of the MI355X (gfx950) forward-completeness scoring
+ * engine for LabDCT grain mapping.
+ *
+ * Plain C types only (pointers, sizes, doubles); no HIP or torch types cross
+ * this boundary.  Every function returns a grs_status (0 on success, a
+ * negative code otherwise) and never aborts the calling process; the text of
+ * the last error is available from grs_last_error().
+ *
+ * Two families of entry points:
+ *
+ *  1. Legacy MEX operators, one per native operator of GrainRecon.  They take
+ *     exactly the single-precision row vectors that the MATLAB wrappers build
+ *     and return the same single-precision output layout, so a MEX gateway
+ *      can forward its mxArrays unchanged.  The inputs are
+ *     promoted to double and scored in FP64.
+ *       grs_cuda_forward_comp          replaces cuda_forward_comp
+ *           (call sites ForwardCalc_OR_space_comp_cuda.m:32,
+ *            fit_OR_constrain_with_local_comp_cuda.m:47,
+ *            fit_OR_constrain_with_local_comp_cuda_2.m:62,
+ *            forward_calc_gridding_OR.m:17)
+ *       grs_cuda_grow_indexed_region   replaces cuda_grow_indexed_region
+ *           (grow_indexed_region_cuda.m:105-107,
+ *            Forward_calc_comp_allvoxel_cuda.m:57-59)
+ *       grs_cuda_forward_comp_compete  replaces cuda_forward_comp_compete
+ *           (grow_unindexed_compete_comp_cuda.m:39-41)
+ *     The MEX operators receive no beam-centre pixel (dety0, detz0) and no hit
+ *     threshold; these entries use dety0 = detysize/2 + 0.5,
+ *     detz0 = detzsize/2 + 0.5 (the convention of
+ *     revise_all_unindexed_voxel_cuda.m:119-120) and a threshold of 0.5 pixel
+ *     (minEucDis = 0.5*mean([pixelysize pixelzsize]), setup_para.m:34,91,133).
+ *
+ *  2. Native FP64 entries on a context that keeps the geometry and the
+ *     distance stack resident in HBM between calls (the MEX path re-uploads
+ *     the stack on every call and resets the device afterwards).
+ *
+ * Layout conventions shared by both families:
+ *  - 3x3 matrices are row-major 9-vectors (MATLAB reshape(X',1,9));
+ *  - hkl is nrhkl triples (h,k,l);
+ *  - the distance stack is MATLAB column-major proj_bin_bw(detz, dety, i):
+ *    element (iz, iy, i) (1-based) at (iz-1) + (iy-1)*nz + (i-1)*nz*ny,
+ *    with nz >= detzsize rows and ny >= detysize columns; values must be
+ *    non-negative and not NaN (bwdist distances in pixels);
+ *  - a distance d is a hit when (double)d <= minEucDis/mean([py pz]).
+ */
+#ifndef GRSCORE_H
+#define GRSCORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t grs_status;
+#define GRS_OK 0
+#define GRS_ERR_ARG (-1)      /* invalid argument, shape or value */
+#define GRS_ERR_DEVICE (-2)   /* a HIP runtime call failed */
+#define GRS_ERR_STATE (-3)    /* geometry or distance stack not set */
+#define GRS_ERR_NOMEM (-4)    /* device allocation failed */
+
+#define GRS_METHOD_FAST 0  /* wave-per-evaluation filtered kernel (default) */
+#define GRS_METHOD_PLAIN 1 /* thread-per-evaluation port (baseline) */
+
+typedef struct grs_ctx grs_ctx;
+
+/* Geometry and selection parameters: the argument list of forward_comp.m:4-6
+ * (forward_comp.m is the CPU scorer every native operator reproduces). */
+typedef struct grs_geometry {
+  double S[9];        /* sample-frame matrix S */
+  double B[9];        /* reciprocal-lattice matrix B = FormB(cell) */
+  double RotDet[9];   /* detector rotation get_det_R(tilt_x, tilt_y, tilt_z) */
+  double Lsam2sou, Lsam2det;      /* mm */
+  double P0y, P0z, RotAxisOffset; /* mm */
+  double dety00, detz00;          /* detector offset, mm */
+  double pixelysize, pixelzsize;  /* mm */
+  double dety0, detz0;            /* beam centre, pixels */
+  double detysize, detzsize;      /* pixels */
+  double BeamStopY[2], BeamStopZ[2]; /* inclusive pixel ranges; [0 -1] = none */
+  double thetamax;                /* degrees */
+  double lambda_min, lambda_max;  /* Angstrom */
+  double minEucDis;               /* mm */
+} grs_geometry;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device`. */
+grs_status grs_create(int32_t device, grs_ctx **out);
+void grs_destroy(grs_ctx *ctx);
+/* Text of the last error on ctx (ctx may be NULL for the legacy entries). */
+const char *grs_last_error(const grs_ctx *ctx);
+const char *grs_version(void);
+
+/* Geometry, reflections and rotation angles (degrees) used by later calls. */
+grs_status grs_set_geometry(grs_ctx *ctx, const grs_geometry *geo, const double *hkl, int64_t nrhkl,
+                            const double *rot_angles, int64_t nrot);
+
+/* Distance stack (see layout above).  src_on_device != 0 means `proj` is a
+ * device pointer; the data is copied either way. */
+grs_status grs_set_distance_stack(grs_ctx *ctx, const float *proj, int64_t nz, int64_t ny, int64_t nrot,
+                                  int32_t src_on_device);
+
+/* ---- native FP64 operators ---------------------------------------------- */
+
+/* One voxel at pos (mm) against N orientations given as rows [q0 q1 q2 q3];
+ * q0 is recomputed from the vector part and rows with |q_vec| > 1 give zeros
+ * (ForwardCalc_OR_space_comp.m:22-53).  Outputs per row: Nr_simu,
+ * Nr_intersect, C = Nr_intersect/Nr_simu (NaN when Nr_simu = 0) and the
+ * median distance (2000 when no spot lands on the detector).  Host pointers.
+ * Replaces cuda_forward_comp (ForwardCalc_OR_space_comp_cuda.m:32). */
+grs_status grs_forward_comp(grs_ctx *ctx, const double *pos, const double *q, int64_t N, int32_t *nr_simu,
+                            int32_t *nr_intersect, double *completeness, float *dis_median, int32_t method);
+
+/* Same on device-resident orientations and outputs (pos stays on the host). */
+grs_status grs_forward_comp_device(grs_ctx *ctx, const double *pos, const double *d_q, int64_t N,
+                                   int32_t *d_nr_simu, int32_t *d_nr_intersect, double *d_completeness,
+                                   float *d_dis_median, int32_t method);
+
+/* One orientation U (row-major) against M voxels; vox rows are
+ * [x y z C mask DisMedian] with 1-based voxel indices, param1 the 14 values
+ * [RecVolumePixel(:,1)' simap_data_flag VoxSize Center' Dimension seed_comp
+ * drop_off maxDmedian] (grow_indexed_region_cuda.m:71-95).  out rows are
+ * [x y z Nr_simu Nr_intersect dis_median growth_flag]; voxels with mask <= 0
+ * are not scored (grow_indexed_region_parallel.m:69) and give zeros.
+ * growth_flag = C > C_old && dis_median <= min(DisMedian, maxDmedian) &&
+ * C >= seed_comp*(1-drop_off) (grow_indexed_region_parallel.m:78-81; the
+ * operator receives no Ninter column, so the equal-C clause is not applied).
+ * Replaces cuda_grow_indexed_region (grow_indexed_region_cuda.m:105). */
+grs_status grs_grow_indexed_region(grs_ctx *ctx, const double *vox, int64_t M, const double *U,
+                                   const double *param1, double *out, int32_t method);
+
+/* M voxels, rows [x y z start end] (1-based inclusive indices into id_all,
+ * find_neighbor_ORs.m:26-29) against the candidate grains id_all (1-based);
+ * euler holds 3 angles (degrees) per grain id.  param1 = the first 11 values
+ * of the grow param1.  out rows [i phi1 PHI phi2 GrainId C Nr_intersect
+ * dis_median] for the first maximum of C (revise_all_unindexed_voxel.m:318).
+ * Replaces cuda_forward_comp_compete (grow_unindexed_compete_comp_cuda.m:39). */
+grs_status grs_forward_comp_compete(grs_ctx *ctx, const double *vox, int64_t M, const double *id_all, int64_t n_id,
+                                    const double *euler, int64_t n_grains, const double *param1, double *out,
+                                    int32_t method);
+
+/* ---- legacy single-precision MEX operators (stateless) ------------------ */
+
+/* cuda_forward_comp(pos, q, S, B, hkl, proj, rot, RotDet, param):
+ * param = [nr_hkl nr_rot Lsam2sou Lsam2det P0y P0z dety00 detz00 pixelysize
+ * pixelzsize detysize detzsize BeamStopY(2) BeamStopZ(2) thetamax lambda_min
+ * lambda_max RotAxisOffset] (ForwardCalc_OR_space_comp_cuda.m:17-27).
+ * out: 4 floats per orientation [Nr_simu Nr_intersect C dis_median]. */
+grs_status grs_cuda_forward_comp(const float *pos, const float *q, int64_t n_q, const float *S, const float *B,
+                                 const float *hkl, int64_t n_hkl, const float *proj, int64_t n_proj,
+                                 const float *rot, int64_t n_rot, const float *RotDet, const float *param,
+                                 int64_t n_param, float *out);
+
+/* cuda_grow_indexed_region(vox6, U, S, B, hkl, proj, rot, RotDet, param, param1):
+ * out: 7 floats per voxel (grow_indexed_region_cuda.m:101-115). */
+grs_status grs_cuda_grow_indexed_region(const float *vox, int64_t n_vox, const float *U, const float *S,
+                                        const float *B, const float *hkl, int64_t n_hkl, const float *proj,
+                                        int64_t n_proj, const float *rot, int64_t n_rot, const float *RotDet,
+                                        const float *param, int64_t n_param, const float *param1,
+                                        int64_t n_param1, float *out);
+
+/* cuda_forward_comp_compete(vox5, id_all, Euler, S, B, hkl, proj, rot,
+ * RotDet, param, param1): out 8 floats per voxel
+ * (grow_unindexed_compete_comp_cuda.m:17-48). */
+grs_status grs_cuda_forward_comp_compete(const float *vox, int64_t n_vox, const float *id_all, int64_t n_id,
+                                         const float *euler, int64_t n_euler, const float *S, const float *B,
+                                         const float *hkl, int64_t n_hkl, const float *proj, int64_t n_proj,
+                                         const float *rot, int64_t n_rot, const float *RotDet,
+                                         const float *param, int64_t n_param, const float *param1,
+                                         int64_t n_param1, float *out);
+
+/* ---- input preparation (no counterpart among the MEX operators) --------- */
+
+/* Forward-simulates binary spot masks with the same geometry: every selected
+ * (omega, hkl) event of voxel i (position pos[3i..3i+2] mm, orientation
+ * U[9*grain[i]..], row-major) sets its pixel in the context's detzsize x
+ * detysize x nrot mask (MATLAB order).  clear != 0 zeroes the mask first. */
+grs_status grs_render_spots(grs_ctx *ctx, const double *pos, const int32_t *grain, int64_t M, const double *U,
+                            int64_t G, int32_t clear);
+/* Copies a binary mask (nonzero = spot pixel, MATLAB order, nz x ny x nrot)
+ * into the context's mask.  src_on_device as for grs_set_distance_stack. */
+grs_status grs_set_binary_stack(grs_ctx *ctx, const uint8_t *mask, int64_t nz, int64_t ny, int64_t nrot,
+                                int32_t src_on_device);
+/* Exact Euclidean distance transform of the context's mask (bwdist of
+ * GrainMapping_fun.m:73-75, as single) installed as the distance stack. */
+grs_status grs_distance_from_mask(grs_ctx *ctx);
+/* Copies the current mask / distance stack to host memory (n elements). */
+grs_status grs_get_mask(grs_ctx *ctx, uint8_t *out, int64_t n);
+grs_status grs_get_distance_stack(grs_ctx *ctx, float *out, int64_t n);
+
+/* Device memory helpers on ctx's device (for callers without a GPU runtime
+ * of their own, e.g. the benchmark keeping orientations resident in HBM). */
+grs_status grs_device_alloc(grs_ctx *ctx, int64_t bytes, void **d_out);
+grs_status grs_device_free(grs_ctx *ctx, void *d);
+grs_status grs_copy_to_device(grs_ctx *ctx, void *d_dst, const void *src, int64_t bytes);
+grs_status grs_copy_to_host(grs_ctx *ctx, void *dst, const void *d_src, int64_t bytes);
+
+/* ---- measurement and diagnostics ---------------------------------------- */
+
+/* Enables HIP-event timing of the scoring kernel launches on ctx's stream. */
+grs_status grs_set_timing(grs_ctx *ctx, int32_t enable);
+/* Accumulated scoring-kernel time (ms) and launch count since the last reset;
+ * reset != 0 clears them after reading. */
+grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, int32_t reset);
+/* Evaluates the engine's FP64 math on the device: fn 0 sin, 1 cos, 2 acos,
+ * 3 atan, 4 round (half away from zero), 5 sqrt, 6 reciprocal.  Host arrays. */
+grs_status grs_math_device(grs_ctx *ctx, int32_t fn, const double *x, double *y, int64_t n);
+/* Blocks until all work on ctx's stream is done. */
+grs_status grs_synchronize(grs_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRSCORE_H */

@@ -1,0 +1,237 @@
+"""GPU parity: the gfx950 engine against the CPU oracle, bit for bit.
+
+Integer counts must be identical; completeness (a ratio of the same
+integers) and the single-precision median must be bitwise identical.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+from grainrecon_amd import engine as E
+from grainrecon_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [_lib.METHOD_FAST, _lib.METHOD_PLAIN]
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+def test_math_host_equals_device(engine):
+    rng = np.random.default_rng(0)
+    cases = {
+        0: np.concatenate([rng.uniform(-7, 7, 20000), np.linspace(0, 2 * np.pi, 721)]),
+        1: np.concatenate([rng.uniform(-7, 7, 20000), np.linspace(0, 2 * np.pi, 721)]),
+        2: np.concatenate([rng.uniform(-1, 1, 20000), 1 - rng.random(2000) ** 8, -1 + rng.random(2000) ** 8]),
+        3: rng.uniform(-10, 10, 20000),
+        4: rng.uniform(-5000, 5000, 20000).round(1) + rng.choice([0.0, 0.5, -0.5], 20000),
+    }
+    for fn, x in cases.items():
+        assert np.array_equal(_bits(engine.math_device(fn, x)), _bits(oracle.math_eval(fn, x))), fn
+    x = np.abs(rng.normal(size=50000)) * 10.0 ** rng.integers(-20, 20, 50000)
+    assert np.array_equal(_bits(engine.math_device(5, x)), _bits(np.sqrt(x)))
+    assert np.array_equal(_bits(engine.math_device(6, x)), _bits(1.0 / x))
+
+
+def _variant_scene(name):
+    kw = dict(seed=31, ngrains=3)
+    if name == "tilted":
+        kw.update(tilts=(0.3, -0.2, 0.25), dety00=0.04, detz00=-0.03)
+    if name == "beamstop":
+        kw.update(beamstop=((110.0, 150.0), (100.0, 160.0)))
+    s = scenes.make_scene(**kw)
+    if name == "offsets":
+        s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.02, -0.015, 0.01
+    return s
+
+
+def _candidate_q(s, rng, n_random=300):
+    """Truth orientations, local perturbations around them (partial C) and random ones."""
+    rows = [s.q]
+    for g in range(s.q.shape[0]):
+        d = rng.normal(scale=0.004, size=(60, 3))
+        qv = s.q[g, 1:] + d
+        qv = qv[(qv ** 2).sum(1) < 1]
+        rows.append(np.hstack([np.sqrt(1 - (qv ** 2).sum(1))[:, None], qv]))
+    rows.append(scenes.random_quats(rng, n_random))
+    rows.append(np.array([[0.0, 0.9, 0.5, 0.3]]))  # |q_vec| > 1: skipped row
+    return np.vstack(rows)
+
+
+def _assert_same(got, ref):
+    ns, ni, C, med = got
+    assert np.array_equal(ns, ref[:, 0].astype(np.int32))
+    assert np.array_equal(ni, ref[:, 1].astype(np.int32))
+    assert np.array_equal(_bits(C), _bits(ref[:, 2]))
+    assert np.array_equal(_bits(med), _bits(ref[:, 3].astype(np.float32)))
+
+
+@pytest.mark.parametrize("variant", ["plain", "tilted", "offsets", "beamstop"])
+@pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
+def test_forward_comp_parity(engine, variant, method):
+    s = _variant_scene(variant)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(5)
+    q = _candidate_q(s, rng)
+    for g, pos in ((0, s.centers[0]), (1, s.voxels[1][5])):
+        ref = oracle.or_space_comp(s.geo, s.stack, pos, q, s.rot, s.hkl)
+        got = engine.forward_comp(pos, q, method)
+        _assert_same(got, ref)
+        if variant != "offsets":  # that scene was rendered before the offsets were applied
+            assert got[2][g] == 1.0  # the grain's own orientation at one of its voxels
+        assert got[0][-1] == 0 and got[2][-1] == 0.0
+
+
+def test_fast_and_plain_agree_and_are_deterministic(engine):
+    s = _variant_scene("plain")
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    q = _candidate_q(s, np.random.default_rng(6), n_random=2000)
+    a = engine.forward_comp(s.centers[2], q, _lib.METHOD_FAST)
+    b = engine.forward_comp(s.centers[2], q, _lib.METHOD_FAST)
+    c = engine.forward_comp(s.centers[2], q, _lib.METHOD_PLAIN)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(_bits(x) if x.dtype.kind == "f" else x, _bits(y) if y.dtype.kind == "f" else y)
+        assert np.array_equal(_bits(x) if x.dtype.kind == "f" else x, _bits(z) if z.dtype.kind == "f" else z)
+
+
+def _p1(n=40, vox=0.005, seed_comp=0.9, drop=0.02, maxdm=5.0):
+    return np.array([1, 1, 1, 1, vox, 0, 0, 0, n, n, n, seed_comp, drop, maxdm], dtype=float)
+
+
+@pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
+def test_grow_parity(engine, method):
+    s = scenes.make_scene(seed=41, ngrains=2, spread_mm=0.02)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(3)
+    idx = np.stack(np.meshgrid(np.arange(14, 27), np.arange(14, 27), np.arange(16, 25), indexing="ij"), -1)
+    idx = idx.reshape(-1, 3).astype(float)
+    vox = np.hstack([idx, rng.uniform(0, 0.6, (idx.shape[0], 1)), (rng.random((idx.shape[0], 1)) < 0.9) * 1.0,
+                     rng.uniform(0, 30, (idx.shape[0], 1))])
+    p1 = _p1()
+    for g in range(2):
+        ref = oracle.grow(s.geo, s.stack, vox, s.U[g], s.rot, s.hkl, p1)
+        got = engine.grow_indexed_region(vox, s.U[g], p1, method)
+        assert np.array_equal(_bits(got), _bits(ref))
+        assert got[:, 6].sum() > 0
+
+
+@pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
+def test_compete_parity(engine, method):
+    s = scenes.make_scene(seed=43, ngrains=3, spread_mm=0.03)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(4)
+    G = 6
+    eu = np.vstack([rng.uniform([0, 0, 0], [360, 180, 360], (G, 3))])
+    for g in range(3):  # three of the candidates are the true grains
+        U = s.U[g]
+        eu[g] = [math.degrees(math.atan2(U[0, 2], -U[1, 2])) % 360, math.degrees(math.acos(U[2, 2])),
+                 math.degrees(math.atan2(U[2, 0], U[2, 1])) % 360]
+    idx = np.stack(np.meshgrid(np.arange(14, 27, 2), np.arange(14, 27, 2), np.arange(16, 25, 2), indexing="ij"), -1)
+    idx = idx.reshape(-1, 3).astype(float)
+    ids, rows = [], []
+    for v in idx:
+        k = int(rng.integers(0, 5))
+        cand = list(rng.integers(1, G + 1, size=k))
+        start = len(ids) + 1
+        ids += cand
+        rows.append([*v, start, start + k - 1])
+    rows.append([10 ** 5, 20, 20, 1, 3])  # far away: every C is NaN
+    vox, ids = np.array(rows, float), np.array(ids, float)
+    p1 = _p1()[:11]
+    ref = oracle.compete(s.geo, s.stack, vox, ids, eu, s.rot, s.hkl, p1)
+    got = engine.forward_comp_compete(vox, ids, eu, p1, method)
+    assert np.array_equal(_bits(got), _bits(ref))
+    assert np.isnan(got[-1, 5])
+
+
+def _f32(x):
+    return np.asarray(x, dtype=np.float32).astype(np.float64)
+
+
+def _legacy_inputs(s):
+    geo = s.geo
+    param = geo.legacy_param(s.hkl.shape[0], s.rot.size).astype(np.float32)
+    # the oracle sees exactly what the engine sees: float32 values promoted to double
+    g = type(geo)(**{k: getattr(geo, k) for k in geo.__dataclass_fields__})
+    g.S, g.B, g.RotDet = _f32(geo.S), _f32(geo.B), _f32(geo.RotDet)
+    for k, j in (("Lsam2sou", 2), ("Lsam2det", 3), ("P0y", 4), ("P0z", 5), ("dety00", 6), ("detz00", 7),
+                 ("pixelysize", 8), ("pixelzsize", 9), ("detysize", 10), ("detzsize", 11), ("thetamax", 16),
+                 ("lambda_min", 17), ("lambda_max", 18), ("RotAxisOffset", 19)):
+        setattr(g, k, float(param[j]))
+    g.BeamStopY = (float(param[12]), float(param[13]))
+    g.BeamStopZ = (float(param[14]), float(param[15]))
+    g.dety0 = g.detysize / 2 + 0.5
+    g.detz0 = g.detzsize / 2 + 0.5
+    g.minEucDis = 0.5 * ((g.pixelysize + g.pixelzsize) / 2)
+    return g, param
+
+
+def test_legacy_forward_comp_matches_oracle_on_rounded_inputs():
+    s = scenes.make_scene(seed=51, ngrains=2)
+    g, param = _legacy_inputs(s)
+    q = _candidate_q(s, np.random.default_rng(8), n_random=200)
+    pos = s.centers[1]
+    # MATLAB passes matrices as reshape(X',1,9), i.e. numpy's row-major order
+    out = E.cuda_forward_comp(pos, q.reshape(-1), np.asarray(s.geo.S).reshape(-1), np.asarray(s.geo.B).reshape(-1),
+                              s.hkl.reshape(-1), E.stack_to_matlab_order(s.proj), s.rot,
+                              np.asarray(s.geo.RotDet).reshape(-1), param)
+    out = out.reshape(-1, 4)
+    ref = oracle.or_space_comp(g, s.stack, _f32(pos), _f32(q), _f32(s.rot), _f32(s.hkl))
+    assert np.array_equal(out[:, 0], ref[:, 0].astype(np.float32))
+    assert np.array_equal(out[:, 1], ref[:, 1].astype(np.float32))
+    assert np.array_equal(_bits(out[:, 2]), _bits(ref[:, 2].astype(np.float32)))
+    assert np.array_equal(_bits(out[:, 3]), _bits(ref[:, 3].astype(np.float32)))
+
+
+def test_legacy_grow_and_compete_match_oracle():
+    s = scenes.make_scene(seed=52, ngrains=2, spread_mm=0.02)
+    g, param = _legacy_inputs(s)
+    idx = np.stack(np.meshgrid(np.arange(16, 25), np.arange(16, 25), np.arange(18, 23), indexing="ij"), -1)
+    idx = idx.reshape(-1, 3).astype(float)
+    vox = np.hstack([idx, np.full((idx.shape[0], 1), 0.25), np.ones((idx.shape[0], 1)), np.full((idx.shape[0], 1), 9.)])
+    p1 = _p1()
+    U = np.asarray(s.U[0], dtype=np.float32)
+    out = E.cuda_grow_indexed_region(vox, U.reshape(-1), np.asarray(s.geo.S).reshape(-1),
+                                     np.asarray(s.geo.B).reshape(-1), s.hkl.reshape(-1),
+                                     E.stack_to_matlab_order(s.proj), s.rot, np.asarray(s.geo.RotDet).reshape(-1),
+                                     param, p1).reshape(-1, 7)
+    ref = oracle.grow(g, s.stack, _f32(vox), _f32(U), _f32(s.rot), _f32(s.hkl), _f32(p1))
+    assert np.array_equal(_bits(out), _bits(ref.astype(np.float32)))
+    eu = np.array([[10.0, 40.0, 70.0], [200.0, 100.0, 30.0]])
+    ids = np.array([1, 2, 2, 1], float)
+    vox5 = np.array([[20, 20, 20, 1, 2], [21, 20, 20, 3, 4], [22, 20, 20, 1, 4]], float)
+    out = E.cuda_forward_comp_compete(vox5, ids, eu.reshape(-1), np.asarray(s.geo.S).reshape(-1),
+                                      np.asarray(s.geo.B).reshape(-1), s.hkl.reshape(-1),
+                                      E.stack_to_matlab_order(s.proj), s.rot, np.asarray(s.geo.RotDet).reshape(-1),
+                                      param, p1[:11]).reshape(-1, 8)
+    ref = oracle.compete(g, s.stack, _f32(vox5), _f32(ids), _f32(eu), _f32(s.rot), _f32(s.hkl), _f32(p1[:11]))
+    assert np.array_equal(_bits(out), _bits(ref.astype(np.float32)))
+
+
+def test_errors_are_reported_not_raised_from_c(engine):
+    s = scenes.make_scene(seed=61, ngrains=1, det=128, pixel_mm=6.72e-3 * 8)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    with pytest.raises(_lib.GrsError):
+        engine.set_distance_stack(np.full((64, 64, s.rot.size), 1.0, np.float32))
+        engine.forward_comp(s.centers[0], s.q)  # stack smaller than the detector
+    with pytest.raises(_lib.GrsError):
+        engine.set_distance_stack(np.full((128, 128, s.rot.size), -1.0, np.float32))  # negative distances
+    engine.set_distance_stack(s.proj)
+    ns, ni, C, med = engine.forward_comp(s.centers[0], np.zeros((0, 4)))
+    assert ns.size == 0
+    out = engine.forward_comp_compete(np.array([[10, 10, 10, 1, 0]], float), np.zeros(0), np.zeros((0, 3)),
+                                      _p1()[:11])
+    assert out[0, 0] == 1 and (out[0, 1:] == 0).all()
+    with pytest.raises(_lib.GrsError):
+        engine.forward_comp_compete(np.array([[10, 10, 10, 1, 2]], float), np.array([1.0]), np.zeros((1, 3)),
+                                    _p1()[:11])
