@@ -161,7 +161,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         pos0 = seeds[args.warmup]
-        qn = q[:8192]
+        qn = q[:65536]
         gout = eng.forward_comp(pos0, qn, _lib.METHOD_FAST)
         shape = (int(cfg.geo.detzsize), int(cfg.geo.detysize), nrot)
         proj = eng.get_distance_stack(shape)

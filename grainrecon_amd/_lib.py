@@ -41,6 +41,10 @@ _SIGS = {
     "grs_forward_comp_device": (c_i32, [c_vp, c_dp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32]),
     "grs_grow_indexed_region": (c_i32, [c_vp, c_dp, c_i64, c_dp, c_dp, c_dp, c_i32]),
     "grs_forward_comp_compete": (c_i32, [c_vp, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64, c_dp, c_dp, c_i32]),
+    "grs_compete_csr": (c_i32, [c_vp, c_dp, c_i64, ctypes.POINTER(c_i64), c_dp, c_i64, c_i32, ctypes.c_double,
+                                ctypes.POINTER(c_i64), c_dp, c_ip, c_ip, c_dp, c_fp]),
+    "grs_set_spots": (c_i32, [c_vp, c_i64, c_dp, c_dp, c_dp, c_ip, c_i64]),
+    "grs_spot_residual": (c_i32, [c_vp, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]),
     "grs_cuda_forward_comp": (c_i32, [c_fp, c_fp, c_i64, c_fp, c_fp, c_fp, c_i64, c_fp, c_i64, c_fp, c_i64, c_fp,
                                       c_fp, c_i64, c_fp]),
     "grs_cuda_grow_indexed_region": (c_i32, [c_fp, c_i64, c_fp, c_fp, c_fp, c_fp, c_i64, c_fp, c_i64, c_fp, c_i64,
@@ -48,6 +52,8 @@ _SIGS = {
     "grs_cuda_forward_comp_compete": (c_i32, [c_fp, c_i64, c_fp, c_i64, c_fp, c_i64, c_fp, c_fp, c_fp, c_i64,
                                               c_fp, c_i64, c_fp, c_i64, c_fp, c_fp, c_i64, c_fp, c_i64, c_fp]),
     "grs_render_spots": (c_i32, [c_vp, c_dp, c_ip, c_i64, c_dp, c_i64, c_i32]),
+    "grs_simulate_spots": (c_i32, [c_vp, c_dp, c_ip, c_i64, c_dp, c_i64, c_i64, c_ip, c_ip, c_ip, c_dp,
+                                   ctypes.POINTER(c_i64)]),
     "grs_set_binary_stack": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_i64, c_i32]),
     "grs_distance_from_mask": (c_i32, [c_vp]),
     "grs_get_mask": (c_i32, [c_vp, c_vp, c_i64]),

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -39,6 +40,11 @@ struct grs_ctx {
   DevBuf d_q, d_sub, d_valid, d_pos, d_vw, d_scratch, d_ns, d_ni, d_C, d_med;
   DevBuf d_rows, d_p1, d_U, d_eu, d_rowstart, d_best, d_evox, d_eu_idx, d_bad, d_x, d_y;
   DevBuf d_mask, d_g1, d_edt_v, d_edt_z, d_grain;
+  DevBuf d_bits, d_rowptr, d_roweoff, d_candq, d_rowbest, d_elist;
+  bool bits_valid = false; /* hit mask matches the current stack and threshold */
+  /* resident matched-spot set of the fits */
+  DevBuf f_hkl, f_om, f_meas, f_grain, f_goff, f_gr6, f_up, f_res, f_contrib, f_jac, f_out;
+  int64_t f_n = 0, f_G = 0;
   int64_t mnz = 0, mny = 0, mnrot = 0; /* shape of d_mask */
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -158,7 +164,9 @@ void grs_destroy(grs_ctx *c) {
                     &c->d_pos, &c->d_vw, &c->d_scratch, &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_rows,
                     &c->d_p1, &c->d_U, &c->d_eu, &c->d_rowstart, &c->d_best, &c->d_evox, &c->d_eu_idx,
                     &c->d_bad, &c->d_x, &c->d_y, &c->d_mask, &c->d_g1, &c->d_edt_v, &c->d_edt_z,
-                    &c->d_grain};
+                    &c->d_grain, &c->d_bits, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest,
+                    &c->d_elist, &c->f_hkl, &c->f_om, &c->f_meas, &c->f_grain, &c->f_goff, &c->f_gr6,
+                    &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out};
   for (DevBuf *b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -221,6 +229,7 @@ grs_status grs_set_geometry(grs_ctx *c, const grs_geometry *geo, const double *h
   hit_key_for(g.thr, &g.hit_key, &g.has_hit);
   c->H = (int32_t)nrhkl;
   c->nrot = (int32_t)nrot;
+  c->bits_valid = false;
   c->hkl.assign(hkl, hkl + 3 * nrhkl);
   ENSURE(c, c->d_hkl, sizeof(double) * 3 * nrhkl);
   ENSURE(c, c->d_rot, sizeof(double) * nrot);
@@ -257,6 +266,7 @@ grs_status grs_set_distance_stack(grs_ctx *c, const float *proj, int64_t nz, int
   c->sny = ny;
   c->snrot = nrot;
   c->stack_set = true;
+  c->bits_valid = false;
   if (c->geo_set) return upload_geo(c);
   return GRS_OK;
 }
@@ -583,6 +593,316 @@ grs_status grs_forward_comp_compete(grs_ctx *c, const double *vox, int64_t M, co
   return GRS_OK;
 }
 
+/* hit mask (1 bit per pixel) of the current stack and threshold */
+static grs_status ensure_bits(grs_ctx *c) {
+  if (c->bits_valid) return GRS_OK;
+  const int64_t n = c->snz * c->sny * c->snrot;
+  ENSURE(c, c->d_bits, sizeof(uint32_t) * (size_t)((n + 31) / 32));
+  HIPCHK(c, grs_launch_pack_hits(P<uint32_t>(c->d_keys), n, c->hgeo.hit_key, c->hgeo.has_hit, P<uint32_t>(c->d_bits),
+                                 c->st));
+  c->bits_valid = true;
+  return GRS_OK;
+}
+
+grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64_t *row_ptr, const double *cand_q,
+                           int64_t K, int32_t nr, double gridsize, int64_t *best, double *best_q, int32_t *ns,
+                           int32_t *ni, double *C, float *med) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || K < 0 || nr < 0 || nr > 40 || !row_ptr || (M > 0 && (!pos || !best || !best_q || !ns || !ni || !C || !med)) ||
+      (K > 0 && !cand_q))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (row_ptr[0] != 0 || row_ptr[M] != K) return fail(c, GRS_ERR_ARG, "row_ptr must start at 0 and end at K");
+  for (int64_t r = 0; r < M; r++)
+    if (row_ptr[r + 1] < row_ptr[r]) return fail(c, GRS_ERR_ARG, "row_ptr must be non-decreasing");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if ((int64_t)c->nrot * c->H > GRS_KEY_MAX_EVENTS)
+    return fail(c, GRS_ERR_ARG, "nrot*nrhkl = %lld exceeds %d events, the exact range of the packed argmax key",
+                (long long)c->nrot * c->H, GRS_KEY_MAX_EVENTS);
+  if (c->H > 240) return fail(c, GRS_ERR_ARG, "at most 240 reflections");
+  if (M == 0) return GRS_OK;
+  s = ensure_bits(c);
+  if (s != GRS_OK) return s;
+  const int64_t S = (int64_t)(nr + 1) * (nr + 1) * (nr + 1);
+  const int64_t chunk = voxel_chunk(c);
+  const int64_t stride = (int64_t)c->nrot * c->H;
+  std::vector<int64_t> rp, eo, el;
+  std::vector<unsigned long long> rb;
+  std::vector<int32_t> hns, hni;
+  std::vector<double> hC;
+  std::vector<float> hmed;
+  for (int64_t r0 = 0; r0 < M; r0 += chunk) {
+    const int64_t m = M - r0 < chunk ? M - r0 : chunk;
+    const int64_t c0 = row_ptr[r0], kc = row_ptr[r0 + m] - c0;
+    rp.resize(m + 1);
+    eo.resize(m + 1);
+    for (int64_t r = 0; r <= m; r++) {
+      rp[r] = row_ptr[r0 + r] - c0;
+      eo[r] = rp[r] * S;
+    }
+    const int64_t E = eo[m];
+    ENSURE(c, c->d_pos, sizeof(double) * 3 * m);
+    ENSURE(c, c->d_rowptr, sizeof(int64_t) * (m + 1));
+    ENSURE(c, c->d_roweoff, sizeof(int64_t) * (m + 1));
+    ENSURE(c, c->d_candq, sizeof(double) * 4 * (kc > 0 ? kc : 1));
+    ENSURE(c, c->d_rowbest, sizeof(unsigned long long) * m);
+    ENSURE(c, c->d_elist, sizeof(int64_t) * m);
+    ENSURE(c, c->d_ns, sizeof(int32_t) * m);
+    ENSURE(c, c->d_ni, sizeof(int32_t) * m);
+    ENSURE(c, c->d_C, sizeof(double) * m);
+    ENSURE(c, c->d_med, sizeof(float) * m);
+    HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos + 3 * r0, sizeof(double) * 3 * m, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_rowptr.p, rp.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_roweoff.p, eo.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, c->st));
+    if (kc > 0)
+      HIPCHK(c, hipMemcpyAsync(c->d_candq.p, cand_q + 4 * c0, sizeof(double) * 4 * kc, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d_rowbest.p, 0, sizeof(unsigned long long) * m, c->st));
+    ENSURE(c, c->d_vw, sizeof(double) * GRS_VW_DOUBLES * (size_t)m * c->nrot);
+    HIPCHK(c, grs_launch_vw_table(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
+                                  P<double>(c->d_vw), c->st));
+    grs_eval_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = P<grs_dgeo>(c->d_geo);
+    a.hkl = P<double>(c->d_hkl);
+    a.cs = P<double>(c->d_cs);
+    a.vw = P<double>(c->d_vw);
+    a.pos = P<double>(c->d_pos);
+    a.keys = P<uint32_t>(c->d_keys);
+    a.hit_bits = P<uint32_t>(c->d_bits);
+    a.scratch_stride = stride;
+    a.csr.row_ptr = P<int64_t>(c->d_rowptr);
+    a.csr.row_eoff = P<int64_t>(c->d_roweoff);
+    a.csr.cand_q = P<double>(c->d_candq);
+    a.csr.nrows = m;
+    a.csr.nr = nr;
+    a.csr.S = (int32_t)S;
+    a.csr.gridsize = gridsize;
+    a.csr.row_best = P<unsigned long long>(c->d_rowbest);
+    int64_t waves = grs_fast_wave_slots(c->device);
+    if (E > 0) {
+      a.n_eval = E;
+      a.map = GRS_MAP_CSR;
+      int64_t w = waves > E ? ((E + 3) / 4) * 4 : waves;
+      if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+      HIPCHK(c, grs_launch_fast_counts(&a, c->H, w, c->st));
+      if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+      if (c->timing) {
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->ms_total += ms;
+        c->launches += 1;
+      }
+    }
+    rb.resize(m);
+    HIPCHK(c, hipMemcpyAsync(rb.data(), c->d_rowbest.p, sizeof(unsigned long long) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    el.resize(m);
+    const unsigned long long jmask = (1ull << GRS_KEY_JBITS) - 1;
+    for (int64_t r = 0; r < m; r++) {
+      if (eo[r + 1] == eo[r]) {
+        el[r] = -1;
+        continue;
+      }
+      int64_t j = (int64_t)(jmask - (rb[r] & jmask));
+      el[r] = eo[r] + j;
+    }
+    /* winners re-scored with the median */
+    HIPCHK(c, hipMemcpyAsync(c->d_elist.p, el.data(), sizeof(int64_t) * m, hipMemcpyHostToDevice, c->st));
+    a.csr.elist = P<int64_t>(c->d_elist);
+    a.n_eval = m;
+    a.map = GRS_MAP_CSR_LIST;
+    a.out_ns = P<int32_t>(c->d_ns);
+    a.out_ni = P<int32_t>(c->d_ni);
+    a.out_C = P<double>(c->d_C);
+    a.out_med = P<float>(c->d_med);
+    int64_t w2 = waves > m ? ((m + 3) / 4) * 4 : waves;
+    ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)w2 * stride);
+    a.scratch = P<uint32_t>(c->d_scratch);
+    HIPCHK(c, grs_launch_fast(&a, c->H, w2, c->st));
+    hns.resize(m);
+    hni.resize(m);
+    hC.resize(m);
+    hmed.resize(m);
+    HIPCHK(c, hipMemcpyAsync(hns.data(), c->d_ns.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hni.data(), c->d_ni.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hC.data(), c->d_C.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hmed.data(), c->d_med.p, sizeof(float) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const double half = (gridsize * (double)nr) / 2.0;
+    for (int64_t r = 0; r < m; r++) {
+      int64_t gr = r0 + r;
+      double *bq = best_q + 4 * gr;
+      if (el[r] < 0) {
+        best[gr] = -1;
+        bq[0] = bq[1] = bq[2] = bq[3] = 0.0;
+        ns[gr] = ni[gr] = 0;
+        C[gr] = 0.0;
+        med[gr] = 0.0f;
+        continue;
+      }
+      int64_t j = el[r] - eo[r];
+      int64_t cand = row_ptr[gr] + j / S;
+      int64_t smp = j % S, n1 = nr + 1;
+      int64_t k[3] = {smp / (n1 * n1), (smp / n1) % n1, smp % n1};
+      const double *q = cand_q + 4 * cand;
+      double v[3];
+      for (int t = 0; t < 3; t++) v[t] = (q[1 + t] - half) + (double)k[t] * gridsize;
+      double ss = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+      bq[0] = ss <= 1.0 ? std::sqrt(1.0 - ss) : 0.0;
+      bq[1] = v[0];
+      bq[2] = v[1];
+      bq[3] = v[2];
+      best[gr] = j;
+      ns[gr] = hns[r];
+      ni[gr] = hni[r];
+      C[gr] = hC[r];
+      med[gr] = hmed[r];
+    }
+  }
+  return GRS_OK;
+}
+
+/* ---- spot-centroid residual and its adjoint ------------------------------ */
+
+grs_status grs_set_spots(grs_ctx *c, int64_t n, const double *hkl, const double *omega, const double *meas,
+                         const int32_t *grain, int64_t G) {
+  if (!c) return GRS_ERR_ARG;
+  if (n < 0 || G < 0 || (n > 0 && (!hkl || !omega || !meas || !grain))) return fail(c, GRS_ERR_ARG, "null argument");
+  std::vector<int64_t> goff(G + 1, 0);
+  for (int64_t i = 0; i < n; i++) {
+    if (grain[i] < 0 || grain[i] >= G) return fail(c, GRS_ERR_ARG, "spot %lld has grain %d outside 0..%lld",
+                                                   (long long)i, grain[i], (long long)(G - 1));
+    if (i > 0 && grain[i] < grain[i - 1]) return fail(c, GRS_ERR_ARG, "spots must be sorted by grain");
+    goff[grain[i] + 1]++;
+  }
+  for (int64_t g = 0; g < G; g++) goff[g + 1] += goff[g];
+  HIPCHK(c, hipSetDevice(c->device));
+  ENSURE(c, c->f_hkl, sizeof(double) * 3 * n);
+  ENSURE(c, c->f_om, sizeof(double) * n);
+  ENSURE(c, c->f_meas, sizeof(double) * 2 * n);
+  ENSURE(c, c->f_grain, sizeof(int32_t) * n);
+  ENSURE(c, c->f_goff, sizeof(int64_t) * (G + 1));
+  if (n > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->f_hkl.p, hkl, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->f_om.p, omega, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->f_meas.p, meas, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->f_grain.p, grain, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->f_goff.p, goff.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->f_n = n;
+  c->f_G = G;
+  return GRS_OK;
+}
+
+/* RotDet = RotZ*RotY*RotX (get_det_R.m) and its derivatives per degree. */
+static void det_rot_and_derivs(const double *t, double *R, double dR[3][9]) {
+  const double d2r = GRS_PI / 180.0;
+  double cx = std::cos(t[0] * d2r), sx = std::sin(t[0] * d2r);
+  double cy = std::cos(t[1] * d2r), sy = std::sin(t[1] * d2r);
+  double cz = std::cos(t[2] * d2r), sz = std::sin(t[2] * d2r);
+  double X[9] = {1, 0, 0, 0, cx, -sx, 0, sx, cx}, dX[9] = {0, 0, 0, 0, -sx * d2r, -cx * d2r, 0, cx * d2r, -sx * d2r};
+  double Y[9] = {cy, 0, sy, 0, 1, 0, -sy, 0, cy}, dY[9] = {-sy * d2r, 0, cy * d2r, 0, 0, 0, -cy * d2r, 0, -sy * d2r};
+  double Z[9] = {cz, -sz, 0, sz, cz, 0, 0, 0, 1}, dZ[9] = {-sz * d2r, -cz * d2r, 0, cz * d2r, -sz * d2r, 0, 0, 0, 0};
+  auto mm = [](const double *A, const double *B, double *C) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+  };
+  double ZY[9], T[9];
+  mm(Z, Y, ZY);
+  mm(ZY, X, R);
+  mm(ZY, dX, dR[0]);
+  mm(Z, dY, T);
+  mm(T, X, dR[1]);
+  mm(dZ, Y, T);
+  mm(T, X, dR[2]);
+}
+
+grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grains6, double *resid,
+                             const double *upstream, double *grad7, double *grad6, double *jac) {
+  if (!c) return GRS_ERR_ARG;
+  if (!c->geo_set) return fail(c, GRS_ERR_STATE, "geometry not set");
+  const int64_t n = c->f_n, G = c->f_G;
+  if (!geo7 || (G > 0 && !grains6) || (n > 0 && !resid)) return fail(c, GRS_ERR_ARG, "null argument");
+  if ((grad7 == nullptr) != (grad6 == nullptr)) return fail(c, GRS_ERR_ARG, "grad7 and grad6 go together");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<char> fc(grs_fitc_size());
+  struct {
+    double S[9], B[9];
+    double P0y, P0z, RAO, py, pz, dety0, detz0;
+    double Lss, Lsd, dy00, dz00;
+    double R[9];
+    double dR[3][9];
+  } f;
+  static_assert(sizeof(f) == 8 * (18 + 7 + 4 + 9 + 27), "layout");
+  if (grs_fitc_size() != sizeof(f)) return fail(c, GRS_ERR_ARG, "internal layout mismatch");
+  std::memcpy(f.S, c->hgeo.S, sizeof(f.S));
+  std::memcpy(f.B, c->hgeo.B, sizeof(f.B));
+  f.P0y = c->hgeo.P0y;
+  f.P0z = c->hgeo.P0z;
+  f.RAO = c->hgeo.RAO;
+  f.py = c->hgeo.py;
+  f.pz = c->hgeo.pz;
+  f.dety0 = c->hgeo.dety0;
+  f.detz0 = c->hgeo.detz0;
+  f.Lss = geo7[0];
+  f.Lsd = geo7[1];
+  f.dy00 = geo7[2];
+  f.dz00 = geo7[3];
+  det_rot_and_derivs(geo7 + 4, f.R, f.dR);
+  if (n == 0) {
+    if (grad7) {
+      for (int k = 0; k < 7; k++) grad7[k] = 0.0;
+      for (int64_t k = 0; k < 6 * G; k++) grad6[k] = 0.0;
+    }
+    return GRS_OK;
+  }
+  ENSURE(c, c->f_gr6, sizeof(double) * 6 * (G > 0 ? G : 1));
+  ENSURE(c, c->f_res, sizeof(double) * 2 * n);
+  HIPCHK(c, hipMemcpyAsync(c->f_gr6.p, grains6, sizeof(double) * 6 * G, hipMemcpyHostToDevice, c->st));
+  double *d_up = nullptr, *d_contrib = nullptr, *d_jac = nullptr;
+  if (grad7) {
+    ENSURE(c, c->f_contrib, sizeof(double) * 13 * n);
+    d_contrib = P<double>(c->f_contrib);
+    if (upstream) {
+      ENSURE(c, c->f_up, sizeof(double) * 2 * n);
+      HIPCHK(c, hipMemcpyAsync(c->f_up.p, upstream, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->st));
+      d_up = P<double>(c->f_up);
+    }
+  }
+  if (jac) {
+    ENSURE(c, c->f_jac, sizeof(double) * 26 * n);
+    d_jac = P<double>(c->f_jac);
+  }
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  HIPCHK(c, grs_launch_spot_residual(&f, n, P<double>(c->f_hkl), P<double>(c->f_om), P<double>(c->f_meas),
+                                     P<int32_t>(c->f_grain), P<double>(c->f_gr6), d_up, P<double>(c->f_res),
+                                     d_contrib, d_jac, c->st));
+  if (grad7) {
+    ENSURE(c, c->f_out, sizeof(double) * (7 + 6 * G));
+    HIPCHK(c, grs_launch_grad_reduce(d_contrib, n, P<int64_t>(c->f_goff), G, P<double>(c->f_out),
+                                     P<double>(c->f_out) + 7, c->st));
+  }
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  HIPCHK(c, hipMemcpyAsync(resid, c->f_res.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->st));
+  if (grad7) {
+    HIPCHK(c, hipMemcpyAsync(grad7, c->f_out.p, sizeof(double) * 7, hipMemcpyDeviceToHost, c->st));
+    if (G > 0)
+      HIPCHK(c, hipMemcpyAsync(grad6, P<double>(c->f_out) + 7, sizeof(double) * 6 * G, hipMemcpyDeviceToHost, c->st));
+  }
+  if (jac) HIPCHK(c, hipMemcpyAsync(jac, c->f_jac.p, sizeof(double) * 26 * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->timing) {
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->ms_total += ms;
+    c->launches += 1;
+  }
+  return GRS_OK;
+}
+
 /* ---- legacy single-precision operators ---------------------------------- */
 
 static std::mutex g_legacy_mu;
@@ -784,6 +1104,70 @@ grs_status grs_render_spots(grs_ctx *c, const double *pos, const int32_t *grain,
   return upload_geo(c);
 }
 
+grs_status grs_simulate_spots(grs_ctx *c, const double *pos, const int32_t *grain, int64_t M, const double *U,
+                              int64_t G, int64_t cap, int32_t *vox_out, int32_t *om_out, int32_t *h_out,
+                              double *yz_out, int64_t *n_out) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || G < 0 || cap < 0 || !n_out || (M > 0 && (!pos || !grain || !U || G == 0)) ||
+      (cap > 0 && (!vox_out || !om_out || !h_out || !yz_out)))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (!c->geo_set) return fail(c, GRS_ERR_STATE, "geometry not set");
+  for (int64_t i = 0; i < M; i++)
+    if (grain[i] < 0 || grain[i] >= G) return fail(c, GRS_ERR_ARG, "grain[%lld] out of range", (long long)i);
+  HIPCHK(c, hipSetDevice(c->device));
+  *n_out = 0;
+  if (M == 0) return GRS_OK;
+  ENSURE(c, c->d_U, sizeof(double) * 9 * G);
+  ENSURE(c, c->d_sub, sizeof(double) * 9 * G);
+  ENSURE(c, c->d_valid, sizeof(int32_t) * G);
+  ENSURE(c, c->d_pos, sizeof(double) * 3 * M);
+  ENSURE(c, c->d_grain, sizeof(int32_t) * M);
+  ENSURE(c, c->d_bad, sizeof(unsigned long long));
+  int64_t cc = cap > 0 ? cap : 1;
+  ENSURE(c, c->d_evox, sizeof(int32_t) * cc);
+  ENSURE(c, c->d_eu_idx, sizeof(int32_t) * cc);
+  ENSURE(c, c->d_ns, sizeof(int32_t) * cc);
+  ENSURE(c, c->d_y, sizeof(double) * 2 * cc);
+  HIPCHK(c, hipMemcpyAsync(c->d_U.p, U, sizeof(double) * 9 * G, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_grain.p, grain, sizeof(int32_t) * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_bad.p, 0, sizeof(unsigned long long), c->st));
+  HIPCHK(c, grs_launch_sub_from_U(P<grs_dgeo>(c->d_geo), P<double>(c->d_U), G, P<double>(c->d_sub),
+                                  P<int32_t>(c->d_valid), c->st));
+  HIPCHK(c, grs_launch_sim_events(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), P<int32_t>(c->d_grain), M, c->nrot,
+                                  P<double>(c->d_sub), P<double>(c->d_hkl), P<double>(c->d_cs), cap,
+                                  P<unsigned long long>(c->d_bad), P<int32_t>(c->d_evox), P<int32_t>(c->d_eu_idx),
+                                  P<int32_t>(c->d_ns), P<double>(c->d_y), c->st));
+  unsigned long long total = 0;
+  HIPCHK(c, hipMemcpyAsync(&total, c->d_bad.p, sizeof(total), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *n_out = (int64_t)total;
+  int64_t k = (int64_t)total < cap ? (int64_t)total : cap;
+  if (k == 0) return GRS_OK;
+  std::vector<int32_t> v(k), o(k), h(k);
+  std::vector<double> yz(2 * k);
+  HIPCHK(c, hipMemcpyAsync(v.data(), c->d_evox.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(o.data(), c->d_eu_idx.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_ns.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(yz.data(), c->d_y.p, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  std::vector<int64_t> idx(k);
+  for (int64_t i = 0; i < k; i++) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+    if (v[a] != v[b]) return v[a] < v[b];
+    if (o[a] != o[b]) return o[a] < o[b];
+    return h[a] < h[b];
+  });
+  for (int64_t i = 0; i < k; i++) {
+    vox_out[i] = v[idx[i]];
+    om_out[i] = o[idx[i]];
+    h_out[i] = h[idx[i]];
+    yz_out[2 * i] = yz[2 * idx[i]];
+    yz_out[2 * i + 1] = yz[2 * idx[i] + 1];
+  }
+  return GRS_OK;
+}
+
 grs_status grs_set_binary_stack(grs_ctx *c, const uint8_t *mask, int64_t nz, int64_t ny, int64_t nrot,
                                 int32_t on_device) {
   if (!c) return GRS_ERR_ARG;
@@ -824,6 +1208,7 @@ grs_status grs_distance_from_mask(grs_ctx *c) {
   c->sny = ny;
   c->snrot = nr;
   c->stack_set = true;
+  c->bits_valid = false;
   if (c->geo_set) return upload_geo(c);
   return GRS_OK;
 }

@@ -198,8 +198,8 @@ GRS_HD double grs_event_beta(const grs_vw *v, const double *Gt, double *Glen_out
  * tilted-detector intersection, pixel rounding and the bounds / beamstop
  * tests (forward_comp.m:33-65).  Returns 1 when the event is selected and
  * writes the 1-based pixel indices. */
-GRS_HD int grs_event_finish(const grs_dgeo *g, const grs_vw *v, const double *Gt, double Glen, double beta,
-                            int32_t *dety_out, int32_t *detz_out) {
+GRS_HD int grs_event_finish_yz(const grs_dgeo *g, const grs_vw *v, const double *Gt, double Glen, double beta,
+                               int32_t *dety_out, int32_t *detz_out, double *yz_out) {
   GRS_NO_CONTRACT
   const double pi = GRS_PI;
   if (!(beta > g->pio2 && beta < g->beta_max)) return 0;
@@ -224,13 +224,24 @@ GRS_HD int grs_event_finish(const grs_dgeo *g, const grs_vw *v, const double *Gt
   double w0 = t * K0 + (-k0), w1 = t * K1 + v->ov1, w2 = t * K2 + v->ov2;
   double Y22 = (R[1] * w0 + R[4] * w1) + R[7] * w2;
   double Z22 = (R[2] * w0 + R[5] * w1) + R[8] * w2;
-  double dety = grs_round((-Y22) / g->py + g->dety0);
-  double detz = grs_round((-Z22) / g->pz + g->detz0);
+  double ry = (-Y22) / g->py + g->dety0;
+  double rz = (-Z22) / g->pz + g->detz0;
+  double dety = grs_round(ry);
+  double detz = grs_round(rz);
   if (!(dety >= 1.0 && dety <= g->detysize && detz >= 1.0 && detz <= g->detzsize)) return 0;
   if (dety >= g->bsy0 && dety <= g->bsy1 && detz >= g->bsz0 && detz <= g->bsz1) return 0;
   *dety_out = (int32_t)dety;
   *detz_out = (int32_t)detz;
+  if (yz_out) {
+    yz_out[0] = ry;
+    yz_out[1] = rz;
+  }
   return 1;
+}
+
+GRS_HD int grs_event_finish(const grs_dgeo *g, const grs_vw *v, const double *Gt, double Glen, double beta,
+                            int32_t *dety_out, int32_t *detz_out) {
+  return grs_event_finish_yz(g, v, Gt, Glen, beta, dety_out, detz_out, (double *)0);
 }
 
 #endif /* GRS_GEOM_H */

@@ -12,7 +12,28 @@ enum grs_eval_map {
   GRS_MAP_ONE_VOXEL = 0, /* voxel 0, orientation e      (cuda_forward_comp) */
   GRS_MAP_ONE_U = 1,     /* voxel e, orientation 0      (cuda_grow_indexed_region) */
   GRS_MAP_PAIRS = 2,     /* voxel eval_vox[e], orientation eval_u[e] (compete) */
+  GRS_MAP_CSR = 3,       /* CSR rows of candidate orientations expanded by local
+                            sampling; e is the global expanded index */
+  GRS_MAP_CSR_LIST = 4,  /* as GRS_MAP_CSR for the expanded indices elist[e]
+                            (elist[e] < 0: empty row, zero output) */
 };
+
+/* CSR candidate rows with local sampling (ori_local_sampling.m:22-34): row r
+ * (a voxel) owns candidates row_ptr[r] .. row_ptr[r+1]-1 (centre quaternions
+ * cand_q[4c..4c+3]); each candidate expands to S = (nr+1)^3 samples
+ * q_i = (c_i - (gridsize*nr)/2) + k_i*gridsize, k in [0, nr]^3 (q1 slowest),
+ * q0 recomputed.  row_eoff[r] = S * row_ptr[r] is the first expanded index. */
+typedef struct grs_csr {
+  const int64_t *row_ptr;
+  const int64_t *row_eoff;
+  const double *cand_q;
+  int64_t nrows;
+  int32_t nr;
+  int32_t S;
+  double gridsize;
+  const int64_t *elist;          /* GRS_MAP_CSR_LIST */
+  unsigned long long *row_best;  /* counts-only: per-row packed argmax key */
+} grs_csr;
 
 typedef struct grs_eval_args {
   const grs_dgeo *g;       /* device copy of the geometry */
@@ -31,7 +52,18 @@ typedef struct grs_eval_args {
   int32_t *out_ns, *out_ni;
   double *out_C;
   float *out_med;
+  const uint32_t *hit_bits; /* counts-only kernels: bit i = (keys[i] <= hit_key) */
+  grs_csr csr;
 } grs_eval_args;
+
+/* Packed per-row argmax key of the counts-only kernel: larger is better.
+ * ((floor(C*2^30) + 1) << 33) | (2^33-1-j) for a scored candidate j with
+ * Nr_simu > 0, (2^33-1-j) when Nr_simu = 0 (C = NaN never beats a number).
+ * Distinct ratios with denominators <= 23170 differ by more than 2^-29, so the
+ * fixed-point part orders C exactly; equal C values tie and the smaller j
+ * (MATLAB max: first maximum) wins. */
+#define GRS_KEY_JBITS 33
+#define GRS_KEY_MAX_EVENTS 23170
 
 /* Launchers (return hipError_t). */
 hipError_t grs_launch_vw_table(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs, int32_t nrot,
@@ -46,6 +78,9 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_euler,
                                      int32_t *d_valid, hipStream_t st);
 hipError_t grs_launch_omega_cs(const double *d_rot, int32_t nrot, double *d_cs, hipStream_t st);
 hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
+hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
+hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,
+                                uint32_t *d_bits, hipStream_t st);
 hipError_t grs_launch_plain(const grs_eval_args *a, int64_t n_threads, hipStream_t st);
 hipError_t grs_launch_compete_reduce(const int64_t *d_row_start, int64_t M, const int32_t *d_ns, const int32_t *d_ni,
                                      const double *d_C, const float *d_med, int64_t *d_best, hipStream_t st);
@@ -55,9 +90,20 @@ hipError_t grs_launch_check_keys(const uint32_t *d_keys, int64_t n, uint32_t *d_
 hipError_t grs_launch_render(const grs_dgeo *d_g, const double *d_pos, const int32_t *d_grain, int64_t M,
                              int32_t nrot, const double *d_SUB, const double *d_hkl, const double *d_cs,
                              uint8_t *d_mask, hipStream_t st);
+hipError_t grs_launch_sim_events(const grs_dgeo *d_g, const double *d_pos, const int32_t *d_grain, int64_t M,
+                                 int32_t nrot, const double *d_SUB, const double *d_hkl, const double *d_cs, int64_t cap,
+                                 unsigned long long *d_counter, int32_t *d_vox, int32_t *d_om, int32_t *d_h,
+                                 double *d_yz, hipStream_t st);
 hipError_t grs_launch_edt_cols(const uint8_t *d_mask, int64_t nz, int64_t ncols, int32_t *d_g1, hipStream_t st);
 hipError_t grs_launch_edt_rows(const int32_t *d_g1, int64_t nz, int64_t ny, int64_t slice0, int64_t nslices,
                                int32_t *d_v, double *d_z, float *d_out, hipStream_t st);
+
+hipError_t grs_launch_spot_residual(const void *fitc, int64_t n, const double *hkl, const double *omega,
+                                    const double *meas, const int32_t *grain, const double *gr6, const double *up,
+                                    double *res, double *contrib, double *jac, hipStream_t st);
+hipError_t grs_launch_grad_reduce(const double *contrib, int64_t n, const int64_t *goff, int64_t G, double *out7,
+                                  double *out6, hipStream_t st);
+size_t grs_fitc_size(void);
 
 /* Wave slots the fast kernel keeps resident (its scratch is sized from this). */
 int64_t grs_fast_wave_slots(int device);

@@ -225,8 +225,61 @@ __device__ float wave_median(const uint32_t *list, int32_t n, int lane) {
 }
 
 /* ---------------------------------------------------------------------- */
+/* orientation of an evaluation                                            */
+
+/* Fills sub (S*U*B) for evaluation e and the voxel slot; returns 1 when the
+ * orientation is valid, 0 when its row is skipped (|q_vec| > 1), -1 when the
+ * evaluation is an empty CSR row (GRS_MAP_CSR_LIST).  row/jloc receive the CSR
+ * row and the index inside the row's expanded list. */
+template <int MAP>
+__device__ __forceinline__ int eval_setup(const grs_eval_args &a, int64_t e, int64_t *vi, double *sub, int64_t *row,
+                                          int64_t *jloc) {
+  if (MAP == GRS_MAP_CSR || MAP == GRS_MAP_CSR_LIST) {
+    const grs_csr &c = a.csr;
+    int64_t ge = e;
+    if (MAP == GRS_MAP_CSR_LIST) {
+      ge = c.elist[e];
+      if (ge < 0) return -1;
+    }
+    int64_t lo = 0, hi = c.nrows;
+    while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (c.row_eoff[mid] <= ge) lo = mid;
+      else hi = mid;
+    }
+    int64_t j = ge - c.row_eoff[lo];
+    int64_t cand = c.row_ptr[lo] + j / c.S;
+    int32_t smp = (int32_t)(j % c.S);
+    int32_t n1 = c.nr + 1;
+    int32_t k1 = smp / (n1 * n1), k2 = (smp / n1) % n1, k3 = smp % n1;
+    const double *q = c.cand_q + 4 * cand;
+    const double half = (c.gridsize * (double)c.nr) / 2.0;
+    double q1 = (q[1] - half) + (double)k1 * c.gridsize;
+    double q2 = (q[2] - half) + (double)k2 * c.gridsize;
+    double q3 = (q[3] - half) + (double)k3 * c.gridsize;
+    double U[9];
+    *vi = lo;
+    *row = lo;
+    *jloc = j;
+    if (!grs_U_from_q(q1, q2, q3, U)) return 0;
+    grs_SUB(a.g, U, sub);
+    return 1;
+  } else {
+    int64_t ui;
+    eval_slots(a, e, vi, &ui);
+    *row = e;
+    *jloc = 0;
+    if (!a.valid[ui]) return 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) sub[k] = a.SUB[ui * 9 + k];
+    return 1;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
 /* k_fast                                                                  */
 
+template <bool MEDIAN, int MAP>
 __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t H) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
@@ -237,7 +290,7 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
   const grs_dgeo *g = a.g;
   const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
   const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
-  uint32_t *wlist = a.scratch + gwave * a.scratch_stride;
+  uint32_t *wlist = MEDIAN ? a.scratch + gwave * a.scratch_stride : nullptr;
   const int32_t nrot = g->nrot;
   const int32_t total = nrot * H;
   const int64_t nz = g->nz, nzny = g->nz * g->ny;
@@ -249,10 +302,15 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
   const double lrel = 1e-9 * lmax;
 
   for (int64_t e = gwave; e < a.n_eval; e += nwaves) {
-    int64_t vi, ui;
-    eval_slots(a, e, &vi, &ui);
-    if (!a.valid[ui]) {
-      if (lane == 0) {
+    int64_t vi, row, jloc;
+    double sub[9];
+    int ok = eval_setup<MAP>(a, e, &vi, sub, &row, &jloc);
+    if (ok <= 0) {
+      if (!MEDIAN) {
+        /* a skipped orientation counts as C = NaN for the argmax */
+        if (lane == 0 && ok == 0)
+          atomicMax(a.csr.row_best + row, (unsigned long long)(((1ull << GRS_KEY_JBITS) - 1) - (uint64_t)jloc));
+      } else if (lane == 0) {
         a.out_ns[e] = 0;
         a.out_ni[e] = 0;
         a.out_C[e] = 0.0;
@@ -260,9 +318,6 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
       }
       continue;
     }
-    double sub[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) sub[k] = a.SUB[ui * 9 + k];
     for (int h = lane; h < H; h += 64) {
       double gw[3];
       grs_Gw(sub, a.hkl + 3 * h, gw);
@@ -305,12 +360,17 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
         int32_t dy = 0, dz = 0;
         sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
         if (sel) {
-          key = a.keys[(int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny];
-          hit = g->has_hit && key <= g->hit_key;
+          int64_t idx = (int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny;
+          if (MEDIAN) {
+            key = a.keys[idx];
+            hit = g->has_hit && key <= g->hit_key;
+          } else {
+            hit = (a.hit_bits[idx >> 5] >> (uint32_t)(idx & 31)) & 1u;
+          }
         }
       }
       uint64_t ms = __ballot(sel);
-      if (sel) wlist[ns + (int32_t)lane_prefix(ms)] = key;
+      if (MEDIAN && sel) wlist[ns + (int32_t)lane_prefix(ms)] = key;
       ns += (int32_t)__popcll(ms);
       ni += (int32_t)__popcll(__ballot(hit));
       head += n;
@@ -347,17 +407,42 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
       drain(n);
       wave_sync();
     }
-    __threadfence_block();
-    float med = 2000.0f;
-    if (ns > 0) med = wave_median(wlist, ns, lane);
-    if (lane == 0) {
-      a.out_ns[e] = ns;
-      a.out_ni[e] = ni;
-      a.out_C[e] = (double)ni / (double)ns;
-      a.out_med[e] = med;
+    if (MEDIAN) {
+      __threadfence_block();
+      float med = 2000.0f;
+      if (ns > 0) med = wave_median(wlist, ns, lane);
+      if (lane == 0) {
+        a.out_ns[e] = ns;
+        a.out_ni[e] = ni;
+        a.out_C[e] = (double)ni / (double)ns;
+        a.out_med[e] = med;
+      }
+    } else if (lane == 0) {
+      uint64_t jpart = ((1ull << GRS_KEY_JBITS) - 1) - (uint64_t)jloc;
+      uint64_t cpart = 0;
+      if (ns > 0) {
+        double C = (double)ni / (double)ns;
+        cpart = (uint64_t)__builtin_floor(C * 1073741824.0) + 1ull;
+      }
+      atomicMax(a.csr.row_best + row, (unsigned long long)((cpart << GRS_KEY_JBITS) | jpart));
     }
     wave_sync();
   }
+}
+
+/* bit i of the hit mask = (keys[i] <= hit_key) */
+__global__ void k_pack_hits(const uint32_t *keys, int64_t n, uint32_t hit_key, int32_t has_hit, uint32_t *bits) {
+  int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t nw = (n + 31) >> 5;
+  if (w >= nw) return;
+  uint32_t b = 0;
+  if (has_hit) {
+    for (int k = 0; k < 32; k++) {
+      int64_t i = (w << 5) + k;
+      if (i < n && keys[i] <= hit_key) b |= 1u << k;
+    }
+  }
+  bits[w] = b;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -513,11 +598,46 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
   return hipGetLastError();
 }
 
+static size_t fast_lds(int32_t H) {
+  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT * sizeof(double) + GRS_QCAP * sizeof(uint32_t));
+}
+
 hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
   if (a->n_eval <= 0) return hipSuccess;
-  int64_t blocks = (n_waves + GRS_WPB - 1) / GRS_WPB;
-  size_t lds = (size_t)GRS_WPB * ((size_t)H * GRS_GWT * sizeof(double) + GRS_QCAP * sizeof(uint32_t));
-  hipLaunchKernelGGL(k_fast, dim3((unsigned)blocks), dim3(64 * GRS_WPB), lds, st, *a, H);
+  dim3 grid((unsigned)((n_waves + GRS_WPB - 1) / GRS_WPB)), block(64 * GRS_WPB);
+  size_t lds = fast_lds(H);
+  switch (a->map) {
+    case GRS_MAP_ONE_VOXEL:
+      hipLaunchKernelGGL((k_fast<true, GRS_MAP_ONE_VOXEL>), grid, block, lds, st, *a, H);
+      break;
+    case GRS_MAP_ONE_U:
+      hipLaunchKernelGGL((k_fast<true, GRS_MAP_ONE_U>), grid, block, lds, st, *a, H);
+      break;
+    case GRS_MAP_PAIRS:
+      hipLaunchKernelGGL((k_fast<true, GRS_MAP_PAIRS>), grid, block, lds, st, *a, H);
+      break;
+    case GRS_MAP_CSR_LIST:
+      hipLaunchKernelGGL((k_fast<true, GRS_MAP_CSR_LIST>), grid, block, lds, st, *a, H);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
+  if (a->n_eval <= 0) return hipSuccess;
+  if (a->map != GRS_MAP_CSR) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((n_waves + GRS_WPB - 1) / GRS_WPB)), block(64 * GRS_WPB);
+  hipLaunchKernelGGL((k_fast<false, GRS_MAP_CSR>), grid, block, fast_lds(H), st, *a, H);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,
+                                uint32_t *d_bits, hipStream_t st) {
+  int64_t nw = (n + 31) >> 5;
+  if (nw <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_hits, dim3(blocks_for(nw, 256)), dim3(256), 0, st, d_keys, n, hit_key, has_hit, d_bits);
   return hipGetLastError();
 }
 

@@ -48,6 +48,41 @@ __global__ void k_render_spots(const grs_dgeo *g, const double *pos, const int32
   }
 }
 
+/* Every selected event of voxel v (orientation SUB[grain[v]]): the omega and
+ * hkl indices and the unrounded pixel coordinates -dety22/py + dety0,
+ * -detz22/pz + detz0.  Slots come from an atomic counter; the host sorts. */
+__global__ void k_sim_events(const grs_dgeo *g, const double *pos, const int32_t *grain, int64_t M,
+                             const double *SUB, const double *hkl, const double *cs, int64_t cap,
+                             unsigned long long *counter, int32_t *ev_vox, int32_t *ev_om, int32_t *ev_h,
+                             double *ev_yz) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t nrot = g->nrot, H = g->H;
+  if (i >= M * nrot) return;
+  int64_t v = i / nrot;
+  int32_t om = (int32_t)(i - v * nrot);
+  const double *sub = SUB + 9 * (int64_t)grain[v];
+  grs_vw t;
+  grs_vw_terms(g, pos + 3 * v, cs[2 * om], cs[2 * om + 1], &t);
+  for (int32_t h = 0; h < H; h++) {
+    double gw[3], Gt[3], Glen;
+    grs_Gw(sub, hkl + 3 * h, gw);
+    grs_Gt(t.c, t.s, gw[0], gw[1], gw[2], Gt);
+    double beta = grs_event_beta(&t, Gt, &Glen);
+    int32_t dy, dz;
+    double yz[2];
+    if (grs_event_finish_yz(g, &t, Gt, Glen, beta, &dy, &dz, yz)) {
+      unsigned long long slot = atomicAdd(counter, 1ull);
+      if (slot < (unsigned long long)cap) {
+        ev_vox[slot] = (int32_t)v;
+        ev_om[slot] = om;
+        ev_h[slot] = h;
+        ev_yz[2 * slot] = yz[0];
+        ev_yz[2 * slot + 1] = yz[1];
+      }
+    }
+  }
+}
+
 /* 1-D distance along detz within each (dety, omega) column. */
 __global__ void k_edt_cols(const uint8_t *mask, int64_t nz, int64_t ncols, int32_t *g1) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,6 +155,17 @@ hipError_t grs_launch_render(const grs_dgeo *d_g, const double *d_pos, const int
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_render_spots, dim3(nblk(n, 128)), dim3(128), 0, st, d_g, d_pos, d_grain, M, d_SUB, d_hkl,
                      d_cs, d_mask);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_sim_events(const grs_dgeo *d_g, const double *d_pos, const int32_t *d_grain, int64_t M,
+                                 int32_t nrot, const double *d_SUB, const double *d_hkl, const double *d_cs, int64_t cap,
+                                 unsigned long long *d_counter, int32_t *d_vox, int32_t *d_om, int32_t *d_h,
+                                 double *d_yz, hipStream_t st) {
+  int64_t n = M * nrot;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sim_events, dim3(nblk(n, 128)), dim3(128), 0, st, d_g, d_pos, d_grain, M, d_SUB, d_hkl, d_cs,
+                     cap, d_counter, d_vox, d_om, d_h, d_yz);
   return hipGetLastError();
 }
 

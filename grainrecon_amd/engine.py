@@ -135,6 +135,29 @@ class Engine:
                                                      _ptr(out, _dp), int(method)))
         return out
 
+    def compete_csr(self, pos, row_ptr, cand_q, nr: int, gridsize: float):
+        """Batched compete with local sampling over CSR rows (grs_compete_csr).
+        Returns (best index in the row's expanded list, best quaternion,
+        Nr_simu, Nr_intersect, C, dis_median) per row."""
+        pos = _d(np.asarray(pos).reshape(-1, 3))
+        rp = np.ascontiguousarray(np.asarray(row_ptr).reshape(-1), dtype=np.int64)
+        cq = _d(np.asarray(cand_q).reshape(-1, 4))
+        M = pos.shape[0]
+        if rp.size != M + 1:
+            raise ValueError("row_ptr must hold M+1 offsets")
+        best = np.zeros(M, np.int64)
+        bq = np.zeros((M, 4))
+        ns = np.zeros(M, np.int32)
+        ni = np.zeros(M, np.int32)
+        C = np.zeros(M)
+        med = np.zeros(M, np.float32)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        self._chk(self._lib.grs_compete_csr(self._h, _ptr(pos, _dp), M, rp.ctypes.data_as(i64p), _ptr(cq, _dp),
+                                            cq.shape[0], int(nr), float(gridsize), best.ctypes.data_as(i64p),
+                                            _ptr(bq, _dp), _ptr(ns, _ip), _ptr(ni, _ip), _ptr(C, _dp),
+                                            _ptr(med, _fp)))
+        return best, bq, ns, ni, C, med
+
     # -- input preparation ------------------------------------------------------
     def render_spots(self, pos, grain, U, clear: bool = True) -> None:
         """Forward-simulates spot masks: voxel i at pos[i] (mm) with orientation U[grain[i]]."""

@@ -139,6 +139,47 @@ grs_status grs_forward_comp_compete(grs_ctx *ctx, const double *vox, int64_t M, 
                                     const double *euler, int64_t n_grains, const double *param1, double *out,
                                     int32_t method);
 
+/* Batched compete over CSR rows with local orientation sampling (one launch
+ * over every candidate of every row).  Row r is a voxel at pos[3r..3r+2] (mm)
+ * whose candidate orientations are cand_q[4c..4c+3] for c in
+ * row_ptr[r] .. row_ptr[r+1]-1 (M+1 offsets, row_ptr[0] = 0, row_ptr[M] = K).
+ * Each candidate expands to (nr+1)^3 samples on the grid of
+ * ori_local_sampling.m:22-30: q_i = (c_i - (gridsize*nr)/2) + k_i*gridsize,
+ * k in [0, nr]^3 with q1 slowest, q0 recomputed.  Per row: the first maximum
+ * of C over the expanded list (revise_all_unindexed_voxel.m:318), as its index
+ * j in that list (best, -1 for an empty row), its quaternion (best_q), counts,
+ * C and median.  The scoring launch counts hits on a bit-packed mask and
+ * reduces a packed 64-bit argmax key per row; only the winners are re-scored
+ * for the median.  Requires nrot*nrhkl <= 23170.  No reference counterpart:
+ * the reference scores these lists in serial 30000-voxel MEX calls
+ * (GrainGrow_engine_compete_comp.m:36-45) without local sampling. */
+grs_status grs_compete_csr(grs_ctx *ctx, const double *pos, int64_t M, const int64_t *row_ptr, const double *cand_q,
+                           int64_t K, int32_t nr, double gridsize, int64_t *best, double *best_q, int32_t *nr_simu,
+                           int32_t *nr_intersect, double *completeness, float *dis_median);
+
+/* ---- spot-centroid residual and its adjoint (geometry / orientation fits) - */
+
+/* Matched spots kept resident for the fits: spot i has reflection hkl[3i..],
+ * rotation angle omega[i] (degrees), measured centroid meas[2i..2i+1]
+ * (dety, detz pixels) and belongs to grain grain[i] (0-based, sorted
+ * non-decreasing, < G).  The SpotsPair columns of
+ * Forward_simu_spots_exp.m:323-325 (hkl 14-16, omega 18, centroid 19-20). */
+grs_status grs_set_spots(grs_ctx *ctx, int64_t n, const double *hkl, const double *omega, const double *meas,
+                         const int32_t *grain, int64_t G);
+
+/* Residual r (2n: unrounded simulated minus measured centroid, pixels) of the
+ * resident spots for geometry geo7 = [Lsam2sou Lsam2det dety00 detz00 tilt_x
+ * tilt_y tilt_z] (mm, degrees; geo_fitting.m:8-17) and per-grain parameters
+ * grains6 = [r1 r2 r3 x y z] (Rodrigues vector, centre of mass in mm;
+ * fit_grain_COM.m:116-118), the other constants from the context geometry.
+ * With grad7/grad6 the hand-written adjoint adds the vector-Jacobian product
+ * J^T u (u = upstream, or r itself when upstream is NULL: the gradient of
+ * 0.5*|r|^2); with jac the 2n x 13 Jacobian rows [d/d geo7, d/d grains6 of the
+ * spot's grain].  Replaces the fminsearchbnd objectives geo_fitting.m:5-79 and
+ * fit_grain_COM.m:113-170 with a differentiable, unrounded residual. */
+grs_status grs_spot_residual(grs_ctx *ctx, const double *geo7, const double *grains6, double *resid,
+                             const double *upstream, double *grad7, double *grad6, double *jac);
+
 /* ---- legacy single-precision MEX operators (stateless) ------------------ */
 
 /* cuda_forward_comp(pos, q, S, B, hkl, proj, rot, RotDet, param):
@@ -177,6 +218,15 @@ grs_status grs_cuda_forward_comp_compete(const float *vox, int64_t n_vox, const 
  * detysize x nrot mask (MATLAB order).  clear != 0 zeroes the mask first. */
 grs_status grs_render_spots(grs_ctx *ctx, const double *pos, const int32_t *grain, int64_t M, const double *U,
                             int64_t G, int32_t clear);
+/* Lists every selected event of each voxel (position pos[3i..] mm,
+ * orientation U[9*grain[i]..]) with its omega index, hkl index and unrounded
+ * pixel coordinates (-dety22/py + dety0, -detz22/pz + detz0), sorted by
+ * (voxel, omega, hkl): the simulated spot list of Forward_simu_spots_exp.m
+ * with the forward_comp geometry.  Writes min(total, cap) events and the
+ * total count to n_out. */
+grs_status grs_simulate_spots(grs_ctx *ctx, const double *pos, const int32_t *grain, int64_t M, const double *U,
+                              int64_t G, int64_t cap, int32_t *vox_out, int32_t *omega_out, int32_t *hkl_out,
+                              double *yz_out, int64_t *n_out);
 /* Copies a binary mask (nonzero = spot pixel, MATLAB order, nz x ny x nrot)
  * into the context's mask.  src_on_device as for grs_set_distance_stack. */
 grs_status grs_set_binary_stack(grs_ctx *ctx, const uint8_t *mask, int64_t nz, int64_t ny, int64_t nrot,

@@ -235,3 +235,58 @@ def test_errors_are_reported_not_raised_from_c(engine):
     with pytest.raises(_lib.GrsError):
         engine.forward_comp_compete(np.array([[10, 10, 10, 1, 2]], float), np.array([1.0]), np.zeros((1, 3)),
                                     _p1()[:11])
+
+
+def _expand_local(qc, nr, g):
+    """The engine's local-sampling grid (q1 slowest), same IEEE operations."""
+    half = (g * nr) / 2.0
+    out = []
+    for k1 in range(nr + 1):
+        for k2 in range(nr + 1):
+            for k3 in range(nr + 1):
+                out.append([0.0, (qc[1] - half) + float(k1) * g, (qc[2] - half) + float(k2) * g,
+                            (qc[3] - half) + float(k3) * g])
+    return np.array(out)
+
+
+def _matlab_argmax(C):
+    best = -1
+    for j, c in enumerate(C):
+        if c == c and (best < 0 or c > C[best]):
+            best = j
+    return 0 if best < 0 and len(C) else best
+
+
+def test_compete_csr_matches_oracle_argmax(engine):
+    s = scenes.make_scene(seed=81, ngrains=3, spread_mm=0.03)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(12)
+    nr, g = 2, 0.002
+    pos, rows, cands = [], [0], []
+    for r in range(14):
+        k = int(rng.integers(0, 5)) if r != 3 else 0  # row 3 is empty
+        grain = r % 3
+        for _ in range(k):
+            if rng.random() < 0.5:
+                qc = s.q[grain] + np.concatenate([[0.0], rng.normal(scale=0.002, size=3)])
+            else:
+                qc = scenes.random_quats(rng, 1)[0]
+            cands.append(qc)
+        rows.append(len(cands))
+        pos.append(s.voxels[grain][int(rng.integers(0, 27))])
+    pos, rows, cands = np.array(pos), np.array(rows), np.array(cands)
+    best, bq, ns, ni, C, med = engine.compete_csr(pos, rows, cands, nr, g)
+    for r in range(len(pos)):
+        if rows[r + 1] == rows[r]:
+            assert best[r] == -1
+            continue
+        qs = np.vstack([_expand_local(cands[c], nr, g) for c in range(rows[r], rows[r + 1])])
+        ref = oracle.or_space_comp(s.geo, s.stack, pos[r], qs, s.rot, s.hkl)
+        j = _matlab_argmax(ref[:, 2])
+        assert best[r] == j, r
+        assert ns[r] == ref[j, 0] and ni[r] == ref[j, 1]
+        assert np.array_equal(np.array([C[r]]).view(np.uint64), ref[j:j + 1, 2].view(np.uint64))
+        assert np.float32(ref[j, 3]).view(np.uint32) == med[r].view(np.uint32)
+        assert np.array_equal(bq[r, 1:], qs[j, 1:])
+    assert (C[rows[1:] > rows[:-1]] > 0).any()

@@ -1,0 +1,114 @@
+"""The spot-centroid residual and its hand-written adjoint on the GPU."""
+import numpy as np
+import pytest
+
+import scenes
+from grainrecon_amd import fit as F
+from grainrecon_amd.geometry import labdct_geometry
+
+pytestmark = pytest.mark.gpu
+
+TILTS = (0.25, -0.15, 0.3)
+
+
+def _rod(q):
+    return q[1:] / q[0]
+
+
+@pytest.fixture(scope="module")
+def problem(engine):
+    s = scenes.make_scene(seed=91, ngrains=4, spread_mm=0.05)
+    geo = labdct_geometry(a=2.8665, lattice="I", det_pixels=256, pixel_mm=6.72e-3 * 4, tilts=TILTS,
+                          dety00=0.03, detz00=-0.02)
+    engine.set_geometry(geo, s.hkl, s.rot)
+    g7_true = np.array([geo.Lsam2sou, geo.Lsam2det, geo.dety00, geo.detz00, *TILTS])
+    gr6_true = np.array([np.concatenate([_rod(s.q[g]), s.centers[g]]) for g in range(4)])
+    sp = F.simulate_matched_spots(engine, g7_true, s.centers, np.stack(s.U))
+    hkl = s.hkl[sp["hkl_idx"]]
+    omega = s.rot[sp["omega_idx"]]
+    fit = F.SpotFit(engine, hkl, omega, sp["meas"], sp["grain"], 4)
+    spots = dict(hkl=fit.hkl, omega=fit.omega, meas=fit.meas, grain=fit.grain)
+    return dict(geo=geo, s=s, fit=fit, spots=spots, g7=g7_true, gr6=gr6_true)
+
+
+def _perturbed(p, rng, scale=1.0):
+    g7 = p["g7"] + scale * np.array([-0.2, 0.3, 0.02, -0.02, 0.1, -0.1, 0.15]) * rng.uniform(0.5, 1.0, 7)
+    gr6 = p["gr6"] + scale * np.concatenate([rng.normal(scale=0.003, size=(4, 3)), rng.normal(scale=0.003, size=(4, 3))],
+                                            axis=1)
+    return g7, gr6
+
+
+def test_residual_zero_at_truth(problem):
+    r = problem["fit"].residual(problem["g7"], problem["gr6"])
+    assert problem["fit"].n > 100
+    assert np.abs(r).max() < 1e-8
+
+
+def test_residual_matches_torch_reference(problem):
+    from oracle import residual_ref
+
+    rng = np.random.default_rng(1)
+    g7, gr6 = _perturbed(problem, rng)
+    r = problem["fit"].residual(g7, gr6)
+    f, d7, d6, rt = residual_ref.loss_and_grad(problem["geo"], problem["spots"], g7, gr6)
+    assert np.abs(r - rt).max() < 1e-8
+
+
+def test_adjoint_matches_autograd(problem):
+    from oracle import residual_ref
+
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        g7, gr6 = _perturbed(problem, rng)
+        r, d7, d6 = problem["fit"].value_and_grad(g7, gr6)
+        _, t7, t6, _ = residual_ref.loss_and_grad(problem["geo"], problem["spots"], g7, gr6)
+        assert np.linalg.norm(d7 - t7) <= 1e-8 * np.linalg.norm(t7)
+        assert np.linalg.norm(d6 - t6) <= 1e-8 * np.linalg.norm(t6)
+
+
+def test_jacobian_matches_central_differences_and_vjp(problem):
+    rng = np.random.default_rng(3)
+    g7, gr6 = _perturbed(problem, rng)
+    fit = problem["fit"]
+    r, J = fit.jacobian(g7, gr6)
+    _, d7, d6 = fit.value_and_grad(g7, gr6)
+    # J^T r equals the adjoint's gradient
+    jt7 = np.einsum("nak,na->k", J[:, :, :7], r)
+    assert np.linalg.norm(jt7 - d7) <= 1e-10 * np.linalg.norm(d7)
+    # central differences, Richardson-extrapolated
+    for k in range(13):
+        scale = 1.0 if k < 7 else 1e-2
+        h = 1e-4 * scale
+        def rr(eps):
+            a, b = g7.copy(), gr6.copy()
+            if k < 7:
+                a[k] += eps
+            else:
+                b[:, k - 7] += eps
+            return fit.residual(a, b)
+        d1 = (rr(h) - rr(-h)) / (2 * h)
+        d2 = (rr(h / 2) - rr(-h / 2)) / h
+        fd = (4 * d2 - d1) / 3
+        an = J[:, :, k]
+        assert np.linalg.norm(fd - an) <= 1e-6 * max(np.linalg.norm(an), 1e-12), k
+
+
+def test_torch_function_gradcheck(problem):
+    import torch
+
+    Fn = F.torch_residual()
+    rng = np.random.default_rng(4)
+    g7, gr6 = _perturbed(problem, rng, 0.3)
+    a = torch.tensor(g7, dtype=torch.float64, requires_grad=True)
+    b = torch.tensor(gr6, dtype=torch.float64, requires_grad=True)
+    fit = problem["fit"]
+    assert torch.autograd.gradcheck(lambda x, y: Fn.apply(x, y, fit), (a, b), eps=1e-6, atol=1e-5, rtol=1e-4)
+
+
+def test_levenberg_marquardt_recovers_geometry(problem):
+    rng = np.random.default_rng(5)
+    g7, gr6 = _perturbed(problem, rng)
+    res = F.levenberg_marquardt(problem["fit"], g7, gr6, max_iter=60)
+    assert res.history[0] > 1.0
+    assert res.rms < 1e-6
+    assert np.abs(res.g7 - problem["g7"]).max() < 1e-6
