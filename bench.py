@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=256,
+                    help="size of a plain device copy timed (and visible to counter runs) for calibration; 0 = off")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by voxel or "
@@ -108,6 +110,19 @@ def main():
     seeds = smp.pos[srng.choice(smp.pos.shape[0], size=total, replace=False)]
 
     eng.set_timing(True)
+    copy_probe = None
+    if args.copy_probe_mb > 0:
+        nb = args.copy_probe_mb << 20
+        a_, b_ = eng.device_alloc(nb), eng.device_alloc(nb)
+        eng.device_copy(b_, a_, nb)  # first touch
+        eng.kernel_time(reset=True)
+        eng.device_copy(b_, a_, nb)
+        cms, _ = eng.kernel_time(reset=True)
+        eng.device_free(a_)
+        eng.device_free(b_)
+        copy_probe = {"bytes_read": nb, "bytes_written": nb, "ms": round(cms, 4),
+                      "GB_per_s": round(2 * nb / (cms * 1e-3) / 1e9, 1)}
+        log(f"[bench] copy probe: {2 * nb / 1e6:.0f} MB moved in {cms:.3f} ms ({copy_probe['GB_per_s']} GB/s)")
     for k in range(args.warmup):
         eng.forward_comp_device(seeds[k], d_q, N, d_ns, d_ni, d_C, d_med, _lib.METHOD_FAST)
     eng.synchronize()
@@ -157,6 +172,8 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 2),
         "roofline": roofline,
     }
+    if copy_probe:
+        result["copy_probe"] = copy_probe
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

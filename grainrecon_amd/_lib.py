@@ -62,6 +62,7 @@ _SIGS = {
     "grs_device_free": (c_i32, [c_vp, c_vp]),
     "grs_copy_to_device": (c_i32, [c_vp, c_vp, c_vp, c_i64]),
     "grs_copy_to_host": (c_i32, [c_vp, c_vp, c_vp, c_i64]),
+    "grs_device_copy": (c_i32, [c_vp, c_vp, c_vp, c_i64]),
     "grs_set_timing": (c_i32, [c_vp, c_i32]),
     "grs_kernel_time": (c_i32, [c_vp, c_dp, ctypes.POINTER(c_i64), c_i32]),
     "grs_math_device": (c_i32, [c_vp, c_i32, c_dp, c_dp, c_i64]),

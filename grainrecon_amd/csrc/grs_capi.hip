@@ -1256,6 +1256,22 @@ grs_status grs_copy_to_device(grs_ctx *c, void *d_dst, const void *src, int64_t 
   return GRS_OK;
 }
 
+grs_status grs_device_copy(grs_ctx *c, void *d_dst, const void *d_src, int64_t bytes) {
+  if (!c || !d_dst || !d_src || bytes < 0 || bytes % 4) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  HIPCHK(c, grs_launch_copy_u32(static_cast<const uint32_t *>(d_src), static_cast<uint32_t *>(d_dst), bytes / 4, c->st));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->timing) {
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->ms_total += ms;
+    c->launches += 1;
+  }
+  return GRS_OK;
+}
+
 grs_status grs_copy_to_host(grs_ctx *c, void *dst, const void *d_src, int64_t bytes) {
   if (!c || !dst || !d_src || bytes < 0) return GRS_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));

@@ -85,6 +85,7 @@ hipError_t grs_launch_plain(const grs_eval_args *a, int64_t n_threads, hipStream
 hipError_t grs_launch_compete_reduce(const int64_t *d_row_start, int64_t M, const int32_t *d_ns, const int32_t *d_ni,
                                      const double *d_C, const float *d_med, int64_t *d_best, hipStream_t st);
 hipError_t grs_launch_math_eval(int fn, const double *d_x, double *d_y, int64_t n, hipStream_t st);
+hipError_t grs_launch_copy_u32(const uint32_t *d_src, uint32_t *d_dst, int64_t n, hipStream_t st);
 hipError_t grs_launch_check_keys(const uint32_t *d_keys, int64_t n, uint32_t *d_bad, hipStream_t st);
 
 hipError_t grs_launch_render(const grs_dgeo *d_g, const double *d_pos, const int32_t *d_grain, int64_t M,

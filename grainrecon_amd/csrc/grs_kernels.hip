@@ -131,6 +131,13 @@ __global__ void k_math_eval(int fn, const double *x, double *y, int64_t n) {
   y[i] = r;
 }
 
+/* Plain device copy (calibrates memory counters: reads and writes n words). */
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 /* Flags keys that are not non-negative, non-NaN floats. */
 __global__ void k_check_keys(const uint32_t *keys, int64_t n, uint32_t *bad) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -658,6 +665,14 @@ hipError_t grs_launch_compete_reduce(const int64_t *d_row_start, int64_t M, cons
 hipError_t grs_launch_math_eval(int fn, const double *d_x, double *d_y, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_math_eval, dim3(blocks_for(n, 256)), dim3(256), 0, st, fn, d_x, d_y, n);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_copy_u32(const uint32_t *d_src, uint32_t *d_dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t b = blocks_for(n, 256);
+  if (b > 65536) b = 65536;
+  hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)b), dim3(256), 0, st, d_src, d_dst, n);
   return hipGetLastError();
 }
 

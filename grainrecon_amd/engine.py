@@ -204,6 +204,9 @@ class Engine:
         self._chk(self._lib.grs_copy_to_device(self._h, ctypes.c_void_p(d), arr.ctypes.data, arr.nbytes))
         return d
 
+    def device_copy(self, d_dst: int, d_src: int, nbytes: int) -> None:
+        self._chk(self._lib.grs_device_copy(self._h, ctypes.c_void_p(d_dst), ctypes.c_void_p(d_src), int(nbytes)))
+
     def to_host(self, dptr: int, shape, dtype) -> np.ndarray:
         out = np.zeros(shape, dtype)
         self._chk(self._lib.grs_copy_to_host(self._h, out.ctypes.data, ctypes.c_void_p(dptr), out.nbytes))

@@ -244,6 +244,9 @@ grs_status grs_device_alloc(grs_ctx *ctx, int64_t bytes, void **d_out);
 grs_status grs_device_free(grs_ctx *ctx, void *d);
 grs_status grs_copy_to_device(grs_ctx *ctx, void *d_dst, const void *src, int64_t bytes);
 grs_status grs_copy_to_host(grs_ctx *ctx, void *dst, const void *d_src, int64_t bytes);
+/* Device-to-device copy by a plain copy kernel (bytes % 4 == 0); calibrates
+ * memory-traffic counters against a transfer of known size. */
+grs_status grs_device_copy(grs_ctx *ctx, void *d_dst, const void *d_src, int64_t bytes);
 
 /* ---- measurement and diagnostics ---------------------------------------- */
 
