@@ -144,7 +144,7 @@ def main():
     eng.synchronize()
     pdt = time.perf_counter() - t
     pms, pn = eng.kernel_time(reset=True)
-    plain_value = args.plain_steps * N / pdt
+    plain_value = args.plain_steps * N / pdt if args.plain_steps else float("nan")
     log(f"[bench] plain: {args.plain_steps} steps in {pdt:.3f}s -> {plain_value:.4g} evals/s; "
         f"kernel {pms / max(pn, 1):.3f} ms/launch")
 
@@ -169,7 +169,7 @@ def main():
                    "orientations": N, "omega": nrot, "hkl": H, "detector": int(cfg.geo.detzsize)},
         "plain": {"value": round(plain_value, 1), "unit": "evals/s", "kernel_ms": round(pms / max(pn, 1), 3),
                   "steps": args.plain_steps},
-        "speedup_vs_plain": round(value / plain_value, 2),
+        "speedup_vs_plain": round(value / plain_value, 2) if args.plain_steps else None,
         "roofline": roofline,
     }
     if copy_probe:
