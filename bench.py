@@ -65,6 +65,86 @@ def cpu_baseline(cfg, proj_host, pos, q, gpu_out, seconds, threads):
     return n / dt, n, dt, exact
 
 
+def geofit_leg(eng, cfg, n_grains, nm_grains, seed=4004):
+    """Geometry refinement on matched spot centroids (the 300-grain Fe sample,
+    0.5 px Gaussian noise, perturbed start): adjoint LM on the GPU, and the
+    reference's procedure -- bounded Nelder-Mead over the 7 geometry values
+    with the grains held -- timed on the CPU on a subset of grains."""
+    import torch
+
+    from grainrecon_amd import fit as F
+    from oracle import residual_ref
+    from oracle.fminsearch import fminsearchbnd
+
+    smp, geo = cfg.sample, cfg.geo
+    rng = np.random.default_rng(seed)
+    px = geo.pixelysize
+    g7_true = np.array([geo.Lsam2sou, geo.Lsam2det, geo.dety00, geo.detz00, 0.0, 0.0, 0.0])
+    G = min(n_grains, smp.q.shape[0])
+    com = np.stack([smp.pos[smp.grain == g].mean(axis=0) for g in range(G)])
+    gr6_true = np.hstack([smp.q[:G, 1:] / smp.q[:G, :1], com])
+    sp = F.simulate_matched_spots(eng, g7_true, com, smp.U[:G], rng=rng, noise_px=0.5)
+    hkl, omega = cfg.hkl[sp["hkl_idx"]], cfg.rot[sp["omega_idx"]]
+    # start: Lsam2sou -0.2 mm, Lsam2det +0.3 mm, dety00/detz00 +-20 px, tilts +-0.2 deg;
+    # grains as indexed (orientation ~0.01 deg, centre of mass ~2 um off)
+    g7_0 = g7_true + np.array([-0.2, 0.3, 20 * px, -20 * px, 0.2, -0.2, 0.2])
+    gr6_0 = gr6_true + np.hstack([rng.normal(scale=1e-4, size=(G, 3)), rng.normal(scale=2e-3, size=(G, 3))])
+
+    def rounded(fit, r):
+        return F.mean_rounded_px(r + fit.meas, fit.meas)
+
+    full = F.SpotFit(eng, hkl, omega, sp["meas"], sp["grain"], G)
+    r0 = full.residual(g7_0, gr6_0)
+    eng.synchronize()
+    joint = F.levenberg_marquardt(full, g7_0, gr6_0, max_iter=100)
+    geo_only = F.levenberg_marquardt(full, g7_0, gr6_0, max_iter=100, fix_grains=True)
+    rj = full.residual(joint.g7, joint.gr6)
+    out = {"grains": G, "spots": int(full.n), "noise_px": 0.5, "start_rms_px": round(F.rms_px(r0), 3),
+           "lm_joint_gpu": {"unknowns": 7 + 6 * G, "rms_px": round(joint.rms, 4),
+                            "mean_rounded_px": round(rounded(full, rj), 4), "seconds": round(joint.seconds, 3),
+                            "iterations": joint.iterations,
+                            "geometry_error": [float(x) for x in np.round(joint.g7 - g7_true, 7)]},
+           "lm_geometry_gpu": {"unknowns": 7, "rms_px": round(geo_only.rms, 4), "seconds": round(geo_only.seconds, 3),
+                               "iterations": geo_only.iterations}}
+    log(f"[bench] geo-fit: {G} grains, {full.n} spots, RMS {F.rms_px(r0):.2f} px at start; joint LM "
+        f"({7 + 6 * G} unknowns) -> {joint.rms:.4f} px in {joint.seconds:.2f}s ({joint.iterations} it); "
+        f"geometry-only LM -> {geo_only.rms:.4f} px in {geo_only.seconds:.2f}s")
+
+    Gs = min(nm_grains, G)
+    sel = sp["grain"] < Gs
+    sub = F.SpotFit(eng, hkl[sel], omega[sel], sp["meas"][sel], sp["grain"][sel], Gs)
+    # L_shift_tilt_fitting.m:23-50 bounds: +-0.5 mm, +-50 px, +-0.5 deg around the start
+    half = np.array([0.5, 0.5, 50 * px, 50 * px, 0.5, 0.5, 0.5])
+    lb, ub = g7_0 - half, g7_0 + half
+    spots = dict(hkl=sub.hkl, omega=sub.omega, meas=sub.meas, grain=sub.grain)
+    held = torch.tensor(gr6_0[:Gs], dtype=torch.float64)
+
+    def obj(x):
+        with torch.no_grad():
+            r = residual_ref.residual(geo, spots, torch.tensor(x, dtype=torch.float64), held)
+        return 0.5 * float((r * r).sum())
+
+    threads = min(16, os.cpu_count() or 1)
+    torch.set_num_threads(threads)
+    t = time.perf_counter()
+    x_nm, f_nm, nfe, nit = fminsearchbnd(obj, g7_0, lb, ub)
+    nm_t = time.perf_counter() - t
+    nm_rms = float(np.sqrt(2 * f_nm / sub.n))
+    lm = F.levenberg_marquardt(sub, g7_0, gr6_0[:Gs], max_iter=100, fix_grains=True)
+    out["fminsearchbnd_subset"] = {
+        "grains": Gs, "spots": int(sub.n), "unknowns": 7,
+        "cpu": {"rms_px": round(nm_rms, 4), "mean_rounded_px": round(rounded(sub, sub.residual(x_nm, gr6_0[:Gs])), 4),
+                "seconds": round(nm_t, 2), "fun_evals": nfe, "iterations": nit, "threads": threads},
+        "gpu_lm": {"rms_px": round(lm.rms, 4), "mean_rounded_px": round(rounded(sub, sub.residual(lm.g7, gr6_0[:Gs])), 4),
+                   "seconds": round(lm.seconds, 3), "iterations": lm.iterations},
+        "rms_gap_px": round(nm_rms - lm.rms, 4),
+        "speedup": round(nm_t / max(lm.seconds, 1e-9), 1),
+    }
+    log(f"[bench] geometry fit on {Gs} grains ({sub.n} spots): fminsearchbnd (CPU, {threads} threads) "
+        f"{nm_rms:.4f} px in {nm_t:.1f}s ({nfe} evaluations); adjoint LM (GPU) {lm.rms:.4f} px in {lm.seconds:.3f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +155,8 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--geo-fit-grains", type=int, default=300, help="grains in the config-4 fit leg; 0 = off")
+    ap.add_argument("--geo-fit-nm-grains", type=int, default=20, help="grains in the Nelder-Mead comparison")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
                     help="size of a plain device copy timed (and visible to counter runs) for calibration; 0 = off")
     args = ap.parse_args()
@@ -191,6 +273,8 @@ def main():
         result["parity"] = {"evals_checked": n, "bit_exact": bool(exact)}
     for d in (d_q, d_ns, d_ni, d_C, d_med):
         eng.device_free(d)
+    if args.geo_fit_grains > 0:
+        result["geo_fit"] = geofit_leg(eng, cfg, args.geo_fit_grains, args.geo_fit_nm_grains)
     eng.close()
     print(json.dumps(result), flush=True)
 

@@ -226,6 +226,8 @@ grs_status grs_set_geometry(grs_ctx *c, const grs_geometry *geo, const double *h
   g.pio2 = GRS_PI / 2.0;
   g.thr = geo->minEucDis / ((geo->pixelysize + geo->pixelzsize) / 2.0);
   g.cos_beta_max = std::cos(g.beta_max);
+  g.ipy = 1.0 / g.py;
+  g.ipz = 1.0 / g.pz;
   hit_key_for(g.thr, &g.hit_key, &g.has_hit);
   c->H = (int32_t)nrhkl;
   c->nrot = (int32_t)nrot;

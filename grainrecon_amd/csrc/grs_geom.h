@@ -35,7 +35,8 @@ typedef struct grs_dgeo {
   double bsy0, bsy1, bsz0, bsz1;
   double beta_max, lambda_min, lambda_max, pio2;
   double thr;        /* minEucDis / mean([py pz]) */
-  double cos_beta_max; /* cos(beta_max), used only by the conservative filter */
+  double cos_beta_max; /* cos(beta_max), used only by the conservative filters */
+  double ipy, ipz;     /* 1/py, 1/pz, used only by the conservative filters */
   uint32_t hit_key;  /* largest key k with (double)as_float(k) <= thr */
   int32_t has_hit;   /* 0 when no non-negative distance can be a hit */
   int32_t H, nrot;

@@ -286,27 +286,89 @@ __device__ __forceinline__ int eval_setup(const grs_eval_args &a, int64_t e, int
 /* ---------------------------------------------------------------------- */
 /* k_fast                                                                  */
 
+/* Refined hardware reciprocal / reciprocal square root (one Newton step):
+ * used only by the conservative position filter. */
+__device__ __forceinline__ double fast_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  return y * __builtin_fma(-x, y, 2.0);
+}
+__device__ __forceinline__ double fast_rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  return y * __builtin_fma(-0.5 * x, y * y, 1.5);
+}
+
+/* Conservative position filter (phase B1).  The spot position of the event
+ * is recomputed with the same formulas in exact arithmetic identities --
+ * sin(theta) = -cos(beta), cos(theta) = sqrt(1 - cos^2 beta),
+ * sin(phix) = sqrt(1 - cos^2 phix), sin(phix - 2 theta) by the difference
+ * formula -- and refined hardware reciprocals, which agree with the exact
+ * pipeline to about 1e-12 relative.  Guards send ill-conditioned events
+ * (|sin(phiy)| < 1e-3, |t denominator| < 1e-3, G nearly along the beam) to
+ * the exact path; otherwise the position error is far below the 1-pixel
+ * margin, so an event is dropped only when its exact pixel is certainly off
+ * the detector or inside the beamstop.  NaNs keep the event. */
+__device__ __forceinline__ bool b1_keep(const grs_dgeo *g, const double *v, double gw0, double gw1, double gw2) {
+  const double c = v[0], s = v[1];
+  const double G0 = c * gw0 - s * gw1, G1 = s * gw0 + c * gw1, G2 = gw2;
+  const double g2 = __builtin_fma(G0, G0, __builtin_fma(G1, G1, G2 * G2));
+  const double cb = __builtin_fma(v[2], G0, __builtin_fma(v[3], G1, v[4] * G2)) * fast_rsq(g2);
+  const double omc = __builtin_fma(-cb, cb, 1.0);
+  const double cth = omc * fast_rsq(omc);
+  const double s2t = -2.0 * cb * cth;
+  const double c2t = __builtin_fma(-2.0 * cb, cb, 1.0);
+  const double nv2 = __builtin_fma(G1, G1, G2 * G2);
+  const double inv = fast_rsq(nv2);
+  const double ap = __builtin_fma(G1, v[7], G2 * v[8]) * inv;
+  double oma = __builtin_fma(-ap, ap, 1.0);
+  oma = oma > 0.0 ? oma : 0.0;
+  const double sph = oma > 0.0 ? oma * fast_rsq(oma) : 0.0;
+  const double sphy = __builtin_fma(sph, c2t, -ap * s2t);
+  if (!(__builtin_fabs(sphy) >= 1e-3) || !(nv2 >= 1e-12 * g2)) return true;
+  const double diffvec = v[9] * s2t * fast_rcp(sphy);
+  const double y22 = __builtin_fma(diffvec * G1, inv, v[5]);
+  const double z22 = __builtin_fma(diffvec * G2, inv, v[6]);
+  const double k0 = v[10], k1 = y22 - v[11], k2 = z22 - v[12];
+  const double ik = fast_rsq(__builtin_fma(k0, k0, __builtin_fma(k1, k1, k2 * k2)));
+  const double K0 = k0 * ik, K1 = k1 * ik, K2 = k2 * ik;
+  const double *R = g->R;
+  const double tden = __builtin_fma(R[0], K0, __builtin_fma(R[3], K1, R[6] * K2));
+  if (!(__builtin_fabs(tden) >= 1e-3)) return true;
+  const double t = v[13] * fast_rcp(tden);
+  const double w0 = __builtin_fma(t, K0, -k0), w1 = __builtin_fma(t, K1, v[14]), w2 = __builtin_fma(t, K2, v[15]);
+  const double Y = __builtin_fma(R[1], w0, __builtin_fma(R[4], w1, R[7] * w2));
+  const double Z = __builtin_fma(R[2], w0, __builtin_fma(R[5], w1, R[8] * w2));
+  const double ry = __builtin_fma(-Y, g->ipy, g->dety0);
+  const double rz = __builtin_fma(-Z, g->ipz, g->detz0);
+  const double m = 1.0; /* pixels */
+  const bool off = ry < 0.5 - m || ry > g->detysize + 0.5 + m || rz < 0.5 - m || rz > g->detzsize + 0.5 + m;
+  const bool in_bs = ry >= g->bsy0 - 0.5 + m && ry <= g->bsy1 + 0.5 - m && rz >= g->bsz0 - 0.5 + m &&
+                     rz <= g->bsz1 + 0.5 - m;
+  return !(off || in_bs);
+}
+
 template <bool MEDIAN, int MAP>
-__global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t H) {
+__global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32_t H) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
-  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_QCAP / 2;
+  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_QCAP; /* two rings of GRS_QCAP u32 */
   double *gwt = smem + wib * per_wave;
-  uint32_t *queue = (uint32_t *)(gwt + (int64_t)H * GRS_GWT);
+  uint32_t *ring1 = (uint32_t *)(gwt + (int64_t)H * GRS_GWT);
+  uint32_t *ring2 = ring1 + GRS_QCAP;
   const grs_dgeo *g = a.g;
   const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
   const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
   uint32_t *wlist = MEDIAN ? a.scratch + gwave * a.scratch_stride : nullptr;
   const int32_t nrot = g->nrot;
-  const int32_t total = nrot * H;
   const int64_t nz = g->nz, nzny = g->nz * g->ny;
   const double m4pi = -4.0 * GRS_PI;
-  /* filter margins: see the header comment */
+  /* phase A margins: see the header comment */
   const double dcb = 1e-12;
   const double cbm = g->cos_beta_max - dcb;
   const double lmin = g->lambda_min, lmax = g->lambda_max;
   const double lrel = 1e-9 * lmax;
+  /* starting (omega, hkl) of each lane in the flattened event space */
+  const int32_t om0 = lane / H, h0 = lane - (lane / H) * H;
 
   for (int64_t e = gwave; e < a.n_eval; e += nwaves) {
     int64_t vi, row, jloc;
@@ -343,17 +405,30 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
     wave_sync();
 
     const double *vwv = a.vw + vi * (int64_t)nrot * GRS_VW_DOUBLES;
-    uint32_t head = 0, tail = 0;
+    uint32_t head1 = 0, tail1 = 0, head2 = 0, tail2 = 0;
     int32_t ns = 0, ni = 0;
+    /* the gather of the previous exact batch is consumed one batch later */
+    bool p_sel = false, p_hit = false;
+    uint32_t p_key = 0;
 
-    /* phase B on n queued events starting at ring position head */
-    auto drain = [&](uint32_t n) {
-      bool sel = false, hit = false;
-      uint32_t key = 0;
+    auto consume = [&]() {
+      uint64_t ms = __ballot(p_sel);
+      if (MEDIAN && p_sel) wlist[ns + (int32_t)lane_prefix(ms)] = p_key;
+      ns += (int32_t)__popcll(ms);
+      bool hit;
+      if (MEDIAN) hit = p_sel && g->has_hit && p_key <= g->hit_key;
+      else hit = p_hit;
+      ni += (int32_t)__popcll(__ballot(hit));
+      p_sel = false;
+      p_hit = false;
+    };
+
+    /* phase B2: exact pipeline on n events of ring2 */
+    auto b2 = [&](uint32_t n) {
+      consume();
       if ((uint32_t)lane < n) {
-        uint32_t ev = queue[(head + lane) & (GRS_QCAP - 1)];
-        int32_t om = (int32_t)(ev / (uint32_t)H);
-        int32_t h = (int32_t)ev - om * H;
+        uint32_t ev = ring2[(head2 + lane) & (GRS_QCAP - 1)];
+        int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
         grs_vw v;
         const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
         double *dst = (double *)&v;
@@ -365,30 +440,44 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
         double Glen;
         double beta = grs_event_beta(&v, Gt, &Glen);
         int32_t dy = 0, dz = 0;
-        sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
-        if (sel) {
+        p_sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
+        if (p_sel) {
           int64_t idx = (int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny;
-          if (MEDIAN) {
-            key = a.keys[idx];
-            hit = g->has_hit && key <= g->hit_key;
-          } else {
-            hit = (a.hit_bits[idx >> 5] >> (uint32_t)(idx & 31)) & 1u;
-          }
+          if (MEDIAN) p_key = a.keys[idx];
+          else p_hit = (a.hit_bits[idx >> 5] >> (uint32_t)(idx & 31)) & 1u;
         }
       }
-      uint64_t ms = __ballot(sel);
-      if (MEDIAN && sel) wlist[ns + (int32_t)lane_prefix(ms)] = key;
-      ns += (int32_t)__popcll(ms);
-      ni += (int32_t)__popcll(__ballot(hit));
-      head += n;
+      head2 += n;
     };
 
-    for (int32_t base = 0; base < total; base += 64) {
-      int32_t ev = base + lane;
+    /* phase B1: conservative position filter on n events of ring1 */
+    auto b1 = [&](uint32_t n) {
+      bool keep = false;
+      uint32_t ev = 0;
+      if ((uint32_t)lane < n) {
+        ev = ring1[(head1 + lane) & (GRS_QCAP - 1)];
+        int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
+        const double *t = gwt + h * GRS_GWT;
+        keep = b1_keep(g, vwv + (int64_t)om * GRS_VW_DOUBLES, t[0], t[1], t[2]);
+      }
+      head1 += n;
+      uint64_t m = __ballot(keep);
+      if (keep) ring2[(tail2 + lane_prefix(m)) & (GRS_QCAP - 1)] = ev;
+      tail2 += (uint32_t)__popcll(m);
+      if (tail2 - head2 >= 64) {
+        wave_sync();
+        b2(64);
+        wave_sync();
+      }
+    };
+
+    /* phase A: wave-uniform steps of 64 consecutive events; lane l owns event
+     * l + 64*k, tracked as (omega, hkl) without divisions */
+    int32_t om = om0, h = h0;
+    const int32_t nsteps = (nrot * H + 63) / 64;
+    for (int32_t k = 0; k < nsteps; ++k) {
       bool pass = false;
-      if (ev < total) {
-        int32_t om = ev / H;
-        int32_t h = ev - om * H;
+      if (om < nrot) {
         const double *vv = vwv + (int64_t)om * GRS_VW_DOUBLES;
         double c = vv[0], s = vv[1], u0 = vv[2], u1 = vv[3], u2 = vv[4];
         const double *t = gwt + h * GRS_GWT;
@@ -399,21 +488,34 @@ __global__ __launch_bounds__(64 * GRS_WPB) void k_fast(grs_eval_args a, int32_t 
         pass = (dot < t[3]) && (dot > t[4]) && (l4 > t[5]) && (l4 < t[6]);
       }
       uint64_t m = __ballot(pass);
-      if (pass) queue[(tail + lane_prefix(m)) & (GRS_QCAP - 1)] = (uint32_t)ev;
-      tail += (uint32_t)__popcll(m);
-      if (tail - head >= 64) {
+      if (pass) ring1[(tail1 + lane_prefix(m)) & (GRS_QCAP - 1)] = ((uint32_t)om << 8) | (uint32_t)h;
+      tail1 += (uint32_t)__popcll(m);
+      if (tail1 - head1 >= 64) {
         wave_sync();
-        drain(64);
+        b1(64);
         wave_sync();
       }
+      h += 64;
+      while (h >= H) {
+        h -= H;
+        om++;
+      }
     }
-    while (tail != head) {
-      uint32_t n = tail - head;
+    while (tail1 != head1) {
+      uint32_t n = tail1 - head1;
       if (n > 64) n = 64;
       wave_sync();
-      drain(n);
+      b1(n);
       wave_sync();
     }
+    while (tail2 != head2) {
+      uint32_t n = tail2 - head2;
+      if (n > 64) n = 64;
+      wave_sync();
+      b2(n);
+      wave_sync();
+    }
+    consume();
     if (MEDIAN) {
       __threadfence_block();
       float med = 2000.0f;
@@ -606,38 +708,60 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
 }
 
 static size_t fast_lds(int32_t H) {
-  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT * sizeof(double) + GRS_QCAP * sizeof(uint32_t));
+  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT * sizeof(double) + 2 * GRS_QCAP * sizeof(uint32_t));
+}
+
+/* Resident blocks per CU of one k_fast instantiation (cached). */
+template <bool MEDIAN, int MAP>
+static int fast_blocks_per_cu(size_t lds) {
+  static int cached = 0;
+  static size_t cached_lds = 0;
+  if (cached == 0 || cached_lds != lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<MEDIAN, MAP>, 64 * GRS_WPB, lds) != hipSuccess ||
+        nb <= 0)
+      nb = 1;
+    cached = nb;
+    cached_lds = lds;
+  }
+  return cached;
+}
+
+static int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  return cus;
+}
+
+/* Persistent launch: at most as many waves as can be resident at once. */
+template <bool MEDIAN, int MAP>
+static hipError_t launch_fast_impl(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
+  size_t lds = fast_lds(H);
+  int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP>(lds);
+  int64_t blocks = (n_waves + GRS_WPB - 1) / GRS_WPB;
+  if (blocks > resident) blocks = resident;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((k_fast<MEDIAN, MAP>), dim3((unsigned)blocks), dim3(64 * GRS_WPB), lds, st, *a, H);
+  return hipGetLastError();
 }
 
 hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
   if (a->n_eval <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n_waves + GRS_WPB - 1) / GRS_WPB)), block(64 * GRS_WPB);
-  size_t lds = fast_lds(H);
   switch (a->map) {
-    case GRS_MAP_ONE_VOXEL:
-      hipLaunchKernelGGL((k_fast<true, GRS_MAP_ONE_VOXEL>), grid, block, lds, st, *a, H);
-      break;
-    case GRS_MAP_ONE_U:
-      hipLaunchKernelGGL((k_fast<true, GRS_MAP_ONE_U>), grid, block, lds, st, *a, H);
-      break;
-    case GRS_MAP_PAIRS:
-      hipLaunchKernelGGL((k_fast<true, GRS_MAP_PAIRS>), grid, block, lds, st, *a, H);
-      break;
-    case GRS_MAP_CSR_LIST:
-      hipLaunchKernelGGL((k_fast<true, GRS_MAP_CSR_LIST>), grid, block, lds, st, *a, H);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case GRS_MAP_ONE_VOXEL: return launch_fast_impl<true, GRS_MAP_ONE_VOXEL>(a, H, n_waves, st);
+    case GRS_MAP_ONE_U: return launch_fast_impl<true, GRS_MAP_ONE_U>(a, H, n_waves, st);
+    case GRS_MAP_PAIRS: return launch_fast_impl<true, GRS_MAP_PAIRS>(a, H, n_waves, st);
+    case GRS_MAP_CSR_LIST: return launch_fast_impl<true, GRS_MAP_CSR_LIST>(a, H, n_waves, st);
+    default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
   if (a->n_eval <= 0) return hipSuccess;
   if (a->map != GRS_MAP_CSR) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((n_waves + GRS_WPB - 1) / GRS_WPB)), block(64 * GRS_WPB);
-  hipLaunchKernelGGL((k_fast<false, GRS_MAP_CSR>), grid, block, fast_lds(H), st, *a, H);
-  return hipGetLastError();
+  return launch_fast_impl<false, GRS_MAP_CSR>(a, H, n_waves, st);
 }
 
 hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,
@@ -685,7 +809,7 @@ hipError_t grs_launch_check_keys(const uint32_t *d_keys, int64_t n, uint32_t *d_
 }
 
 int64_t grs_fast_wave_slots(int device) {
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
-  return (int64_t)cus * 8 * GRS_WPB;
+  (void)device;
+  /* upper bound over the instantiations: the scratch is indexed by wave */
+  return (int64_t)device_cus() * 16 * GRS_WPB;
 }

@@ -96,8 +96,10 @@ class FitResult:
 
 
 def levenberg_marquardt(fit: SpotFit, g7, gr6, max_iter: int = 50, tol: float = 1e-10, lam: float = 1e-3,
-                        fix_geometry: bool = False) -> FitResult:
-    """Joint refinement of geometry and grains (block-arrow LM)."""
+                        fix_geometry: bool = False, fix_grains: bool = False) -> FitResult:
+    """Joint refinement of geometry and grains (block-arrow LM); either block
+    can be held fixed (fix_grains=True is the reference's geometry-only fit,
+    L_shift_tilt_fitting.m:23-50)."""
     t0 = time.perf_counter()
     g7 = np.array(g7, float)
     gr6 = np.array(gr6, float).reshape(fit.G, 6)
@@ -130,11 +132,16 @@ def levenberg_marquardt(fit: SpotFit, g7, gr6, max_iter: int = 50, tol: float = 
             Ak_inv = np.linalg.inv(Ak)
             if fix_geometry:
                 dg = np.zeros(7)
+            elif fix_grains:
+                dg = np.linalg.solve(Agg + Dg, -bg)
             else:
                 Sg = Agg + Dg - np.einsum("gij,gjk,glk->il", Agk, Ak_inv, Agk)
                 rhs = -(bg - np.einsum("gij,gjk,gk->i", Agk, Ak_inv, bk))
                 dg = np.linalg.solve(Sg, rhs)
-            dk = -np.einsum("gij,gj->gi", Ak_inv, bk + np.einsum("gji,j->gi", Agk, dg))
+            if fix_grains:
+                dk = np.zeros_like(gr6)
+            else:
+                dk = -np.einsum("gij,gj->gi", Ak_inv, bk + np.einsum("gji,j->gi", Agk, dg))
             g7n, gr6n = g7 + dg, gr6 + dk
             rn, Jn = fit.jacobian(g7n, gr6n)
             costn = 0.5 * float(np.sum(rn * rn))

@@ -112,3 +112,12 @@ def test_levenberg_marquardt_recovers_geometry(problem):
     assert res.history[0] > 1.0
     assert res.rms < 1e-6
     assert np.abs(res.g7 - problem["g7"]).max() < 1e-6
+
+
+def test_geometry_only_fit_with_grains_held(problem):
+    rng = np.random.default_rng(6)
+    g7, _ = _perturbed(problem, rng)
+    res = F.levenberg_marquardt(problem["fit"], g7, problem["gr6"], max_iter=60, fix_grains=True)
+    assert np.array_equal(res.gr6, problem["gr6"])
+    assert res.rms < 1e-6
+    assert np.abs(res.g7 - problem["g7"]).max() < 1e-6
