@@ -110,6 +110,8 @@ def geofit_leg(eng, cfg, n_grains, nm_grains, seed=4004):
         f"({7 + 6 * G} unknowns) -> {joint.rms:.4f} px in {joint.seconds:.2f}s ({joint.iterations} it); "
         f"geometry-only LM -> {geo_only.rms:.4f} px in {geo_only.seconds:.2f}s")
 
+    if nm_grains <= 0:
+        return out
     Gs = min(nm_grains, G)
     sel = sp["grain"] < Gs
     sub = F.SpotFit(eng, hkl[sel], omega[sel], sp["meas"][sel], sp["grain"][sel], Gs)
@@ -145,6 +147,79 @@ def geofit_leg(eng, cfg, n_grains, nm_grains, seed=4004):
     return out
 
 
+def expand_local(qc, nr, g):
+    """The engine's local-sampling grid for one candidate (q1 slowest; q0 is
+    recomputed by the scorer), same IEEE operations as the device."""
+    half = (g * nr) / 2.0
+    k = np.arange(nr + 1, dtype=float) * g
+    a, b, c = (qc[1] - half) + k, (qc[2] - half) + k, (qc[3] - half) + k
+    A, B, C = np.meshgrid(a, b, c, indexing="ij")
+    return np.stack([np.zeros(A.size), A.reshape(-1), B.reshape(-1), C.reshape(-1)], axis=1)
+
+
+def batched_leg(eng, cfg, n_voxels, steps, warmup, seed=2003):
+    """Batched compete with local sampling (CSR rows of neighbour orientations,
+    each expanded to 7^3 samples spanning +-1.5 deg in 0.5 deg steps) in one
+    persistent launch per batch."""
+    smp = cfg.sample
+    rng = np.random.default_rng(seed)
+    G = smp.q.shape[0]
+    nr, gridsize = 6, float(np.sin(np.deg2rad(0.25)))
+    batches = []
+    vox = rng.choice(smp.pos.shape[0], size=n_voxels * (steps + warmup), replace=False)
+    for b in range(steps + warmup):
+        v = vox[b * n_voxels:(b + 1) * n_voxels]
+        k = np.minimum(rng.geometric(1.0 / 8.0, size=n_voxels), 64)
+        rows, cands = [0], []
+        for i, vi in enumerate(v):
+            own = int(smp.grain[vi])
+            others = rng.choice(np.delete(np.arange(G), own), size=k[i] - 1, replace=False)
+            ids = np.sort(np.concatenate([[own], others]))   # find_neighbor_ORs order: ascending grain id
+            cands.append(smp.q[ids])
+            rows.append(rows[-1] + ids.size)
+        batches.append((smp.pos[v], np.array(rows), np.vstack(cands)))
+    S = (nr + 1) ** 3
+    for b in range(warmup):
+        eng.compete_csr(*batches[b], nr, gridsize)
+    eng.synchronize()
+    eng.kernel_time(reset=True)
+    t = time.perf_counter()
+    outs = []
+    for b in range(warmup, warmup + steps):
+        outs.append(eng.compete_csr(*batches[b], nr, gridsize))
+    eng.synchronize()
+    dt = time.perf_counter() - t
+    kms, kn = eng.kernel_time(reset=True)
+    evals = sum(int(batches[b][1][-1]) * S for b in range(warmup, warmup + steps))
+    C = np.concatenate([o[4] for o in outs])
+    res = {"value": round(evals / (kms * 1e-3), 1), "unit": "evals/s", "wall_evals_per_s": round(evals / dt, 1),
+           "evals": evals, "voxels_per_step": n_voxels, "steps": steps, "samples_per_candidate": S,
+           "mean_candidates_per_voxel": round(evals / S / (n_voxels * steps), 3), "kernel_ms_total": round(kms, 3),
+           "launches": kn, "winners_C1": int(np.sum(C == 1.0)), "kernel": "k_fast<counts-only, CSR>"}
+    log(f"[bench] batched: {evals} evals in {kms:.1f} ms of scoring kernels ({res['value']:.4g} evals/s; wall "
+        f"{dt:.2f}s incl. host CSR/winner pass), {res['winners_C1']} of {n_voxels * steps} voxels at C = 1")
+    return res, batches[warmup], outs[0], (nr, gridsize)
+
+
+def check_batched_rows(cfg, proj, batch, out, nrg, rows, threads):
+    """Oracle argmax over the expanded lists of a few rows."""
+    import oracle
+
+    pos, rp, cq = batch
+    nr, g = nrg
+    stack = oracle.Stack(proj)
+    ok = True
+    for r in rows:
+        qs = np.vstack([expand_local(cq[c], nr, g) for c in range(rp[r], rp[r + 1])])
+        ref = oracle.or_space_comp(cfg.geo, stack, pos[r], qs, cfg.rot, cfg.hkl, nthreads=threads)
+        Cr = ref[:, 2]
+        j = int(np.nanargmax(Cr)) if np.isfinite(Cr).any() else 0
+        best, _, ns, ni, C, med = out
+        ok &= bool(best[r] == j and ns[r] == ref[j, 0] and ni[r] == ref[j, 1] and C[r] == Cr[j]
+                   and np.float32(med[r]) == np.float32(ref[j, 3]))
+    return ok
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +230,8 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-voxels", type=int, default=2048, help="voxels per batched compete step; 0 = off")
+    ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--geo-fit-grains", type=int, default=300, help="grains in the config-4 fit leg; 0 = off")
     ap.add_argument("--geo-fit-nm-grains", type=int, default=20, help="grains in the Nelder-Mead comparison")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
@@ -256,6 +333,13 @@ def main():
     }
     if copy_probe:
         result["copy_probe"] = copy_probe
+    batched = None
+    if args.batched_voxels > 0:
+        bres, bbatch, bout, nrg = batched_leg(eng, cfg, args.batched_voxels, args.batched_steps, 1)
+        if args.plain_steps:
+            bres["speedup_vs_plain"] = round(bres["value"] / plain_value, 2)
+        result["batched"] = bres
+        batched = (bbatch, bout, nrg)
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -271,10 +355,19 @@ def main():
                                   "sample": f"{n} orientations x 1 seed voxel of the same workload "
                                             f"(oracle/grs_oracle.c, gcc -O2, OpenMP)"}
         result["parity"] = {"evals_checked": n, "bit_exact": bool(exact)}
+        if batched:
+            bbatch, bout, nrg = batched
+            sizes = np.diff(bbatch[1])
+            rows = [int(r) for r in np.argsort(sizes, kind="stable")[:3]]
+            t = time.perf_counter()
+            ok = check_batched_rows(cfg, proj, bbatch, bout, nrg, rows, threads)
+            result["batched"]["parity"] = {"rows_checked": rows, "evals_checked": int(sizes[rows].sum()) * 343,
+                                           "argmax_and_counts_equal": ok}
+            log(f"[bench] batched parity on rows {rows}: {ok} ({time.perf_counter() - t:.1f}s)")
     for d in (d_q, d_ns, d_ni, d_C, d_med):
         eng.device_free(d)
     if args.geo_fit_grains > 0:
-        result["geo_fit"] = geofit_leg(eng, cfg, args.geo_fit_grains, args.geo_fit_nm_grains)
+        result["geo_fit"] = geofit_leg(eng, cfg, args.geo_fit_grains, 0 if args.no_cpu else args.geo_fit_nm_grains)
     eng.close()
     print(json.dumps(result), flush=True)
 

@@ -47,6 +47,8 @@ struct grs_ctx {
   int64_t f_n = 0, f_G = 0;
   int64_t mnz = 0, mny = 0, mnrot = 0; /* shape of d_mask */
   bool timing = false;
+  bool audit = false;
+  DevBuf d_audit;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
   int64_t launches = 0;
@@ -249,6 +251,9 @@ grs_status grs_set_distance_stack(grs_ctx *c, const float *proj, int64_t nz, int
                                   int32_t on_device) {
   if (!c) return GRS_ERR_ARG;
   if (!proj || nz < 1 || ny < 1 || nrot < 1) return fail(c, GRS_ERR_ARG, "invalid distance stack shape");
+  if (nz > ((int64_t)1 << 32) / ny / nrot)
+    return fail(c, GRS_ERR_ARG, "distance stack of %lld x %lld x %lld exceeds 2^32 entries", (long long)nz,
+                (long long)ny, (long long)nrot);
   HIPCHK(c, hipSetDevice(c->device));
   int64_t n = nz * ny * nrot;
   ENSURE(c, c->d_keys, sizeof(uint32_t) * n);
@@ -320,8 +325,11 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
     ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)waves * stride);
     a.vw = P<double>(c->d_vw);
     a.scratch = P<uint32_t>(c->d_scratch);
+    const bool audit = c->audit && map == GRS_MAP_ONE_VOXEL;
+    if (audit) a.audit = P<unsigned long long>(c->d_audit);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, grs_launch_fast(&a, c->H, waves, c->st));
+    if (audit) HIPCHK(c, grs_launch_fast_audit(&a, c->H, waves, c->st));
+    else HIPCHK(c, grs_launch_fast(&a, c->H, waves, c->st));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
   } else if (method == GRS_METHOD_PLAIN) {
     int64_t T = n_eval < 32768 ? n_eval : 32768;
@@ -1287,6 +1295,29 @@ grs_status grs_copy_to_host(grs_ctx *c, void *dst, const void *d_src, int64_t by
 grs_status grs_set_timing(grs_ctx *c, int32_t enable) {
   if (!c) return GRS_ERR_ARG;
   c->timing = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_set_audit(grs_ctx *c, int32_t enable) {
+  if (!c) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (enable) {
+    ENSURE(c, c->d_audit, sizeof(unsigned long long) * GRS_AUDIT_WORDS);
+    HIPCHK(c, hipMemsetAsync(c->d_audit.p, 0, sizeof(unsigned long long) * GRS_AUDIT_WORDS, c->st));
+  }
+  c->audit = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_get_audit(grs_ctx *c, int64_t *counts, double *max_dev) {
+  if (!c || !counts || !max_dev) return GRS_ERR_ARG;
+  if (!c->d_audit.p) return fail(c, GRS_ERR_STATE, "audit never enabled");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned long long w[GRS_AUDIT_WORDS];
+  HIPCHK(c, hipMemcpyAsync(w, c->d_audit.p, sizeof(w), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int k = 0; k < 5; k++) counts[k] = (int64_t)w[k];
+  std::memcpy(max_dev, &w[5], sizeof(double));
   return GRS_OK;
 }
 

@@ -54,7 +54,14 @@ typedef struct grs_eval_args {
   float *out_med;
   const uint32_t *hit_bits; /* counts-only kernels: bit i = (keys[i] <= hit_key) */
   grs_csr csr;
+  unsigned long long *audit; /* GRS_AUDIT_WORDS counters, audit launches only */
 } grs_eval_args;
+
+/* Audit counters of the fast kernel's resolution stage: events seen by phase
+ * B1, resolved as selected, resolved as not selected, sent to the exact
+ * pipeline, disagreements with the exact pipeline, and the largest
+ * |coordinate difference| (pixels, as the bits of a non-negative double). */
+#define GRS_AUDIT_WORDS 6
 
 /* Packed per-row argmax key of the counts-only kernel: larger is better.
  * ((floor(C*2^30) + 1) << 33) | (2^33-1-j) for a scored candidate j with
@@ -79,6 +86,7 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_euler,
 hipError_t grs_launch_omega_cs(const double *d_rot, int32_t nrot, double *d_cs, hipStream_t st);
 hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
 hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
+hipError_t grs_launch_fast_audit(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
 hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,
                                 uint32_t *d_bits, hipStream_t st);
 hipError_t grs_launch_plain(const grs_eval_args *a, int64_t n_threads, hipStream_t st);

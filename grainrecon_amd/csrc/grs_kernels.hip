@@ -15,9 +15,15 @@
  *            1e-9 relative on lambda, far above the FP64 rounding of the exact
  *            path).  Survivors are compacted into a 128-slot LDS ring with
  *            ballot + mbcnt.
- *   phase B: whenever 64 survivors are queued the wave runs the exact FP64
- *            event pipeline on them with every lane busy, gathers the
- *            distance keys and counts hits with ballot/popcount.
+ *   phase B1: whenever 64 survivors are queued the wave recomputes their
+ *            spot positions through exact identities and refined hardware
+ *            reciprocals (b1_classify) and decides every event whose pixel
+ *            and window tests are unambiguous by a wide margin; the rest
+ *            (about 1e-4 of them) are queued for
+ *   phase B2: the exact FP64 event pipeline of forward_comp.m, 64 at a time.
+ *   gather:  selected events are queued as stack indices; 64 at a time the
+ *            wave gathers their distance keys (or hit bits), consumed one
+ *            batch later, and counts hits with ballot/popcount.
  *   median : selected keys go to a per-wave list in HBM; the median is found
  *            by a 32-step radix bisection over the keys held in registers.
  *   Decisions are those of the exact pipeline, so the counts are bit-exact.
@@ -33,6 +39,7 @@
 #define GRS_WPB 4      /* waves per block of k_fast */
 #define GRS_QCAP 128   /* LDS ring slots per wave */
 #define GRS_GWT 8      /* doubles per hkl row in LDS */
+#define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
 
 __device__ __forceinline__ void wave_sync() {
@@ -297,33 +304,61 @@ __device__ __forceinline__ double fast_rsq(double x) {
   return y * __builtin_fma(-0.5 * x, y * y, 1.5);
 }
 
-/* Conservative position filter (phase B1).  The spot position of the event
- * is recomputed with the same formulas in exact arithmetic identities --
- * sin(theta) = -cos(beta), cos(theta) = sqrt(1 - cos^2 beta),
- * sin(phix) = sqrt(1 - cos^2 phix), sin(phix - 2 theta) by the difference
- * formula -- and refined hardware reciprocals, which agree with the exact
- * pipeline to about 1e-12 relative.  Guards send ill-conditioned events
- * (|sin(phiy)| < 1e-3, |t denominator| < 1e-3, G nearly along the beam) to
- * the exact path; otherwise the position error is far below the 1-pixel
- * margin, so an event is dropped only when its exact pixel is certainly off
- * the detector or inside the beamstop.  NaNs keep the event. */
-__device__ __forceinline__ bool b1_keep(const grs_dgeo *g, const double *v, double gw0, double gw1, double gw2) {
+/* Phase B1: resolve an event without the exact pipeline where that is safe.
+ *
+ * The spot position is recomputed from the same quantities through exact
+ * identities -- sin(theta) = -cos(beta), cos(theta) = sqrt(1 - cos^2 beta),
+ * lambda = -4 pi cos(beta) / |G|, sin(phix) = sqrt(1 - cos^2 phix),
+ * sin(phix - 2 theta) by the difference formula -- with refined hardware
+ * reciprocals (relative error about 1e-16).  Guards send ill-conditioned
+ * events to the exact pipeline: |sin(phiy)| < 1e-3, |t denominator| < 1e-3,
+ * |cos(phix)| > 0.99, G nearly along the beam.  Under the guards both this
+ * value and the exact pipeline's are within about 1e-8 pixel of the true spot
+ * position, so when the unrounded coordinate is at least GRS_RES_TAU (1e-4
+ * pixel) from every half-integer the two round to the same pixel; the window
+ * tests are decided here only when cos(beta) and lambda clear their bounds by
+ * the phase-A margins.  Everything else (and every NaN) returns 2 and goes
+ * through the exact pipeline, so the decisions are those of forward_comp.m.
+ * Returns 0 (not selected), 1 (selected, pixel in *dy, *dz) or 2 (exact). */
+#define GRS_RES_TAU 1e-4
+
+struct b1_out {
+  double ry, rz;
+  int32_t dy, dz;
+};
+
+__device__ __forceinline__ bool res_round(double r, double *k) {
+  const double f = r + 0.5;
+  const double fl = __builtin_floor(f);
+  const double fr = f - fl;
+  *k = fl;
+  return fr >= GRS_RES_TAU && fr <= 1.0 - GRS_RES_TAU;
+}
+
+__device__ __forceinline__ int b1_classify(const grs_dgeo *g, const double *v, double gw0, double gw1, double gw2,
+                                           b1_out *o) {
   const double c = v[0], s = v[1];
   const double G0 = c * gw0 - s * gw1, G1 = s * gw0 + c * gw1, G2 = gw2;
   const double g2 = __builtin_fma(G0, G0, __builtin_fma(G1, G1, G2 * G2));
-  const double cb = __builtin_fma(v[2], G0, __builtin_fma(v[3], G1, v[4] * G2)) * fast_rsq(g2);
+  const double rg = fast_rsq(g2);
+  const double cb = __builtin_fma(v[2], G0, __builtin_fma(v[3], G1, v[4] * G2)) * rg;
+  const double lam = (-4.0 * GRS_PI) * cb * rg;
+  const double lm = 1e-9 * g->lambda_max;
+  if (!(cb < -1e-12 && cb > g->cos_beta_max + 1e-12 && lam > g->lambda_min + lm && lam < g->lambda_max - lm))
+    return 2;
   const double omc = __builtin_fma(-cb, cb, 1.0);
   const double cth = omc * fast_rsq(omc);
   const double s2t = -2.0 * cb * cth;
   const double c2t = __builtin_fma(-2.0 * cb, cb, 1.0);
   const double nv2 = __builtin_fma(G1, G1, G2 * G2);
+  if (!(nv2 >= 1e-12 * g2)) return 2;
   const double inv = fast_rsq(nv2);
   const double ap = __builtin_fma(G1, v[7], G2 * v[8]) * inv;
-  double oma = __builtin_fma(-ap, ap, 1.0);
-  oma = oma > 0.0 ? oma : 0.0;
-  const double sph = oma > 0.0 ? oma * fast_rsq(oma) : 0.0;
+  if (!(__builtin_fabs(ap) <= 0.99)) return 2;
+  const double oma = __builtin_fma(-ap, ap, 1.0);
+  const double sph = oma * fast_rsq(oma);
   const double sphy = __builtin_fma(sph, c2t, -ap * s2t);
-  if (!(__builtin_fabs(sphy) >= 1e-3) || !(nv2 >= 1e-12 * g2)) return true;
+  if (!(__builtin_fabs(sphy) >= 1e-3)) return 2;
   const double diffvec = v[9] * s2t * fast_rcp(sphy);
   const double y22 = __builtin_fma(diffvec * G1, inv, v[5]);
   const double z22 = __builtin_fma(diffvec * G2, inv, v[6]);
@@ -332,29 +367,36 @@ __device__ __forceinline__ bool b1_keep(const grs_dgeo *g, const double *v, doub
   const double K0 = k0 * ik, K1 = k1 * ik, K2 = k2 * ik;
   const double *R = g->R;
   const double tden = __builtin_fma(R[0], K0, __builtin_fma(R[3], K1, R[6] * K2));
-  if (!(__builtin_fabs(tden) >= 1e-3)) return true;
+  if (!(__builtin_fabs(tden) >= 1e-3)) return 2;
   const double t = v[13] * fast_rcp(tden);
   const double w0 = __builtin_fma(t, K0, -k0), w1 = __builtin_fma(t, K1, v[14]), w2 = __builtin_fma(t, K2, v[15]);
   const double Y = __builtin_fma(R[1], w0, __builtin_fma(R[4], w1, R[7] * w2));
   const double Z = __builtin_fma(R[2], w0, __builtin_fma(R[5], w1, R[8] * w2));
   const double ry = __builtin_fma(-Y, g->ipy, g->dety0);
   const double rz = __builtin_fma(-Z, g->ipz, g->detz0);
-  const double m = 1.0; /* pixels */
-  const bool off = ry < 0.5 - m || ry > g->detysize + 0.5 + m || rz < 0.5 - m || rz > g->detzsize + 0.5 + m;
-  const bool in_bs = ry >= g->bsy0 - 0.5 + m && ry <= g->bsy1 + 0.5 - m && rz >= g->bsz0 - 0.5 + m &&
-                     rz <= g->bsz1 + 0.5 - m;
-  return !(off || in_bs);
+  o->ry = ry;
+  o->rz = rz;
+  double ky, kz;
+  if (!res_round(ry, &ky) || !res_round(rz, &kz)) return 2;
+  /* ky, kz are the rounded pixel: floor(r + 0.5) equals round-half-away for
+   * every r that is not a half-integer, and negative values are off anyway */
+  if (!(ky >= 1.0 && ky <= g->detysize && kz >= 1.0 && kz <= g->detzsize)) return 0;
+  if (ky >= g->bsy0 && ky <= g->bsy1 && kz >= g->bsz0 && kz <= g->bsz1) return 0;
+  o->dy = (int32_t)ky;
+  o->dz = (int32_t)kz;
+  return 1;
 }
 
-template <bool MEDIAN, int MAP>
+template <bool MEDIAN, int MAP, bool AUDIT>
 __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32_t H) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
-  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_QCAP; /* two rings of GRS_QCAP u32 */
+  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_RING_DOUBLES;
   double *gwt = smem + wib * per_wave;
-  uint32_t *ring1 = (uint32_t *)(gwt + (int64_t)H * GRS_GWT);
-  uint32_t *ring2 = ring1 + GRS_QCAP;
+  uint32_t *ring1 = (uint32_t *)(gwt + (int64_t)H * GRS_GWT); /* phase-A survivors (omega << 8 | hkl) */
+  uint32_t *ring2 = ring1 + GRS_QCAP;                          /* events for the exact pipeline */
+  uint32_t *ring3 = ring2 + GRS_QCAP;                          /* selected events: stack index */
   const grs_dgeo *g = a.g;
   const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
   const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
@@ -405,9 +447,9 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
     wave_sync();
 
     const double *vwv = a.vw + vi * (int64_t)nrot * GRS_VW_DOUBLES;
-    uint32_t head1 = 0, tail1 = 0, head2 = 0, tail2 = 0;
+    uint32_t head1 = 0, tail1 = 0, head2 = 0, tail2 = 0, head3 = 0, tail3 = 0;
     int32_t ns = 0, ni = 0;
-    /* the gather of the previous exact batch is consumed one batch later */
+    /* the gather of the previous batch is consumed one batch later */
     bool p_sel = false, p_hit = false;
     uint32_t p_key = 0;
 
@@ -423,9 +465,33 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
       p_hit = false;
     };
 
+    /* gather: distance key (or hit bit) of n selected events of ring3 */
+    auto gather = [&](uint32_t n) {
+      consume();
+      if ((uint32_t)lane < n) {
+        uint32_t idx = ring3[(head3 + lane) & (GRS_QCAP - 1)];
+        p_sel = true;
+        if (MEDIAN) p_key = a.keys[idx];
+        else p_hit = (a.hit_bits[idx >> 5] >> (idx & 31u)) & 1u;
+      }
+      head3 += n;
+    };
+
+    auto push3 = [&](bool sel, uint32_t idx) {
+      uint64_t m = __ballot(sel);
+      if (sel) ring3[(tail3 + lane_prefix(m)) & (GRS_QCAP - 1)] = idx;
+      tail3 += (uint32_t)__popcll(m);
+      if (tail3 - head3 >= 64) {
+        wave_sync();
+        gather(64);
+        wave_sync();
+      }
+    };
+
     /* phase B2: exact pipeline on n events of ring2 */
     auto b2 = [&](uint32_t n) {
-      consume();
+      bool sel = false;
+      uint32_t idx = 0;
       if ((uint32_t)lane < n) {
         uint32_t ev = ring2[(head2 + lane) & (GRS_QCAP - 1)];
         int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
@@ -440,30 +506,62 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
         double Glen;
         double beta = grs_event_beta(&v, Gt, &Glen);
         int32_t dy = 0, dz = 0;
-        p_sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
-        if (p_sel) {
-          int64_t idx = (int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny;
-          if (MEDIAN) p_key = a.keys[idx];
-          else p_hit = (a.hit_bits[idx >> 5] >> (uint32_t)(idx & 31)) & 1u;
-        }
+        sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
+        if (sel) idx = (uint32_t)((int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny);
       }
       head2 += n;
+      push3(sel, idx);
     };
 
-    /* phase B1: conservative position filter on n events of ring1 */
+    /* phase B1: resolve n events of ring1 or queue them for the exact pipeline */
     auto b1 = [&](uint32_t n) {
-      bool keep = false;
-      uint32_t ev = 0;
+      int st = 0;
+      uint32_t ev = 0, idx = 0;
+      bool bad = false;
+      double dev = 0.0;
       if ((uint32_t)lane < n) {
         ev = ring1[(head1 + lane) & (GRS_QCAP - 1)];
         int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
         const double *t = gwt + h * GRS_GWT;
-        keep = b1_keep(g, vwv + (int64_t)om * GRS_VW_DOUBLES, t[0], t[1], t[2]);
+        b1_out o;
+        o.ry = o.rz = 0.0;
+        o.dy = o.dz = 0;
+        st = b1_classify(g, vwv + (int64_t)om * GRS_VW_DOUBLES, t[0], t[1], t[2], &o);
+        if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
+        if (AUDIT) {
+          grs_vw v;
+          const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
+          double *dst = (double *)&v;
+#pragma unroll
+          for (int k = 0; k < GRS_VW_DOUBLES; k++) dst[k] = src[k];
+          double Gt[3], Glen, yz[2] = {0.0, 0.0};
+          grs_Gt(v.c, v.s, t[0], t[1], t[2], Gt);
+          double beta = grs_event_beta(&v, Gt, &Glen);
+          int32_t edy = 0, edz = 0;
+          int es = grs_event_finish_yz(g, &v, Gt, Glen, beta, &edy, &edz, yz);
+          bad = (st == 0 && es) || (st == 1 && (!es || edy != o.dy || edz != o.dz));
+          if (st == 1 && es) dev = __builtin_fmax(__builtin_fabs(o.ry - yz[0]), __builtin_fabs(o.rz - yz[1]));
+        }
+      }
+      if (AUDIT) {
+        /* whole-wave reductions, outside the divergent block */
+        uint64_t mb = __ballot(bad), m1 = __ballot(st == 1), m0 = __ballot((uint32_t)lane < n && st == 0);
+        uint64_t m2 = __ballot(st == 2);
+        for (int off = 32; off >= 1; off >>= 1) dev = __builtin_fmax(dev, __shfl_xor(dev, off, 64));
+        if (lane == 0) {
+          atomicAdd(a.audit + 0, (unsigned long long)n);
+          atomicAdd(a.audit + 1, (unsigned long long)__popcll(m1));
+          atomicAdd(a.audit + 2, (unsigned long long)__popcll(m0));
+          atomicAdd(a.audit + 3, (unsigned long long)__popcll(m2));
+          atomicAdd(a.audit + 4, (unsigned long long)__popcll(mb));
+          atomicMax(a.audit + 5, (unsigned long long)__double_as_longlong(dev));
+        }
       }
       head1 += n;
-      uint64_t m = __ballot(keep);
-      if (keep) ring2[(tail2 + lane_prefix(m)) & (GRS_QCAP - 1)] = ev;
-      tail2 += (uint32_t)__popcll(m);
+      uint64_t m2 = __ballot(st == 2);
+      if (st == 2) ring2[(tail2 + lane_prefix(m2)) & (GRS_QCAP - 1)] = ev;
+      tail2 += (uint32_t)__popcll(m2);
+      push3(st == 1, idx);
       if (tail2 - head2 >= 64) {
         wave_sync();
         b2(64);
@@ -513,6 +611,13 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
       if (n > 64) n = 64;
       wave_sync();
       b2(n);
+      wave_sync();
+    }
+    while (tail3 != head3) {
+      uint32_t n = tail3 - head3;
+      if (n > 64) n = 64;
+      wave_sync();
+      gather(n);
       wave_sync();
     }
     consume();
@@ -708,17 +813,17 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
 }
 
 static size_t fast_lds(int32_t H) {
-  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT * sizeof(double) + 2 * GRS_QCAP * sizeof(uint32_t));
+  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES) * sizeof(double);
 }
 
 /* Resident blocks per CU of one k_fast instantiation (cached). */
-template <bool MEDIAN, int MAP>
+template <bool MEDIAN, int MAP, bool AUDIT>
 static int fast_blocks_per_cu(size_t lds) {
   static int cached = 0;
   static size_t cached_lds = 0;
   if (cached == 0 || cached_lds != lds) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<MEDIAN, MAP>, 64 * GRS_WPB, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<MEDIAN, MAP, AUDIT>, 64 * GRS_WPB, lds) != hipSuccess ||
         nb <= 0)
       nb = 1;
     cached = nb;
@@ -736,14 +841,14 @@ static int device_cus() {
 }
 
 /* Persistent launch: at most as many waves as can be resident at once. */
-template <bool MEDIAN, int MAP>
+template <bool MEDIAN, int MAP, bool AUDIT = false>
 static hipError_t launch_fast_impl(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
   size_t lds = fast_lds(H);
-  int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP>(lds);
+  int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP, AUDIT>(lds);
   int64_t blocks = (n_waves + GRS_WPB - 1) / GRS_WPB;
   if (blocks > resident) blocks = resident;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_fast<MEDIAN, MAP>), dim3((unsigned)blocks), dim3(64 * GRS_WPB), lds, st, *a, H);
+  hipLaunchKernelGGL((k_fast<MEDIAN, MAP, AUDIT>), dim3((unsigned)blocks), dim3(64 * GRS_WPB), lds, st, *a, H);
   return hipGetLastError();
 }
 
@@ -762,6 +867,14 @@ hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_w
   if (a->n_eval <= 0) return hipSuccess;
   if (a->map != GRS_MAP_CSR) return hipErrorInvalidValue;
   return launch_fast_impl<false, GRS_MAP_CSR>(a, H, n_waves, st);
+}
+
+/* One-voxel evaluation that also runs the exact pipeline on every event of
+ * phase B1 and counts disagreements (a->audit). */
+hipError_t grs_launch_fast_audit(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
+  if (a->n_eval <= 0) return hipSuccess;
+  if (a->map != GRS_MAP_ONE_VOXEL || !a->audit) return hipErrorInvalidValue;
+  return launch_fast_impl<true, GRS_MAP_ONE_VOXEL, true>(a, H, n_waves, st);
 }
 
 hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,

@@ -222,6 +222,20 @@ class Engine:
         self._chk(self._lib.grs_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0))
         return ms.value, n.value
 
+    def set_audit(self, on: bool = True) -> None:
+        """Arms (and clears) the audit of the fast kernel's resolution stage."""
+        self._chk(self._lib.grs_set_audit(self._h, 1 if on else 0))
+
+    def audit(self) -> dict:
+        """Audit counters since set_audit(True): events examined by the
+        resolution stage, resolved selected / not selected, sent to the exact
+        pipeline, disagreements, and the largest coordinate difference (px)."""
+        counts = (ctypes.c_int64 * 5)()
+        dev = ctypes.c_double()
+        self._chk(self._lib.grs_get_audit(self._h, counts, ctypes.byref(dev)))
+        return {"examined": counts[0], "resolved_selected": counts[1], "resolved_rejected": counts[2],
+                "exact": counts[3], "mismatched": counts[4], "max_dev_px": dev.value}
+
     def math_device(self, fn: int, x) -> np.ndarray:
         x = _d(np.asarray(x).reshape(-1))
         y = np.zeros_like(x)

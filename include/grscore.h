@@ -255,6 +255,17 @@ grs_status grs_set_timing(grs_ctx *ctx, int32_t enable);
 /* Accumulated scoring-kernel time (ms) and launch count since the last reset;
  * reset != 0 clears them after reading. */
 grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, int32_t reset);
+/* Audit of the fast kernel's resolution stage (one-voxel calls only, i.e.
+ * grs_forward_comp / grs_forward_comp_device with the fast method): every
+ * event that the kernel decides without the exact forward_comp.m pipeline is
+ * also run through that pipeline and compared.  enable != 0 clears and arms
+ * the counters.  No reference counterpart (a diagnostic of this engine). */
+grs_status grs_set_audit(grs_ctx *ctx, int32_t enable);
+/* counts[5]: events examined, resolved as selected, resolved as not selected,
+ * sent to the exact pipeline, disagreements with it; max_dev: the largest
+ * difference (pixels) between the resolved and the exact unrounded spot
+ * coordinates of the selected events. */
+grs_status grs_get_audit(grs_ctx *ctx, int64_t *counts, double *max_dev);
 /* Evaluates the engine's FP64 math on the device: fn 0 sin, 1 cos, 2 acos,
  * 3 atan, 4 round (half away from zero), 5 sqrt, 6 reciprocal.  Host arrays. */
 grs_status grs_math_device(grs_ctx *ctx, int32_t fn, const double *x, double *y, int64_t n);

@@ -290,3 +290,30 @@ def test_compete_csr_matches_oracle_argmax(engine):
         assert np.float32(ref[j, 3]).view(np.uint32) == med[r].view(np.uint32)
         assert np.array_equal(bq[r, 1:], qs[j, 1:])
     assert (C[rows[1:] > rows[:-1]] > 0).any()
+
+
+@pytest.mark.parametrize("variant", ["plain", "tilted_offsets"])
+def test_resolution_stage_agrees_with_exact_pipeline(engine, variant):
+    """Every event the fast kernel decides without the exact pipeline is re-run
+    through it on the device: no disagreement, coordinates far inside the
+    1e-4 pixel resolution margin, and only a small share sent to the exact path."""
+    kw = {} if variant == "plain" else dict(tilts=(0.25, -0.15, 0.3), dety00=0.03, detz00=-0.02)
+    s = scenes.make_scene(seed=61, ngrains=3, spread_mm=0.05, **kw)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(7)
+    q = np.vstack([s.q, scenes.random_quats(rng, 4000)])
+    ref = engine.forward_comp(s.voxels[0][13], q)
+    engine.set_audit(True)
+    try:
+        out = engine.forward_comp(s.voxels[0][13], q)
+        au = engine.audit()
+    finally:
+        engine.set_audit(False)
+    for x, y in zip(out, ref):
+        np.testing.assert_array_equal(x, y)
+    assert au["examined"] == au["resolved_selected"] + au["resolved_rejected"] + au["exact"]
+    assert au["resolved_selected"] > 10000
+    assert au["mismatched"] == 0
+    assert au["max_dev_px"] < 1e-6
+    assert au["exact"] <= 0.01 * au["examined"]
