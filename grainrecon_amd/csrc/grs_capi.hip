@@ -41,6 +41,7 @@ struct grs_ctx {
   DevBuf d_rows, d_p1, d_U, d_eu, d_rowstart, d_best, d_evox, d_eu_idx, d_bad, d_x, d_y;
   DevBuf d_mask, d_g1, d_edt_v, d_edt_z, d_grain;
   DevBuf d_bits, d_rowptr, d_roweoff, d_candq, d_rowbest, d_elist;
+  DevBuf c_cen, c_grid, c_gn, c_out, c_n, c_gid, c_vox, c_ids, c_nout;
   bool bits_valid = false; /* hit mask matches the current stack and threshold */
   /* resident matched-spot set of the fits */
   DevBuf f_hkl, f_om, f_meas, f_grain, f_goff, f_gr6, f_up, f_res, f_contrib, f_jac, f_out;
@@ -1063,6 +1064,138 @@ grs_status grs_cuda_forward_comp_compete(const float *vox, int64_t n_vox, const 
                                GRS_METHOD_FAST);
   if (s != GRS_OK) return legacy_ret(c, s);
   for (int64_t k = 0; k < 8 * M; k++) out[k] = (float)o[k];
+  return GRS_OK;
+}
+
+/* ---- candidate generation ------------------------------------------------- */
+
+/* MATLAB a:d:b, following MathWorks' published description of the colon
+ * operator: n = round((b-a)/d) intervals, one fewer when a+n*d overshoots b
+ * by more than tol = 2*eps*max(|a|,|b|); the end point snapped to b within
+ * tol; values a+k*d from the left, last-k*d from the right, (a+last)/2 in the
+ * middle of an even count.  Returns the count, -1 if above cap or not finite. */
+static int64_t matlab_colon(double a, double d, double b, double *v, int64_t cap) {
+  if (!std::isfinite(a) || !std::isfinite(d) || !std::isfinite(b)) return -1;
+  if (d == 0.0 || (a < b && d < 0.0) || (b < a && d > 0.0)) return 0;
+  const double tol = 2.0 * 2.220446049250313e-16 * std::max(std::fabs(a), std::fabs(b));
+  const double sig = d > 0.0 ? 1.0 : -1.0;
+  double n;
+  if (a == std::floor(a) && d == 1.0) {
+    n = std::floor(b) - a;
+  } else if (a == std::floor(a) && d == std::floor(d)) {
+    const double q = std::floor(a / d);
+    const double r = a - q * d;
+    n = std::floor((b - r) / d) - q;
+  } else {
+    n = std::round((b - a) / d);
+    if (sig * ((a + n * d) - b) > tol) n = n - 1.0;
+  }
+  if (!(n + 1.0 <= (double)cap)) return -1;
+  double last = a + n * d;
+  if (sig * (last - b) > -tol) last = b;
+  const int64_t N = (int64_t)n;
+  for (int64_t k = 0; k <= N / 2; k++) v[k] = a + (double)k * d;
+  for (int64_t k = 0; k <= N / 2; k++) v[N - k] = last - (double)k * d;
+  if (N % 2 == 0) v[N / 2] = (a + last) / 2.0;
+  return N + 1;
+}
+
+grs_status grs_local_sampling(grs_ctx *c, const double *centres, int64_t K, int32_t nr, double gridsize,
+                              double *out_q, int64_t cap, int64_t *out_n) {
+  if (!c) return GRS_ERR_ARG;
+  if (K < 0 || nr < 0 || cap < 1 || (K > 0 && (!centres || !out_q || !out_n)) || !std::isfinite(gridsize))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (K == 0) return GRS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> grids((size_t)K * 3 * GRS_LS_MAX_GRID, 0.0);
+  std::vector<int32_t> gn((size_t)K * 3);
+  const double half = (gridsize * (double)nr) / 2.0;
+  for (int64_t k = 0; k < K; k++) {
+    int64_t rows = 1;
+    for (int j = 0; j < 3; j++) {
+      const double cj = centres[4 * k + 1 + j];
+      const int64_t n = matlab_colon(cj - half, gridsize, cj + half, &grids[(size_t)(3 * k + j) * GRS_LS_MAX_GRID],
+                                     GRS_LS_MAX_GRID);
+      if (n < 0) return fail(c, GRS_ERR_ARG, "centre %lld: grid of more than %d values", (long long)k, GRS_LS_MAX_GRID);
+      gn[3 * k + j] = (int32_t)n;
+      rows *= n;
+    }
+    if (1 + rows > GRS_LS_MAX_ROWS)
+      return fail(c, GRS_ERR_ARG, "centre %lld: %lld rows exceed %d", (long long)k, (long long)(1 + rows),
+                  GRS_LS_MAX_ROWS);
+    if (1 + rows > cap)
+      return fail(c, GRS_ERR_ARG, "centre %lld: up to %lld rows, capacity %lld", (long long)k, (long long)(1 + rows),
+                  (long long)cap);
+  }
+  ENSURE(c, c->c_cen, sizeof(double) * 4 * K);
+  ENSURE(c, c->c_grid, sizeof(double) * grids.size());
+  ENSURE(c, c->c_gn, sizeof(int32_t) * gn.size());
+  ENSURE(c, c->c_out, sizeof(double) * 4 * (size_t)K * cap);
+  ENSURE(c, c->c_n, sizeof(int64_t) * K);
+  ENSURE(c, c->d_bad, sizeof(uint32_t));
+  HIPCHK(c, hipMemcpyAsync(c->c_cen.p, centres, sizeof(double) * 4 * K, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->c_grid.p, grids.data(), sizeof(double) * grids.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->c_gn.p, gn.data(), sizeof(int32_t) * gn.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_bad.p, 0, sizeof(uint32_t), c->st));
+  HIPCHK(c, grs_launch_local_sampling(P<double>(c->c_cen), P<double>(c->c_grid), P<int32_t>(c->c_gn), K,
+                                      P<double>(c->c_out), cap, P<int64_t>(c->c_n), P<uint32_t>(c->d_bad), c->st));
+  uint32_t bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(out_n, c->c_n.p, sizeof(int64_t) * K, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(out_q, c->c_out.p, sizeof(double) * 4 * (size_t)K * cap, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (bad) return fail(c, GRS_ERR_ARG, "a grid point has q1^2+q2^2+q3^2 > 1");
+  return GRS_OK;
+}
+
+grs_status grs_find_neighbor_ors(grs_ctx *c, const int32_t *grain_id, int64_t nx, int64_t ny, int64_t nz,
+                                 const int32_t *vox, int64_t M, int32_t search_radius, int32_t nr_max,
+                                 int32_t *id_all, int64_t *ind, int64_t *n_total) {
+  if (!c) return GRS_ERR_ARG;
+  if (nx < 1 || ny < 1 || nz < 1 || M < 0 || search_radius < 0 || nr_max < 1 || !grain_id || !n_total ||
+      (M > 0 && (!vox || !id_all || !ind)))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  const int64_t nvol = nx * ny * nz;
+  int32_t maxid = 0;
+  for (int64_t i = 0; i < nvol; i++) maxid = grain_id[i] > maxid ? grain_id[i] : maxid;
+  if (maxid <= 0) return fail(c, GRS_ERR_ARG, "the grain-id volume holds no positive id");
+  if (maxid >= 32 * 16384) return fail(c, GRS_ERR_ARG, "grain ids above %d are not supported", 32 * 16384 - 1);
+  const int64_t dims[3] = {nx, ny, nz};
+  for (int64_t m = 0; m < M; m++)
+    for (int j = 0; j < 3; j++)
+      if (vox[3 * m + j] < 1 || vox[3 * m + j] > dims[j])
+        return fail(c, GRS_ERR_ARG, "voxel %lld lies outside the volume", (long long)m);
+  *n_total = 0;
+  if (M == 0) return GRS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int32_t W = (maxid >> 5) + 1;
+  ENSURE(c, c->c_gid, sizeof(int32_t) * nvol);
+  ENSURE(c, c->c_vox, sizeof(int32_t) * 3 * M);
+  ENSURE(c, c->c_ids, sizeof(int32_t) * (size_t)M * nr_max);
+  ENSURE(c, c->c_nout, sizeof(int32_t) * M);
+  ENSURE(c, c->d_bad, sizeof(uint32_t));
+  HIPCHK(c, hipMemcpyAsync(c->c_gid.p, grain_id, sizeof(int32_t) * nvol, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->c_vox.p, vox, sizeof(int32_t) * 3 * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_bad.p, 0, sizeof(uint32_t), c->st));
+  HIPCHK(c, grs_launch_find_neighbors(P<int32_t>(c->c_gid), nx, ny, nz, P<int32_t>(c->c_vox), M, search_radius, nr_max,
+                                      W, P<int32_t>(c->c_ids), P<int32_t>(c->c_nout), P<uint32_t>(c->d_bad), c->st));
+  std::vector<int32_t> nout((size_t)M), ids((size_t)M * nr_max);
+  uint32_t bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(nout.data(), c->c_nout.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(ids.data(), c->c_ids.p, sizeof(int32_t) * ids.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (bad) return fail(c, GRS_ERR_STATE, "a search box grew to the whole volume without a positive id");
+  /* pack one voxel after the other, 1-based [first last] (find_neighbor_ORs.m:26-29) */
+  int64_t count = 0;
+  for (int64_t m = 0; m < M; m++) {
+    const int32_t n = nout[m];
+    for (int32_t k = 0; k < n; k++) id_all[count + k] = ids[(size_t)m * nr_max + k];
+    count += n;
+    ind[2 * m] = count - n + 1;
+    ind[2 * m + 1] = count;
+  }
+  *n_total = count;
   return GRS_OK;
 }
 

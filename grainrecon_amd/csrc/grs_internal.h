@@ -96,6 +96,15 @@ hipError_t grs_launch_math_eval(int fn, const double *d_x, double *d_y, int64_t 
 hipError_t grs_launch_copy_u32(const uint32_t *d_src, uint32_t *d_dst, int64_t n, hipStream_t st);
 hipError_t grs_launch_check_keys(const uint32_t *d_keys, int64_t n, uint32_t *d_bad, hipStream_t st);
 
+/* candidate generation */
+#define GRS_LS_MAX_ROWS 2048 /* centre + grid rows of one local sampling */
+#define GRS_LS_MAX_GRID 64   /* values per (q1, q2, q3) grid */
+hipError_t grs_launch_local_sampling(const double *d_cen, const double *d_grids, const int32_t *d_gn, int64_t K,
+                                     double *d_out, int64_t cap, int64_t *d_n, uint32_t *d_bad, hipStream_t st);
+hipError_t grs_launch_find_neighbors(const int32_t *d_gid, int64_t nx, int64_t ny, int64_t nz, const int32_t *d_vox,
+                                     int64_t M, int32_t radius, int32_t nr_max, int32_t W, int32_t *d_ids,
+                                     int32_t *d_nout, uint32_t *d_bad, hipStream_t st);
+
 hipError_t grs_launch_render(const grs_dgeo *d_g, const double *d_pos, const int32_t *d_grain, int64_t M,
                              int32_t nrot, const double *d_SUB, const double *d_hkl, const double *d_cs,
                              uint8_t *d_mask, hipStream_t st);

@@ -158,6 +158,44 @@ class Engine:
                                             _ptr(med, _fp)))
         return best, bq, ns, ni, C, med
 
+    # -- candidate generation ---------------------------------------------------
+    def local_sampling(self, centres, nr: int, gridsize: float = 0.004) -> list:
+        """ori_local_sampling.m:12-34 for each centre row [q0 q1 q2 q3] (a
+        3-vector gets q0 = sqrt(1-(q1^2+q2^2+q3^2)), :12-14).  Returns one
+        (n, 4) array of sorted unique quaternion rows (OR_local.q) per centre."""
+        c = np.asarray(centres, dtype=float)
+        c = c.reshape(1, -1) if c.ndim == 1 else c
+        if c.shape[1] == 3:
+            q0 = np.sqrt(1.0 - ((c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1]) + c[:, 2] * c[:, 2]))
+            c = np.column_stack([q0, c])
+        c = _d(c.reshape(-1, 4))
+        K = c.shape[0]
+        cap = 1 + (int(nr) + 3) ** 3
+        out = np.zeros((K, cap, 4))
+        n = np.zeros(K, np.int64)
+        self._chk(self._lib.grs_local_sampling(self._h, _ptr(c, _dp), K, int(nr), float(gridsize), _ptr(out, _dp), cap,
+                                               n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return [out[k, :n[k]].copy() for k in range(K)]
+
+    def find_neighbor_ors(self, grain_id: np.ndarray, vox, search_radius: int = 20, nr_max: int = 30):
+        """find_neighbor_ORs.m:3-96.  grain_id: (nx, ny, nz) indexed like
+        DS_in.GrainId(x, y, z); vox: (M, >=3) 1-based [x y z ...].  Returns
+        (id_neigb_all (K,), id_neigb_ind (M, 2) 1-based [first last])."""
+        g = np.asarray(grain_id)
+        nx, ny, nz = g.shape
+        gf = np.ascontiguousarray(g.reshape(-1, order="F"), dtype=np.int32)
+        v = np.ascontiguousarray(np.asarray(vox)[:, :3], dtype=np.int32)
+        M = v.shape[0]
+        ids = np.zeros(max(M * nr_max, 1), np.int32)
+        ind = np.zeros((M, 2), np.int64)
+        n = ctypes.c_int64()
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        i64 = ctypes.POINTER(ctypes.c_int64)
+        self._chk(self._lib.grs_find_neighbor_ors(self._h, gf.ctypes.data_as(i32), nx, ny, nz, v.ctypes.data_as(i32), M,
+                                                  int(search_radius), int(nr_max), ids.ctypes.data_as(i32),
+                                                  ind.ctypes.data_as(i64), ctypes.byref(n)))
+        return ids[:n.value].copy(), ind
+
     # -- input preparation ------------------------------------------------------
     def render_spots(self, pos, grain, U, clear: bool = True) -> None:
         """Forward-simulates spot masks: voxel i at pos[i] (mm) with orientation U[grain[i]]."""

@@ -248,6 +248,33 @@ grs_status grs_copy_to_host(grs_ctx *ctx, void *dst, const void *d_src, int64_t 
  * memory-traffic counters against a transfer of known size. */
 grs_status grs_device_copy(grs_ctx *ctx, void *d_dst, const void *d_src, int64_t bytes);
 
+/* ---- candidate generation ------------------------------------------------ */
+
+/* ori_local_sampling.m:12-34 for K centres [q0 q1 q2 q3] (used as given,
+ * :16): the grids c - gridsize*nr/2 : gridsize : c + gridsize*nr/2 of q1, q2,
+ * q3 (MATLAB colon operator), the centre row followed by the rows
+ * [sqrt(1-(b^2+c^2+d^2)) b c d], sorted and de-duplicated like
+ * unique(q,'rows').  Centre k's rows go to out_q[4*(k*cap + r) ..], their
+ * number to out_n[k].  At most 2048 rows per centre before de-duplication;
+ * cap must hold them.  Replaces the host loops of ori_local_sampling.m
+ * (called from fit_OR_constrain_with_local_comp_cuda.m:20 and
+ * GrainMapping_engine_gpu_comp.m:87). */
+grs_status grs_local_sampling(grs_ctx *ctx, const double *centres, int64_t K, int32_t nr, double gridsize,
+                              double *out_q, int64_t cap, int64_t *out_n);
+
+/* find_neighbor_ORs.m:3-96: grain_id is the nx x ny x nz GrainId volume
+ * (x fastest, DS_in.GrainId(x,y,z)); vox holds M rows of 1-based [x y z].
+ * For each voxel: the positive ids in the box of half-width search_radius
+ * clipped to the volume (grown by 2 voxels per try while empty), unique in
+ * ascending order, at most nr_max (the reference's 30).  id_all (capacity
+ * M*nr_max) receives them voxel after voxel, ind M rows of 1-based
+ * [first last] positions, n_total their count.  Grain ids must be below
+ * 524288.  Replaces the host search of GrainGrow_engine_compete_comp.m:47 and
+ * revise_all_unindexed_voxel_cuda.m:88. */
+grs_status grs_find_neighbor_ors(grs_ctx *ctx, const int32_t *grain_id, int64_t nx, int64_t ny, int64_t nz,
+                                 const int32_t *vox, int64_t M, int32_t search_radius, int32_t nr_max,
+                                 int32_t *id_all, int64_t *ind, int64_t *n_total);
+
 /* ---- measurement and diagnostics ---------------------------------------- */
 
 /* Enables HIP-event timing of the scoring kernel launches on ctx's stream. */

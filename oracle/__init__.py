@@ -90,6 +90,14 @@ def lib():
         L.oracle_set_libm.argtypes = [ctypes.c_int]
         L.oracle_euler2u_deg.restype = None
         L.oracle_euler2u_deg.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.oracle_colon.restype = _i64
+        L.oracle_colon.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _i64]
+        L.oracle_local_sampling.restype = _i64
+        L.oracle_local_sampling.argtypes = [_dp, ctypes.c_int32, ctypes.c_double, _dp, _i64]
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        L.oracle_find_neighbor_ors.restype = _i64
+        L.oracle_find_neighbor_ors.argtypes = [_i32p, _i64, _i64, _i64, _i32p, _i64, ctypes.c_int32, ctypes.c_int32,
+                                               _i32p, ctypes.POINTER(_i64)]
         _lib = L
     return _lib
 
@@ -204,3 +212,46 @@ def euler2u_deg(p1, P, p2) -> np.ndarray:
     U = np.zeros(9)
     lib().oracle_euler2u_deg(float(p1), float(P), float(p2), U.ctypes.data_as(_dp))
     return U.reshape(3, 3)
+
+
+def colon(a: float, d: float, b: float) -> np.ndarray:
+    """MATLAB a:d:b (oracle_colon)."""
+    v = np.zeros(4096)
+    n = lib().oracle_colon(float(a), float(d), float(b), v.ctypes.data_as(_dp), v.size)
+    if n < 0:
+        raise ValueError("colon: too many elements or non-finite argument")
+    return v[:n].copy()
+
+
+def local_sampling(centre, nr: int, gridsize: float) -> np.ndarray:
+    """ori_local_sampling.m:12-34 -> OR_local.q (sorted unique rows)."""
+    c = np.asarray(centre, dtype=float).reshape(-1)
+    if c.size == 3:
+        c = np.array([np.sqrt(1.0 - ((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2])), *c])
+    c = _d(c)
+    cap = 1 + (int(nr) + 3) ** 3
+    out = np.zeros((cap, 4))
+    n = lib().oracle_local_sampling(c.ctypes.data_as(_dp), int(nr), float(gridsize), out.ctypes.data_as(_dp), cap)
+    if n < 0:
+        raise ValueError("local_sampling failed")
+    return out[:n].copy()
+
+
+def find_neighbor_ors(grain_id: np.ndarray, vox, search_radius: int, nr_max: int = 30):
+    """find_neighbor_ORs.m:3-96.  grain_id: (nx, ny, nz) array indexed like
+    DS_in.GrainId(x, y, z); vox: (M, 3) 1-based [x y z].  Returns
+    (id_neigb_all (K,), id_neigb_ind (M, 2) 1-based [first last])."""
+    g = np.asarray(grain_id)
+    nx, ny, nz = g.shape
+    gf = np.ascontiguousarray(g.reshape(-1, order="F"), dtype=np.int32)
+    v = np.ascontiguousarray(np.asarray(vox).reshape(-1, 3), dtype=np.int32)
+    M = v.shape[0]
+    ids = np.zeros(max(M * nr_max, 1), np.int32)
+    ind = np.zeros((M, 2), np.int64)
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    n = lib().oracle_find_neighbor_ors(gf.ctypes.data_as(i32), nx, ny, nz, v.ctypes.data_as(i32), M,
+                                       int(search_radius), int(nr_max), ids.ctypes.data_as(i32),
+                                       ind.ctypes.data_as(ctypes.POINTER(_i64)))
+    if n < 0:
+        raise ValueError("the grain-id volume holds no positive id")
+    return ids[:n].copy(), ind
