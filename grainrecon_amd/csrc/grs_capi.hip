@@ -17,6 +17,7 @@
 
 #include "../../include/grscore.h"
 #include "grs_internal.h"
+#include "grs_nm.h"
 
 namespace {
 
@@ -1064,6 +1065,99 @@ grs_status grs_cuda_forward_comp_compete(const float *vox, int64_t n_vox, const 
                                GRS_METHOD_FAST);
   if (s != GRS_OK) return legacy_ret(c, s);
   for (int64_t k = 0; k < 8 * M; k++) out[k] = (float)o[k];
+  return GRS_OK;
+}
+
+/* ---- orientation fit --------------------------------------------------------- */
+
+grs_status grs_fit_or_local(grs_ctx *c, const double *pos, const double *x0, int64_t M, double half_width,
+                            int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
+                            double *fval, int32_t *n_fun_evals, int32_t *n_iter) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || (M > 0 && (!pos || !x0 || !x_out || !fval || !n_fun_evals || !n_iter)) || !(half_width > 0.0) ||
+      max_fun_evals < 1 || max_iter < 1 || !(tol_fun >= 0.0) || !(tol_x >= 0.0))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (M == 0) return GRS_OK;
+  if (M > INT32_MAX / 4) return fail(c, GRS_ERR_ARG, "too many fits");
+  std::vector<grs_nm> nm((size_t)M);
+  for (int64_t m = 0; m < M; m++) {
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+      /* fit_OR_constrain_with_local_comp_cuda.m:124-125 */
+      const double a = x0[3 * m + k] - half_width, b = x0[3 * m + k] + half_width;
+      lo[k] = grs_nm_max(a, -1.0);
+      hi[k] = (b != b) ? 1.0 : (b < 1.0 ? b : 1.0);
+    }
+    nm[m].max_fe = max_fun_evals;
+    nm[m].max_it = max_iter;
+    nm[m].tol_f = tol_fun;
+    nm[m].tol_x = tol_x;
+    nm[m].start(3, x0 + 3 * m, lo, hi);
+  }
+  ENSURE(c, c->d_pos, sizeof(double) * 3 * M);
+  HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * M, hipMemcpyHostToDevice, c->st));
+  std::vector<double> q;
+  std::vector<int32_t> evox, eu, hns, hni;
+  std::vector<int64_t> first;
+  std::vector<double> fvals;
+  for (;;) {
+    q.clear();
+    evox.clear();
+    first.assign((size_t)M + 1, 0);
+    for (int64_t m = 0; m < M; m++) {
+      first[m] = (int64_t)evox.size();
+      if (nm[m].done()) continue;
+      for (int p = 0; p < nm[m].npend; p++) {
+        double x[3];
+        nm[m].xform(nm[m].pend[p], x);
+        q.push_back(0.0); /* q0 is recomputed from the vector part (fit_OR_fun_local.m:9) */
+        q.push_back(x[0]);
+        q.push_back(x[1]);
+        q.push_back(x[2]);
+        evox.push_back((int32_t)m);
+      }
+    }
+    first[M] = (int64_t)evox.size();
+    const int64_t E = (int64_t)evox.size();
+    if (E == 0) break;
+    eu.resize((size_t)E);
+    for (int64_t e = 0; e < E; e++) eu[e] = (int32_t)e;
+    ENSURE(c, c->d_q, sizeof(double) * 4 * E);
+    ENSURE(c, c->d_sub, sizeof(double) * 9 * E);
+    ENSURE(c, c->d_valid, sizeof(int32_t) * E);
+    ENSURE(c, c->d_evox, sizeof(int32_t) * E);
+    ENSURE(c, c->d_eu_idx, sizeof(int32_t) * E);
+    ENSURE(c, c->d_ns, sizeof(int32_t) * E);
+    ENSURE(c, c->d_ni, sizeof(int32_t) * E);
+    ENSURE(c, c->d_C, sizeof(double) * E);
+    ENSURE(c, c->d_med, sizeof(float) * E);
+    HIPCHK(c, hipMemcpyAsync(c->d_q.p, q.data(), sizeof(double) * 4 * E, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_evox.p, evox.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_eu_idx.p, eu.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), P<double>(c->d_q), E, P<double>(c->d_sub),
+                                    P<int32_t>(c->d_valid), c->st));
+    s = run_eval(c, GRS_MAP_PAIRS, E, M, P<int32_t>(c->d_evox), P<int32_t>(c->d_eu_idx), GRS_METHOD_FAST,
+                 P<int32_t>(c->d_ns), P<int32_t>(c->d_ni), P<double>(c->d_C), P<float>(c->d_med));
+    if (s != GRS_OK) return s;
+    hns.resize((size_t)E);
+    hni.resize((size_t)E);
+    HIPCHK(c, hipMemcpyAsync(hns.data(), c->d_ns.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hni.data(), c->d_ni.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    fvals.resize((size_t)E);
+    for (int64_t e = 0; e < E; e++) /* fit_OR_fun_local.m:98; 0/0 (no event, or a skipped row) is NaN */
+      fvals[e] = 1.0 - (double)hni[e] / (double)hns[e];
+    for (int64_t m = 0; m < M; m++)
+      if (first[m + 1] > first[m]) nm[m].feed(&fvals[first[m]]);
+  }
+  for (int64_t m = 0; m < M; m++) {
+    nm[m].result(x_out + 3 * m, fval + m);
+    n_fun_evals[m] = nm[m].nf;
+    n_iter[m] = nm[m].it;
+  }
   return GRS_OK;
 }
 

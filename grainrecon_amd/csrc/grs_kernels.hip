@@ -133,6 +133,7 @@ __global__ void k_math_eval(int fn, const double *x, double *y, int64_t n) {
     case 3: r = grs_atan(v); break;
     case 4: r = grs_round(v); break;
     case 5: r = grs_sqrt(v); break;
+    case 7: r = grs_asin(v); break;
     default: r = 1.0 / v; break;
   }
   y[i] = r;

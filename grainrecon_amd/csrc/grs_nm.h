@@ -1,0 +1,249 @@
+/*
+ * grs_nm.h -- bounded Nelder-Mead as a resumable state machine (host code).
+ *
+ * fminsearchbnd (fminsearchbnd.m:152-165 start point, :263-268 sine
+ * transform for two-sided bounds) over MATLAB's fminsearch.  The method
+ * (Lagarias, Reeds, Wright, Wright, SIAM J. Optim. 9 (1998) 112-147, with
+ * fminsearch's documented constants) is restated here:
+ *   simplex x0 and x0 with coordinate j scaled by 1.05 (0.00025 when 0),
+ *   sorted by f (stable, NaN last), iteration count 1;
+ *   loop while evaluations < MaxFunEvals and iterations < MaxIter; stop when
+ *   max|f1-fj| <= max(TolFun, 10 eps(f1)) and max|vj-v1| <= max(TolX,
+ *   10 eps(max(v1))) (max ignores NaN);
+ *   xbar = (sum of the n best) / n; reflect 2 xbar - v_end; expand 3 xbar -
+ *   2 v_end if the reflection beats the best; keep the reflection if it beats
+ *   the second worst; else contract outside 1.5 xbar - 0.5 v_end (reflection
+ *   beats the worst; kept if <= the reflection) or inside 0.5 xbar +
+ *   0.5 v_end (kept if it beats the worst); otherwise shrink towards the best
+ *   by 1/2; sort again.
+ * The caller evaluates the points of each request in any order (typically
+ * many fits in one device launch) and feeds the values back, so the
+ * evaluations of independent fits are batched while each fit follows the
+ * sequential method exactly.  sin/asin are the engine's FP64 functions.
+ */
+#ifndef GRS_NM_H
+#define GRS_NM_H
+
+#include <cmath>
+#include <cstring>
+
+#include "grs_math.h"
+
+#define GRS_NM_NMAX 8
+
+static inline double grs_nm_eps(double x) {
+  if (x != x) return x;
+  x = std::fabs(x);
+  if (std::isinf(x)) return std::nan("");
+  if (x < 2.2250738585072014e-308) return 4.9406564584124654e-324;
+  int e;
+  std::frexp(x, &e);
+  return std::ldexp(1.0, e - 53);
+}
+
+/* MATLAB max of two values: NaN ignored unless both are NaN */
+static inline double grs_nm_max(double a, double b) {
+  if (a != a) return b;
+  if (b != b) return a;
+  return a > b ? a : b;
+}
+
+struct grs_nm {
+  enum State { INIT, REFLECT, EXPAND, CONTRACT_OUT, CONTRACT_IN, SHRINK, DONE };
+  int n = 0;
+  double lb[GRS_NM_NMAX], ub[GRS_NM_NMAX];
+  int max_fe = 2000, max_it = 500;
+  double tol_f = 1e-10, tol_x = 1e-4;
+  double v[GRS_NM_NMAX + 1][GRS_NM_NMAX], fv[GRS_NM_NMAX + 1];
+  double xbar[GRS_NM_NMAX], xr[GRS_NM_NMAX], fxr = 0.0;
+  int nf = 0, it = 0;
+  State st = DONE;
+  int npend = 0;
+  double pend[GRS_NM_NMAX + 1][GRS_NM_NMAX]; /* requested points, transformed (u) space */
+
+  /* u -> x (fminsearchbnd.m:263-268) */
+  void xform(const double *u, double *x) const {
+    GRS_NO_CONTRACT
+    for (int k = 0; k < n; k++) {
+      double t = (grs_sin(u[k]) + 1.0) / 2.0;
+      t = t * (ub[k] - lb[k]) + lb[k];
+      x[k] = std::fmax(lb[k], std::fmin(ub[k], t));
+    }
+  }
+
+  void start(int dims, const double *x0, const double *lo, const double *hi) {
+    GRS_NO_CONTRACT
+    n = dims;
+    std::memcpy(lb, lo, sizeof(double) * n);
+    std::memcpy(ub, hi, sizeof(double) * n);
+    double u0[GRS_NM_NMAX];
+    for (int k = 0; k < n; k++) {
+      if (x0[k] <= lb[k]) {
+        u0[k] = -GRS_PI / 2.0;
+      } else if (x0[k] >= ub[k]) {
+        u0[k] = GRS_PI / 2.0;
+      } else {
+        double z = 2.0 * (x0[k] - lb[k]) / (ub[k] - lb[k]) - 1.0;
+        z = z < -1.0 ? -1.0 : (z > 1.0 ? 1.0 : z);
+        u0[k] = 2.0 * GRS_PI + grs_asin(z);
+      }
+    }
+    std::memcpy(pend[0], u0, sizeof(double) * n);
+    for (int j = 0; j < n; j++) {
+      std::memcpy(pend[j + 1], u0, sizeof(double) * n);
+      pend[j + 1][j] = u0[j] != 0.0 ? (1.0 + 0.05) * u0[j] : 0.00025;
+    }
+    npend = n + 1;
+    st = INIT;
+  }
+
+  bool done() const { return st == DONE; }
+
+  void sort_simplex() {
+    /* stable insertion sort, NaN last */
+    for (int i = 1; i <= n; i++) {
+      double f = fv[i], row[GRS_NM_NMAX];
+      std::memcpy(row, v[i], sizeof(double) * n);
+      int j = i - 1;
+      while (j >= 0 && (f == f) && (fv[j] != fv[j] || fv[j] > f)) {
+        fv[j + 1] = fv[j];
+        std::memcpy(v[j + 1], v[j], sizeof(double) * n);
+        j--;
+      }
+      fv[j + 1] = f;
+      std::memcpy(v[j + 1], row, sizeof(double) * n);
+    }
+  }
+
+  void begin_iteration() {
+    GRS_NO_CONTRACT
+    if (!(nf < max_fe && it < max_it)) {
+      st = DONE;
+      npend = 0;
+      return;
+    }
+    double dfv = std::nan(""), dx = std::nan(""), vmax = std::nan("");
+    for (int j = 1; j <= n; j++) dfv = grs_nm_max(dfv, std::fabs(fv[0] - fv[j]));
+    for (int j = 1; j <= n; j++)
+      for (int k = 0; k < n; k++) dx = grs_nm_max(dx, std::fabs(v[j][k] - v[0][k]));
+    for (int k = 0; k < n; k++) vmax = grs_nm_max(vmax, v[0][k]);
+    if (dfv <= grs_nm_max(tol_f, 10.0 * grs_nm_eps(fv[0])) && dx <= grs_nm_max(tol_x, 10.0 * grs_nm_eps(vmax))) {
+      st = DONE;
+      npend = 0;
+      return;
+    }
+    for (int k = 0; k < n; k++) {
+      double acc = 0.0;
+      for (int j = 0; j < n; j++) acc += v[j][k];
+      xbar[k] = acc / (double)n;
+    }
+    for (int k = 0; k < n; k++) xr[k] = (1.0 + 1.0) * xbar[k] - 1.0 * v[n][k];
+    std::memcpy(pend[0], xr, sizeof(double) * n);
+    npend = 1;
+    st = REFLECT;
+  }
+
+  void end_iteration() {
+    sort_simplex();
+    it += 1;
+    begin_iteration();
+  }
+
+  void request_shrink() {
+    GRS_NO_CONTRACT
+    for (int j = 1; j <= n; j++) {
+      for (int k = 0; k < n; k++) v[j][k] = v[0][k] + 0.5 * (v[j][k] - v[0][k]);
+      std::memcpy(pend[j - 1], v[j], sizeof(double) * n);
+    }
+    npend = n;
+    st = SHRINK;
+  }
+
+  /* values of the npend requested points, in request order */
+  void feed(const double *f) {
+    GRS_NO_CONTRACT
+    switch (st) {
+      case INIT:
+        for (int j = 0; j <= n; j++) {
+          std::memcpy(v[j], pend[j], sizeof(double) * n);
+          fv[j] = f[j];
+        }
+        nf = n + 1;
+        sort_simplex();
+        it = 1;
+        begin_iteration();
+        return;
+      case REFLECT:
+        fxr = f[0];
+        nf += 1;
+        if (fxr < fv[0]) {
+          for (int k = 0; k < n; k++) pend[0][k] = (1.0 + 2.0) * xbar[k] - 2.0 * v[n][k];
+          npend = 1;
+          st = EXPAND;
+          return;
+        }
+        if (fxr < fv[n - 1]) {
+          std::memcpy(v[n], xr, sizeof(double) * n);
+          fv[n] = fxr;
+          end_iteration();
+          return;
+        }
+        if (fxr < fv[n]) {
+          for (int k = 0; k < n; k++) pend[0][k] = (1.0 + 0.5) * xbar[k] - 0.5 * v[n][k];
+          npend = 1;
+          st = CONTRACT_OUT;
+        } else {
+          for (int k = 0; k < n; k++) pend[0][k] = (1.0 - 0.5) * xbar[k] + 0.5 * v[n][k];
+          npend = 1;
+          st = CONTRACT_IN;
+        }
+        return;
+      case EXPAND:
+        nf += 1;
+        if (f[0] < fxr) {
+          std::memcpy(v[n], pend[0], sizeof(double) * n);
+          fv[n] = f[0];
+        } else {
+          std::memcpy(v[n], xr, sizeof(double) * n);
+          fv[n] = fxr;
+        }
+        end_iteration();
+        return;
+      case CONTRACT_OUT:
+        nf += 1;
+        if (f[0] <= fxr) {
+          std::memcpy(v[n], pend[0], sizeof(double) * n);
+          fv[n] = f[0];
+          end_iteration();
+        } else {
+          request_shrink();
+        }
+        return;
+      case CONTRACT_IN:
+        nf += 1;
+        if (f[0] < fv[n]) {
+          std::memcpy(v[n], pend[0], sizeof(double) * n);
+          fv[n] = f[0];
+          end_iteration();
+        } else {
+          request_shrink();
+        }
+        return;
+      case SHRINK:
+        for (int j = 1; j <= n; j++) fv[j] = f[j - 1];
+        nf += n;
+        end_iteration();
+        return;
+      case DONE:
+        return;
+    }
+  }
+
+  /* result in the original variables */
+  void result(double *x, double *fval) const {
+    xform(v[0], x);
+    *fval = fv[0];
+  }
+};
+
+#endif /* GRS_NM_H */

@@ -158,6 +158,26 @@ class Engine:
                                             _ptr(med, _fp)))
         return best, bq, ns, ni, C, med
 
+    # -- orientation fit ----------------------------------------------------------
+    def fit_or_local(self, pos, x0, half_width: float = 0.02, max_fun_evals: int = 2000, max_iter: int = 2000,
+                     tol_fun: float = 1e-10, tol_x: float = 1e-4):
+        """Batched fminsearchbnd of 1 - C over (q1, q2, q3) per voxel
+        (fit_OR_constrain_with_local_comp_cuda.m:116-126).  pos, x0: (M, 3).
+        Returns (x (M, 3), fval (M,), n_fun_evals (M,), n_iter (M,))."""
+        pos = _d(np.asarray(pos).reshape(-1, 3))
+        x0 = _d(np.asarray(x0).reshape(-1, 3))
+        M = pos.shape[0]
+        if x0.shape[0] != M:
+            raise ValueError("pos and x0 must have the same number of rows")
+        x = np.zeros((M, 3))
+        fv = np.zeros(M)
+        nf = np.zeros(M, np.int32)
+        it = np.zeros(M, np.int32)
+        self._chk(self._lib.grs_fit_or_local(self._h, _ptr(pos, _dp), _ptr(x0, _dp), M, float(half_width),
+                                             int(max_fun_evals), int(max_iter), float(tol_fun), float(tol_x),
+                                             _ptr(x, _dp), _ptr(fv, _dp), _ptr(nf, _ip), _ptr(it, _ip)))
+        return x, fv, nf, it
+
     # -- candidate generation ---------------------------------------------------
     def local_sampling(self, centres, nr: int, gridsize: float = 0.004) -> list:
         """ori_local_sampling.m:12-34 for each centre row [q0 q1 q2 q3] (a

@@ -248,6 +248,23 @@ grs_status grs_copy_to_host(grs_ctx *ctx, void *dst, const void *d_src, int64_t 
  * memory-traffic counters against a transfer of known size. */
 grs_status grs_device_copy(grs_ctx *ctx, void *d_dst, const void *d_src, int64_t bytes);
 
+/* ---- orientation fit ------------------------------------------------------- */
+
+/* M independent orientation fits (fit_OR_constrain_with_local_comp_cuda.m:
+ * 116-126 with fit_OR_fun_local.m:2-99): fit m minimises
+ * 1 - Nr_intersect/Nr_simu of the voxel at pos[3m..] over x = (q1, q2, q3)
+ * (q0 = sqrt(1 - (q1^2+q2^2+q3^2))), starting at x0[3m..] inside the bounds
+ * max(x0 - half_width, -1) .. min(x0 + half_width, 1) (the reference uses
+ * 0.02), by fminsearchbnd with MaxFunEvals, MaxIter, TolFun, TolX (the
+ * reference passes 2000, 2000, 1e-10 and the default 1e-4).  Each fit follows
+ * the sequential method exactly; the evaluations of all active fits are
+ * scored together in one device launch per step.  A point with no selected
+ * event or |q_vec| > 1 has the value NaN.  Outputs: x_out (3 per fit), fval,
+ * the number of evaluations and of iterations. */
+grs_status grs_fit_or_local(grs_ctx *ctx, const double *pos, const double *x0, int64_t M, double half_width,
+                            int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
+                            double *fval, int32_t *n_fun_evals, int32_t *n_iter);
+
 /* ---- candidate generation ------------------------------------------------ */
 
 /* ori_local_sampling.m:12-34 for K centres [q0 q1 q2 q3] (used as given,
@@ -294,7 +311,8 @@ grs_status grs_set_audit(grs_ctx *ctx, int32_t enable);
  * coordinates of the selected events. */
 grs_status grs_get_audit(grs_ctx *ctx, int64_t *counts, double *max_dev);
 /* Evaluates the engine's FP64 math on the device: fn 0 sin, 1 cos, 2 acos,
- * 3 atan, 4 round (half away from zero), 5 sqrt, 6 reciprocal.  Host arrays. */
+ * 3 atan, 4 round (half away from zero), 5 sqrt, 6 reciprocal, 7 asin.  Host
+ * arrays. */
 grs_status grs_math_device(grs_ctx *ctx, int32_t fn, const double *x, double *y, int64_t n);
 /* Blocks until all work on ctx's stream is done. */
 grs_status grs_synchronize(grs_ctx *ctx);

@@ -196,7 +196,7 @@ def compete(geo, stack: Stack, vox5, id_all, euler, rot, hkl, param1, nthreads: 
 
 
 def math_eval(fn: int, x) -> np.ndarray:
-    """The engine's FP64 math evaluated on the host: 0 sin, 1 cos, 2 acos, 3 atan, 4 round."""
+    """The engine's FP64 math evaluated on the host: 0 sin, 1 cos, 2 acos, 3 atan, 4 round, 7 asin."""
     x = _d(np.asarray(x).reshape(-1))
     y = np.zeros_like(x)
     lib().oracle_math_eval(int(fn), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), x.size)

@@ -1,18 +1,31 @@
-"""Bounded Nelder-Mead with fminsearchbnd semantics -- CPU BASELINE ONLY.
+"""Bounded Nelder-Mead with fminsearchbnd semantics -- TEST INFRASTRUCTURE /
+CPU BASELINE ONLY.
 
-The reference fits geometry and grains with fminsearchbnd
-(L_shift_tilt_fitting.m:195-222, fit_grain_COM.m:25-28): MATLAB's
-fminsearch (the Nelder-Mead variant of J. C. Lagarias, J. A. Reeds, M. H.
-Wright, P. E. Wright, "Convergence properties of the Nelder-Mead simplex
-method in low dimensions", SIAM J. Optim. 9 (1998)) over variables mapped
-through a sine transform for two-sided bounds (fminsearchbnd.m:38-41).  This
-is this project's restatement, used only to time the derivative-free baseline
-against the adjoint-based fit.
+The reference fits geometry, grains and orientations with fminsearchbnd
+(L_shift_tilt_fitting.m:195-222, fit_grain_COM.m:25-28,
+fit_OR_constrain_with_local_comp_cuda.m:117-126): MATLAB's fminsearch (the
+Nelder-Mead method as analysed by J. C. Lagarias, J. A. Reeds, M. H. Wright,
+P. E. Wright, SIAM J. Optim. 9 (1998) 112-147) over variables mapped through
+a sine for two-sided bounds (fminsearchbnd.m:152-165 for the start point,
+:263-268 for the transform).
 
-Defaults follow fminsearch: reflection 1, expansion 2, contraction 1/2,
-shrink 1/2, initial simplex x0 with each coordinate scaled by 1.05 (0.00025
-for zeros), stop when the simplex diameter <= TolX and the spread of f <=
-TolFun, or after MaxFunEvals / MaxIter.
+fminsearch's published behaviour, restated step by step:
+  * simplex: x0 and x0 with coordinate j scaled by 1.05 (0.00025 when 0),
+    sorted by f (stable; NaN last); iteration count 1, n+1 evaluations;
+  * loop while evaluations < MaxFunEvals and iterations < MaxIter; stop when
+    max|f1 - fj| <= max(TolFun, 10 eps(f1)) and
+    max|v_j - v_1| (all coordinates) <= max(TolX, 10 eps(max(v_1)));
+  * xbar = (sum of the n best vertices) / n; reflection 2 xbar - v_end;
+    expansion 3 xbar - 2 v_end when the reflection beats the best (kept if
+    it beats the reflection); the reflection is kept if it beats the second
+    worst; otherwise an outside contraction 1.5 xbar - 0.5 v_end (kept if
+    <= the reflection) when the reflection beats the worst, else an inside
+    contraction 0.5 xbar + 0.5 v_end (kept if it beats the worst); a failed
+    contraction shrinks every vertex towards the best one by 1/2;
+  * after every iteration the simplex is sorted again;
+  * max() ignores NaN as MATLAB's does.
+Sums run in index order.  sin and asin are injectable so that the checker
+of the engine's batched fit can use the engine's own FP64 functions.
 """
 from __future__ import annotations
 
@@ -20,77 +33,117 @@ import math
 
 import numpy as np
 
-
-def _to_bounded(u, lb, ub):
-    # fminsearchbnd xtransform for two-sided bounds
-    x = (np.sin(u) + 1.0) / 2.0 * (ub - lb) + lb
-    return np.clip(x, lb, ub)
+_EPS = 2.220446049250313e-16
 
 
-def _from_bounded(x, lb, ub):
-    z = 2.0 * (x - lb) / (ub - lb) - 1.0
-    return 2.0 * math.pi + np.arcsin(np.clip(z, -1.0, 1.0))
+def _eps(x: float) -> float:
+    """MATLAB eps(x): the spacing of doubles at |x|."""
+    if x != x:
+        return float("nan")
+    x = abs(x)
+    if x == math.inf:
+        return float("nan")
+    if x < 2.2250738585072014e-308:
+        return 5e-324
+    m, e = math.frexp(x)
+    return math.ldexp(1.0, e - 53)
 
 
-def fminsearchbnd(fun, x0, lb, ub, max_fun_evals=2000, max_iter=500, tol_fun=1e-10, tol_x=1e-4):
-    """Returns (x, fval, n_fun_evals, n_iter)."""
-    x0 = np.asarray(x0, float)
-    lb = np.asarray(lb, float)
-    ub = np.asarray(ub, float)
-    n = x0.size
-    f = lambda u: fun(_to_bounded(u, lb, ub))  # noqa: E731
-    u0 = _from_bounded(x0, lb, ub)
-    sim = np.empty((n + 1, n))
-    sim[0] = u0
+def _nanmax(vals):
+    """MATLAB max: NaN entries are ignored unless every entry is NaN."""
+    vals = [v for v in vals if v == v]
+    return max(vals) if vals else float("nan")
+
+
+def _sort(fv, sim):
+    order = sorted(range(len(fv)), key=lambda i: (fv[i] != fv[i], fv[i] if fv[i] == fv[i] else 0.0))
+    return [fv[i] for i in order], [sim[i] for i in order]
+
+
+def fminsearchbnd(fun, x0, lb, ub, max_fun_evals=2000, max_iter=500, tol_fun=1e-10, tol_x=1e-4,
+                  sin=math.sin, asin=math.asin):
+    """Returns (x, fval, n_fun_evals, n_iter); every bound must be finite."""
+    x0 = [float(v) for v in np.asarray(x0, float).reshape(-1)]
+    lb = [float(v) for v in np.asarray(lb, float).reshape(-1)]
+    ub = [float(v) for v in np.asarray(ub, float).reshape(-1)]
+    n = len(x0)
+
+    def xform(u):
+        out = []
+        for k in range(n):
+            t = (sin(u[k]) + 1.0) / 2.0
+            t = t * (ub[k] - lb[k]) + lb[k]
+            out.append(max(lb[k], min(ub[k], t)))
+        return np.array(out)
+
+    u0 = []
     for k in range(n):
-        y = u0.copy()
-        y[k] = y[k] * 1.05 if y[k] != 0.0 else 0.00025
-        sim[k + 1] = y
-    fv = np.array([f(v) for v in sim])
-    nf = n + 1
-    it = 0
-    while nf < max_fun_evals and it < max_iter:
-        order = np.argsort(fv, kind="stable")
-        sim, fv = sim[order], fv[order]
-        if np.max(np.abs(fv[0] - fv[1:])) <= tol_fun and np.max(np.abs(sim[1:] - sim[0])) <= tol_x:
-            break
-        it += 1
-        xbar = sim[:-1].mean(axis=0)
-        xr = 2.0 * xbar - sim[-1]
-        fr = f(xr)
-        nf += 1
-        if fr < fv[0]:
-            xe = 3.0 * xbar - 2.0 * sim[-1]
-            fe = f(xe)
-            nf += 1
-            if fe < fr:
-                sim[-1], fv[-1] = xe, fe
-            else:
-                sim[-1], fv[-1] = xr, fr
-        elif fr < fv[-2]:
-            sim[-1], fv[-1] = xr, fr
+        if x0[k] <= lb[k]:
+            u0.append(-math.pi / 2)
+        elif x0[k] >= ub[k]:
+            u0.append(math.pi / 2)
         else:
-            shrink = False
-            if fr < fv[-1]:
-                xc = 1.5 * xbar - 0.5 * sim[-1]
-                fc = f(xc)
-                nf += 1
-                if fc <= fr:
-                    sim[-1], fv[-1] = xc, fc
-                else:
-                    shrink = True
+            z = 2.0 * (x0[k] - lb[k]) / (ub[k] - lb[k]) - 1.0
+            u0.append(2.0 * math.pi + asin(max(-1.0, min(1.0, z))))
+
+    def f(u):
+        return float(fun(xform(u)))
+
+    sim = [list(u0)]
+    for j in range(n):
+        y = list(u0)
+        y[j] = (1.0 + 0.05) * y[j] if y[j] != 0.0 else 0.00025
+        sim.append(y)
+    fv = [f(v) for v in sim]
+    fv, sim = _sort(fv, sim)
+    nf, it = n + 1, 1
+    while nf < max_fun_evals and it < max_iter:
+        dfv = _nanmax([abs(fv[0] - fv[j]) for j in range(1, n + 1)])
+        dx = _nanmax([abs(sim[j][k] - sim[0][k]) for j in range(1, n + 1) for k in range(n)])
+        if dfv <= _nanmax([tol_fun, 10.0 * _eps(fv[0])]) and dx <= _nanmax([tol_x, 10.0 * _eps(_nanmax(sim[0]))]):
+            break
+        xbar = []
+        for k in range(n):
+            acc = 0.0
+            for j in range(n):
+                acc += sim[j][k]
+            xbar.append(acc / n)
+        vend = sim[n]
+        xr = [(1.0 + 1.0) * xbar[k] - 1.0 * vend[k] for k in range(n)]
+        fxr = f(xr)
+        nf += 1
+        shrink = False
+        if fxr < fv[0]:
+            xe = [(1.0 + 2.0) * xbar[k] - 2.0 * vend[k] for k in range(n)]
+            fxe = f(xe)
+            nf += 1
+            if fxe < fxr:
+                sim[n], fv[n] = xe, fxe
             else:
-                xcc = 0.5 * xbar + 0.5 * sim[-1]
-                fcc = f(xcc)
-                nf += 1
-                if fcc < fv[-1]:
-                    sim[-1], fv[-1] = xcc, fcc
-                else:
-                    shrink = True
-            if shrink:
-                for k in range(1, n + 1):
-                    sim[k] = sim[0] + 0.5 * (sim[k] - sim[0])
-                    fv[k] = f(sim[k])
-                    nf += 1
-    k = int(np.argmin(fv))
-    return _to_bounded(sim[k], lb, ub), float(fv[k]), nf, it
+                sim[n], fv[n] = xr, fxr
+        elif fxr < fv[n - 1]:
+            sim[n], fv[n] = xr, fxr
+        elif fxr < fv[n]:
+            xc = [(1.0 + 0.5) * xbar[k] - 0.5 * vend[k] for k in range(n)]
+            fxc = f(xc)
+            nf += 1
+            if fxc <= fxr:
+                sim[n], fv[n] = xc, fxc
+            else:
+                shrink = True
+        else:
+            xcc = [(1.0 - 0.5) * xbar[k] + 0.5 * vend[k] for k in range(n)]
+            fxcc = f(xcc)
+            nf += 1
+            if fxcc < fv[n]:
+                sim[n], fv[n] = xcc, fxcc
+            else:
+                shrink = True
+        if shrink:
+            for j in range(1, n + 1):
+                sim[j] = [sim[0][k] + 0.5 * (sim[j][k] - sim[0][k]) for k in range(n)]
+                fv[j] = f(sim[j])
+            nf += n
+        fv, sim = _sort(fv, sim)
+        it += 1
+    return xform(sim[0]), fv[0], nf, it

@@ -48,6 +48,8 @@ def test_header_coefficients_match_generator():
     (1, mp.cos, -0.8, 0.8),
     (2, mp.acos, -1.0, 1.0),
     (3, mp.atan, -8.0, 8.0),
+    (7, mp.asin, -1.0, 1.0),
+    (7, mp.asin, -0.5, 0.5),
 ])
 def test_under_one_ulp(fn_id, mpfn, lo, hi):
     rng = np.random.default_rng(fn_id * 7 + int(hi))
@@ -94,3 +96,19 @@ def test_round_half_away_from_zero():
     want = np.array([1.0, 2.0, 3.0, -1.0, -2.0, -3.0, 0.0, -0.0, 2.0 ** 52 + 1, -(2.0 ** 52) - 1, 1e300, 4.0,
                      -4.0, 0.0])
     assert np.array_equal(oracle.math_eval(4, xs), want)
+
+
+def test_asin_near_endpoints_and_branch_points():
+    xs = []
+    for c in (-1.0, -0.5, 0.0, 0.5, 1.0):
+        for k in range(1, 60):
+            for s in (-1, 1):
+                x = c + s * 2.0 ** (-k)
+                if -1.0 <= x <= 1.0:
+                    xs.append(x)
+    xs += [-1.0, -0.5, 0.0, 0.5, 1.0, 1e-300, -1e-300]
+    assert max_ulp(7, mp.asin, np.array(xs)) < 1.0
+    y = oracle.math_eval(7, np.array([1.0, -1.0, -0.0, 1.0000000000000002, np.nan]))
+    assert y[0] == math.pi / 2 and y[1] == -math.pi / 2
+    assert y[2] == 0.0 and math.copysign(1.0, y[2]) < 0
+    assert np.isnan(y[3]) and np.isnan(y[4])
