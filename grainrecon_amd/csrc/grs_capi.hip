@@ -38,7 +38,7 @@ struct grs_ctx {
   int32_t H = 0, nrot = 0;
   int64_t snz = 0, sny = 0, snrot = 0;
   DevBuf d_geo, d_hkl, d_rot, d_cs, d_keys;
-  DevBuf d_q, d_sub, d_valid, d_pos, d_vw, d_scratch, d_ns, d_ni, d_C, d_med;
+  DevBuf d_q, d_sub, d_valid, d_pos, d_vw, d_vb, d_scratch, d_ns, d_ni, d_C, d_med;
   DevBuf d_rows, d_p1, d_U, d_eu, d_rowstart, d_best, d_evox, d_eu_idx, d_bad, d_x, d_y;
   DevBuf d_mask, d_g1, d_edt_v, d_edt_z, d_grain;
   DevBuf d_bits, d_rowptr, d_roweoff, d_candq, d_rowbest, d_elist;
@@ -232,6 +232,34 @@ grs_status grs_set_geometry(grs_ctx *c, const grs_geometry *geo, const double *h
   g.cos_beta_max = std::cos(g.beta_max);
   g.ipy = 1.0 / g.py;
   g.ipz = 1.0 / g.pz;
+  /* omega solve: uniform ascending rotation angles spanning at most one turn
+   * (plus two steps); corners of the detector in the frame of SamposW:
+   * D0 + Y r2 + Z r3 with D0 = (Lsd, dety00 - RAO, detz00) and (Y, Z) the
+   * detector coordinates of pixel positions 2 pixels outside the area */
+  g.theta_win = g.beta_max - g.pio2;
+  g.om_solve = 0;
+  if (nrot >= 2) {
+    const double step = (rot[nrot - 1] - rot[0]) / (double)(nrot - 1);
+    bool uni = step > 0.0 && (rot[nrot - 1] - rot[0]) <= 360.0 + 2.0 * step;
+    for (int64_t i = 0; uni && i < nrot; i++)
+      uni = std::fabs(rot[i] - (rot[0] + (double)i * step)) <= 1e-9 * (1.0 + std::fabs(rot[i]));
+    if (uni) {
+      g.om_solve = 1;
+      g.om0 = rot[0] * (GRS_PI / 180.0);
+      g.dom = step * (GRS_PI / 180.0);
+      g.om_last = rot[nrot - 1] * (GRS_PI / 180.0);
+    }
+  }
+  {
+    const double r2[3] = {g.R[1], g.R[4], g.R[7]}, r3[3] = {g.R[2], g.R[5], g.R[8]};
+    const double D0[3] = {g.Lsd, g.dety00 - g.RAO, g.detz00};
+    const double ys[2] = {0.5 - 2.0, g.detysize + 0.5 + 2.0}, zs[2] = {0.5 - 2.0, g.detzsize + 0.5 + 2.0};
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        const double Y = (g.dety0 - ys[a]) * g.py, Z = (g.detz0 - zs[b]) * g.pz;
+        for (int k = 0; k < 3; k++) g.corner[3 * (2 * a + b) + k] = D0[k] + Y * r2[k] + Z * r3[k];
+      }
+  }
   hit_key_for(g.thr, &g.hit_key, &g.has_hit);
   c->H = (int32_t)nrhkl;
   c->nrot = (int32_t)nrot;
@@ -320,12 +348,14 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
   if (method == GRS_METHOD_FAST) {
     if (c->H > 240) return fail(c, GRS_ERR_ARG, "the fast method supports at most 240 reflections (got %d)", c->H);
     ENSURE(c, c->d_vw, sizeof(double) * GRS_VW_DOUBLES * (size_t)V * c->nrot);
+    ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)V);
     HIPCHK(c, grs_launch_vw_table(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
-                                  P<double>(c->d_vw), c->st));
+                                  P<double>(c->d_vw), P<double>(c->d_vb), c->st));
     int64_t waves = grs_fast_wave_slots(c->device);
     if (waves > n_eval) waves = ((n_eval + 3) / 4) * 4;
     ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)waves * stride);
     a.vw = P<double>(c->d_vw);
+    a.vb = P<double>(c->d_vb);
     a.scratch = P<uint32_t>(c->d_scratch);
     const bool audit = c->audit && map == GRS_MAP_ONE_VOXEL;
     if (audit) a.audit = P<unsigned long long>(c->d_audit);
@@ -671,14 +701,16 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
       HIPCHK(c, hipMemcpyAsync(c->d_candq.p, cand_q + 4 * c0, sizeof(double) * 4 * kc, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_rowbest.p, 0, sizeof(unsigned long long) * m, c->st));
     ENSURE(c, c->d_vw, sizeof(double) * GRS_VW_DOUBLES * (size_t)m * c->nrot);
+    ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)m);
     HIPCHK(c, grs_launch_vw_table(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
-                                  P<double>(c->d_vw), c->st));
+                                  P<double>(c->d_vw), P<double>(c->d_vb), c->st));
     grs_eval_args a;
     std::memset(&a, 0, sizeof(a));
     a.g = P<grs_dgeo>(c->d_geo);
     a.hkl = P<double>(c->d_hkl);
     a.cs = P<double>(c->d_cs);
     a.vw = P<double>(c->d_vw);
+    a.vb = P<double>(c->d_vb);
     a.pos = P<double>(c->d_pos);
     a.keys = P<uint32_t>(c->d_keys);
     a.hit_bits = P<uint32_t>(c->d_bits);

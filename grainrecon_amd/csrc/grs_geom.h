@@ -37,6 +37,15 @@ typedef struct grs_dgeo {
   double thr;        /* minEucDis / mean([py pz]) */
   double cos_beta_max; /* cos(beta_max), used only by the conservative filters */
   double ipy, ipz;     /* 1/py, 1/pz, used only by the conservative filters */
+  /* omega solve of the candidate filter (used only when om_solve != 0):
+   * rotation angles om0 + i*dom (radians, ascending, span <= 2 pi + 2 dom),
+   * detector corners (x, y, z) in the SamposW frame, 2 pixels outside the
+   * pixel area, and the largest Bragg angle of the window */
+  double om0, dom, om_last;
+  double corner[12];
+  double theta_win;
+  int32_t om_solve;
+  int32_t pad_;
   uint32_t hit_key;  /* largest key k with (double)as_float(k) <= thr */
   int32_t has_hit;   /* 0 when no non-negative distance can be a hit */
   int32_t H, nrot;

@@ -40,6 +40,7 @@
 #define GRS_QCAP 128   /* LDS ring slots per wave */
 #define GRS_GWT 8      /* doubles per hkl row in LDS */
 #define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
+#define GRS_RANGE_DOUBLES(H) (4 * (int64_t)(H) + 1) /* 8H + 2 u32: omega-solve ranges */
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
 
 __device__ __forceinline__ void wave_sync() {
@@ -93,6 +94,51 @@ __global__ void k_vw_table(const grs_dgeo *g, const double *pos, int64_t V, cons
   const double *src = (const double *)&t;
 #pragma unroll
   for (int k = 0; k < GRS_VW_DOUBLES; k++) o[k] = src[k];
+}
+
+/* Per-voxel bounds of the omega-solve candidate filter:
+ *   vb[2v]     >= max over omega of |u - x|, u the unit beam direction at the
+ *               voxel (grainpos / norm(grainpos)) and x the unit x vector;
+ *   vb[2v + 1] >= max over omega of sin^2(gamma/2), gamma the largest angle
+ *               between u and the rays from the voxel to the detector corners
+ *               (2 pixels outside the area); 1 (no detector bound) when a
+ *               corner is 90 degrees or more off the beam or when the window
+ *               admits 2 theta + angle(u, x) >= pi/2 - 1e-3, where the spot
+ *               construction of forward_comp.m:40-55 no longer puts the
+ *               diffracted ray at 2 theta from u.
+ * A spot on the detector lies on a ray at 2 theta from u (the sine-rule
+ * triangle of forward_comp.m:40-47 has angle 2 theta at the voxel whenever
+ * 0 < phiy < pi), so sin^2(theta) = cos^2(beta) <= vb[2v + 1]. */
+__global__ void k_voxel_bounds(const grs_dgeo *g, const double *vw, int64_t V, int32_t nrot, double *vb) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  double d2max = 0.0, kmax = 0.0;
+  bool front = true;
+  for (int32_t om = 0; om < nrot; om++) {
+    const double *t = vw + (v * nrot + om) * GRS_VW_DOUBLES;
+    const double u0 = t[2], u1 = t[3], u2 = t[4];
+    const double sw0 = g->Lsd - t[10], sw1 = t[11], sw2 = t[12];
+    const double d2 = (1.0 - u0) * (1.0 - u0) + u1 * u1 + u2 * u2;
+    d2max = d2 > d2max ? d2 : d2max;
+    double cmin = 1.0;
+    for (int j = 0; j < 4; j++) {
+      const double dx = g->corner[3 * j] - sw0, dy = g->corner[3 * j + 1] - sw1, dz = g->corner[3 * j + 2] - sw2;
+      const double cj = (u0 * dx + u1 * dy + u2 * dz) / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+      front = front && (cj > 1e-3);
+      cmin = cj < cmin ? cj : cmin;
+    }
+    const double k = (1.0 - cmin) * 0.5;
+    kmax = k > kmax ? k : kmax;
+  }
+  const double delta = __builtin_sqrt(d2max) * (1.0 + 1e-9) + 1e-12;
+  const double alpha = 2.0 * grs_asin(delta * 0.5 < 1.0 ? delta * 0.5 : 1.0) + 1e-9;
+  double K = 1.0;
+  if (front && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3) {
+    K = kmax * (1.0 + 1e-6) + 1e-9;
+    K = K < 1.0 ? K : 1.0;
+  }
+  vb[2 * v] = delta;
+  vb[2 * v + 1] = K;
 }
 
 __global__ void k_sub_from_q(const grs_dgeo *g, const double *q, int64_t N, double *SUB, int32_t *valid) {
@@ -317,8 +363,9 @@ __device__ __forceinline__ double fast_rsq(double x) {
  * value and the exact pipeline's are within about 1e-8 pixel of the true spot
  * position, so when the unrounded coordinate is at least GRS_RES_TAU (1e-4
  * pixel) from every half-integer the two round to the same pixel; the window
- * tests are decided here only when cos(beta) and lambda clear their bounds by
- * the phase-A margins.  Everything else (and every NaN) returns 2 and goes
+ * tests are decided here when cos(beta) and lambda clear their bounds by the
+ * phase-A margins (1e-12 on cos(beta), 1e-9 lambda_max on lambda), either
+ * way.  Everything else (and every NaN) returns 2 and goes
  * through the exact pipeline, so the decisions are those of forward_comp.m.
  * Returns 0 (not selected), 1 (selected, pixel in *dy, *dz) or 2 (exact). */
 #define GRS_RES_TAU 1e-4
@@ -345,6 +392,10 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, const double *v, d
   const double cb = __builtin_fma(v[2], G0, __builtin_fma(v[3], G1, v[4] * G2)) * rg;
   const double lam = (-4.0 * GRS_PI) * cb * rg;
   const double lm = 1e-9 * g->lambda_max;
+  /* outside the window by more than the margins: not selected */
+  if (!(cb < 1e-12 && cb > g->cos_beta_max - 1e-12 && lam > g->lambda_min - lm && lam < g->lambda_max + lm))
+    return cb == cb && lam == lam ? 0 : 2;
+  /* inside by less than the margins: the exact pipeline decides */
   if (!(cb < -1e-12 && cb > g->cos_beta_max + 1e-12 && lam > g->lambda_min + lm && lam < g->lambda_max - lm))
     return 2;
   const double omc = __builtin_fma(-cb, cb, 1.0);
@@ -388,16 +439,153 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, const double *v, d
   return 1;
 }
 
+/* The exact forward_comp.m:28-65 pipeline of one event (omega index om,
+ * reflection h): returns 1 and the stack index when the event is selected;
+ * yz receives the unrounded pixel coordinates of a selected event.  Kept out
+ * of line: it runs for about 1e-4 of the events, and inlining it would raise
+ * the register pressure of the whole kernel. */
+__device__ __noinline__ int exact_event(const grs_dgeo *g, const double *vwv, const double *gwt, uint32_t ev,
+                                        uint32_t *idx, double *yz) {
+  const int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
+  grs_vw v;
+  const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
+  double *dst = (double *)&v;
+#pragma unroll
+  for (int k = 0; k < GRS_VW_DOUBLES; k++) dst[k] = src[k];
+  const double *t = gwt + h * GRS_GWT;
+  double Gt[3];
+  grs_Gt(v.c, v.s, t[0], t[1], t[2], Gt);
+  double Glen;
+  double beta = grs_event_beta(&v, Gt, &Glen);
+  int32_t dy = 0, dz = 0;
+  double w[2] = {0.0, 0.0};
+  if (!grs_event_finish_yz(g, &v, Gt, Glen, beta, &dy, &dz, w)) return 0;
+  *idx = (uint32_t)((int64_t)(dz - 1) + (int64_t)(dy - 1) * g->nz + (int64_t)om * g->nz * g->ny);
+  yz[0] = w[0];
+  yz[1] = w[1];
+  return 1;
+}
+
+/* acos and atan2 for the candidate filter (errors ~1e-16 rad; the filter
+ * widens its intervals by 1e-6 rad).  Out of line to keep the register
+ * pressure of the main loop low; they run a few times per evaluation. */
+__device__ __noinline__ double filt_acos(double x) { return grs_acos(x); }
+
+__device__ __noinline__ double filt_atan2(double y, double x) {
+  const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+  double a;
+  if (ax >= ay) a = ax == 0.0 ? 0.0 : grs_atan(ay / ax);
+  else a = GRS_PI / 2.0 - grs_atan(ax / ay);
+  if (x < 0.0) a = GRS_PI - a;
+  return y < 0.0 ? -a : a;
+}
+
+/* Omega solve of one reflection.  With g = S*U*B*hkl, |G| = gn and
+ * r = |(g0, g1)|, the x component of the unit G at omega is
+ * r cos(omega + phi) / gn, phi = atan2(g1, g0); the cos(beta) of the event
+ * (forward_comp.m:31) differs from it by at most delta = max |u - x|.  The
+ * event can pass the window and reach the detector only if cos(beta) lies in
+ * [lo_cb, hi_cb], so omega + phi must lie in the arcs where
+ * cos(psi) is in [(lo_cb - delta) gn / r, (hi_cb + delta) gn / r] (the caller
+ * folds delta in).  The arcs are widened by 1e-6 rad and the index bounds by
+ * 1e-6 steps; the resulting ranges of rotation indices [i0, i1] are sorted
+ * and disjoint.  Returns their number (at most 4) or -1 when more would be
+ * needed (the caller then scans every event). */
+__device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1, double gn, double lo_cb,
+                                         double hi_cb, int32_t nrot, int32_t *i0, int32_t *i1) {
+  const double twopi = 2.0 * GRS_PI, eps = 1e-6;
+  const double r = __builtin_sqrt(g0 * g0 + g1 * g1);
+  if (lo_cb != lo_cb || hi_cb != hi_cb || r != r || !(gn > 0.0)) return -1;
+  if (!(hi_cb > lo_cb)) return 0;
+  double lo[2], hi[2];
+  int niv = 0;
+  bool whole = false;
+  if (!(r > 1e-12 * gn)) {
+    /* |cos(beta) x component| <= r / gn <= 1e-12 at every omega */
+    whole = lo_cb <= 1e-12 && hi_cb >= -1e-12;
+    if (!whole) return 0;
+  } else {
+    const double a = lo_cb * gn / r, b = hi_cb * gn / r;
+    if (!(a < 1.0) || !(b > -1.0)) return 0;
+    const double pa = a <= -1.0 ? GRS_PI : filt_acos(a);
+    const double pb = b >= 1.0 ? 0.0 : filt_acos(b);
+    const double phi = filt_atan2(g1, g0);
+    if (pb <= eps && pa >= GRS_PI - eps) {
+      whole = true;
+    } else if (pb <= eps) {
+      lo[0] = (-pa - eps) - phi;
+      hi[0] = (pa + eps) - phi;
+      niv = 1;
+    } else if (pa >= GRS_PI - eps) {
+      lo[0] = (pb - eps) - phi;
+      hi[0] = (twopi - pb + eps) - phi;
+      niv = 1;
+    } else {
+      lo[0] = (pb - eps) - phi;
+      hi[0] = (pa + eps) - phi;
+      lo[1] = (-pa - eps) - phi;
+      hi[1] = (-pb + eps) - phi;
+      niv = 2;
+    }
+  }
+  if (whole) {
+    i0[0] = 0;
+    i1[0] = nrot - 1;
+    return 1;
+  }
+  int n = 0;
+  for (int k = 0; k < niv; k++) {
+    double m = __builtin_floor((g->om0 - hi[k]) / twopi);
+    for (int it = 0; it < 4; it++, m += 1.0) {
+      const double L = lo[k] + twopi * m, U = hi[k] + twopi * m;
+      if (L > g->om_last + 1e-9) break;
+      if (U < g->om0 - 1e-9) continue;
+      double f0 = __builtin_ceil((L - g->om0) / g->dom - 1e-6);
+      double f1 = __builtin_floor((U - g->om0) / g->dom + 1e-6);
+      f0 = f0 > 0.0 ? f0 : 0.0;
+      f1 = f1 < (double)(nrot - 1) ? f1 : (double)(nrot - 1);
+      if (f0 <= f1) {
+        if (n == 4) return -1;
+        i0[n] = (int32_t)f0;
+        i1[n] = (int32_t)f1;
+        n++;
+      }
+    }
+  }
+  /* sort by start, then make the ranges disjoint */
+  for (int a = 1; a < n; a++)
+    for (int b = a; b > 0 && i0[b] < i0[b - 1]; b--) {
+      int32_t t0 = i0[b], t1 = i1[b];
+      i0[b] = i0[b - 1];
+      i1[b] = i1[b - 1];
+      i0[b - 1] = t0;
+      i1[b - 1] = t1;
+    }
+  int m = 0;
+  int32_t last = -1;
+  for (int k = 0; k < n; k++) {
+    const int32_t a0 = i0[k] > last ? i0[k] : last + 1;
+    if (a0 > i1[k]) continue;
+    i0[m] = a0;
+    i1[m] = i1[k];
+    last = i1[k] > last ? i1[k] : last;
+    m++;
+  }
+  return m;
+}
+
 template <bool MEDIAN, int MAP, bool AUDIT>
 __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32_t H) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
-  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_RING_DOUBLES;
+  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H);
   double *gwt = smem + wib * per_wave;
   uint32_t *ring1 = (uint32_t *)(gwt + (int64_t)H * GRS_GWT); /* phase-A survivors (omega << 8 | hkl) */
   uint32_t *ring2 = ring1 + GRS_QCAP;                          /* events for the exact pipeline */
   uint32_t *ring3 = ring2 + GRS_QCAP;                          /* selected events: stack index */
+  uint32_t *rstart = ring3 + GRS_QCAP;                         /* omega-solve ranges: (i0 << 8 | hkl) */
+  uint32_t *rcum = rstart + 4 * H;                             /* exclusive prefix of range lengths */
   const grs_dgeo *g = a.g;
   const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
   const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
@@ -430,22 +618,85 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
       }
       continue;
     }
-    for (int h = lane; h < H; h += 64) {
-      double gw[3];
-      grs_Gw(sub, a.hkl + 3 * h, gw);
-      double n2 = (gw[0] * gw[0] + gw[1] * gw[1]) + gw[2] * gw[2];
-      double gn = __builtin_sqrt(n2);
-      double *t = gwt + h * GRS_GWT;
-      t[0] = gw[0];
-      t[1] = gw[1];
-      t[2] = gw[2];
-      t[3] = dcb * gn;                              /* keep if dot <  tb0 */
-      t[4] = cbm * gn;                              /* keep if dot >  tb1 */
-      double ml = lrel + 1e-12 * (4.0 * GRS_PI) / gn; /* absolute lambda margin */
-      t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
-      t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
+    /* per-reflection terms; with the omega solve also the candidate ranges */
+    const bool solve = g->om_solve != 0;
+    double delta = 0.0, sK = 1.0;
+    if (solve) {
+      delta = a.vb[2 * vi];
+      sK = __builtin_sqrt(a.vb[2 * vi + 1]);
+    }
+    uint32_t R = 0;
+    bool overflow = false;
+    for (int hb = 0; hb < H; hb += 64) {
+      const int h = hb + lane;
+      int nr_h = 0;
+      int32_t ri0[4], ri1[4];
+      if (h < H) {
+        double gw[3];
+        grs_Gw(sub, a.hkl + 3 * h, gw);
+        double n2 = (gw[0] * gw[0] + gw[1] * gw[1]) + gw[2] * gw[2];
+        double gn = __builtin_sqrt(n2);
+        double *t = gwt + h * GRS_GWT;
+        t[0] = gw[0];
+        t[1] = gw[1];
+        t[2] = gw[2];
+        t[3] = dcb * gn;                              /* keep if dot <  tb0 */
+        t[4] = cbm * gn;                              /* keep if dot >  tb1 */
+        double ml = lrel + 1e-12 * (4.0 * GRS_PI) / gn; /* absolute lambda margin */
+        t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
+        t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
+        if (solve) {
+          /* cos(beta) window: beta and lambda bounds with the phase-A margins,
+           * the detector cone, then widened by delta */
+          double lo_cb = cbm;
+          double lw = -(lmax + ml) * gn / (4.0 * GRS_PI);
+          lo_cb = lw > lo_cb ? lw : lo_cb;
+          lo_cb = -sK > lo_cb ? -sK : lo_cb;
+          double hi_cb = dcb;
+          double hw = -(lmin - ml) * gn / (4.0 * GRS_PI);
+          hi_cb = hw < hi_cb ? hw : hi_cb;
+          nr_h = om_ranges(g, gw[0], gw[1], gn, lo_cb - delta - 1e-12, hi_cb + delta + 1e-12, nrot, ri0, ri1);
+        }
+      }
+      if (solve) {
+        overflow = overflow || (__ballot(nr_h < 0) != 0);
+        const int cnt = nr_h > 0 ? nr_h : 0;
+        int incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+          int v = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += v;
+        }
+        const uint32_t pos = R + (uint32_t)(incl - cnt);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (k < cnt) {
+            rstart[pos + k] = ((uint32_t)ri0[k] << 8) | (uint32_t)h;
+            rcum[pos + k] = (uint32_t)(ri1[k] - ri0[k] + 1);
+          }
+        }
+        R += (uint32_t)__shfl(incl, 63, 64);
+      }
     }
     wave_sync();
+    uint32_t total = 0;
+    if (solve && !overflow) {
+      /* lengths -> exclusive prefix sums */
+      for (uint32_t cb = 0; cb < R; cb += 64) {
+        const uint32_t r = cb + lane;
+        const uint32_t len = r < R ? rcum[r] : 0u;
+        uint32_t incl = len;
+        for (int off = 1; off < 64; off <<= 1) {
+          uint32_t v = (uint32_t)__shfl_up((int)incl, off, 64);
+          if (lane >= off) incl += v;
+        }
+        wave_sync();
+        if (r < R) rcum[r] = total + incl - len;
+        total += (uint32_t)__shfl((int)incl, 63, 64);
+        wave_sync();
+      }
+      if (lane == 0) rcum[R] = total;
+      wave_sync();
+    }
 
     const double *vwv = a.vw + vi * (int64_t)nrot * GRS_VW_DOUBLES;
     uint32_t head1 = 0, tail1 = 0, head2 = 0, tail2 = 0, head3 = 0, tail3 = 0;
@@ -494,34 +745,20 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
       bool sel = false;
       uint32_t idx = 0;
       if ((uint32_t)lane < n) {
-        uint32_t ev = ring2[(head2 + lane) & (GRS_QCAP - 1)];
-        int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
-        grs_vw v;
-        const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
-        double *dst = (double *)&v;
-#pragma unroll
-        for (int k = 0; k < GRS_VW_DOUBLES; k++) dst[k] = src[k];
-        const double *t = gwt + h * GRS_GWT;
-        double Gt[3];
-        grs_Gt(v.c, v.s, t[0], t[1], t[2], Gt);
-        double Glen;
-        double beta = grs_event_beta(&v, Gt, &Glen);
-        int32_t dy = 0, dz = 0;
-        sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
-        if (sel) idx = (uint32_t)((int64_t)(dz - 1) + (int64_t)(dy - 1) * nz + (int64_t)om * nzny);
+        double yz[2];
+        sel = exact_event(g, vwv, gwt, ring2[(head2 + lane) & (GRS_QCAP - 1)], &idx, yz) != 0;
       }
       head2 += n;
       push3(sel, idx);
     };
 
     /* phase B1: resolve n events of ring1 or queue them for the exact pipeline */
-    auto b1 = [&](uint32_t n) {
+    auto b1_core = [&](bool valid, uint32_t ev, uint32_t n) {
       int st = 0;
-      uint32_t ev = 0, idx = 0;
+      uint32_t idx = 0;
       bool bad = false;
       double dev = 0.0;
-      if ((uint32_t)lane < n) {
-        ev = ring1[(head1 + lane) & (GRS_QCAP - 1)];
+      if (valid) {
         int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
         const double *t = gwt + h * GRS_GWT;
         b1_out o;
@@ -530,23 +767,16 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
         st = b1_classify(g, vwv + (int64_t)om * GRS_VW_DOUBLES, t[0], t[1], t[2], &o);
         if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
         if (AUDIT) {
-          grs_vw v;
-          const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
-          double *dst = (double *)&v;
-#pragma unroll
-          for (int k = 0; k < GRS_VW_DOUBLES; k++) dst[k] = src[k];
-          double Gt[3], Glen, yz[2] = {0.0, 0.0};
-          grs_Gt(v.c, v.s, t[0], t[1], t[2], Gt);
-          double beta = grs_event_beta(&v, Gt, &Glen);
-          int32_t edy = 0, edz = 0;
-          int es = grs_event_finish_yz(g, &v, Gt, Glen, beta, &edy, &edz, yz);
-          bad = (st == 0 && es) || (st == 1 && (!es || edy != o.dy || edz != o.dz));
+          double yz[2] = {0.0, 0.0};
+          uint32_t eidx = 0;
+          const int es = exact_event(g, vwv, gwt, ev, &eidx, yz);
+          bad = (st == 0 && es) || (st == 1 && (!es || eidx != idx));
           if (st == 1 && es) dev = __builtin_fmax(__builtin_fabs(o.ry - yz[0]), __builtin_fabs(o.rz - yz[1]));
         }
       }
       if (AUDIT) {
         /* whole-wave reductions, outside the divergent block */
-        uint64_t mb = __ballot(bad), m1 = __ballot(st == 1), m0 = __ballot((uint32_t)lane < n && st == 0);
+        uint64_t mb = __ballot(bad), m1 = __ballot(st == 1), m0 = __ballot(valid && st == 0);
         uint64_t m2 = __ballot(st == 2);
         for (int off = 32; off >= 1; off >>= 1) dev = __builtin_fmax(dev, __shfl_xor(dev, off, 64));
         if (lane == 0) {
@@ -558,7 +788,6 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
           atomicMax(a.audit + 5, (unsigned long long)__double_as_longlong(dev));
         }
       }
-      head1 += n;
       uint64_t m2 = __ballot(st == 2);
       if (st == 2) ring2[(tail2 + lane_prefix(m2)) & (GRS_QCAP - 1)] = ev;
       tail2 += (uint32_t)__popcll(m2);
@@ -570,10 +799,38 @@ __global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32
       }
     };
 
-    /* phase A: wave-uniform steps of 64 consecutive events; lane l owns event
-     * l + 64*k, tracked as (omega, hkl) without divisions */
+    auto b1 = [&](uint32_t n) {
+      const bool valid = (uint32_t)lane < n;
+      const uint32_t ev = valid ? ring1[(head1 + lane) & (GRS_QCAP - 1)] : 0u;
+      head1 += n;
+      b1_core(valid, ev, n);
+    };
+
+    if (solve && !overflow) {
+      /* candidates of the omega solve, 64 per step, straight to phase B1 */
+      uint32_t lo_r = 0;
+      for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t idx = base + (uint32_t)lane;
+        const bool valid = idx < total;
+        uint32_t ev = 0;
+        if (valid) {
+          uint32_t lo = lo_r, hi = R;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (rcum[mid] <= idx) lo = mid;
+            else hi = mid;
+          }
+          lo_r = lo;
+          const uint32_t st0 = rstart[lo];
+          ev = (((st0 >> 8) + (idx - rcum[lo])) << 8) | (st0 & 255u);
+        }
+        b1_core(valid, ev, total - base < 64u ? total - base : 64u);
+      }
+    }
+    /* phase A (no omega solve): wave-uniform steps of 64 consecutive events;
+     * lane l owns event l + 64*k, tracked as (omega, hkl) without divisions */
     int32_t om = om0, h = h0;
-    const int32_t nsteps = (nrot * H + 63) / 64;
+    const int32_t nsteps = (solve && !overflow) ? 0 : (nrot * H + 63) / 64;
     for (int32_t k = 0; k < nsteps; ++k) {
       bool pass = false;
       if (om < nrot) {
@@ -785,10 +1042,13 @@ hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t stride,
 }
 
 hipError_t grs_launch_vw_table(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs, int32_t nrot,
-                               double *d_vw, hipStream_t st) {
+                               double *d_vw, double *d_vb, hipStream_t st) {
   int64_t n = V * nrot;
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_vw_table, dim3(blocks_for(n, 128)), dim3(128), 0, st, d_g, d_pos, V, d_cs, nrot, d_vw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !d_vb) return e;
+  hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 64)), dim3(64), 0, st, d_g, d_vw, V, nrot, d_vb);
   return hipGetLastError();
 }
 
@@ -814,7 +1074,7 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
 }
 
 static size_t fast_lds(int32_t H) {
-  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES) * sizeof(double);
+  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H)) * sizeof(double);
 }
 
 /* Resident blocks per CU of one k_fast instantiation (cached). */
