@@ -37,6 +37,9 @@
 #include "grs_internal.h"
 
 #define GRS_WPB 4      /* waves per block of k_fast */
+#ifndef GRS_FAST_MIN_WAVES
+#define GRS_FAST_MIN_WAVES 3 /* resident waves per SIMD k_fast is compiled for */
+#endif
 #define GRS_QCAP 128   /* LDS ring slots per wave */
 #define GRS_GWT 8      /* doubles per hkl row in LDS */
 #define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
@@ -240,47 +243,57 @@ __device__ uint32_t wave_select_global(const uint32_t *list, int32_t n, int32_t 
   return res;
 }
 
-__device__ float wave_median(const uint32_t *list, int32_t n, int lane) {
+/* Median of n <= 64*R keys held R per lane in registers: the k-th smallest
+ * key by bisection on its bits (count of keys below a candidate by ballot). */
+template <int R>
+__device__ float wave_median_reg(const uint32_t *list, int32_t n, int lane) {
   const int32_t k1 = (n & 1) ? (n >> 1) : (n >> 1) - 1;
-  uint32_t v1, v2;
-  if (n <= 64 * GRS_REG_KEYS) {
-    uint32_t r[GRS_REG_KEYS];
+  uint32_t r[R];
 #pragma unroll
-    for (int j = 0; j < GRS_REG_KEYS; j++) {
-      int32_t idx = j * 64 + lane;
-      r[j] = idx < n ? list[idx] : 0xFFFFFFFFu;
-    }
-    uint32_t res = 0;
-    for (int bit = 31; bit >= 0; --bit) {
-      uint32_t cand = res | (1u << bit);
-      int32_t c = 0;
-#pragma unroll
-      for (int j = 0; j < GRS_REG_KEYS; j++) c += __popcll(__ballot(r[j] < cand));
-      if (c <= k1) res = cand;
-    }
-    v1 = res;
-    if (n & 1) return key_to_float(v1);
-    int32_t cle = 0;
-    uint32_t mn = 0xFFFFFFFFu;
-#pragma unroll
-    for (int j = 0; j < GRS_REG_KEYS; j++) {
-      cle += __popcll(__ballot(r[j] <= v1));
-      if (r[j] > v1 && r[j] < mn) mn = r[j];
-    }
-    v2 = cle > k1 + 1 ? v1 : wave_min_u32(mn);
-  } else {
-    v1 = wave_select_global(list, n, k1, lane);
-    if (n & 1) return key_to_float(v1);
-    int32_t cle = 0;
-    uint32_t mn = 0xFFFFFFFFu;
-    for (int32_t b = 0; b < n; b += 64) {
-      int32_t j = b + lane;
-      uint32_t kv = j < n ? list[j] : 0xFFFFFFFFu;
-      cle += __popcll(__ballot(j < n && kv <= v1));
-      if (j < n && kv > v1 && kv < mn) mn = kv;
-    }
-    v2 = cle > k1 + 1 ? v1 : wave_min_u32(mn);
+  for (int j = 0; j < R; j++) {
+    int32_t idx = j * 64 + lane;
+    r[j] = idx < n ? list[idx] : 0xFFFFFFFFu;
   }
+  uint32_t res = 0;
+  /* keys are non-negative floats: bit 31 is clear */
+  for (int bit = 30; bit >= 0; --bit) {
+    uint32_t cand = res | (1u << bit);
+    int32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < R; j++) c += __popcll(__ballot(r[j] < cand));
+    if (c <= k1) res = cand;
+  }
+  const uint32_t v1 = res;
+  if (n & 1) return key_to_float(v1);
+  int32_t cle = 0;
+  uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    cle += __popcll(__ballot(r[j] <= v1));
+    if (r[j] > v1 && r[j] < mn) mn = r[j];
+  }
+  const uint32_t v2 = cle > k1 + 1 ? v1 : wave_min_u32(mn);
+  float a = key_to_float(v1), b = key_to_float(v2);
+  return a + (b - a) / 2.0f;
+}
+
+__device__ float wave_median(const uint32_t *list, int32_t n, int lane) {
+  if (n <= 128) return wave_median_reg<2>(list, n, lane);
+  if (n <= 256) return wave_median_reg<4>(list, n, lane);
+  if (n <= 512) return wave_median_reg<8>(list, n, lane);
+  if (n <= 64 * GRS_REG_KEYS) return wave_median_reg<GRS_REG_KEYS>(list, n, lane);
+  const int32_t k1 = (n & 1) ? (n >> 1) : (n >> 1) - 1;
+  uint32_t v1 = wave_select_global(list, n, k1, lane), v2;
+  if (n & 1) return key_to_float(v1);
+  int32_t cle = 0;
+  uint32_t mn = 0xFFFFFFFFu;
+  for (int32_t b = 0; b < n; b += 64) {
+    int32_t j = b + lane;
+    uint32_t kv = j < n ? list[j] : 0xFFFFFFFFu;
+    cle += __popcll(__ballot(j < n && kv <= v1));
+    if (j < n && kv > v1 && kv < mn) mn = kv;
+  }
+  v2 = cle > k1 + 1 ? v1 : wave_min_u32(mn);
   float a = key_to_float(v1), b = key_to_float(v2);
   return a + (b - a) / 2.0f;
 }
@@ -575,7 +588,7 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
 }
 
 template <bool MEDIAN, int MAP, bool AUDIT>
-__global__ __launch_bounds__(64 * GRS_WPB, 3) void k_fast(grs_eval_args a, int32_t H) {
+__global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_eval_args a, int32_t H) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
