@@ -85,8 +85,12 @@ __global__ void k_voxel_pos(const double *rows, int64_t M, int32_t stride, const
   grs_voxel_pos(r[0], r[1], r[2], p1, pos + 3 * i);
 }
 
+/* vw: the per (voxel, omega) terms of the exact pipeline, one record per
+ * (voxel, omega); bt: the six terms phase B1 reads (u0 u1 u2 Lsd-SW0 SW1 SW2)
+ * as separate rows per voxel (bt[(v*6 + f)*nrot + omega]) so that a wave's
+ * loads of one term over consecutive omegas are contiguous. */
 __global__ void k_vw_table(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
-                           double *vw) {
+                           double *vw, double *bt) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= V * nrot) return;
   int64_t v = i / nrot;
@@ -97,6 +101,13 @@ __global__ void k_vw_table(const grs_dgeo *g, const double *pos, int64_t V, cons
   const double *src = (const double *)&t;
 #pragma unroll
   for (int k = 0; k < GRS_VW_DOUBLES; k++) o[k] = src[k];
+  double *b = bt + v * 6 * (int64_t)nrot + om;
+  b[0] = t.u0;
+  b[nrot] = t.u1;
+  b[2 * (int64_t)nrot] = t.u2;
+  b[3 * (int64_t)nrot] = t.k0;
+  b[4 * (int64_t)nrot] = t.SW1;
+  b[5 * (int64_t)nrot] = t.SW2;
 }
 
 /* Per-voxel bounds of the omega-solve candidate filter:
@@ -396,13 +407,13 @@ __device__ __forceinline__ bool res_round(double r, double *k) {
   return fr >= GRS_RES_TAU && fr <= 1.0 - GRS_RES_TAU;
 }
 
-__device__ __forceinline__ int b1_classify(const grs_dgeo *g, const double *v, double gw0, double gw1, double gw2,
-                                           b1_out *o) {
-  const double c = v[0], s = v[1];
+__device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *bt, int32_t om,
+                                           int32_t nrot, double gw0, double gw1, double gw2, b1_out *o) {
+  const double u0 = bt[om], u1 = bt[nrot + om], u2 = bt[2 * nrot + om];
   const double G0 = c * gw0 - s * gw1, G1 = s * gw0 + c * gw1, G2 = gw2;
   const double g2 = __builtin_fma(G0, G0, __builtin_fma(G1, G1, G2 * G2));
   const double rg = fast_rsq(g2);
-  const double cb = __builtin_fma(v[2], G0, __builtin_fma(v[3], G1, v[4] * G2)) * rg;
+  const double cb = __builtin_fma(u0, G0, __builtin_fma(u1, G1, u2 * G2)) * rg;
   const double lam = (-4.0 * GRS_PI) * cb * rg;
   const double lm = 1e-9 * g->lambda_max;
   /* outside the window by more than the margins: not selected */
@@ -417,24 +428,33 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, const double *v, d
   const double c2t = __builtin_fma(-2.0 * cb, cb, 1.0);
   const double nv2 = __builtin_fma(G1, G1, G2 * G2);
   if (!(nv2 >= 1e-12 * g2)) return 2;
+  /* the remaining per (voxel, omega) terms from u and SamposW: center is
+   * grainpos * L / grainpos(1), so center/|center| = u, center(2:3) =
+   * L u(2:3) / u0, L2 = (Lsd - SW1) / u0 (cos(alpha) = u0), and the t
+   * numerator and offsets of forward_comp.m:51-54 */
+  const double k0 = bt[3 * nrot + om], sw1 = bt[4 * nrot + om], sw2 = bt[5 * nrot + om];
+  const double iu0 = fast_rcp(u0);
+  const double ctr1 = g->L * u1 * iu0, ctr2 = g->L * u2 * iu0, L2 = k0 * iu0;
+  const double *R = g->R;
+  const double tnum = __builtin_fma(R[0], k0, __builtin_fma(R[3], (g->dety00 - g->RAO) - sw1, R[6] * (g->detz00 - sw2)));
+  const double ov1 = (sw1 - g->dety00) + g->RAO, ov2 = sw2 - g->detz00;
   const double inv = fast_rsq(nv2);
-  const double ap = __builtin_fma(G1, v[7], G2 * v[8]) * inv;
+  const double ap = __builtin_fma(G1, u1, G2 * u2) * inv;
   if (!(__builtin_fabs(ap) <= 0.99)) return 2;
   const double oma = __builtin_fma(-ap, ap, 1.0);
   const double sph = oma * fast_rsq(oma);
   const double sphy = __builtin_fma(sph, c2t, -ap * s2t);
   if (!(__builtin_fabs(sphy) >= 1e-3)) return 2;
-  const double diffvec = v[9] * s2t * fast_rcp(sphy);
-  const double y22 = __builtin_fma(diffvec * G1, inv, v[5]);
-  const double z22 = __builtin_fma(diffvec * G2, inv, v[6]);
-  const double k0 = v[10], k1 = y22 - v[11], k2 = z22 - v[12];
+  const double diffvec = L2 * s2t * fast_rcp(sphy);
+  const double y22 = __builtin_fma(diffvec * G1, inv, ctr1);
+  const double z22 = __builtin_fma(diffvec * G2, inv, ctr2);
+  const double k1 = y22 - sw1, k2 = z22 - sw2;
   const double ik = fast_rsq(__builtin_fma(k0, k0, __builtin_fma(k1, k1, k2 * k2)));
   const double K0 = k0 * ik, K1 = k1 * ik, K2 = k2 * ik;
-  const double *R = g->R;
   const double tden = __builtin_fma(R[0], K0, __builtin_fma(R[3], K1, R[6] * K2));
   if (!(__builtin_fabs(tden) >= 1e-3)) return 2;
-  const double t = v[13] * fast_rcp(tden);
-  const double w0 = __builtin_fma(t, K0, -k0), w1 = __builtin_fma(t, K1, v[14]), w2 = __builtin_fma(t, K2, v[15]);
+  const double t = tnum * fast_rcp(tden);
+  const double w0 = __builtin_fma(t, K0, -k0), w1 = __builtin_fma(t, K1, ov1), w2 = __builtin_fma(t, K2, ov2);
   const double Y = __builtin_fma(R[1], w0, __builtin_fma(R[4], w1, R[7] * w2));
   const double Z = __builtin_fma(R[2], w0, __builtin_fma(R[5], w1, R[8] * w2));
   const double ry = __builtin_fma(-Y, g->ipy, g->dety0);
@@ -712,6 +732,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     }
 
     const double *vwv = a.vw + vi * (int64_t)nrot * GRS_VW_DOUBLES;
+    const double *btv = a.bt + vi * 6 * (int64_t)nrot;
     uint32_t head1 = 0, tail1 = 0, head2 = 0, tail2 = 0, head3 = 0, tail3 = 0;
     int32_t ns = 0, ni = 0;
     /* the gather of the previous batch is consumed one batch later */
@@ -777,7 +798,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         b1_out o;
         o.ry = o.rz = 0.0;
         o.dy = o.dz = 0;
-        st = b1_classify(g, vwv + (int64_t)om * GRS_VW_DOUBLES, t[0], t[1], t[2], &o);
+        st = b1_classify(g, a.cs[2 * om], a.cs[2 * om + 1], btv, om, nrot, t[0], t[1], t[2], &o);
         if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
         if (AUDIT) {
           double yz[2] = {0.0, 0.0};
@@ -1055,10 +1076,10 @@ hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t stride,
 }
 
 hipError_t grs_launch_vw_table(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs, int32_t nrot,
-                               double *d_vw, double *d_vb, hipStream_t st) {
+                               double *d_vw, double *d_bt, double *d_vb, hipStream_t st) {
   int64_t n = V * nrot;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vw_table, dim3(blocks_for(n, 128)), dim3(128), 0, st, d_g, d_pos, V, d_cs, nrot, d_vw);
+  hipLaunchKernelGGL(k_vw_table, dim3(blocks_for(n, 128)), dim3(128), 0, st, d_g, d_pos, V, d_cs, nrot, d_vw, d_bt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !d_vb) return e;
   hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 64)), dim3(64), 0, st, d_g, d_vw, V, nrot, d_vb);
