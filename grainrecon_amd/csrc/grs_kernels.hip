@@ -619,7 +619,13 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   uint32_t *ring3 = ring2 + GRS_QCAP;                          /* selected events: stack index */
   uint32_t *rstart = ring3 + GRS_QCAP;                         /* omega-solve ranges: (i0 << 8 | hkl) */
   uint32_t *rcum = rstart + 4 * H;                             /* exclusive prefix of range lengths */
-  const grs_dgeo *g = a.g;
+  /* the geometry constants live in LDS: the loop bodies read ~30 of them per
+   * batch, and neither scalar registers (spilled) nor vector loads (the
+   * compiler cannot prove them unchanged across the loop's stores) serve */
+  __shared__ grs_dgeo sg;
+  if (threadIdx.x == 0) sg = *a.g;
+  __syncthreads();
+  const grs_dgeo *g = &sg;
   const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
   const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
   uint32_t *wlist = MEDIAN ? a.scratch + gwave * a.scratch_stride : nullptr;
@@ -780,7 +786,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       uint32_t idx = 0;
       if ((uint32_t)lane < n) {
         double yz[2];
-        sel = exact_event(g, vwv, gwt, ring2[(head2 + lane) & (GRS_QCAP - 1)], &idx, yz) != 0;
+        sel = exact_event(a.g, vwv, gwt, ring2[(head2 + lane) & (GRS_QCAP - 1)], &idx, yz) != 0;
       }
       head2 += n;
       push3(sel, idx);
@@ -803,7 +809,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         if (AUDIT) {
           double yz[2] = {0.0, 0.0};
           uint32_t eidx = 0;
-          const int es = exact_event(g, vwv, gwt, ev, &eidx, yz);
+          const int es = exact_event(a.g, vwv, gwt, ev, &eidx, yz);
           bad = (st == 0 && es) || (st == 1 && (!es || eidx != idx));
           if (st == 1 && es) dev = __builtin_fmax(__builtin_fabs(o.ry - yz[0]), __builtin_fabs(o.rz - yz[1]));
         }
