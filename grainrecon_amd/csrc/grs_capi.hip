@@ -38,7 +38,7 @@ struct grs_ctx {
   int32_t H = 0, nrot = 0;
   int64_t snz = 0, sny = 0, snrot = 0;
   DevBuf d_geo, d_hkl, d_rot, d_cs, d_keys;
-  DevBuf d_q, d_sub, d_valid, d_pos, d_vw, d_vb, d_bt, d_scratch, d_ns, d_ni, d_C, d_med;
+  DevBuf d_q, d_sub, d_valid, d_pos, d_vb, d_scratch, d_ns, d_ni, d_C, d_med;
   DevBuf d_rows, d_p1, d_U, d_eu, d_rowstart, d_best, d_evox, d_eu_idx, d_bad, d_x, d_y;
   DevBuf d_mask, d_g1, d_edt_v, d_edt_z, d_grain;
   DevBuf d_bits, d_rowptr, d_roweoff, d_candq, d_rowbest, d_elist;
@@ -165,7 +165,7 @@ void grs_destroy(grs_ctx *c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->st);
   DevBuf *bufs[] = {&c->d_geo, &c->d_hkl, &c->d_rot, &c->d_cs, &c->d_keys, &c->d_q, &c->d_sub, &c->d_valid,
-                    &c->d_pos, &c->d_vw, &c->d_scratch, &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_rows,
+                    &c->d_pos, &c->d_vb, &c->d_scratch, &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_rows,
                     &c->d_p1, &c->d_U, &c->d_eu, &c->d_rowstart, &c->d_best, &c->d_evox, &c->d_eu_idx,
                     &c->d_bad, &c->d_x, &c->d_y, &c->d_mask, &c->d_g1, &c->d_edt_v, &c->d_edt_z,
                     &c->d_grain, &c->d_bits, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest,
@@ -347,17 +347,13 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
   a.out_med = d_med;
   if (method == GRS_METHOD_FAST) {
     if (c->H > 240) return fail(c, GRS_ERR_ARG, "the fast method supports at most 240 reflections (got %d)", c->H);
-    ENSURE(c, c->d_vw, sizeof(double) * GRS_VW_DOUBLES * (size_t)V * c->nrot);
     ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)V);
-    ENSURE(c, c->d_bt, sizeof(double) * 6 * (size_t)V * c->nrot);
-    HIPCHK(c, grs_launch_vw_table(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
-                                  P<double>(c->d_vw), P<double>(c->d_bt), P<double>(c->d_vb), c->st));
+    HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
+                                      P<double>(c->d_vb), c->st));
     int64_t waves = grs_fast_wave_slots(c->device);
     if (waves > n_eval) waves = ((n_eval + 3) / 4) * 4;
     ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)waves * stride);
-    a.vw = P<double>(c->d_vw);
     a.vb = P<double>(c->d_vb);
-    a.bt = P<double>(c->d_bt);
     a.scratch = P<uint32_t>(c->d_scratch);
     const bool audit = c->audit && map == GRS_MAP_ONE_VOXEL;
     if (audit) a.audit = P<unsigned long long>(c->d_audit);
@@ -442,12 +438,11 @@ grs_status grs_forward_comp_device(grs_ctx *c, const double *pos, const double *
 }
 
 /* voxel rows per device chunk (bounds the per-(voxel, omega) table) */
+/* Voxels per launch of the grow / compete entries: device buffers per voxel
+ * are a few dozen bytes (positions, bounds, outputs), so chunks are large. */
 static int64_t voxel_chunk(const grs_ctx *c) {
-  int64_t per = (int64_t)c->nrot * GRS_VW_DOUBLES * 8;
-  int64_t m = ((int64_t)1 << 30) / (per > 0 ? per : 1);
-  if (m < 256) m = 256;
-  if (m > 32768) m = 32768;
-  return m;
+  (void)c;
+  return (int64_t)1 << 22;
 }
 
 grs_status grs_grow_indexed_region(grs_ctx *c, const double *vox, int64_t M, const double *U, const double *p1,
@@ -702,19 +697,15 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     if (kc > 0)
       HIPCHK(c, hipMemcpyAsync(c->d_candq.p, cand_q + 4 * c0, sizeof(double) * 4 * kc, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemsetAsync(c->d_rowbest.p, 0, sizeof(unsigned long long) * m, c->st));
-    ENSURE(c, c->d_vw, sizeof(double) * GRS_VW_DOUBLES * (size_t)m * c->nrot);
     ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)m);
-    ENSURE(c, c->d_bt, sizeof(double) * 6 * (size_t)m * c->nrot);
-    HIPCHK(c, grs_launch_vw_table(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
-                                  P<double>(c->d_vw), P<double>(c->d_bt), P<double>(c->d_vb), c->st));
+    HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
+                                      P<double>(c->d_vb), c->st));
     grs_eval_args a;
     std::memset(&a, 0, sizeof(a));
     a.g = P<grs_dgeo>(c->d_geo);
     a.hkl = P<double>(c->d_hkl);
     a.cs = P<double>(c->d_cs);
-    a.vw = P<double>(c->d_vw);
     a.vb = P<double>(c->d_vb);
-    a.bt = P<double>(c->d_bt);
     a.pos = P<double>(c->d_pos);
     a.keys = P<uint32_t>(c->d_keys);
     a.hit_bits = P<uint32_t>(c->d_bits);

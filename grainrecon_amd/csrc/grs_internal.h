@@ -39,10 +39,8 @@ typedef struct grs_eval_args {
   const grs_dgeo *g;       /* device copy of the geometry */
   const double *hkl;       /* H x 3 */
   const double *cs;        /* nrot x 2: cos, sin of omega */
-  const double *vw;        /* V x nrot x GRS_VW_DOUBLES (fast kernel) */
   const double *vb;        /* V x 2 omega-solve bounds (fast kernel) */
-  const double *bt;        /* V x 6 x nrot phase-B1 terms (fast kernel) */
-  const double *pos;       /* V x 3 voxel positions in mm (plain kernel) */
+  const double *pos;       /* V x 3 voxel positions in mm */
   const double *SUB;       /* NU x 9 */
   const int32_t *valid;    /* NU: 0 -> orientation row skipped */
   const uint32_t *keys;    /* distance stack as ordered keys, nz*ny*nrot */
@@ -75,9 +73,9 @@ typedef struct grs_eval_args {
 #define GRS_KEY_MAX_EVENTS 23170
 
 /* Launchers (return hipError_t). */
-/* also fills d_vb (2 doubles per voxel, the omega-solve bounds) when non-null */
-hipError_t grs_launch_vw_table(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs, int32_t nrot,
-                               double *d_vw, double *d_bt, double *d_vb, hipStream_t st);
+/* d_vb: 2 doubles per voxel, the omega-solve bounds */
+hipError_t grs_launch_voxel_bounds(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs,
+                                   int32_t nrot, double *d_vb, hipStream_t st);
 hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t row_stride, const double *d_p1,
                                 double *d_pos, hipStream_t st);
 hipError_t grs_launch_sub_from_q(const grs_dgeo *d_g, const double *d_q, int64_t N, double *d_SUB, int32_t *d_valid,

@@ -85,29 +85,28 @@ __global__ void k_voxel_pos(const double *rows, int64_t M, int32_t stride, const
   grs_voxel_pos(r[0], r[1], r[2], p1, pos + 3 * i);
 }
 
-/* vw: the per (voxel, omega) terms of the exact pipeline, one record per
- * (voxel, omega); bt: the six terms phase B1 reads (u0 u1 u2 Lsd-SW0 SW1 SW2)
- * as separate rows per voxel (bt[(v*6 + f)*nrot + omega]) so that a wave's
- * loads of one term over consecutive omegas are contiguous. */
-__global__ void k_vw_table(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
-                           double *vw, double *bt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= V * nrot) return;
-  int64_t v = i / nrot;
-  int32_t om = (int32_t)(i - v * nrot);
-  grs_vw t;
-  grs_vw_terms(g, pos + 3 * v, cs[2 * om], cs[2 * om + 1], &t);
-  double *o = vw + i * GRS_VW_DOUBLES;
-  const double *src = (const double *)&t;
-#pragma unroll
-  for (int k = 0; k < GRS_VW_DOUBLES; k++) o[k] = src[k];
-  double *b = bt + v * 6 * (int64_t)nrot + om;
-  b[0] = t.u0;
-  b[nrot] = t.u1;
-  b[2 * (int64_t)nrot] = t.u2;
-  b[3 * (int64_t)nrot] = t.k0;
-  b[4 * (int64_t)nrot] = t.SW1;
-  b[5 * (int64_t)nrot] = t.SW2;
+/* P = S * [pos0, pos1 - RotAxisOffset, pos2] of a voxel: SamposW at omega is
+ * Omega * P (forward_comp.m:17,22 up to the order of the products, which
+ * only the exact pipeline needs). */
+__device__ __forceinline__ void voxel_P(const grs_dgeo *g, const double *pos, double *P) {
+  const double x = pos[0], y = pos[1] - g->RAO, z = pos[2];
+  for (int i = 0; i < 3; i++) P[i] = __builtin_fma(g->S[3 * i], x, __builtin_fma(g->S[3 * i + 1], y, g->S[3 * i + 2] * z));
+}
+
+/* SamposW and the unit beam direction u = grainpos / |grainpos| at
+ * (cos omega, sin omega), to ~1e-16 relative (filters only). */
+__device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, double c, double s, double *sw,
+                                           double *u) {
+  sw[0] = c * P[0] - s * P[1];
+  sw[1] = s * P[0] + c * P[1];
+  sw[2] = P[2];
+  const double gp0 = g->Lss + sw[0], gp1 = (sw[1] - g->P0y) + g->RAO, gp2 = sw[2] - g->P0z;
+  const double n2 = __builtin_fma(gp0, gp0, __builtin_fma(gp1, gp1, gp2 * gp2));
+  double r = __builtin_amdgcn_rsq(n2);
+  r = r * __builtin_fma(-0.5 * n2, r * r, 1.5);
+  u[0] = gp0 * r;
+  u[1] = gp1 * r;
+  u[2] = gp2 * r;
 }
 
 /* Per-voxel bounds of the omega-solve candidate filter:
@@ -123,15 +122,19 @@ __global__ void k_vw_table(const grs_dgeo *g, const double *pos, int64_t V, cons
  * A spot on the detector lies on a ray at 2 theta from u (the sine-rule
  * triangle of forward_comp.m:40-47 has angle 2 theta at the voxel whenever
  * 0 < phiy < pi), so sin^2(theta) = cos^2(beta) <= vb[2v + 1]. */
-__global__ void k_voxel_bounds(const grs_dgeo *g, const double *vw, int64_t V, int32_t nrot, double *vb) {
+__global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
+                               double *vb) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
+  double P[3];
+  voxel_P(g, pos + 3 * v, P);
   double d2max = 0.0, kmax = 0.0;
   bool front = true;
   for (int32_t om = 0; om < nrot; om++) {
-    const double *t = vw + (v * nrot + om) * GRS_VW_DOUBLES;
-    const double u0 = t[2], u1 = t[3], u2 = t[4];
-    const double sw0 = g->Lsd - t[10], sw1 = t[11], sw2 = t[12];
+    double sw[3], u[3];
+    beam_terms(g, P, cs[2 * om], cs[2 * om + 1], sw, u);
+    const double u0 = u[0], u1 = u[1], u2 = u[2];
+    const double sw0 = sw[0], sw1 = sw[1], sw2 = sw[2];
     const double d2 = (1.0 - u0) * (1.0 - u0) + u1 * u1 + u2 * u2;
     d2max = d2 > d2max ? d2 : d2max;
     double cmin = 1.0;
@@ -407,9 +410,11 @@ __device__ __forceinline__ bool res_round(double r, double *k) {
   return fr >= GRS_RES_TAU && fr <= 1.0 - GRS_RES_TAU;
 }
 
-__device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *bt, int32_t om,
-                                           int32_t nrot, double gw0, double gw1, double gw2, b1_out *o) {
-  const double u0 = bt[om], u1 = bt[nrot + om], u2 = bt[2 * nrot + om];
+__device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *P, double gw0,
+                                           double gw1, double gw2, b1_out *o) {
+  double swv[3], uv[3];
+  beam_terms(g, P, c, s, swv, uv);
+  const double u0 = uv[0], u1 = uv[1], u2 = uv[2];
   const double G0 = c * gw0 - s * gw1, G1 = s * gw0 + c * gw1, G2 = gw2;
   const double g2 = __builtin_fma(G0, G0, __builtin_fma(G1, G1, G2 * G2));
   const double rg = fast_rsq(g2);
@@ -432,7 +437,7 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
    * grainpos * L / grainpos(1), so center/|center| = u, center(2:3) =
    * L u(2:3) / u0, L2 = (Lsd - SW1) / u0 (cos(alpha) = u0), and the t
    * numerator and offsets of forward_comp.m:51-54 */
-  const double k0 = bt[3 * nrot + om], sw1 = bt[4 * nrot + om], sw2 = bt[5 * nrot + om];
+  const double k0 = g->Lsd - swv[0], sw1 = swv[1], sw2 = swv[2];
   const double iu0 = fast_rcp(u0);
   const double ctr1 = g->L * u1 * iu0, ctr2 = g->L * u2 * iu0, L2 = k0 * iu0;
   const double *R = g->R;
@@ -474,17 +479,15 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
 
 /* The exact forward_comp.m:28-65 pipeline of one event (omega index om,
  * reflection h): returns 1 and the stack index when the event is selected;
- * yz receives the unrounded pixel coordinates of a selected event.  Kept out
- * of line: it runs for about 1e-4 of the events, and inlining it would raise
+ * yz receives the unrounded pixel coordinates of a selected event.  The
+ * per (voxel, omega) terms are recomputed here with the function of the
+ * exact pipeline (grs_vw_terms).  Kept out of line: it runs for about 1e-4 of the events, and inlining it would raise
  * the register pressure of the whole kernel. */
-__device__ __noinline__ int exact_event(const grs_dgeo *g, const double *vwv, const double *gwt, uint32_t ev,
-                                        uint32_t *idx, double *yz) {
+__device__ __noinline__ int exact_event(const grs_dgeo *g, const double *pos, const double *cs, const double *gwt,
+                                        uint32_t ev, uint32_t *idx, double *yz) {
   const int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
   grs_vw v;
-  const double *src = vwv + (int64_t)om * GRS_VW_DOUBLES;
-  double *dst = (double *)&v;
-#pragma unroll
-  for (int k = 0; k < GRS_VW_DOUBLES; k++) dst[k] = src[k];
+  grs_vw_terms(g, pos, cs[2 * om], cs[2 * om + 1], &v);
   const double *t = gwt + h * GRS_GWT;
   double Gt[3];
   grs_Gt(v.c, v.s, t[0], t[1], t[2], Gt);
@@ -612,13 +615,14 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
-  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H);
+  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4;
   double *gwt = smem + wib * per_wave;
   uint32_t *ring1 = (uint32_t *)(gwt + (int64_t)H * GRS_GWT); /* phase-A survivors (omega << 8 | hkl) */
   uint32_t *ring2 = ring1 + GRS_QCAP;                          /* events for the exact pipeline */
   uint32_t *ring3 = ring2 + GRS_QCAP;                          /* selected events: stack index */
   uint32_t *rstart = ring3 + GRS_QCAP;                         /* omega-solve ranges: (i0 << 8 | hkl) */
   uint32_t *rcum = rstart + 4 * H;                             /* exclusive prefix of range lengths */
+  double *Pw = gwt + (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H); /* S*pos of the voxel */
   /* the geometry constants live in LDS: the loop bodies read ~30 of them per
    * batch, and neither scalar registers (spilled) nor vector loads (the
    * compiler cannot prove them unchanged across the loop's stores) serve */
@@ -659,6 +663,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     }
     /* per-reflection terms; with the omega solve also the candidate ranges */
     const bool solve = g->om_solve != 0;
+    if (lane == 0) voxel_P(g, a.pos + 3 * vi, Pw);
     double delta = 0.0, sK = 1.0;
     if (solve) {
       delta = a.vb[2 * vi];
@@ -737,8 +742,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       wave_sync();
     }
 
-    const double *vwv = a.vw + vi * (int64_t)nrot * GRS_VW_DOUBLES;
-    const double *btv = a.bt + vi * 6 * (int64_t)nrot;
+    const double *vpos = a.pos + 3 * vi;
     uint32_t head1 = 0, tail1 = 0, head2 = 0, tail2 = 0, head3 = 0, tail3 = 0;
     int32_t ns = 0, ni = 0;
     /* the gather of the previous batch is consumed one batch later */
@@ -786,7 +790,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       uint32_t idx = 0;
       if ((uint32_t)lane < n) {
         double yz[2];
-        sel = exact_event(a.g, vwv, gwt, ring2[(head2 + lane) & (GRS_QCAP - 1)], &idx, yz) != 0;
+        sel = exact_event(a.g, vpos, a.cs, gwt, ring2[(head2 + lane) & (GRS_QCAP - 1)], &idx, yz) != 0;
       }
       head2 += n;
       push3(sel, idx);
@@ -804,12 +808,12 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         b1_out o;
         o.ry = o.rz = 0.0;
         o.dy = o.dz = 0;
-        st = b1_classify(g, a.cs[2 * om], a.cs[2 * om + 1], btv, om, nrot, t[0], t[1], t[2], &o);
+        st = b1_classify(g, a.cs[2 * om], a.cs[2 * om + 1], Pw, t[0], t[1], t[2], &o);
         if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
         if (AUDIT) {
           double yz[2] = {0.0, 0.0};
           uint32_t eidx = 0;
-          const int es = exact_event(a.g, vwv, gwt, ev, &eidx, yz);
+          const int es = exact_event(a.g, vpos, a.cs, gwt, ev, &eidx, yz);
           bad = (st == 0 && es) || (st == 1 && (!es || eidx != idx));
           if (st == 1 && es) dev = __builtin_fmax(__builtin_fabs(o.ry - yz[0]), __builtin_fabs(o.rz - yz[1]));
         }
@@ -874,8 +878,10 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     for (int32_t k = 0; k < nsteps; ++k) {
       bool pass = false;
       if (om < nrot) {
-        const double *vv = vwv + (int64_t)om * GRS_VW_DOUBLES;
-        double c = vv[0], s = vv[1], u0 = vv[2], u1 = vv[3], u2 = vv[4];
+        const double c = a.cs[2 * om], s = a.cs[2 * om + 1];
+        double swv[3], uv[3];
+        beam_terms(g, Pw, c, s, swv, uv);
+        const double u0 = uv[0], u1 = uv[1], u2 = uv[2];
         const double *t = gwt + h * GRS_GWT;
         double Gt0 = c * t[0] - s * t[1];
         double Gt1 = s * t[0] + c * t[1];
@@ -1081,14 +1087,10 @@ hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t stride,
   return hipGetLastError();
 }
 
-hipError_t grs_launch_vw_table(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs, int32_t nrot,
-                               double *d_vw, double *d_bt, double *d_vb, hipStream_t st) {
-  int64_t n = V * nrot;
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vw_table, dim3(blocks_for(n, 128)), dim3(128), 0, st, d_g, d_pos, V, d_cs, nrot, d_vw, d_bt);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !d_vb) return e;
-  hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 64)), dim3(64), 0, st, d_g, d_vw, V, nrot, d_vb);
+hipError_t grs_launch_voxel_bounds(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs,
+                                   int32_t nrot, double *d_vb, hipStream_t st) {
+  if (V <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 64)), dim3(64), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
   return hipGetLastError();
 }
 
@@ -1114,7 +1116,7 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
 }
 
 static size_t fast_lds(int32_t H) {
-  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H)) * sizeof(double);
+  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4) * sizeof(double);
 }
 
 /* Resident blocks per CU of one k_fast instantiation (cached). */
