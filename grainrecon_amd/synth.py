@@ -153,6 +153,39 @@ def fe_config(scale: str = "cfg2", seed: int = 1002) -> Config:
     return Config(scale, geo, cubic_hkl("I", 6), np.arange(0.0, 361.0, 2.0), sample)
 
 
+def al_config(seed: int = 1005) -> Config:
+    """The FCC Al configuration: a = 4.0495 A, families {111} {200} {220}
+    {311} {222} {400} (64 reflections), 361 omega steps of 1 degree, 2048^2
+    detector at 3.36 um, Lsam2sou = Lsam2det = 14 mm, 10-70 keV; a 400 um
+    cylinder at 2.5 um voxels, 310 um high (about 2.5 M voxels), 1000
+    Voronoi grains."""
+    geo = labdct_geometry(a=4.0495, lattice="F", det_pixels=2048, pixel_mm=3.36e-3)
+    sample = voronoi_cylinder(400.0, 310.0, 2.5, 1000, seed)
+    return Config("cfg5", geo, cubic_hkl("F", 6), np.arange(0.0, 361.0, 1.0), sample)
+
+
+def subvolumes(sample: Sample, fractions) -> np.ndarray:
+    """Splits the voxels into z slabs holding the given fractions of the
+    voxels (uneven subvolumes); returns the slab index of every voxel."""
+    order = np.argsort(sample.index[:, 2], kind="stable")
+    bounds = np.round(np.cumsum(np.asarray(fractions, float) / np.sum(fractions)) * order.size).astype(np.int64)
+    slab = np.empty(order.size, np.int32)
+    start = 0
+    for k, b in enumerate(bounds):
+        slab[order[start:b]] = k
+        start = b
+    return slab
+
+
+def grain_id_volume(sample: Sample) -> np.ndarray:
+    """DS.GrainId-style volume: 1-based grain id inside the sample, 0 outside,
+    indexed (x, y, z) by the 1-based voxel indices of the sample."""
+    vol = np.zeros(sample.dims, np.int32)
+    i = sample.index
+    vol[i[:, 0] - 1, i[:, 1] - 1, i[:, 2] - 1] = sample.grain + 1
+    return vol
+
+
 def load_on_engine(eng, cfg: Config) -> None:
     """Geometry + forward-simulated spots + GPU distance transform on `eng`."""
     eng.set_geometry(cfg.geo, cfg.hkl, cfg.rot)
