@@ -51,6 +51,13 @@ struct grs_ctx {
   bool timing = false;
   bool audit = false;
   DevBuf d_audit;
+  /* HIP graph of the batched-compete pipeline of one chunk, replayed while
+   * its shape and buffers are unchanged */
+  bool graphs = true;
+  hipGraphExec_t g_exec = nullptr;
+  std::vector<uint64_t> g_key;
+  int64_t g_captures = 0, g_replays = 0;
+  DevBuf d_bestidx, d_bestq;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
   int64_t launches = 0;
@@ -170,7 +177,10 @@ void grs_destroy(grs_ctx *c) {
                     &c->d_bad, &c->d_x, &c->d_y, &c->d_mask, &c->d_g1, &c->d_edt_v, &c->d_edt_z,
                     &c->d_grain, &c->d_bits, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest,
                     &c->d_elist, &c->f_hkl, &c->f_om, &c->f_meas, &c->f_grain, &c->f_goff, &c->f_gr6,
-                    &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out};
+                    &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out, &c->c_cen, &c->c_grid,
+                    &c->c_gn, &c->c_out, &c->c_n, &c->c_gid, &c->c_vox, &c->c_ids, &c->c_nout, &c->d_audit,
+                    &c->d_bestidx, &c->d_bestq};
+  if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (DevBuf *b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -666,11 +676,7 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
   const int64_t S = (int64_t)(nr + 1) * (nr + 1) * (nr + 1);
   const int64_t chunk = voxel_chunk(c);
   const int64_t stride = (int64_t)c->nrot * c->H;
-  std::vector<int64_t> rp, eo, el;
-  std::vector<unsigned long long> rb;
-  std::vector<int32_t> hns, hni;
-  std::vector<double> hC;
-  std::vector<float> hmed;
+  std::vector<int64_t> rp, eo;
   for (int64_t r0 = 0; r0 < M; r0 += chunk) {
     const int64_t m = M - r0 < chunk ? M - r0 : chunk;
     const int64_t c0 = row_ptr[r0], kc = row_ptr[r0 + m] - c0;
@@ -696,10 +702,13 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     HIPCHK(c, hipMemcpyAsync(c->d_roweoff.p, eo.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, c->st));
     if (kc > 0)
       HIPCHK(c, hipMemcpyAsync(c->d_candq.p, cand_q + 4 * c0, sizeof(double) * 4 * kc, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemsetAsync(c->d_rowbest.p, 0, sizeof(unsigned long long) * m, c->st));
     ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)m);
-    HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
-                                      P<double>(c->d_vb), c->st));
+    ENSURE(c, c->d_bestidx, sizeof(int64_t) * m);
+    ENSURE(c, c->d_bestq, sizeof(double) * 4 * m);
+    int64_t waves = grs_fast_wave_slots(c->device);
+    const int64_t w1 = waves > E ? ((E + 3) / 4) * 4 : waves;
+    const int64_t w2 = waves > m ? ((m + 3) / 4) * 4 : waves;
+    ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)(w2 > 0 ? w2 : 1) * stride);
     grs_eval_args a;
     std::memset(&a, 0, sizeof(a));
     a.g = P<grs_dgeo>(c->d_geo);
@@ -710,6 +719,7 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     a.keys = P<uint32_t>(c->d_keys);
     a.hit_bits = P<uint32_t>(c->d_bits);
     a.scratch_stride = stride;
+    a.scratch = P<uint32_t>(c->d_scratch);
     a.csr.row_ptr = P<int64_t>(c->d_rowptr);
     a.csr.row_eoff = P<int64_t>(c->d_roweoff);
     a.csr.cand_q = P<double>(c->d_candq);
@@ -718,87 +728,92 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     a.csr.S = (int32_t)S;
     a.csr.gridsize = gridsize;
     a.csr.row_best = P<unsigned long long>(c->d_rowbest);
-    int64_t waves = grs_fast_wave_slots(c->device);
-    if (E > 0) {
-      a.n_eval = E;
-      a.map = GRS_MAP_CSR;
-      int64_t w = waves > E ? ((E + 3) / 4) * 4 : waves;
-      if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
-      HIPCHK(c, grs_launch_fast_counts(&a, c->H, w, c->st));
-      if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
-      if (c->timing) {
-        HIPCHK(c, hipEventSynchronize(c->ev1));
-        float ms = 0.0f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        c->ms_total += ms;
-        c->launches += 1;
-      }
-    }
-    rb.resize(m);
-    HIPCHK(c, hipMemcpyAsync(rb.data(), c->d_rowbest.p, sizeof(unsigned long long) * m, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    el.resize(m);
-    const unsigned long long jmask = (1ull << GRS_KEY_JBITS) - 1;
-    for (int64_t r = 0; r < m; r++) {
-      if (eo[r + 1] == eo[r]) {
-        el[r] = -1;
-        continue;
-      }
-      int64_t j = (int64_t)(jmask - (rb[r] & jmask));
-      el[r] = eo[r] + j;
-    }
-    /* winners re-scored with the median */
-    HIPCHK(c, hipMemcpyAsync(c->d_elist.p, el.data(), sizeof(int64_t) * m, hipMemcpyHostToDevice, c->st));
     a.csr.elist = P<int64_t>(c->d_elist);
-    a.n_eval = m;
-    a.map = GRS_MAP_CSR_LIST;
     a.out_ns = P<int32_t>(c->d_ns);
     a.out_ni = P<int32_t>(c->d_ni);
     a.out_C = P<double>(c->d_C);
     a.out_med = P<float>(c->d_med);
-    int64_t w2 = waves > m ? ((m + 3) / 4) * 4 : waves;
-    ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)w2 * stride);
-    a.scratch = P<uint32_t>(c->d_scratch);
-    HIPCHK(c, grs_launch_fast(&a, c->H, w2, c->st));
-    hns.resize(m);
-    hni.resize(m);
-    hC.resize(m);
-    hmed.resize(m);
-    HIPCHK(c, hipMemcpyAsync(hns.data(), c->d_ns.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(hni.data(), c->d_ni.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(hC.data(), c->d_C.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(hmed.data(), c->d_med.p, sizeof(float) * m, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    const double half = (gridsize * (double)nr) / 2.0;
-    for (int64_t r = 0; r < m; r++) {
-      int64_t gr = r0 + r;
-      double *bq = best_q + 4 * gr;
-      if (el[r] < 0) {
-        best[gr] = -1;
-        bq[0] = bq[1] = bq[2] = bq[3] = 0.0;
-        ns[gr] = ni[gr] = 0;
-        C[gr] = 0.0;
-        med[gr] = 0.0f;
-        continue;
+    /* the whole chunk on the device: clear keys, bounds, counts-only scoring
+     * of every expanded candidate, winners, median re-scoring of the winners,
+     * winner index and quaternion */
+    auto enqueue = [&]() -> hipError_t {
+      hipError_t e;
+      if ((e = grs_launch_fill_u64(P<unsigned long long>(c->d_rowbest), m, 0ull, c->st)) != hipSuccess) return e;
+      if ((e = grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
+                                       P<double>(c->d_vb), c->st)) != hipSuccess)
+        return e;
+      if (E > 0) {
+        grs_eval_args a1 = a;
+        a1.n_eval = E;
+        a1.map = GRS_MAP_CSR;
+        if ((e = grs_launch_fast_counts(&a1, c->H, w1, c->st)) != hipSuccess) return e;
       }
-      int64_t j = el[r] - eo[r];
-      int64_t cand = row_ptr[gr] + j / S;
-      int64_t smp = j % S, n1 = nr + 1;
-      int64_t k[3] = {smp / (n1 * n1), (smp / n1) % n1, smp % n1};
-      const double *q = cand_q + 4 * cand;
-      double v[3];
-      for (int t = 0; t < 3; t++) v[t] = (q[1 + t] - half) + (double)k[t] * gridsize;
-      double ss = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
-      bq[0] = ss <= 1.0 ? std::sqrt(1.0 - ss) : 0.0;
-      bq[1] = v[0];
-      bq[2] = v[1];
-      bq[3] = v[2];
-      best[gr] = j;
-      ns[gr] = hns[r];
-      ni[gr] = hni[r];
-      C[gr] = hC[r];
-      med[gr] = hmed[r];
+      if ((e = grs_launch_winner_list(P<unsigned long long>(c->d_rowbest), P<int64_t>(c->d_roweoff), m,
+                                      P<int64_t>(c->d_elist), c->st)) != hipSuccess)
+        return e;
+      grs_eval_args a2 = a;
+      a2.n_eval = m;
+      a2.map = GRS_MAP_CSR_LIST;
+      if ((e = grs_launch_fast(&a2, c->H, w2, c->st)) != hipSuccess) return e;
+      return grs_launch_winner_out(P<int64_t>(c->d_elist), P<int64_t>(c->d_roweoff), P<int64_t>(c->d_rowptr),
+                                   P<double>(c->d_candq), nr, gridsize, m, P<int64_t>(c->d_bestidx),
+                                   P<double>(c->d_bestq), c->st);
+    };
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    if (c->graphs) {
+      double gs = gridsize;
+      uint64_t gbits;
+      std::memcpy(&gbits, &gs, sizeof(gbits));
+      std::vector<uint64_t> key = {(uint64_t)m, (uint64_t)E, (uint64_t)S, (uint64_t)nr, gbits, (uint64_t)c->H,
+                                   (uint64_t)c->nrot, (uint64_t)w1, (uint64_t)w2, (uint64_t)stride};
+      for (const DevBuf *bp : {&c->d_geo, &c->d_hkl, &c->d_cs, &c->d_vb, &c->d_pos, &c->d_keys, &c->d_bits,
+                               &c->d_scratch, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest, &c->d_elist,
+                               &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_bestidx, &c->d_bestq})
+        key.push_back((uint64_t)(uintptr_t)bp->p);
+      if (!c->g_exec || key != c->g_key) {
+        if (c->g_exec) {
+          HIPCHK(c, hipGraphExecDestroy(c->g_exec));
+          c->g_exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue();
+        hipError_t e2 = hipStreamEndCapture(c->st, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+          if (graph) hipGraphDestroy(graph);
+          return fail(c, GRS_ERR_DEVICE, "graph capture failed: %s",
+                      hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        e = hipGraphInstantiate(&c->g_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          c->g_exec = nullptr;
+          return fail(c, GRS_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        c->g_key = key;
+        c->g_captures += 1;
+      } else {
+        c->g_replays += 1;
+      }
+      HIPCHK(c, hipGraphLaunch(c->g_exec, c->st));
+    } else {
+      HIPCHK(c, enqueue());
     }
+    if (c->timing) {
+      HIPCHK(c, hipEventRecord(c->ev1, c->st));
+      HIPCHK(c, hipEventSynchronize(c->ev1));
+      float ms = 0.0f;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->ms_total += ms;
+      c->launches += 1;
+    }
+    HIPCHK(c, hipMemcpyAsync(best + r0, c->d_bestidx.p, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(best_q + 4 * r0, c->d_bestq.p, sizeof(double) * 4 * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ns + r0, c->d_ns.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ni + r0, c->d_ni.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(C + r0, c->d_C.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(med + r0, c->d_med.p, sizeof(float) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
   }
   return GRS_OK;
 }
@@ -1549,6 +1564,19 @@ grs_status grs_copy_to_host(grs_ctx *c, void *dst, const void *d_src, int64_t by
 grs_status grs_set_timing(grs_ctx *c, int32_t enable) {
   if (!c) return GRS_ERR_ARG;
   c->timing = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_set_graphs(grs_ctx *c, int32_t enable) {
+  if (!c) return GRS_ERR_ARG;
+  c->graphs = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_graph_stats(grs_ctx *c, int64_t *captures, int64_t *replays) {
+  if (!c || !captures || !replays) return GRS_ERR_ARG;
+  *captures = c->g_captures;
+  *replays = c->g_replays;
   return GRS_OK;
 }
 

@@ -88,6 +88,12 @@ hipError_t grs_launch_omega_cs(const double *d_rot, int32_t nrot, double *d_cs, 
 hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
 hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
 hipError_t grs_launch_fast_audit(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
+hipError_t grs_launch_fill_u64(unsigned long long *d_p, int64_t n, unsigned long long v, hipStream_t st);
+hipError_t grs_launch_winner_list(const unsigned long long *d_row_best, const int64_t *d_row_eoff, int64_t m,
+                                  int64_t *d_elist, hipStream_t st);
+hipError_t grs_launch_winner_out(const int64_t *d_elist, const int64_t *d_row_eoff, const int64_t *d_row_ptr,
+                                 const double *d_cand_q, int32_t nr, double gridsize, int64_t m, int64_t *d_best,
+                                 double *d_best_q, hipStream_t st);
 hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,
                                 uint32_t *d_bits, hipStream_t st);
 hipError_t grs_launch_plain(const grs_eval_args *a, int64_t n_threads, hipStream_t st);

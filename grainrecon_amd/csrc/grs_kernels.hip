@@ -948,6 +948,58 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   }
 }
 
+/* ---------------------------------------------------------------------- */
+/* batched compete: device-side steps between and after the two launches   */
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+/* winner of each CSR row from its packed argmax key: the index of the
+ * first maximum in the expanded list (-1 for an empty row) */
+__global__ void k_winner_list(const unsigned long long *row_best, const int64_t *row_eoff, int64_t m, int64_t *elist) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  if (row_eoff[r + 1] == row_eoff[r]) {
+    elist[r] = -1;
+    return;
+  }
+  const unsigned long long jmask = (1ull << GRS_KEY_JBITS) - 1;
+  elist[r] = row_eoff[r] + (int64_t)(jmask - (row_best[r] & jmask));
+}
+
+/* best index in the row's expanded list and its quaternion (the local
+ * sample of the grid, q0 from the vector part, 0 when |q_vec| > 1) */
+__global__ void k_winner_out(const int64_t *elist, const int64_t *row_eoff, const int64_t *row_ptr,
+                             const double *cand_q, int32_t nr, double gridsize, int64_t m, int64_t *best,
+                             double *best_q) {
+  GRS_NO_CONTRACT
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  double *bq = best_q + 4 * r;
+  if (elist[r] < 0) {
+    best[r] = -1;
+    bq[0] = bq[1] = bq[2] = bq[3] = 0.0;
+    return;
+  }
+  const int64_t n1 = nr + 1, S = n1 * n1 * n1;
+  const int64_t j = elist[r] - row_eoff[r];
+  const int64_t cand = row_ptr[r] + j / S;
+  const int64_t smp = j % S;
+  const int64_t k[3] = {smp / (n1 * n1), (smp / n1) % n1, smp % n1};
+  const double *q = cand_q + 4 * cand;
+  const double half = (gridsize * (double)nr) / 2.0;
+  double v[3];
+  for (int t = 0; t < 3; t++) v[t] = (q[1 + t] - half) + (double)k[t] * gridsize;
+  const double ss = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+  bq[0] = ss <= 1.0 ? __builtin_sqrt(1.0 - ss) : 0.0;
+  bq[1] = v[0];
+  bq[2] = v[1];
+  bq[3] = v[2];
+  best[r] = j;
+}
+
 /* bit i of the hit mask = (keys[i] <= hit_key) */
 __global__ void k_pack_hits(const uint32_t *keys, int64_t n, uint32_t hit_key, int32_t has_hit, uint32_t *bits) {
   int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1228,4 +1280,26 @@ int64_t grs_fast_wave_slots(int device) {
   (void)device;
   /* upper bound over the instantiations: the scratch is indexed by wave */
   return (int64_t)device_cus() * 16 * GRS_WPB;
+}
+
+hipError_t grs_launch_fill_u64(unsigned long long *d_p, int64_t n, unsigned long long v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_winner_list(const unsigned long long *d_row_best, const int64_t *d_row_eoff, int64_t m,
+                                  int64_t *d_elist, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_winner_list, dim3(blocks_for(m, 256)), dim3(256), 0, st, d_row_best, d_row_eoff, m, d_elist);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_winner_out(const int64_t *d_elist, const int64_t *d_row_eoff, const int64_t *d_row_ptr,
+                                 const double *d_cand_q, int32_t nr, double gridsize, int64_t m, int64_t *d_best,
+                                 double *d_best_q, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_winner_out, dim3(blocks_for(m, 256)), dim3(256), 0, st, d_elist, d_row_eoff, d_row_ptr, d_cand_q,
+                     nr, gridsize, m, d_best, d_best_q);
+  return hipGetLastError();
 }

@@ -280,6 +280,16 @@ class Engine:
         self._chk(self._lib.grs_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0))
         return ms.value, n.value
 
+    def set_graphs(self, on: bool = True) -> None:
+        """HIP-graph capture and replay of the batched compete pipeline."""
+        self._chk(self._lib.grs_set_graphs(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        """(captures, replays) of the batched-compete graphs so far."""
+        cap, rep = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._lib.grs_graph_stats(self._h, ctypes.byref(cap), ctypes.byref(rep)))
+        return cap.value, rep.value
+
     def set_audit(self, on: bool = True) -> None:
         """Arms (and clears) the audit of the fast kernel's resolution stage."""
         self._chk(self._lib.grs_set_audit(self._h, 1 if on else 0))

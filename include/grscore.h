@@ -150,7 +150,8 @@ grs_status grs_forward_comp_compete(grs_ctx *ctx, const double *vox, int64_t M, 
  * j in that list (best, -1 for an empty row), its quaternion (best_q), counts,
  * C and median.  The scoring launch counts hits on a bit-packed mask and
  * reduces a packed 64-bit argmax key per row; only the winners are re-scored
- * for the median.  Requires nrot*nrhkl <= 23170.  No reference counterpart:
+ * for the median.  Each chunk of rows runs on the device without host
+ * round trips, as a replayed HIP graph (grs_set_graphs).  Requires nrot*nrhkl <= 23170.  No reference counterpart:
  * the reference scores these lists in serial 30000-voxel MEX calls
  * (GrainGrow_engine_compete_comp.m:36-45) without local sampling. */
 grs_status grs_compete_csr(grs_ctx *ctx, const double *pos, int64_t M, const int64_t *row_ptr, const double *cand_q,
@@ -299,6 +300,14 @@ grs_status grs_set_timing(grs_ctx *ctx, int32_t enable);
 /* Accumulated scoring-kernel time (ms) and launch count since the last reset;
  * reset != 0 clears them after reading. */
 grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, int32_t reset);
+/* HIP graphs for grs_compete_csr (default on): each chunk's device pipeline
+ * (key clearing, bounds, counts-only scoring, winners, median re-scoring,
+ * winner outputs) is captured once and replayed while its shape and device
+ * buffers are unchanged; enable = 0 launches the same kernels directly. */
+grs_status grs_set_graphs(grs_ctx *ctx, int32_t enable);
+/* Graph captures and replays since the context was created. */
+grs_status grs_graph_stats(grs_ctx *ctx, int64_t *captures, int64_t *replays);
+
 /* Audit of the fast kernel's resolution stage (one-voxel calls only, i.e.
  * grs_forward_comp / grs_forward_comp_device with the fast method): every
  * event that the kernel decides without the exact forward_comp.m pipeline is

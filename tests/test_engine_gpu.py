@@ -318,3 +318,34 @@ def test_resolution_stage_agrees_with_exact_pipeline(engine, variant):
     assert au["mismatched"] == 0
     assert au["max_dev_px"] < 1e-6
     assert au["exact"] <= 0.01 * au["examined"]
+
+
+def test_compete_csr_graph_replays_match_direct_launches(engine):
+    """The batched compete runs as a captured HIP graph: replays with new data
+    of the same shape, interleaved with other work, equal direct launches."""
+    s = scenes.make_scene(seed=83, ngrains=3, spread_mm=0.03)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(14)
+    nr, g = 2, 0.002
+    rows = np.array([0, 3, 3, 5, 9, 10, 13, 16])   # fixed shape: 7 rows, 16 candidates (one empty row)
+    datasets = []
+    for _ in range(3):
+        pos = np.array([s.voxels[r % 3][int(rng.integers(0, 27))] for r in range(7)])
+        cands = np.array([s.q[c % 3] + np.concatenate([[0.0], rng.normal(scale=0.002, size=3)]) for c in range(16)])
+        datasets.append((pos, cands))
+    engine.set_graphs(False)
+    try:
+        ref = [engine.compete_csr(p, rows, c, nr, g) for p, c in datasets]
+    finally:
+        engine.set_graphs(True)
+    cap0, rep0 = engine.graph_stats()
+    for rnd in range(3):
+        for k, (p, c) in enumerate(datasets):
+            out = engine.compete_csr(p, rows, c, nr, g)
+            for x, y in zip(out, ref[k]):
+                np.testing.assert_array_equal(x, y)
+            engine.forward_comp(s.voxels[0][0], s.q[:2])   # eager work between replays
+            engine.synchronize()
+    cap1, rep1 = engine.graph_stats()
+    assert rep1 - rep0 >= 8 and cap1 - cap0 <= 1
