@@ -457,6 +457,7 @@ def main():
             log(f"[bench] batched parity on rows {rows}: {ok} ({time.perf_counter() - t:.1f}s)")
     for d in (d_q, d_ns, d_ni, d_C, d_med):
         eng.device_free(d)
+    result["autotune"] = {"median_kernel": eng.tuning(0), "counts_kernel": eng.tuning(1)}
     if args.geo_fit_grains > 0:
         result["geo_fit"] = geofit_leg(eng, cfg, args.geo_fit_grains, 0 if args.no_cpu else args.geo_fit_nm_grains)
     if args.al:

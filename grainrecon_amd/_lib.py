@@ -67,6 +67,8 @@ _SIGS = {
     "grs_kernel_time": (c_i32, [c_vp, c_dp, ctypes.POINTER(c_i64), c_i32]),
     "grs_set_audit": (c_i32, [c_vp, c_i32]),
     "grs_set_graphs": (c_i32, [c_vp, c_i32]),
+    "grs_set_autotune": (c_i32, [c_vp, c_i32]),
+    "grs_get_tuning": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), c_dp]),
     "grs_graph_stats": (c_i32, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     "grs_fit_or_local": (c_i32, [c_vp, c_dp, c_dp, c_i64, ctypes.c_double, c_i32, c_i32, ctypes.c_double,
                                  ctypes.c_double, c_dp, c_dp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -27,6 +28,12 @@ struct DevBuf {
 };
 
 }  // namespace
+
+struct grs_tune_state {
+  bool done = false;
+  int32_t wpb = 4, keys = 0, ncfg = 0;
+  double ms_per_eval[6] = {0, 0, 0, 0, 0, 0};
+};
 
 struct grs_ctx {
   int device = 0;
@@ -54,6 +61,11 @@ struct grs_ctx {
   /* HIP graph of the batched-compete pipeline of one chunk, replayed while
    * its shape and buffers are unchanged */
   bool graphs = true;
+  /* launch autotuning of k_fast per (kind, H, nrot): waves per block and,
+   * for the counts-only kind, the hit-test source (bit mask or keys) */
+  bool autotune = true;
+  std::map<uint64_t, struct grs_tune_state> tunes;
+  std::vector<hipEvent_t> tev;
   hipGraphExec_t g_exec = nullptr;
   std::vector<uint64_t> g_key;
   int64_t g_captures = 0, g_replays = 0;
@@ -181,6 +193,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->c_gn, &c->c_out, &c->c_n, &c->c_gid, &c->c_vox, &c->c_ids, &c->c_nout, &c->d_audit,
                     &c->d_bestidx, &c->d_bestq};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
+  for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (DevBuf *b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -332,6 +345,97 @@ static grs_status check_ready(grs_ctx *c) {
 
 /* Runs the scoring kernel on evaluations already described on the device.
  * d_pos holds V voxel positions; d_sub/d_valid the orientations. */
+/* ---- launch autotuning ------------------------------------------------------ */
+
+/* configurations: waves per block x hit-test source (counts-only kind only) */
+static const int32_t k_cfg_wpb[6] = {4, 2, 1, 4, 2, 1};
+static const int32_t k_cfg_keys[6] = {0, 0, 0, 1, 1, 1};
+
+/* evaluations per tuning part: enough to keep every resident wave busy
+ * (3 waves per SIMD, 4 SIMDs per CU) for several evaluations */
+static int64_t tune_part_min(const grs_ctx *c) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+  return (int64_t)cus * 12 * 8;
+}
+
+static uint64_t tune_key(const grs_ctx *c, int kind) {
+  return (uint64_t)kind | ((uint64_t)c->H << 4) | ((uint64_t)c->nrot << 16);
+}
+
+/* Launches k_fast over evaluations [0, n_eval) of a (kind 0: with median,
+ * kind 1: counts-only CSR).  The first call of a (kind, H, nrot) class with
+ * enough evaluations splits the range into 2*ncfg+1 equal parts: the first
+ * warms the caches with the default configuration, the others run the
+ * configurations in the order 0..ncfg-1, ncfg-1..0 (so that each sees an
+ * early and a late part); every part is timed with HIP events and the
+ * configuration with the least time per evaluation is kept.  Every part computes
+ * its own evaluations, so the results do not depend on the configuration. */
+static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t waves) {
+  grs_tune_state &t = c->tunes[tune_key(c, kind)];
+  const int ncfg = kind == 1 ? 6 : 3;
+  const int64_t E = a.n_eval;
+  auto launch = [&](const grs_eval_args &x, int32_t wpb) -> hipError_t {
+    return kind == 1 ? grs_launch_fast_counts(&x, c->H, waves, wpb, c->st) : grs_launch_fast(&x, c->H, waves, wpb, c->st);
+  };
+  if (!c->autotune || t.done || E < (int64_t)(2 * ncfg + 1) * tune_part_min(c)) {
+    a.count_keys = t.done ? t.keys : 0;
+    HIPCHK(c, launch(a, t.done ? t.wpb : 4));
+    return GRS_OK;
+  }
+  /* part 0 warms caches with the default configuration and is not timed;
+   * parts 1 .. 2*ncfg run the configurations in mirrored order */
+  const int nparts = 2 * ncfg + 1;
+  while ((int)c->tev.size() < nparts + 1) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->tev.push_back(e);
+  }
+  const int64_t part = E / nparts;
+  double ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t cnt[6] = {0, 0, 0, 0, 0, 0};
+  int cfg_of[13];
+  for (int p = 0; p < nparts; p++) {
+    const int q = p - 1;
+    const int k = p == 0 ? 0 : (q < ncfg ? q : 2 * ncfg - 1 - q);
+    cfg_of[p] = k;
+    grs_eval_args x = a;
+    x.e0 = (int64_t)p * part;
+    x.n_eval = p == nparts - 1 ? E : (int64_t)(p + 1) * part;
+    x.count_keys = k_cfg_keys[k];
+    HIPCHK(c, hipEventRecord(c->tev[p], c->st));
+    HIPCHK(c, launch(x, k_cfg_wpb[k]));
+  }
+  HIPCHK(c, hipEventRecord(c->tev[nparts], c->st));
+  HIPCHK(c, hipEventSynchronize(c->tev[nparts]));
+  for (int p = 1; p < nparts; p++) {
+    float f = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->tev[p], c->tev[p + 1]));
+    const int64_t n = (p == nparts - 1 ? E : (int64_t)(p + 1) * part) - (int64_t)p * part;
+    ms[cfg_of[p]] += f;
+    cnt[cfg_of[p]] += n;
+  }
+  int best = 0;
+  for (int k = 0; k < ncfg; k++) {
+    t.ms_per_eval[k] = ms[k] / (double)(cnt[k] > 0 ? cnt[k] : 1);
+    if (t.ms_per_eval[k] < t.ms_per_eval[best]) best = k;
+  }
+  t.wpb = k_cfg_wpb[best];
+  t.keys = k_cfg_keys[best];
+  t.ncfg = ncfg;
+  t.done = true;
+  return GRS_OK;
+}
+
+static bool tune_pending(const grs_ctx *c, int kind, int64_t E, int64_t waves) {
+  if (!c->autotune) return false;
+  auto it = c->tunes.find(tune_key(c, kind));
+  if (it != c->tunes.end() && it->second.done) return false;
+  const int ncfg = kind == 1 ? 6 : 3;
+  (void)waves;
+  return E >= (int64_t)(2 * ncfg + 1) * tune_part_min(c);
+}
+
 static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, const int32_t *d_evox,
                            const int32_t *d_eu, int32_t method, int32_t *d_ns, int32_t *d_ni, double *d_C,
                            float *d_med) {
@@ -368,8 +472,12 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
     const bool audit = c->audit && map == GRS_MAP_ONE_VOXEL;
     if (audit) a.audit = P<unsigned long long>(c->d_audit);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    if (audit) HIPCHK(c, grs_launch_fast_audit(&a, c->H, waves, c->st));
-    else HIPCHK(c, grs_launch_fast(&a, c->H, waves, c->st));
+    if (audit) {
+      HIPCHK(c, grs_launch_fast_audit(&a, c->H, waves, c->st));
+    } else {
+      grs_status ts = launch_tuned(c, a, 0, waves);
+      if (ts != GRS_OK) return ts;
+    }
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
   } else if (method == GRS_METHOD_PLAIN) {
     int64_t T = n_eval < 32768 ? n_eval : 32768;
@@ -736,36 +844,41 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     /* the whole chunk on the device: clear keys, bounds, counts-only scoring
      * of every expanded candidate, winners, median re-scoring of the winners,
      * winner index and quaternion */
-    auto enqueue = [&]() -> hipError_t {
-      hipError_t e;
-      if ((e = grs_launch_fill_u64(P<unsigned long long>(c->d_rowbest), m, 0ull, c->st)) != hipSuccess) return e;
-      if ((e = grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
-                                       P<double>(c->d_vb), c->st)) != hipSuccess)
-        return e;
+    auto enqueue = [&]() -> grs_status {
+      HIPCHK(c, grs_launch_fill_u64(P<unsigned long long>(c->d_rowbest), m, 0ull, c->st));
+      HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), m, P<double>(c->d_cs), c->nrot,
+                                        P<double>(c->d_vb), c->st));
+      grs_status ls;
       if (E > 0) {
         grs_eval_args a1 = a;
         a1.n_eval = E;
         a1.map = GRS_MAP_CSR;
-        if ((e = grs_launch_fast_counts(&a1, c->H, w1, c->st)) != hipSuccess) return e;
+        if ((ls = launch_tuned(c, a1, 1, w1)) != GRS_OK) return ls;
       }
-      if ((e = grs_launch_winner_list(P<unsigned long long>(c->d_rowbest), P<int64_t>(c->d_roweoff), m,
-                                      P<int64_t>(c->d_elist), c->st)) != hipSuccess)
-        return e;
+      HIPCHK(c, grs_launch_winner_list(P<unsigned long long>(c->d_rowbest), P<int64_t>(c->d_roweoff), m,
+                                       P<int64_t>(c->d_elist), c->st));
       grs_eval_args a2 = a;
       a2.n_eval = m;
       a2.map = GRS_MAP_CSR_LIST;
-      if ((e = grs_launch_fast(&a2, c->H, w2, c->st)) != hipSuccess) return e;
-      return grs_launch_winner_out(P<int64_t>(c->d_elist), P<int64_t>(c->d_roweoff), P<int64_t>(c->d_rowptr),
-                                   P<double>(c->d_candq), nr, gridsize, m, P<int64_t>(c->d_bestidx),
-                                   P<double>(c->d_bestq), c->st);
+      if ((ls = launch_tuned(c, a2, 0, w2)) != GRS_OK) return ls;
+      HIPCHK(c, grs_launch_winner_out(P<int64_t>(c->d_elist), P<int64_t>(c->d_roweoff), P<int64_t>(c->d_rowptr),
+                                      P<double>(c->d_candq), nr, gridsize, m, P<int64_t>(c->d_bestidx),
+                                      P<double>(c->d_bestq), c->st));
+      return GRS_OK;
     };
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    if (c->graphs) {
+    /* a chunk that first tunes a launch configuration runs directly: the
+     * tuning times launches, which a captured graph cannot */
+    const bool tuning = tune_pending(c, 1, E, w1) || tune_pending(c, 0, m, w2);
+    if (c->graphs && !tuning) {
       double gs = gridsize;
       uint64_t gbits;
       std::memcpy(&gbits, &gs, sizeof(gbits));
+      const grs_tune_state &t1 = c->tunes[tune_key(c, 1)], &t0 = c->tunes[tune_key(c, 0)];
       std::vector<uint64_t> key = {(uint64_t)m, (uint64_t)E, (uint64_t)S, (uint64_t)nr, gbits, (uint64_t)c->H,
-                                   (uint64_t)c->nrot, (uint64_t)w1, (uint64_t)w2, (uint64_t)stride};
+                                   (uint64_t)c->nrot, (uint64_t)w1, (uint64_t)w2, (uint64_t)stride,
+                                   (uint64_t)(t1.done ? t1.wpb * 2 + t1.keys : 0), (uint64_t)(t0.done ? t0.wpb : 0),
+                                   (uint64_t)c->autotune};
       for (const DevBuf *bp : {&c->d_geo, &c->d_hkl, &c->d_cs, &c->d_vb, &c->d_pos, &c->d_keys, &c->d_bits,
                                &c->d_scratch, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest, &c->d_elist,
                                &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_bestidx, &c->d_bestq})
@@ -777,14 +890,14 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
         }
         hipGraph_t graph = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = enqueue();
+        grs_status es = enqueue();
         hipError_t e2 = hipStreamEndCapture(c->st, &graph);
-        if (e != hipSuccess || e2 != hipSuccess) {
+        if (es != GRS_OK || e2 != hipSuccess) {
           if (graph) hipGraphDestroy(graph);
-          return fail(c, GRS_ERR_DEVICE, "graph capture failed: %s",
-                      hipGetErrorString(e != hipSuccess ? e : e2));
+          if (es != GRS_OK) return es;
+          return fail(c, GRS_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e2));
         }
-        e = hipGraphInstantiate(&c->g_exec, graph, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&c->g_exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (e != hipSuccess) {
           c->g_exec = nullptr;
@@ -797,7 +910,8 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
       }
       HIPCHK(c, hipGraphLaunch(c->g_exec, c->st));
     } else {
-      HIPCHK(c, enqueue());
+      grs_status es = enqueue();
+      if (es != GRS_OK) return es;
     }
     if (c->timing) {
       HIPCHK(c, hipEventRecord(c->ev1, c->st));
@@ -1564,6 +1678,25 @@ grs_status grs_copy_to_host(grs_ctx *c, void *dst, const void *d_src, int64_t by
 grs_status grs_set_timing(grs_ctx *c, int32_t enable) {
   if (!c) return GRS_ERR_ARG;
   c->timing = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_set_autotune(grs_ctx *c, int32_t enable) {
+  if (!c) return GRS_ERR_ARG;
+  c->autotune = enable != 0;
+  return GRS_OK;
+}
+
+grs_status grs_get_tuning(grs_ctx *c, int32_t kind, int32_t *waves_per_block, int32_t *hit_keys, int32_t *n_configs,
+                          double *ms_per_eval) {
+  if (!c || (kind != 0 && kind != 1) || !waves_per_block || !hit_keys || !n_configs || !ms_per_eval)
+    return GRS_ERR_ARG;
+  auto it = c->tunes.find(tune_key(c, kind));
+  if (it == c->tunes.end() || !it->second.done) return fail(c, GRS_ERR_STATE, "not tuned yet");
+  *waves_per_block = it->second.wpb;
+  *hit_keys = it->second.keys;
+  *n_configs = it->second.ncfg;
+  for (int k = 0; k < 6; k++) ms_per_eval[k] = it->second.ms_per_eval[k];
   return GRS_OK;
 }
 

@@ -55,6 +55,8 @@ typedef struct grs_eval_args {
   const uint32_t *hit_bits; /* counts-only kernels: bit i = (keys[i] <= hit_key) */
   grs_csr csr;
   unsigned long long *audit; /* GRS_AUDIT_WORDS counters, audit launches only */
+  int64_t e0;              /* first evaluation of this launch (evaluations e0 .. n_eval-1) */
+  int32_t count_keys;      /* counts-only kernels: 1 = hit test on the keys, 0 = on hit_bits */
 } grs_eval_args;
 
 /* Audit counters of the fast kernel's resolution stage: events seen by phase
@@ -85,8 +87,9 @@ hipError_t grs_launch_sub_from_U(const grs_dgeo *d_g, const double *d_U, int64_t
 hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_euler, int64_t N, double *d_SUB,
                                      int32_t *d_valid, hipStream_t st);
 hipError_t grs_launch_omega_cs(const double *d_rot, int32_t nrot, double *d_cs, hipStream_t st);
-hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
-hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
+/* wpb: waves per block, 1, 2 or 4; n_waves: a multiple of 4 (wave slots of the scratch) */
+hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, int32_t wpb, hipStream_t st);
+hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, int32_t wpb, hipStream_t st);
 hipError_t grs_launch_fast_audit(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st);
 hipError_t grs_launch_fill_u64(unsigned long long *d_p, int64_t n, unsigned long long v, hipStream_t st);
 hipError_t grs_launch_winner_list(const unsigned long long *d_row_best, const int64_t *d_row_eoff, int64_t m,

@@ -36,7 +36,7 @@
 
 #include "grs_internal.h"
 
-#define GRS_WPB 4      /* waves per block of k_fast */
+#define GRS_WPB 4      /* largest number of waves per block of k_fast (the launch chooses 1, 2 or 4) */
 #ifndef GRS_FAST_MIN_WAVES
 #define GRS_FAST_MIN_WAVES 3 /* resident waves per SIMD k_fast is compiled for */
 #endif
@@ -630,8 +630,9 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   if (threadIdx.x == 0) sg = *a.g;
   __syncthreads();
   const grs_dgeo *g = &sg;
-  const int64_t gwave = (int64_t)blockIdx.x * GRS_WPB + wib;
-  const int64_t nwaves = (int64_t)gridDim.x * GRS_WPB;
+  const int wpb = (int)(blockDim.x >> 6);
+  const int64_t gwave = (int64_t)blockIdx.x * wpb + wib;
+  const int64_t nwaves = (int64_t)gridDim.x * wpb;
   uint32_t *wlist = MEDIAN ? a.scratch + gwave * a.scratch_stride : nullptr;
   const int32_t nrot = g->nrot;
   const int64_t nz = g->nz, nzny = g->nz * g->ny;
@@ -644,7 +645,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   /* starting (omega, hkl) of each lane in the flattened event space */
   const int32_t om0 = lane / H, h0 = lane - (lane / H) * H;
 
-  for (int64_t e = gwave; e < a.n_eval; e += nwaves) {
+  for (int64_t e = a.e0 + gwave; e < a.n_eval; e += nwaves) {
     int64_t vi, row, jloc;
     double sub[9];
     int ok = eval_setup<MAP>(a, e, &vi, sub, &row, &jloc);
@@ -768,6 +769,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         uint32_t idx = ring3[(head3 + lane) & (GRS_QCAP - 1)];
         p_sel = true;
         if (MEDIAN) p_key = a.keys[idx];
+        else if (a.count_keys) p_hit = g->has_hit && a.keys[idx] <= g->hit_key;
         else p_hit = (a.hit_bits[idx >> 5] >> (idx & 31u)) & 1u;
       }
       head3 += n;
@@ -1167,24 +1169,25 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
   return hipGetLastError();
 }
 
-static size_t fast_lds(int32_t H) {
-  return (size_t)GRS_WPB * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4) * sizeof(double);
+static size_t fast_lds(int32_t H, int wpb) {
+  return (size_t)wpb * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4) * sizeof(double);
 }
 
-/* Resident blocks per CU of one k_fast instantiation (cached). */
+/* Resident blocks per CU of one k_fast instantiation at wpb waves per block
+ * (cached per wpb). */
 template <bool MEDIAN, int MAP, bool AUDIT>
-static int fast_blocks_per_cu(size_t lds) {
-  static int cached = 0;
-  static size_t cached_lds = 0;
-  if (cached == 0 || cached_lds != lds) {
+static int fast_blocks_per_cu(size_t lds, int wpb) {
+  static int cached[GRS_WPB + 1] = {0};
+  static size_t cached_lds[GRS_WPB + 1] = {0};
+  if (cached[wpb] == 0 || cached_lds[wpb] != lds) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<MEDIAN, MAP, AUDIT>, 64 * GRS_WPB, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast<MEDIAN, MAP, AUDIT>, 64 * wpb, lds) != hipSuccess ||
         nb <= 0)
       nb = 1;
-    cached = nb;
-    cached_lds = lds;
+    cached[wpb] = nb;
+    cached_lds[wpb] = lds;
   }
-  return cached;
+  return cached[wpb];
 }
 
 static int device_cus() {
@@ -1197,39 +1200,40 @@ static int device_cus() {
 
 /* Persistent launch: at most as many waves as can be resident at once. */
 template <bool MEDIAN, int MAP, bool AUDIT = false>
-static hipError_t launch_fast_impl(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
-  size_t lds = fast_lds(H);
-  int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP, AUDIT>(lds);
-  int64_t blocks = (n_waves + GRS_WPB - 1) / GRS_WPB;
+static hipError_t launch_fast_impl(const grs_eval_args *a, int32_t H, int64_t n_waves, int wpb, hipStream_t st) {
+  if (wpb != 1 && wpb != 2 && wpb != 4) return hipErrorInvalidValue;
+  size_t lds = fast_lds(H, wpb);
+  int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP, AUDIT>(lds, wpb);
+  int64_t blocks = n_waves / wpb; /* n_waves is a multiple of 4: no wave beyond the scratch slots */
   if (blocks > resident) blocks = resident;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_fast<MEDIAN, MAP, AUDIT>), dim3((unsigned)blocks), dim3(64 * GRS_WPB), lds, st, *a, H);
+  hipLaunchKernelGGL((k_fast<MEDIAN, MAP, AUDIT>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, *a, H);
   return hipGetLastError();
 }
 
-hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
-  if (a->n_eval <= 0) return hipSuccess;
+hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, int32_t wpb, hipStream_t st) {
+  if (a->n_eval <= a->e0) return hipSuccess;
   switch (a->map) {
-    case GRS_MAP_ONE_VOXEL: return launch_fast_impl<true, GRS_MAP_ONE_VOXEL>(a, H, n_waves, st);
-    case GRS_MAP_ONE_U: return launch_fast_impl<true, GRS_MAP_ONE_U>(a, H, n_waves, st);
-    case GRS_MAP_PAIRS: return launch_fast_impl<true, GRS_MAP_PAIRS>(a, H, n_waves, st);
-    case GRS_MAP_CSR_LIST: return launch_fast_impl<true, GRS_MAP_CSR_LIST>(a, H, n_waves, st);
+    case GRS_MAP_ONE_VOXEL: return launch_fast_impl<true, GRS_MAP_ONE_VOXEL>(a, H, n_waves, wpb, st);
+    case GRS_MAP_ONE_U: return launch_fast_impl<true, GRS_MAP_ONE_U>(a, H, n_waves, wpb, st);
+    case GRS_MAP_PAIRS: return launch_fast_impl<true, GRS_MAP_PAIRS>(a, H, n_waves, wpb, st);
+    case GRS_MAP_CSR_LIST: return launch_fast_impl<true, GRS_MAP_CSR_LIST>(a, H, n_waves, wpb, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
-  if (a->n_eval <= 0) return hipSuccess;
+hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, int32_t wpb, hipStream_t st) {
+  if (a->n_eval <= a->e0) return hipSuccess;
   if (a->map != GRS_MAP_CSR) return hipErrorInvalidValue;
-  return launch_fast_impl<false, GRS_MAP_CSR>(a, H, n_waves, st);
+  return launch_fast_impl<false, GRS_MAP_CSR>(a, H, n_waves, wpb, st);
 }
 
 /* One-voxel evaluation that also runs the exact pipeline on every event of
  * phase B1 and counts disagreements (a->audit). */
 hipError_t grs_launch_fast_audit(const grs_eval_args *a, int32_t H, int64_t n_waves, hipStream_t st) {
-  if (a->n_eval <= 0) return hipSuccess;
+  if (a->n_eval <= a->e0) return hipSuccess;
   if (a->map != GRS_MAP_ONE_VOXEL || !a->audit) return hipErrorInvalidValue;
-  return launch_fast_impl<true, GRS_MAP_ONE_VOXEL, true>(a, H, n_waves, st);
+  return launch_fast_impl<true, GRS_MAP_ONE_VOXEL, true>(a, H, n_waves, GRS_WPB, st);
 }
 
 hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,

@@ -280,6 +280,21 @@ class Engine:
         self._chk(self._lib.grs_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0))
         return ms.value, n.value
 
+    def set_autotune(self, on: bool = True) -> None:
+        """Launch autotuning of the scoring kernel (waves per block, hit-test source)."""
+        self._chk(self._lib.grs_set_autotune(self._h, 1 if on else 0))
+
+    def tuning(self, kind: int):
+        """Chosen configuration of kind 0 (with median) or 1 (counts-only):
+        dict with waves_per_block, hit_keys and ms_per_eval of every candidate;
+        None before the first large enough call."""
+        w, k, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = (ctypes.c_double * 6)()
+        st = self._lib.grs_get_tuning(self._h, int(kind), ctypes.byref(w), ctypes.byref(k), ctypes.byref(n), ms)
+        if st != 0:
+            return None
+        return {"waves_per_block": w.value, "hit_keys": bool(k.value), "ms_per_eval": list(ms)[:n.value]}
+
     def set_graphs(self, on: bool = True) -> None:
         """HIP-graph capture and replay of the batched compete pipeline."""
         self._chk(self._lib.grs_set_graphs(self._h, 1 if on else 0))

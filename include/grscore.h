@@ -300,6 +300,21 @@ grs_status grs_set_timing(grs_ctx *ctx, int32_t enable);
 /* Accumulated scoring-kernel time (ms) and launch count since the last reset;
  * reset != 0 clears them after reading. */
 grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, int32_t reset);
+/* Launch autotuning (default on).  The first call of a workload class --
+ * kind 0, the kernel with the median (grs_forward_comp*, grow, compete,
+ * winners), or kind 1, the counts-only CSR kernel of grs_compete_csr -- for
+ * the current geometry (reflection and rotation counts) that is large enough
+ * splits its evaluations among the configurations (waves per block 4, 2, 1;
+ * for kind 1 also the hit test on the bit mask or on the distance keys),
+ * times them and keeps the fastest.  Results never depend on it. */
+grs_status grs_set_autotune(grs_ctx *ctx, int32_t enable);
+/* The chosen configuration of a kind (GRS_ERR_STATE before tuning) and the
+ * measured milliseconds per evaluation of each of its n_configs candidates
+ * (ms_per_eval holds 6 values; order: wpb 4, 2, 1 on the bit mask, then on
+ * the keys). */
+grs_status grs_get_tuning(grs_ctx *ctx, int32_t kind, int32_t *waves_per_block, int32_t *hit_keys,
+                          int32_t *n_configs, double *ms_per_eval);
+
 /* HIP graphs for grs_compete_csr (default on): each chunk's device pipeline
  * (key clearing, bounds, counts-only scoring, winners, median re-scoring,
  * winner outputs) is captured once and replayed while its shape and device

@@ -349,3 +349,35 @@ def test_compete_csr_graph_replays_match_direct_launches(engine):
             engine.synchronize()
     cap1, rep1 = engine.graph_stats()
     assert rep1 - rep0 >= 8 and cap1 - cap0 <= 1
+
+
+def test_autotuned_launches_give_identical_results(engine):
+    """The first large call of each kernel kind times its launch configurations
+    on parts of the work; the outputs equal those of the untuned launch."""
+    s = scenes.make_scene(seed=85, ngrains=3, spread_mm=0.03, nrot=19)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(21)
+    q = np.vstack([s.q, scenes.random_quats(rng, 180000)])
+    pos = s.voxels[1][13]
+    rows = np.arange(0, 481, 4)                    # 120 rows x 4 candidates x 729 samples
+    cands = np.array([s.q[c % 3] + np.concatenate([[0.0], rng.normal(scale=0.004, size=3)]) for c in range(480)])
+    cpos = np.array([s.voxels[r % 3][int(rng.integers(0, 27))] for r in range(120)])
+    engine.set_autotune(False)
+    try:
+        ref = engine.forward_comp(pos, q)
+        ref_c = engine.compete_csr(cpos, rows, cands, 8, 0.002)
+    finally:
+        engine.set_autotune(True)
+    out = engine.forward_comp(pos, q)
+    out_c = engine.compete_csr(cpos, rows, cands, 8, 0.002)
+    for x, y in zip(out, ref):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(out_c, ref_c):
+        np.testing.assert_array_equal(x, y)
+    t0, t1 = engine.tuning(0), engine.tuning(1)
+    assert t0 is not None and t0["waves_per_block"] in (1, 2, 4) and len(t0["ms_per_eval"]) == 3
+    assert t1 is not None and len(t1["ms_per_eval"]) == 6 and all(v > 0 for v in t1["ms_per_eval"])
+    # tuned launches afterwards, still identical
+    for x, y in zip(engine.compete_csr(cpos, rows, cands, 8, 0.002), ref_c):
+        np.testing.assert_array_equal(x, y)
