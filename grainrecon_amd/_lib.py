@@ -73,6 +73,11 @@ _SIGS = {
     "grs_fit_or_local": (c_i32, [c_vp, c_dp, c_dp, c_i64, ctypes.c_double, c_i32, c_i32, ctypes.c_double,
                                  ctypes.c_double, c_dp, c_dp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
     "grs_local_sampling": (c_i32, [c_vp, c_dp, c_i64, c_i32, ctypes.c_double, c_dp, c_i64, ctypes.POINTER(c_i64)]),
+    "grs_set_orientation_set": (c_i32, [c_vp, c_dp, c_i64]),
+    "grs_index_seeds": (c_i32, [c_vp, c_dp, c_i64, c_i32, c_i32, ctypes.c_double, c_i32, ctypes.c_double, c_i32,
+                                c_i32, ctypes.c_double, ctypes.c_double, c_dp, ctypes.POINTER(c_i32), c_dp,
+                                ctypes.POINTER(c_i32), c_dp]),
+    "grs_fit_orientations": (c_i32, [c_dp, c_i64, c_dp, c_dp, ctypes.POINTER(c_i32)]),
     "grs_find_neighbor_ors": (c_i32, [c_vp, ctypes.POINTER(c_i32), c_i64, c_i64, c_i64, ctypes.POINTER(c_i32), c_i64,
                                       c_i32, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
                                       ctypes.POINTER(c_i64)]),

@@ -19,6 +19,7 @@
 #include "../../include/grscore.h"
 #include "grs_internal.h"
 #include "grs_nm.h"
+#include "grs_orient.h"
 
 namespace {
 
@@ -70,6 +71,16 @@ struct grs_ctx {
   std::vector<uint64_t> g_key;
   int64_t g_captures = 0, g_replays = 0;
   DevBuf d_bestidx, d_bestq;
+  /* resident orientation set of the seed indexing (N x 4 as given) with its
+   * S*U*B matrices for geometry generation o_gen */
+  DevBuf o_q, o_sub, o_valid;
+  int64_t o_n = 0;
+  uint64_t geo_gen = 0, o_gen = 0;
+  /* seed-indexing pipeline buffers and its graphs (per shape and buffers) */
+  DevBuf x_pos, x_vb, x_key1, x_ns1, x_ni1, x_top, x_cen, x_grid, x_gn, x_ls, x_lsn, x_owner, x_Rmax, x_q2, x_sub2,
+      x_valid2, x_key2, x_ns2, x_ni2, x_gcount, x_gkey, x_grow, x_fitkey, x_x0, x_fgroup, x_fns, x_fni, x_sidx, x_sC,
+      x_bad;
+  std::map<std::vector<uint64_t>, hipGraphExec_t> gi_cache;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
   int64_t launches = 0;
@@ -191,8 +202,13 @@ void grs_destroy(grs_ctx *c) {
                     &c->d_elist, &c->f_hkl, &c->f_om, &c->f_meas, &c->f_grain, &c->f_goff, &c->f_gr6,
                     &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out, &c->c_cen, &c->c_grid,
                     &c->c_gn, &c->c_out, &c->c_n, &c->c_gid, &c->c_vox, &c->c_ids, &c->c_nout, &c->d_audit,
-                    &c->d_bestidx, &c->d_bestq};
+                    &c->d_bestidx, &c->d_bestq, &c->o_q, &c->o_sub, &c->o_valid, &c->x_pos, &c->x_vb,
+                    &c->x_key1, &c->x_ns1, &c->x_ni1, &c->x_top, &c->x_cen, &c->x_grid, &c->x_gn, &c->x_ls,
+                    &c->x_lsn, &c->x_owner, &c->x_Rmax, &c->x_q2, &c->x_sub2, &c->x_valid2, &c->x_key2,
+                    &c->x_ns2, &c->x_ni2, &c->x_gcount, &c->x_gkey, &c->x_grow, &c->x_fitkey, &c->x_x0,
+                    &c->x_fgroup, &c->x_fns, &c->x_fni, &c->x_sidx, &c->x_sC, &c->x_bad};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
+  for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (DevBuf *b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -297,6 +313,7 @@ grs_status grs_set_geometry(grs_ctx *c, const grs_geometry *geo, const double *h
   grs_status s = upload_geo(c);
   if (s != GRS_OK) return s;
   c->geo_set = true;
+  c->geo_gen += 1;
   return GRS_OK;
 }
 
@@ -364,7 +381,7 @@ static uint64_t tune_key(const grs_ctx *c, int kind) {
 }
 
 /* Launches k_fast over evaluations [0, n_eval) of a (kind 0: with median,
- * kind 1: counts-only CSR).  The first call of a (kind, H, nrot) class with
+ * kind 1: counts-only CSR, kind 2: counts-only grid).  The first call of a (kind, H, nrot) class with
  * enough evaluations splits the range into 2*ncfg+1 equal parts: the first
  * warms the caches with the default configuration, the others run the
  * configurations in the order 0..ncfg-1, ncfg-1..0 (so that each sees an
@@ -373,10 +390,10 @@ static uint64_t tune_key(const grs_ctx *c, int kind) {
  * its own evaluations, so the results do not depend on the configuration. */
 static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t waves) {
   grs_tune_state &t = c->tunes[tune_key(c, kind)];
-  const int ncfg = kind == 1 ? 6 : 3;
+  const int ncfg = kind >= 1 ? 6 : 3;
   const int64_t E = a.n_eval;
   auto launch = [&](const grs_eval_args &x, int32_t wpb) -> hipError_t {
-    return kind == 1 ? grs_launch_fast_counts(&x, c->H, waves, wpb, c->st) : grs_launch_fast(&x, c->H, waves, wpb, c->st);
+    return kind >= 1 ? grs_launch_fast_counts(&x, c->H, waves, wpb, c->st) : grs_launch_fast(&x, c->H, waves, wpb, c->st);
   };
   if (!c->autotune || t.done || E < (int64_t)(2 * ncfg + 1) * tune_part_min(c)) {
     a.count_keys = t.done ? t.keys : 0;
@@ -431,7 +448,7 @@ static bool tune_pending(const grs_ctx *c, int kind, int64_t E, int64_t waves) {
   if (!c->autotune) return false;
   auto it = c->tunes.find(tune_key(c, kind));
   if (it != c->tunes.end() && it->second.done) return false;
-  const int ncfg = kind == 1 ? 6 : 3;
+  const int ncfg = kind >= 1 ? 6 : 3;
   (void)waves;
   return E >= (int64_t)(2 * ncfg + 1) * tune_part_min(c);
 }
@@ -1226,17 +1243,11 @@ grs_status grs_cuda_forward_comp_compete(const float *vox, int64_t n_vox, const 
 
 /* ---- orientation fit --------------------------------------------------------- */
 
-grs_status grs_fit_or_local(grs_ctx *c, const double *pos, const double *x0, int64_t M, double half_width,
+/* M fits, fit m at pos[3m..] from x0[3m..] (the body of grs_fit_or_local) */
+static grs_status fit_batch(grs_ctx *c, const double *pos, const double *x0, int64_t M, double half_width,
                             int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
                             double *fval, int32_t *n_fun_evals, int32_t *n_iter) {
-  if (!c) return GRS_ERR_ARG;
-  if (M < 0 || (M > 0 && (!pos || !x0 || !x_out || !fval || !n_fun_evals || !n_iter)) || !(half_width > 0.0) ||
-      max_fun_evals < 1 || max_iter < 1 || !(tol_fun >= 0.0) || !(tol_x >= 0.0))
-    return fail(c, GRS_ERR_ARG, "invalid argument");
-  HIPCHK(c, hipSetDevice(c->device));
-  grs_status s = check_ready(c);
-  if (s != GRS_OK) return s;
-  if (M == 0) return GRS_OK;
+  grs_status s = GRS_OK;
   if (M > INT32_MAX / 4) return fail(c, GRS_ERR_ARG, "too many fits");
   std::vector<grs_nm> nm((size_t)M);
   for (int64_t m = 0; m < M; m++) {
@@ -1317,38 +1328,351 @@ grs_status grs_fit_or_local(grs_ctx *c, const double *pos, const double *x0, int
   return GRS_OK;
 }
 
-/* ---- candidate generation ------------------------------------------------- */
-
-/* MATLAB a:d:b, following MathWorks' published description of the colon
- * operator: n = round((b-a)/d) intervals, one fewer when a+n*d overshoots b
- * by more than tol = 2*eps*max(|a|,|b|); the end point snapped to b within
- * tol; values a+k*d from the left, last-k*d from the right, (a+last)/2 in the
- * middle of an even count.  Returns the count, -1 if above cap or not finite. */
-static int64_t matlab_colon(double a, double d, double b, double *v, int64_t cap) {
-  if (!std::isfinite(a) || !std::isfinite(d) || !std::isfinite(b)) return -1;
-  if (d == 0.0 || (a < b && d < 0.0) || (b < a && d > 0.0)) return 0;
-  const double tol = 2.0 * 2.220446049250313e-16 * std::max(std::fabs(a), std::fabs(b));
-  const double sig = d > 0.0 ? 1.0 : -1.0;
-  double n;
-  if (a == std::floor(a) && d == 1.0) {
-    n = std::floor(b) - a;
-  } else if (a == std::floor(a) && d == std::floor(d)) {
-    const double q = std::floor(a / d);
-    const double r = a - q * d;
-    n = std::floor((b - r) / d) - q;
-  } else {
-    n = std::round((b - a) / d);
-    if (sig * ((a + n * d) - b) > tol) n = n - 1.0;
-  }
-  if (!(n + 1.0 <= (double)cap)) return -1;
-  double last = a + n * d;
-  if (sig * (last - b) > -tol) last = b;
-  const int64_t N = (int64_t)n;
-  for (int64_t k = 0; k <= N / 2; k++) v[k] = a + (double)k * d;
-  for (int64_t k = 0; k <= N / 2; k++) v[N - k] = last - (double)k * d;
-  if (N % 2 == 0) v[N / 2] = (a + last) / 2.0;
-  return N + 1;
+grs_status grs_fit_or_local(grs_ctx *c, const double *pos, const double *x0, int64_t M, double half_width,
+                            int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
+                            double *fval, int32_t *n_fun_evals, int32_t *n_iter) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || (M > 0 && (!pos || !x0 || !x_out || !fval || !n_fun_evals || !n_iter)) || !(half_width > 0.0) ||
+      max_fun_evals < 1 || max_iter < 1 || !(tol_fun >= 0.0) || !(tol_x >= 0.0))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (M == 0) return GRS_OK;
+  return fit_batch(c, pos, x0, M, half_width, max_fun_evals, max_iter, tol_fun, tol_x, x_out, fval, n_fun_evals,
+                   n_iter);
 }
+
+/* ---- seed indexing ---------------------------------------------------------- */
+
+grs_status grs_fit_orientations(const double *x, int64_t M, double *out10, double *U, int32_t *ok) {
+  if (M < 0 || (M > 0 && (!x || !out10 || !U || !ok))) return GRS_ERR_ARG;
+  for (int64_t m = 0; m < M; m++) ok[m] = grs_fit_orientation(x + 3 * m, out10 + 10 * m, U + 9 * m);
+  return GRS_OK;
+}
+
+grs_status grs_set_orientation_set(grs_ctx *c, const double *q, int64_t N) {
+  if (!c) return GRS_ERR_ARG;
+  if (N < 1 || N > 0x7FFFFFFFll || !q) return fail(c, GRS_ERR_ARG, "need 1 .. 2^31-1 orientations");
+  HIPCHK(c, hipSetDevice(c->device));
+  ENSURE(c, c->o_q, sizeof(double) * 4 * (size_t)N);
+  ENSURE(c, c->o_sub, sizeof(double) * 9 * (size_t)N);
+  ENSURE(c, c->o_valid, sizeof(int32_t) * (size_t)N);
+  HIPCHK(c, hipMemcpyAsync(c->o_q.p, q, sizeof(double) * 4 * (size_t)N, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->o_n = N;
+  c->o_gen = 0; /* S*U*B computed on first use under the current geometry */
+  return GRS_OK;
+}
+
+/* seed positions per device pass: the pipeline keeps per seed the grid
+ * scores of the orientation set and Nr_seed*cap local rows */
+static int64_t index_chunk(int64_t N, int64_t K, int64_t cap) {
+  const int64_t Rcap = K * cap;
+  const int64_t per = N * 16 + K * (48 + 24 * GRS_LS_MAX_GRID + 32 * cap + 64) + Rcap * 132;
+  const int64_t budget = (int64_t)6 << 30;
+  int64_t m = budget / (per > 0 ? per : 1);
+  return m < 1 ? 1 : m;
+}
+
+grs_status grs_index_seeds(grs_ctx *c, const double *pos, int64_t n_pos, int32_t nr_seed, int32_t nr, double gridsize,
+                           int32_t n_fit, double half_width, int32_t max_fun_evals, int32_t max_iter, double tol_fun,
+                           double tol_x, double *indexing, int32_t *candidate, double *fit_info, int32_t *seed_idx,
+                           double *seed_C) {
+  if (!c) return GRS_ERR_ARG;
+  if (n_pos < 0 || (n_pos > 0 && (!pos || !indexing || !candidate)) || nr_seed < 1 || nr < 0 || nr > 11 ||
+      !(gridsize > 0.0) || !std::isfinite(gridsize) || n_fit < 1 || !(half_width > 0.0) || max_fun_evals < 1 ||
+      max_iter < 1 || !(tol_fun >= 0.0) || !(tol_x >= 0.0))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (c->o_n < 1) return fail(c, GRS_ERR_STATE, "orientation set not set");
+  if ((int64_t)c->nrot * c->H > GRS_KEY_MAX_EVENTS)
+    return fail(c, GRS_ERR_ARG, "nrot*nrhkl = %lld exceeds %d events, the exact range of the packed sort key",
+                (long long)c->nrot * c->H, GRS_KEY_MAX_EVENTS);
+  if (c->H > 240) return fail(c, GRS_ERR_ARG, "at most 240 reflections");
+  const int64_t N = c->o_n;
+  const int64_t K = nr_seed < N ? nr_seed : N;
+  if (K > GRS_TOPK_MAX) return fail(c, GRS_ERR_ARG, "Nr_seed above %d", GRS_TOPK_MAX);
+  const int32_t nfit = n_fit < K ? n_fit : (int32_t)K;
+  const int32_t cap_axis = nr + 1;
+  const int64_t cap = (int64_t)cap_axis * cap_axis * cap_axis + 1;
+  const int64_t Rcap = K * cap;
+  if (n_pos == 0) return GRS_OK;
+  if (c->o_gen != c->geo_gen) {
+    HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), P<double>(c->o_q), N, P<double>(c->o_sub),
+                                    P<int32_t>(c->o_valid), c->st));
+    c->o_gen = c->geo_gen;
+  }
+  s = ensure_bits(c);
+  if (s != GRS_OK) return s;
+  const int64_t stride = (int64_t)c->nrot * c->H;
+  const int64_t chunk = index_chunk(N, K, cap);
+  for (int64_t p0 = 0; p0 < n_pos; p0 += chunk) {
+    const int64_t m = n_pos - p0 < chunk ? n_pos - p0 : chunk;
+    const int64_t E1 = m * N, E2 = m * Rcap;
+    ENSURE(c, c->x_pos, sizeof(double) * 3 * m);
+    ENSURE(c, c->x_vb, sizeof(double) * 2 * m);
+    ENSURE(c, c->x_key1, sizeof(unsigned long long) * E1);
+    ENSURE(c, c->x_ns1, sizeof(int32_t) * E1);
+    ENSURE(c, c->x_ni1, sizeof(int32_t) * E1);
+    ENSURE(c, c->x_top, sizeof(unsigned long long) * m * K);
+    ENSURE(c, c->x_cen, sizeof(double) * 4 * m * K);
+    ENSURE(c, c->x_grid, sizeof(double) * 3 * GRS_LS_MAX_GRID * m * K);
+    ENSURE(c, c->x_gn, sizeof(int32_t) * 3 * m * K);
+    ENSURE(c, c->x_ls, sizeof(double) * 4 * m * K * cap);
+    ENSURE(c, c->x_lsn, sizeof(int64_t) * m * K);
+    ENSURE(c, c->x_owner, sizeof(int32_t) * E2);
+    ENSURE(c, c->x_Rmax, sizeof(unsigned long long) * m);
+    ENSURE(c, c->x_q2, sizeof(double) * 4 * E2);
+    ENSURE(c, c->x_sub2, sizeof(double) * 9 * E2);
+    ENSURE(c, c->x_valid2, sizeof(int32_t) * E2);
+    ENSURE(c, c->x_key2, sizeof(unsigned long long) * E2);
+    ENSURE(c, c->x_ns2, sizeof(int32_t) * E2);
+    ENSURE(c, c->x_ni2, sizeof(int32_t) * E2);
+    ENSURE(c, c->x_gcount, sizeof(uint32_t) * m * K);
+    ENSURE(c, c->x_gkey, sizeof(unsigned long long) * m * K);
+    ENSURE(c, c->x_grow, sizeof(int64_t) * m * K);
+    ENSURE(c, c->x_fitkey, sizeof(unsigned long long) * m * nfit);
+    ENSURE(c, c->x_x0, sizeof(double) * 3 * m * nfit);
+    ENSURE(c, c->x_fgroup, sizeof(int32_t) * m * nfit);
+    ENSURE(c, c->x_fns, sizeof(int32_t) * m * nfit);
+    ENSURE(c, c->x_fni, sizeof(int32_t) * m * nfit);
+    ENSURE(c, c->x_sidx, sizeof(int32_t) * m * K);
+    ENSURE(c, c->x_sC, sizeof(double) * m * K);
+    ENSURE(c, c->x_bad, sizeof(uint32_t));
+    const int64_t slots = grs_fast_wave_slots(c->device);
+    const int64_t w1 = slots > E1 ? ((E1 + 3) / 4) * 4 : slots;
+    const int64_t w2 = slots > E2 ? ((E2 + 3) / 4) * 4 : slots;
+    ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)(w1 > w2 ? w1 : w2) * stride);
+    HIPCHK(c, hipMemcpyAsync(c->x_pos.p, pos + 3 * p0, sizeof(double) * 3 * m, hipMemcpyHostToDevice, c->st));
+    grs_eval_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = P<grs_dgeo>(c->d_geo);
+    a.hkl = P<double>(c->d_hkl);
+    a.cs = P<double>(c->d_cs);
+    a.vb = P<double>(c->x_vb);
+    a.pos = P<double>(c->x_pos);
+    a.keys = P<uint32_t>(c->d_keys);
+    a.hit_bits = P<uint32_t>(c->d_bits);
+    a.scratch_stride = stride;
+    a.scratch = P<uint32_t>(c->d_scratch);
+    a.map = GRS_MAP_GRID;
+    auto enqueue = [&]() -> grs_status {
+      HIPCHK(c, grs_launch_fill_u32(P<uint32_t>(c->x_owner), E2, 0u, c->st));
+      HIPCHK(c, grs_launch_fill_u32(P<uint32_t>(c->x_gcount), m * K, 0u, c->st));
+      HIPCHK(c, grs_launch_fill_u32(P<uint32_t>(c->x_bad), 1, 0u, c->st));
+      HIPCHK(c, grs_launch_fill_u64(P<unsigned long long>(c->x_Rmax), m, 0ull, c->st));
+      HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->x_pos), m, P<double>(c->d_cs), c->nrot,
+                                        P<double>(c->x_vb), c->st));
+      /* ForwardCalc_OR_space_comp_cuda over the orientation set */
+      grs_eval_args a1 = a;
+      a1.SUB = P<double>(c->o_sub);
+      a1.valid = P<int32_t>(c->o_valid);
+      a1.n_eval = E1;
+      a1.n_u = N;
+      a1.u_stride = 0;
+      a1.out_key = P<unsigned long long>(c->x_key1);
+      a1.out_ns = P<int32_t>(c->x_ns1);
+      a1.out_ni = P<int32_t>(c->x_ni1);
+      grs_status ls = launch_tuned(c, a1, 2, w1);
+      if (ls != GRS_OK) return ls;
+      /* sortrows(...,3,'descend'), rows 1:Nr_seed */
+      HIPCHK(c, grs_launch_topk(P<unsigned long long>(c->x_key1), m, N, N, (int32_t)K,
+                                P<unsigned long long>(c->x_top), c->st));
+      HIPCHK(c, grs_launch_seed_out(P<unsigned long long>(c->x_top), P<int32_t>(c->x_ns1), P<int32_t>(c->x_ni1),
+                                    P<int32_t>(c->o_valid), m, K, N, P<int32_t>(c->x_sidx), P<double>(c->x_sC),
+                                    c->st));
+      /* local sampling around every seed, laid out as q_all */
+      HIPCHK(c, grs_launch_gather_centres(P<double>(c->o_q), P<unsigned long long>(c->x_top), m * K,
+                                          P<double>(c->x_cen), c->st));
+      HIPCHK(c, grs_launch_ls_grids(P<double>(c->x_cen), m * K, nr, gridsize, cap_axis, P<double>(c->x_grid),
+                                    P<int32_t>(c->x_gn), P<uint32_t>(c->x_bad), c->st));
+      HIPCHK(c, grs_launch_local_sampling(P<double>(c->x_cen), P<double>(c->x_grid), P<int32_t>(c->x_gn), m * K,
+                                          P<double>(c->x_ls), cap, P<int64_t>(c->x_lsn), P<uint32_t>(c->x_bad),
+                                          c->st));
+      HIPCHK(c, grs_launch_qall(P<double>(c->x_ls), P<int64_t>(c->x_lsn), m, K, cap, P<int32_t>(c->x_owner),
+                                P<unsigned long long>(c->x_Rmax), P<double>(c->x_q2), P<uint32_t>(c->x_gcount),
+                                c->st));
+      HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), P<double>(c->x_q2), E2, P<double>(c->x_sub2),
+                                      P<int32_t>(c->x_valid2), c->st));
+      /* cuda_forward_comp over q_all */
+      grs_eval_args a2 = a;
+      a2.SUB = P<double>(c->x_sub2);
+      a2.valid = P<int32_t>(c->x_valid2);
+      a2.n_eval = E2;
+      a2.n_u = Rcap;
+      a2.u_stride = Rcap;
+      a2.out_key = P<unsigned long long>(c->x_key2);
+      a2.out_ns = P<int32_t>(c->x_ns2);
+      a2.out_ni = P<int32_t>(c->x_ni2);
+      if ((ls = launch_tuned(c, a2, 2, w2)) != GRS_OK) return ls;
+      /* per-seed maxima, sortrows(Output_max,3,'descend'), rows to fit */
+      HIPCHK(c, grs_launch_group_max(P<unsigned long long>(c->x_key2), P<int64_t>(c->x_lsn),
+                                     P<uint32_t>(c->x_gcount), m, K, Rcap, P<unsigned long long>(c->x_gkey),
+                                     P<int64_t>(c->x_grow), c->st));
+      HIPCHK(c, grs_launch_topk(P<unsigned long long>(c->x_gkey), m, K, K, nfit, P<unsigned long long>(c->x_fitkey),
+                                c->st));
+      HIPCHK(c, grs_launch_fit_starts(P<unsigned long long>(c->x_fitkey), P<int64_t>(c->x_grow), P<double>(c->x_q2),
+                                      P<int32_t>(c->x_owner), P<int32_t>(c->x_ns2), P<int32_t>(c->x_ni2), m, K, nfit,
+                                      Rcap, P<double>(c->x_x0), P<int32_t>(c->x_fgroup), P<int32_t>(c->x_fns),
+                                      P<int32_t>(c->x_fni), c->st));
+      return GRS_OK;
+    };
+    const bool tuning = tune_pending(c, 2, E1, w1) || tune_pending(c, 2, E2, w2);
+    if (c->graphs && !tuning) {
+      double gs = gridsize;
+      uint64_t gbits;
+      std::memcpy(&gbits, &gs, sizeof(gbits));
+      const grs_tune_state &t2 = c->tunes[tune_key(c, 2)];
+      std::vector<uint64_t> key = {(uint64_t)m, (uint64_t)N, (uint64_t)K, (uint64_t)nr, gbits, (uint64_t)nfit,
+                                   (uint64_t)c->H, (uint64_t)c->nrot, (uint64_t)w1, (uint64_t)w2, (uint64_t)stride,
+                                   (uint64_t)(t2.done ? t2.wpb * 2 + t2.keys : 0), (uint64_t)c->autotune};
+      for (const DevBuf *bp :
+           {&c->d_geo, &c->d_hkl, &c->d_cs, &c->d_keys, &c->d_bits, &c->d_scratch, &c->o_q, &c->o_sub, &c->o_valid,
+            &c->x_pos, &c->x_vb, &c->x_key1, &c->x_ns1, &c->x_ni1, &c->x_top, &c->x_cen, &c->x_grid, &c->x_gn,
+            &c->x_ls, &c->x_lsn, &c->x_owner, &c->x_Rmax, &c->x_q2, &c->x_sub2, &c->x_valid2, &c->x_key2, &c->x_ns2,
+            &c->x_ni2, &c->x_gcount, &c->x_gkey, &c->x_grow, &c->x_fitkey, &c->x_x0, &c->x_fgroup, &c->x_fns,
+            &c->x_fni, &c->x_sidx, &c->x_sC, &c->x_bad})
+        key.push_back((uint64_t)(uintptr_t)bp->p);
+      auto it = c->gi_cache.find(key);
+      hipGraphExec_t exec = nullptr;
+      if (it == c->gi_cache.end()) {
+        if (c->gi_cache.size() >= 4) {
+          for (auto &kv : c->gi_cache) HIPCHK(c, hipGraphExecDestroy(kv.second));
+          c->gi_cache.clear();
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        grs_status es = enqueue();
+        hipError_t e2 = hipStreamEndCapture(c->st, &graph);
+        if (es != GRS_OK || e2 != hipSuccess) {
+          if (graph) hipGraphDestroy(graph);
+          if (es != GRS_OK) return es;
+          return fail(c, GRS_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e2));
+        }
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c, GRS_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+        c->gi_cache[key] = exec;
+        c->g_captures += 1;
+      } else {
+        exec = it->second;
+        c->g_replays += 1;
+      }
+      HIPCHK(c, hipGraphLaunch(exec, c->st));
+    } else {
+      grs_status es = enqueue();
+      if (es != GRS_OK) return es;
+    }
+    const int64_t F = m * nfit;
+    std::vector<double> x0((size_t)F * 3), fpos((size_t)F * 3);
+    std::vector<int32_t> fgroup((size_t)F), fns((size_t)F), fni((size_t)F);
+    uint32_t bad = 0;
+    HIPCHK(c, hipMemcpyAsync(x0.data(), c->x_x0.p, sizeof(double) * 3 * F, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(fgroup.data(), c->x_fgroup.p, sizeof(int32_t) * F, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&bad, c->x_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+    std::vector<int32_t> sidx;
+    std::vector<double> sC;
+    if (seed_idx || seed_C) {
+      sidx.resize((size_t)m * K);
+      sC.resize((size_t)m * K);
+      HIPCHK(c, hipMemcpyAsync(sidx.data(), c->x_sidx.p, sizeof(int32_t) * m * K, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipMemcpyAsync(sC.data(), c->x_sC.p, sizeof(double) * m * K, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (bad & 1u) return fail(c, GRS_ERR_ARG, "a local-sampling grid point has q1^2+q2^2+q3^2 > 1");
+    if (bad & 2u) return fail(c, GRS_ERR_ARG, "a local-sampling grid has more than nr+1 values");
+    for (int64_t p = 0; p < m; p++) {
+      for (int64_t r = 0; r < nr_seed; r++) {
+        if (seed_idx) seed_idx[(p0 + p) * nr_seed + r] = r < K ? sidx[(size_t)(p * K + r)] : -1;
+        if (seed_C) seed_C[(p0 + p) * nr_seed + r] = r < K ? sC[(size_t)(p * K + r)] : std::nan("");
+      }
+      for (int32_t f = 0; f < nfit; f++)
+        for (int k = 0; k < 3; k++) fpos[(size_t)(p * nfit + f) * 3 + k] = pos[3 * (p0 + p) + k];
+    }
+    /* fminsearchbnd from every start (fit_OR_constrain_with_local_comp_cuda.m:115-126) */
+    std::vector<double> fx((size_t)F * 3), fval((size_t)F);
+    std::vector<int32_t> fnf((size_t)F), fit_((size_t)F);
+    s = fit_batch(c, fpos.data(), x0.data(), F, half_width, max_fun_evals, max_iter, tol_fun, tol_x, fx.data(),
+                  fval.data(), fnf.data(), fit_.data());
+    if (s != GRS_OK) return s;
+    /* Euler angles, Rodrigues vector, quaternion, U (:133-148) and the
+     * verification counts with that U (:151-153) */
+    std::vector<double> o10((size_t)F * 10), U((size_t)F * 9);
+    std::vector<int32_t> ok((size_t)F), evox((size_t)F), eu((size_t)F), vns((size_t)F), vni((size_t)F);
+    std::vector<double> vC((size_t)F);
+    std::vector<float> vmed((size_t)F);
+    for (int64_t f = 0; f < F; f++) {
+      ok[f] = grs_fit_orientation(&fx[(size_t)f * 3], &o10[(size_t)f * 10], &U[(size_t)f * 9]);
+      evox[f] = (int32_t)f;
+      eu[f] = (int32_t)f;
+    }
+    ENSURE(c, c->d_pos, sizeof(double) * 3 * F);
+    ENSURE(c, c->d_U, sizeof(double) * 9 * F);
+    ENSURE(c, c->d_sub, sizeof(double) * 9 * F);
+    ENSURE(c, c->d_valid, sizeof(int32_t) * F);
+    ENSURE(c, c->d_evox, sizeof(int32_t) * F);
+    ENSURE(c, c->d_eu_idx, sizeof(int32_t) * F);
+    ENSURE(c, c->d_ns, sizeof(int32_t) * F);
+    ENSURE(c, c->d_ni, sizeof(int32_t) * F);
+    ENSURE(c, c->d_C, sizeof(double) * F);
+    ENSURE(c, c->d_med, sizeof(float) * F);
+    HIPCHK(c, hipMemcpyAsync(c->d_pos.p, fpos.data(), sizeof(double) * 3 * F, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_U.p, U.data(), sizeof(double) * 9 * F, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_evox.p, evox.data(), sizeof(int32_t) * F, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_eu_idx.p, eu.data(), sizeof(int32_t) * F, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_sub_from_U(P<grs_dgeo>(c->d_geo), P<double>(c->d_U), F, P<double>(c->d_sub),
+                                    P<int32_t>(c->d_valid), c->st));
+    s = run_eval(c, GRS_MAP_PAIRS, F, F, P<int32_t>(c->d_evox), P<int32_t>(c->d_eu_idx), GRS_METHOD_FAST,
+                 P<int32_t>(c->d_ns), P<int32_t>(c->d_ni), P<double>(c->d_C), P<float>(c->d_med));
+    if (s != GRS_OK) return s;
+    HIPCHK(c, hipMemcpyAsync(vns.data(), c->d_ns.p, sizeof(int32_t) * F, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(vni.data(), c->d_ni.p, sizeof(int32_t) * F, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(vmed.data(), c->d_med.p, sizeof(float) * F, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int64_t p = 0; p < m; p++) {
+      /* [~,ind] = max(indexing_result(:,7)) (:161): NaN ignored, first maximum */
+      int32_t best = 0;
+      double bestC = std::nan("");
+      for (int32_t f = 0; f < nfit; f++) {
+        const int64_t i = p * nfit + f;
+        const double Cf = ok[i] ? (double)vni[i] / (double)vns[i] : std::nan("");
+        vC[i] = Cf;
+        if (Cf == Cf && (bestC != bestC || Cf > bestC)) {
+          best = f;
+          bestC = Cf;
+        }
+      }
+      const int64_t i = p * nfit + best;
+      double *row = indexing + (p0 + p) * 18;
+      row[0] = (double)(p0 + p + 1);
+      for (int k = 0; k < 3; k++) row[1 + k] = pos[3 * (p0 + p) + k];
+      if (ok[i]) {
+        row[4] = (double)vns[i];
+        row[5] = (double)vni[i];
+        row[6] = vC[i];
+        row[7] = (double)vmed[i];
+      } else {
+        row[4] = row[5] = row[6] = row[7] = std::nan("");
+      }
+      for (int k = 0; k < 10; k++) row[8 + k] = o10[(size_t)i * 10 + k];
+      candidate[p0 + p] = fgroup[i];
+      if (fit_info) {
+        double *fi = fit_info + (p0 + p) * 9;
+        for (int k = 0; k < 3; k++) fi[k] = x0[(size_t)i * 3 + k];
+        for (int k = 0; k < 3; k++) fi[3 + k] = fx[(size_t)i * 3 + k];
+        fi[6] = fval[i];
+        fi[7] = (double)fnf[i];
+        fi[8] = (double)fit_[i];
+      }
+    }
+  }
+  return GRS_OK;
+}
+
+/* ---- candidate generation ------------------------------------------------- */
 
 grs_status grs_local_sampling(grs_ctx *c, const double *centres, int64_t K, int32_t nr, double gridsize,
                               double *out_q, int64_t cap, int64_t *out_n) {
@@ -1364,7 +1688,7 @@ grs_status grs_local_sampling(grs_ctx *c, const double *centres, int64_t K, int3
     int64_t rows = 1;
     for (int j = 0; j < 3; j++) {
       const double cj = centres[4 * k + 1 + j];
-      const int64_t n = matlab_colon(cj - half, gridsize, cj + half, &grids[(size_t)(3 * k + j) * GRS_LS_MAX_GRID],
+      const int64_t n = grs_colon(cj - half, gridsize, cj + half, &grids[(size_t)(3 * k + j) * GRS_LS_MAX_GRID],
                                      GRS_LS_MAX_GRID);
       if (n < 0) return fail(c, GRS_ERR_ARG, "centre %lld: grid of more than %d values", (long long)k, GRS_LS_MAX_GRID);
       gn[3 * k + j] = (int32_t)n;
@@ -1689,7 +2013,7 @@ grs_status grs_set_autotune(grs_ctx *c, int32_t enable) {
 
 grs_status grs_get_tuning(grs_ctx *c, int32_t kind, int32_t *waves_per_block, int32_t *hit_keys, int32_t *n_configs,
                           double *ms_per_eval) {
-  if (!c || (kind != 0 && kind != 1) || !waves_per_block || !hit_keys || !n_configs || !ms_per_eval)
+  if (!c || kind < 0 || kind > 2 || !waves_per_block || !hit_keys || !n_configs || !ms_per_eval)
     return GRS_ERR_ARG;
   auto it = c->tunes.find(tune_key(c, kind));
   if (it == c->tunes.end() || !it->second.done) return fail(c, GRS_ERR_STATE, "not tuned yet");

@@ -254,4 +254,37 @@ GRS_HD int grs_event_finish(const grs_dgeo *g, const grs_vw *v, const double *Gt
   return grs_event_finish_yz(g, v, Gt, Glen, beta, dety_out, detz_out, (double *)0);
 }
 
+/* MATLAB a:d:b, following MathWorks' published description of the colon
+ * operator: n = round((b-a)/d) intervals, one fewer when a+n*d overshoots b
+ * by more than tol = 2*eps*max(|a|,|b|); the end point snapped to b within
+ * tol; values a+k*d from the left, last-k*d from the right, (a+last)/2 in the
+ * middle of an even count.  Returns the count, -1 if above cap or not finite. */
+GRS_HD int64_t grs_colon(double a, double d, double b, double *v, int64_t cap) {
+  GRS_NO_CONTRACT
+  if (!(a - a == 0.0) || !(d - d == 0.0) || !(b - b == 0.0)) return -1; /* NaN or infinite */
+  if (d == 0.0 || (a < b && d < 0.0) || (b < a && d > 0.0)) return 0;
+  const double fa = grs_fabs(a), fb = grs_fabs(b);
+  const double tol = 2.0 * 2.220446049250313e-16 * (fa > fb ? fa : fb);
+  const double sig = d > 0.0 ? 1.0 : -1.0;
+  double n;
+  if (a == __builtin_floor(a) && d == 1.0) {
+    n = __builtin_floor(b) - a;
+  } else if (a == __builtin_floor(a) && d == __builtin_floor(d)) {
+    const double q = __builtin_floor(a / d);
+    const double r = a - q * d;
+    n = __builtin_floor((b - r) / d) - q;
+  } else {
+    n = __builtin_round((b - a) / d);
+    if (sig * ((a + n * d) - b) > tol) n = n - 1.0;
+  }
+  if (!(n + 1.0 <= (double)cap)) return -1;
+  double last = a + n * d;
+  if (sig * (last - b) > -tol) last = b;
+  const int64_t N = (int64_t)n;
+  for (int64_t k = 0; k <= N / 2; k++) v[k] = a + (double)k * d;
+  for (int64_t k = 0; k <= N / 2; k++) v[N - k] = last - (double)k * d;
+  if (N % 2 == 0) v[N / 2] = (a + last) / 2.0;
+  return N + 1;
+}
+
 #endif /* GRS_GEOM_H */

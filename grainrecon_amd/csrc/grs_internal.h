@@ -16,6 +16,9 @@ enum grs_eval_map {
                             sampling; e is the global expanded index */
   GRS_MAP_CSR_LIST = 4,  /* as GRS_MAP_CSR for the expanded indices elist[e]
                             (elist[e] < 0: empty row, zero output) */
+  GRS_MAP_GRID = 5,      /* voxel v = e / n_u, orientation (e % n_u) + v*u_stride:
+                            every voxel against one shared list (u_stride 0) or
+                            against its own block of n_u rows (u_stride n_u) */
 };
 
 /* CSR candidate rows with local sampling (ori_local_sampling.m:22-34): row r
@@ -57,6 +60,8 @@ typedef struct grs_eval_args {
   unsigned long long *audit; /* GRS_AUDIT_WORDS counters, audit launches only */
   int64_t e0;              /* first evaluation of this launch (evaluations e0 .. n_eval-1) */
   int32_t count_keys;      /* counts-only kernels: 1 = hit test on the keys, 0 = on hit_bits */
+  int64_t n_u, u_stride;   /* GRS_MAP_GRID */
+  unsigned long long *out_key; /* counts-only GRS_MAP_GRID: per-evaluation sort key */
 } grs_eval_args;
 
 /* Audit counters of the fast kernel's resolution stage: events seen by phase
@@ -73,6 +78,8 @@ typedef struct grs_eval_args {
  * (MATLAB max: first maximum) wins. */
 #define GRS_KEY_JBITS 33
 #define GRS_KEY_MAX_EVENTS 23170
+/* largest k of the device top-k (sortrows + first rows) */
+#define GRS_TOPK_MAX 4096
 
 /* Launchers (return hipError_t). */
 /* d_vb: 2 doubles per voxel, the omega-solve bounds */
@@ -135,5 +142,27 @@ size_t grs_fitc_size(void);
 
 /* Wave slots the fast kernel keeps resident (its scratch is sized from this). */
 int64_t grs_fast_wave_slots(int device);
+
+/* seed indexing (grs_index.hip) */
+hipError_t grs_launch_fill_u32(uint32_t *d_p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t grs_launch_topk(const unsigned long long *d_keys, int64_t segs, int64_t n, int64_t seg_stride, int32_t k,
+                           unsigned long long *d_out, hipStream_t st);
+hipError_t grs_launch_gather_centres(const double *d_set_q, const unsigned long long *d_topkey, int64_t n,
+                                     double *d_cen, hipStream_t st);
+hipError_t grs_launch_ls_grids(const double *d_cen, int64_t n, int32_t nr, double gridsize, int32_t cap_axis,
+                               double *d_grids, int32_t *d_gn, uint32_t *d_bad, hipStream_t st);
+hipError_t grs_launch_qall(const double *d_ls_q, const int64_t *d_ls_n, int64_t P, int64_t K, int64_t cap,
+                           int32_t *d_owner, unsigned long long *d_Rmax, double *d_q_rows, uint32_t *d_gcount,
+                           hipStream_t st);
+hipError_t grs_launch_group_max(const unsigned long long *d_key2, const int64_t *d_ls_n, const uint32_t *d_gcount,
+                                int64_t P, int64_t K, int64_t Rcap, unsigned long long *d_gkey, int64_t *d_grow,
+                                hipStream_t st);
+hipError_t grs_launch_fit_starts(const unsigned long long *d_fitkey, const int64_t *d_grow, const double *d_q_rows,
+                                 const int32_t *d_owner, const int32_t *d_ns2, const int32_t *d_ni2, int64_t P,
+                                 int64_t K, int32_t n_fit, int64_t Rcap, double *d_x0, int32_t *d_fgroup,
+                                 int32_t *d_fns, int32_t *d_fni, hipStream_t st);
+hipError_t grs_launch_seed_out(const unsigned long long *d_topkey, const int32_t *d_ns1, const int32_t *d_ni1,
+                               const int32_t *d_set_valid, int64_t P, int64_t K, int64_t N, int32_t *d_seed_idx,
+                               double *d_seed_C, hipStream_t st);
 
 #endif

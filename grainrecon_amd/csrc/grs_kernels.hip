@@ -228,6 +228,10 @@ __device__ __forceinline__ void eval_slots(const grs_eval_args &a, int64_t e, in
   } else if (a.map == GRS_MAP_ONE_U) {
     *vi = e;
     *ui = 0;
+  } else if (a.map == GRS_MAP_GRID) {
+    const int64_t v = e / a.n_u;
+    *vi = v;
+    *ui = (e - v * a.n_u) + v * a.u_stride;
   } else {
     *vi = a.eval_vox[e];
     *ui = a.eval_u[e];
@@ -366,6 +370,29 @@ __device__ __forceinline__ int eval_setup(const grs_eval_args &a, int64_t e, int
 
 /* ---------------------------------------------------------------------- */
 /* k_fast                                                                  */
+
+/* Counts-only output of a grid evaluation: the counts and the key
+ * (class << 32) | (2^32 - 1 - i) of orientation i of its voxel, where class
+ * orders C = ni/ns as MATLAB's sortrows(...,'descend') does: NaN (ns = 0)
+ * first as 2^31, else floor(C * 2^30) + 1, which separates every two distinct
+ * ratios of counts up to GRS_KEY_MAX_EVENTS; a skipped orientation has C = 0
+ * (class 1).  Equal classes order by ascending i (a stable sort). */
+__device__ __forceinline__ void grid_counts_out(const grs_eval_args &a, int64_t e, int32_t ns, int32_t ni,
+                                                bool valid) {
+  if (a.out_ns) {
+    a.out_ns[e] = ns;
+    a.out_ni[e] = ni;
+  }
+  if (a.out_key) {
+    uint64_t cls = 1;
+    if (valid) {
+      if (ns > 0) cls = (uint64_t)__builtin_floor(((double)ni / (double)ns) * 1073741824.0) + 1ull;
+      else cls = 1ull << 31;
+    }
+    const uint64_t i = (uint64_t)(e % a.n_u);
+    a.out_key[e] = (unsigned long long)((cls << 32) | (0xFFFFFFFFull - i));
+  }
+}
 
 /* Refined hardware reciprocal / reciprocal square root (one Newton step):
  * used only by the conservative position filter. */
@@ -651,9 +678,13 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     int ok = eval_setup<MAP>(a, e, &vi, sub, &row, &jloc);
     if (ok <= 0) {
       if (!MEDIAN) {
-        /* a skipped orientation counts as C = NaN for the argmax */
-        if (lane == 0 && ok == 0)
+        /* a skipped orientation counts as C = NaN for the argmax; in a grid
+         * it has C = 0 */
+        if (MAP == GRS_MAP_GRID) {
+          if (lane == 0) grid_counts_out(a, e, 0, 0, false);
+        } else if (lane == 0 && ok == 0) {
           atomicMax(a.csr.row_best + row, (unsigned long long)(((1ull << GRS_KEY_JBITS) - 1) - (uint64_t)jloc));
+        }
       } else if (lane == 0) {
         a.out_ns[e] = 0;
         a.out_ni[e] = 0;
@@ -937,6 +968,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         a.out_C[e] = (double)ni / (double)ns;
         a.out_med[e] = med;
       }
+    } else if (MAP == GRS_MAP_GRID) {
+      if (lane == 0) grid_counts_out(a, e, ns, ni, true);
     } else if (lane == 0) {
       uint64_t jpart = ((1ull << GRS_KEY_JBITS) - 1) - (uint64_t)jloc;
       uint64_t cpart = 0;
@@ -1224,8 +1257,9 @@ hipError_t grs_launch_fast(const grs_eval_args *a, int32_t H, int64_t n_waves, i
 
 hipError_t grs_launch_fast_counts(const grs_eval_args *a, int32_t H, int64_t n_waves, int32_t wpb, hipStream_t st) {
   if (a->n_eval <= a->e0) return hipSuccess;
-  if (a->map != GRS_MAP_CSR) return hipErrorInvalidValue;
-  return launch_fast_impl<false, GRS_MAP_CSR>(a, H, n_waves, wpb, st);
+  if (a->map == GRS_MAP_CSR) return launch_fast_impl<false, GRS_MAP_CSR>(a, H, n_waves, wpb, st);
+  if (a->map == GRS_MAP_GRID && a->n_u > 0) return launch_fast_impl<false, GRS_MAP_GRID>(a, H, n_waves, wpb, st);
+  return hipErrorInvalidValue;
 }
 
 /* One-voxel evaluation that also runs the exact pipeline on every event of

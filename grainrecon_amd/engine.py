@@ -48,6 +48,21 @@ def stack_to_matlab_order(proj: np.ndarray) -> np.ndarray:
     return proj.reshape(-1, order="F")
 
 
+def fit_orientations(x):
+    """fit_OR_constrain_with_local_comp_cuda.m:133-148 on the host for fitted
+    vector parts x (M, 3): returns (out10 (M, 10) = [Euler ZXZ deg, Rodrigues,
+    quaternion], U (M, 3, 3), ok (M,))."""
+    x = _d(np.asarray(x).reshape(-1, 3))
+    M = x.shape[0]
+    out = np.zeros((M, 10))
+    U = np.zeros((M, 9))
+    ok = np.zeros(M, np.int32)
+    code = L.lib().grs_fit_orientations(_ptr(x, _dp), M, _ptr(out, _dp), _ptr(U, _dp), _ptr(ok, _ip))
+    if code != 0:
+        raise ValueError("grs_fit_orientations: invalid argument")
+    return out, U.reshape(M, 3, 3), ok
+
+
 class Engine:
     """FP64 scoring context on one GPU (grs_ctx)."""
 
@@ -177,6 +192,37 @@ class Engine:
                                              int(max_fun_evals), int(max_iter), float(tol_fun), float(tol_x),
                                              _ptr(x, _dp), _ptr(fv, _dp), _ptr(nf, _ip), _ptr(it, _ip)))
         return x, fv, nf, it
+
+    # -- seed indexing ------------------------------------------------------------
+    def set_orientation_set(self, q) -> None:
+        """Resident orientation set OR.q (N, 4) of the seed indexing."""
+        q = _d(np.asarray(q).reshape(-1, 4))
+        self._chk(self._lib.grs_set_orientation_set(self._h, _ptr(q, _dp), q.shape[0]))
+
+    def index_seeds(self, pos, nr_seed: int, nr: int = 8, gridsize: float = 0.004, n_fit: int = 1,
+                    half_width: float = 0.02, max_fun_evals: int = 2000, max_iter: int = 2000,
+                    tol_fun: float = 1e-10, tol_x: float = 1e-4) -> dict:
+        """GrainIndexing_engine_gpu_comp.m:46-66 with
+        fit_OR_constrain_with_local_comp_cuda.m for every seed position
+        (pos (P, 3) mm) against the resident orientation set.  Returns a dict:
+        indexing (P, 18) [seed number, pos, Nr_simu, Nr_intersect, C,
+        dis_median, Euler ZXZ deg, Rodrigues, quaternion], candidate (P,),
+        x0 / x (P, 3), fval, n_fun_evals, n_iter (P,), seed_idx / seed_C
+        (P, nr_seed)."""
+        pos = _d(np.asarray(pos).reshape(-1, 3))
+        P = pos.shape[0]
+        ind = np.zeros((P, 18))
+        cand = np.zeros(P, np.int32)
+        info = np.zeros((P, 9))
+        sidx = np.zeros((P, int(nr_seed)), np.int32)
+        sC = np.zeros((P, int(nr_seed)))
+        self._chk(self._lib.grs_index_seeds(self._h, _ptr(pos, _dp), P, int(nr_seed), int(nr), float(gridsize),
+                                            int(n_fit), float(half_width), int(max_fun_evals), int(max_iter),
+                                            float(tol_fun), float(tol_x), _ptr(ind, _dp), _ptr(cand, _ip),
+                                            _ptr(info, _dp), _ptr(sidx, _ip), _ptr(sC, _dp)))
+        return dict(indexing=ind, candidate=cand, x0=info[:, 0:3], x=info[:, 3:6], fval=info[:, 6],
+                    n_fun_evals=info[:, 7].astype(np.int64), n_iter=info[:, 8].astype(np.int64), seed_idx=sidx,
+                    seed_C=sC)
 
     # -- candidate generation ---------------------------------------------------
     def local_sampling(self, centres, nr: int, gridsize: float = 0.004) -> list:

@@ -266,6 +266,53 @@ grs_status grs_fit_or_local(grs_ctx *ctx, const double *pos, const double *x0, i
                             int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
                             double *fval, int32_t *n_fun_evals, int32_t *n_iter);
 
+/* ---- seed indexing ---------------------------------------------------------- */
+
+/* Makes the orientation set of the seed indexing (OR.q of
+ * GrainIndexing_engine_gpu_comp.m:46, N rows [q0 q1 q2 q3]) resident on the
+ * device; its S*U*B matrices (q0 recomputed from the vector part) follow the
+ * current geometry. */
+grs_status grs_set_orientation_set(grs_ctx *ctx, const double *q, int64_t N);
+
+/* GrainIndexing_engine_gpu_comp.m:46-66 for n_pos seed positions (pos,
+ * n_pos x 3 mm),
+ * each with fit_OR_constrain_with_local_comp_cuda.m:12-165 (Nr_fit = n_fit):
+ *   scores of every orientation of the set at the seed; the first
+ *   min(nr_seed, N) rows of sortrows(Output,3,'descend') (NaN first, ties in
+ *   set order); ori_local_sampling(seed q, nr, gridsize) around each, laid out
+ *   as q_all (seed j at rows (j-1)*len_j+1 .. j*len_j, later seeds
+ *   overwriting, unwritten rows zero); their scores; per seed the first
+ *   maximum of C over rows 1+nr_per_OR*(j-1) .. nr_per_OR*j (nr_per_OR the
+ *   last seed's row count); sortrows(Output_max,3,'descend'); fminsearchbnd
+ *   of 1 - C from the first n_fit rows (x0 = their q1..q3, bounds
+ *   x0 -/+ half_width clipped to [-1, 1], MaxFunEvals, MaxIter, TolFun, TolX);
+ *   Euler angles / Rodrigues vector / quaternion of each fit (:133-147), the
+ *   verification counts with U = quaternion2U(quat) (the counts and median of
+ *   forward_comp_pair_spots.m:3-93, 2000 when nothing is selected) and the fit
+ *   of highest C (:161, first maximum, NaN ignored).
+ * The steps up to the fit starts run on the device without host round trips
+ * (captured in a HIP graph and replayed for further batches of the same
+ * shape); the fits of all seeds are batched.
+ * Outputs: indexing (n_pos x 18) = [seed number (1-based), pos, Nr_simu,
+ * Nr_intersect, C, dis_median, Euler ZXZ (degrees), Rodrigues (3),
+ * quaternion (4)]; candidate (n_pos) = column 9 of the fitted Output_max row
+ * (the 1-based rank of the seed whose local sampling gave it, 0 for an
+ * unwritten row); optional fit_info (n_pos x 9) = [x0 (3), x (3), fval,
+ * n_fun_evals, n_iter]; optional seed_idx / seed_C (n_pos x nr_seed): 0-based
+ * orientation-set indices of the index_seeds rows and their C (-1 / NaN
+ * beyond N).  Requires nrot*nrhkl <= 23170, nr <= 11, min(nr_seed, N) <= 4096. */
+grs_status grs_index_seeds(grs_ctx *ctx, const double *pos, int64_t n_pos, int32_t nr_seed, int32_t nr, double gridsize,
+                           int32_t n_fit, double half_width, int32_t max_fun_evals, int32_t max_iter, double tol_fun,
+                           double tol_x, double *indexing, int32_t *candidate, double *fit_info, int32_t *seed_idx,
+                           double *seed_C);
+
+/* The orientation bookkeeping of fit_OR_constrain_with_local_comp_cuda.m:
+ * 133-148 on the host for M fitted vector parts x (M x 3): out10 (M x 10) =
+ * [Euler ZXZ (degrees, in [0, 360)), Rodrigues (3), quaternion (4)], U (M x 9,
+ * quaternion2U of that quaternion), ok (M) = 0 where rod2angle's gimbal-lock
+ * branch or a complex intermediate would be needed (outputs NaN). */
+grs_status grs_fit_orientations(const double *x, int64_t M, double *out10, double *U, int32_t *ok);
+
 /* ---- candidate generation ------------------------------------------------ */
 
 /* ori_local_sampling.m:12-34 for K centres [q0 q1 q2 q3] (used as given,
