@@ -331,7 +331,8 @@ class Engine:
         self._chk(self._lib.grs_set_autotune(self._h, 1 if on else 0))
 
     def tuning(self, kind: int):
-        """Chosen configuration of kind 0 (with median) or 1 (counts-only):
+        """Chosen configuration of kind 0 (with median), 1 (counts-only CSR) or
+        2 (counts-only grid, seed indexing):
         dict with waves_per_block, hit_keys and ms_per_eval of every candidate;
         None before the first large enough call."""
         w, k, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
@@ -342,7 +343,8 @@ class Engine:
         return {"waves_per_block": w.value, "hit_keys": bool(k.value), "ms_per_eval": list(ms)[:n.value]}
 
     def set_graphs(self, on: bool = True) -> None:
-        """HIP-graph capture and replay of the batched compete pipeline."""
+        """HIP-graph capture and replay of the batched compete and seed-indexing
+        pipelines."""
         self._chk(self._lib.grs_set_graphs(self._h, 1 if on else 0))
 
     def graph_stats(self):

@@ -349,11 +349,12 @@ grs_status grs_set_timing(grs_ctx *ctx, int32_t enable);
 grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, int32_t reset);
 /* Launch autotuning (default on).  The first call of a workload class --
  * kind 0, the kernel with the median (grs_forward_comp*, grow, compete,
- * winners), or kind 1, the counts-only CSR kernel of grs_compete_csr -- for
- * the current geometry (reflection and rotation counts) that is large enough
- * splits its evaluations among the configurations (waves per block 4, 2, 1;
- * for kind 1 also the hit test on the bit mask or on the distance keys),
- * times them and keeps the fastest.  Results never depend on it. */
+ * winners), kind 1, the counts-only CSR kernel of grs_compete_csr, or kind 2,
+ * the counts-only grid kernel of grs_index_seeds -- for the current geometry
+ * (reflection and rotation counts) that is large enough splits its
+ * evaluations among the configurations (waves per block 4, 2, 1; for kinds 1
+ * and 2 also the hit test on the bit mask or on the distance keys), times
+ * them and keeps the fastest.  Results never depend on it. */
 grs_status grs_set_autotune(grs_ctx *ctx, int32_t enable);
 /* The chosen configuration of a kind (GRS_ERR_STATE before tuning) and the
  * measured milliseconds per evaluation of each of its n_configs candidates
@@ -362,10 +363,12 @@ grs_status grs_set_autotune(grs_ctx *ctx, int32_t enable);
 grs_status grs_get_tuning(grs_ctx *ctx, int32_t kind, int32_t *waves_per_block, int32_t *hit_keys,
                           int32_t *n_configs, double *ms_per_eval);
 
-/* HIP graphs for grs_compete_csr (default on): each chunk's device pipeline
- * (key clearing, bounds, counts-only scoring, winners, median re-scoring,
- * winner outputs) is captured once and replayed while its shape and device
- * buffers are unchanged; enable = 0 launches the same kernels directly. */
+/* HIP graphs for grs_compete_csr and grs_index_seeds (default on): each
+ * chunk's device pipeline (for the compete: key clearing, bounds, counts-only
+ * scoring, winners, median re-scoring, winner outputs; for the indexing: the
+ * steps up to the fit starts) is captured once and replayed while its shape
+ * and device buffers are unchanged; enable = 0 launches the same kernels
+ * directly. */
 grs_status grs_set_graphs(grs_ctx *ctx, int32_t enable);
 /* Graph captures and replays since the context was created. */
 grs_status grs_graph_stats(grs_ctx *ctx, int64_t *captures, int64_t *replays);
