@@ -81,6 +81,9 @@ struct grs_ctx {
       x_valid2, x_key2, x_ns2, x_ni2, x_gcount, x_gkey, x_grow, x_fitkey, x_x0, x_fgroup, x_fns, x_fni, x_sidx, x_sC,
       x_bad;
   std::map<std::vector<uint64_t>, hipGraphExec_t> gi_cache;
+  /* experimental spots per projection (CSR) for the spot pairing */
+  DevBuf p_fptr, p_fsp, p_flag, p_sid, p_lam, p_dyz, p_dis, p_off, p_cnt, p_rowptr, p_rows;
+  int64_t p_nframes = -1, p_n = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
   int64_t launches = 0;
@@ -206,7 +209,9 @@ void grs_destroy(grs_ctx *c) {
                     &c->x_key1, &c->x_ns1, &c->x_ni1, &c->x_top, &c->x_cen, &c->x_grid, &c->x_gn, &c->x_ls,
                     &c->x_lsn, &c->x_owner, &c->x_Rmax, &c->x_q2, &c->x_sub2, &c->x_valid2, &c->x_key2,
                     &c->x_ns2, &c->x_ni2, &c->x_gcount, &c->x_gkey, &c->x_grow, &c->x_fitkey, &c->x_x0,
-                    &c->x_fgroup, &c->x_fns, &c->x_fni, &c->x_sidx, &c->x_sC, &c->x_bad};
+                    &c->x_fgroup, &c->x_fns, &c->x_fni, &c->x_sidx, &c->x_sC, &c->x_bad, &c->p_fptr,
+                    &c->p_fsp, &c->p_flag, &c->p_sid, &c->p_lam, &c->p_dyz, &c->p_dis, &c->p_off, &c->p_cnt,
+                    &c->p_rowptr, &c->p_rows};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1669,6 +1674,114 @@ grs_status grs_index_seeds(grs_ctx *c, const double *pos, int64_t n_pos, int32_t
       }
     }
   }
+  return GRS_OK;
+}
+
+/* ---- spot pairing ----------------------------------------------------------- */
+
+grs_status grs_set_frame_spots(grs_ctx *c, const int64_t *frame_ptr, int64_t nframes, const double *spots,
+                               int64_t n) {
+  if (!c) return GRS_ERR_ARG;
+  if (nframes < 1 || !frame_ptr || n < 0 || (n > 0 && !spots)) return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (frame_ptr[0] != 0 || frame_ptr[nframes] != n) return fail(c, GRS_ERR_ARG, "frame_ptr must run from 0 to n");
+  for (int64_t i = 0; i < nframes; i++)
+    if (frame_ptr[i + 1] < frame_ptr[i]) return fail(c, GRS_ERR_ARG, "frame_ptr must be non-decreasing");
+  HIPCHK(c, hipSetDevice(c->device));
+  ENSURE(c, c->p_fptr, sizeof(int64_t) * (size_t)(nframes + 1));
+  ENSURE(c, c->p_fsp, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  HIPCHK(c, hipMemcpyAsync(c->p_fptr.p, frame_ptr, sizeof(int64_t) * (size_t)(nframes + 1), hipMemcpyHostToDevice,
+                           c->st));
+  if (n > 0)
+    HIPCHK(c, hipMemcpyAsync(c->p_fsp.p, spots, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->p_nframes = nframes;
+  c->p_n = n;
+  return GRS_OK;
+}
+
+grs_status grs_pair_spots(grs_ctx *c, const double *pos, const double *U, int64_t M, int32_t *ns, int32_t *ni,
+                          float *med, int64_t *row_ptr, double *rows, int64_t cap) {
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || !row_ptr || (M > 0 && (!pos || !U || !ns || !ni || !med)) || cap < 0)
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (c->p_nframes != c->nrot)
+    return fail(c, GRS_ERR_STATE, "experimental spots of %d projections needed (grs_set_frame_spots)", c->nrot);
+  row_ptr[0] = 0;
+  if (M == 0) return GRS_OK;
+  /* Nr_simu, Nr_intersect, dis_median: the scoring kernel */
+  std::vector<int32_t> ident((size_t)M);
+  for (int64_t m = 0; m < M; m++) ident[m] = (int32_t)m;
+  ENSURE(c, c->d_pos, sizeof(double) * 3 * M);
+  ENSURE(c, c->d_U, sizeof(double) * 9 * M);
+  ENSURE(c, c->d_sub, sizeof(double) * 9 * M);
+  ENSURE(c, c->d_valid, sizeof(int32_t) * M);
+  ENSURE(c, c->d_evox, sizeof(int32_t) * M);
+  ENSURE(c, c->d_eu_idx, sizeof(int32_t) * M);
+  ENSURE(c, c->d_ns, sizeof(int32_t) * M);
+  ENSURE(c, c->d_ni, sizeof(int32_t) * M);
+  ENSURE(c, c->d_C, sizeof(double) * M);
+  ENSURE(c, c->d_med, sizeof(float) * M);
+  HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_U.p, U, sizeof(double) * 9 * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_evox.p, ident.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_eu_idx.p, ident.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_sub_from_U(P<grs_dgeo>(c->d_geo), P<double>(c->d_U), M, P<double>(c->d_sub),
+                                  P<int32_t>(c->d_valid), c->st));
+  s = run_eval(c, GRS_MAP_PAIRS, M, M, P<int32_t>(c->d_evox), P<int32_t>(c->d_eu_idx), GRS_METHOD_FAST,
+               P<int32_t>(c->d_ns), P<int32_t>(c->d_ni), P<double>(c->d_C), P<float>(c->d_med));
+  if (s != GRS_OK) return s;
+  HIPCHK(c, hipMemcpyAsync(ns, c->d_ns.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(ni, c->d_ni.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(med, c->d_med.p, sizeof(float) * M, hipMemcpyDeviceToHost, c->st));
+  /* the pairs, in chunks of orientations */
+  const int64_t E = (int64_t)c->nrot * c->H;
+  int64_t chunk = ((int64_t)1 << 26) / E;
+  if (chunk < 1) chunk = 1;
+  std::vector<int64_t> cnt;
+  for (int64_t m0 = 0; m0 < M; m0 += chunk) {
+    const int64_t m = M - m0 < chunk ? M - m0 : chunk;
+    const int64_t me = m * E;
+    ENSURE(c, c->p_flag, sizeof(int32_t) * me);
+    ENSURE(c, c->p_sid, sizeof(int64_t) * me);
+    ENSURE(c, c->p_lam, sizeof(double) * me);
+    ENSURE(c, c->p_dyz, sizeof(int32_t) * 2 * me);
+    ENSURE(c, c->p_dis, sizeof(float) * me);
+    ENSURE(c, c->p_off, sizeof(int32_t) * me);
+    ENSURE(c, c->p_cnt, sizeof(int64_t) * (m + 1));
+    ENSURE(c, c->p_rowptr, sizeof(int64_t) * (m + 1));
+    HIPCHK(c, grs_launch_pair_events(P<grs_dgeo>(c->d_geo), P<double>(c->d_hkl), P<double>(c->d_cs),
+                                     P<uint32_t>(c->d_keys), P<double>(c->d_pos) + 3 * m0,
+                                     P<double>(c->d_sub) + 9 * m0, m, E, P<int64_t>(c->p_fptr), P<double>(c->p_fsp),
+                                     P<int32_t>(c->p_flag), P<int64_t>(c->p_sid), P<double>(c->p_lam),
+                                     P<int32_t>(c->p_dyz), P<float>(c->p_dis), c->st));
+    HIPCHK(c, grs_launch_pair_scan(P<int32_t>(c->p_flag), m, E, P<int32_t>(c->p_off), P<int64_t>(c->p_cnt), c->st));
+    cnt.resize((size_t)m);
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->p_cnt.p, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<int64_t> rp((size_t)m + 1, 0);
+    for (int64_t k = 0; k < m; k++) {
+      rp[k + 1] = rp[k] + cnt[k];
+      row_ptr[m0 + k + 1] = row_ptr[m0 + k] + cnt[k];
+    }
+    if (!rows || rp[m] == 0) continue;
+    if (row_ptr[m0 + m] > cap)
+      return fail(c, GRS_ERR_ARG, "%lld pair rows do not fit the capacity %lld", (long long)row_ptr[m0 + m],
+                  (long long)cap);
+    ENSURE(c, c->p_rows, sizeof(double) * 23 * rp[m]);
+    HIPCHK(c, hipMemcpyAsync(c->p_rowptr.p, rp.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_pair_rows(P<grs_dgeo>(c->d_geo), P<double>(c->d_hkl), P<double>(c->d_rot),
+                                   P<double>(c->d_pos) + 3 * m0, P<double>(c->d_U) + 9 * m0, m, E,
+                                   P<double>(c->p_fsp), P<int32_t>(c->p_flag), P<int64_t>(c->p_sid),
+                                   P<double>(c->p_lam), P<int32_t>(c->p_dyz), P<float>(c->p_dis),
+                                   P<int32_t>(c->p_off), P<int64_t>(c->p_rowptr), P<double>(c->p_rows), c->st));
+    HIPCHK(c, hipMemcpyAsync(rows + 23 * row_ptr[m0], c->p_rows.p, sizeof(double) * 23 * rp[m],
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return GRS_OK;
 }
 

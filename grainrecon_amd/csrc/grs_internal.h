@@ -165,4 +165,16 @@ hipError_t grs_launch_seed_out(const unsigned long long *d_topkey, const int32_t
                                const int32_t *d_set_valid, int64_t P, int64_t K, int64_t N, int32_t *d_seed_idx,
                                double *d_seed_C, hipStream_t st);
 
+/* spot pairing (grs_pair.hip) */
+hipError_t grs_launch_pair_events(const grs_dgeo *d_g, const double *d_hkl, const double *d_cs,
+                                  const uint32_t *d_keys, const double *d_pos, const double *d_SUB, int64_t M,
+                                  int64_t E, const int64_t *d_fptr, const double *d_fsp, int32_t *d_flag,
+                                  int64_t *d_sid, double *d_lam, int32_t *d_dyz, float *d_dis, hipStream_t st);
+hipError_t grs_launch_pair_scan(const int32_t *d_flag, int64_t M, int64_t E, int32_t *d_off, int64_t *d_count,
+                                hipStream_t st);
+hipError_t grs_launch_pair_rows(const grs_dgeo *d_g, const double *d_hkl, const double *d_rot, const double *d_pos,
+                                const double *d_U, int64_t M, int64_t E, const double *d_fsp, const int32_t *d_flag,
+                                const int64_t *d_sid, const double *d_lam, const int32_t *d_dyz, const float *d_dis,
+                                const int32_t *d_off, const int64_t *d_row_ptr, double *d_rows, hipStream_t st);
+
 #endif

@@ -224,6 +224,36 @@ class Engine:
                     n_fun_evals=info[:, 7].astype(np.int64), n_iter=info[:, 8].astype(np.int64), seed_idx=sidx,
                     seed_C=sC)
 
+    # -- spot pairing ---------------------------------------------------------------
+    def set_frame_spots(self, frames) -> None:
+        """Experimental spots per projection: a list (one entry per rotation
+        angle) of (n_i, 3) arrays [centroid y, centroid z, area]
+        (Spots{i}(:,9:11))."""
+        rows = [np.asarray(f, dtype=float).reshape(-1, 3) for f in frames]
+        fp = np.zeros(len(rows) + 1, np.int64)
+        fp[1:] = np.cumsum([r.shape[0] for r in rows])
+        sp = _d(np.concatenate(rows) if fp[-1] else np.zeros((1, 3)))
+        self._chk(self._lib.grs_set_frame_spots(self._h, fp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                len(rows), _ptr(sp, _dp), int(fp[-1])))
+
+    def pair_spots(self, pos, U):
+        """forward_comp_pair_spots.m for (pos (M, 3), U (M, 3, 3)).  Returns
+        (ns, ni, med, [HittedSpots rows (n_m, 23) per pair])."""
+        pos = _d(np.asarray(pos).reshape(-1, 3))
+        U = _d(np.asarray(U).reshape(-1, 9))
+        M = pos.shape[0]
+        ns = np.zeros(M, np.int32)
+        ni = np.zeros(M, np.int32)
+        med = np.zeros(M, np.float32)
+        rp = np.zeros(M + 1, np.int64)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        self._chk(self._lib.grs_pair_spots(self._h, _ptr(pos, _dp), _ptr(U, _dp), M, _ptr(ns, _ip), _ptr(ni, _ip),
+                                           _ptr(med, _fp), rp.ctypes.data_as(i64p), None, 0))
+        rows = np.zeros((max(int(rp[-1]), 1), 23))
+        self._chk(self._lib.grs_pair_spots(self._h, _ptr(pos, _dp), _ptr(U, _dp), M, _ptr(ns, _ip), _ptr(ni, _ip),
+                                           _ptr(med, _fp), rp.ctypes.data_as(i64p), _ptr(rows, _dp), rows.shape[0]))
+        return ns, ni, med, [rows[rp[m]:rp[m + 1]].copy() for m in range(M)]
+
     # -- candidate generation ---------------------------------------------------
     def local_sampling(self, centres, nr: int, gridsize: float = 0.004) -> list:
         """ori_local_sampling.m:12-34 for each centre row [q0 q1 q2 q3] (a

@@ -313,6 +313,31 @@ grs_status grs_index_seeds(grs_ctx *ctx, const double *pos, int64_t n_pos, int32
  * branch or a complex intermediate would be needed (outputs NaN). */
 grs_status grs_fit_orientations(const double *x, int64_t M, double *out10, double *U, int32_t *ok);
 
+/* ---- spot pairing ------------------------------------------------------------ */
+
+/* Experimental spots per projection (Spots{i}(:,9:11) of
+ * forward_comp_pair_spots.m:73-79): nframes + 1 offsets (frame i owns rows
+ * frame_ptr[i] .. frame_ptr[i+1]-1 of spots) and n rows [centroid y,
+ * centroid z, area] in pixels.  nframes must equal the number of rotation
+ * angles when pairing. */
+grs_status grs_set_frame_spots(grs_ctx *ctx, const int64_t *frame_ptr, int64_t nframes, const double *spots,
+                               int64_t n);
+
+/* forward_comp_pair_spots.m:3-93 for M (pos, U) pairs (pos M x 3 mm, U M x 9
+ * row-major): Nr_simu, Nr_intersect and dis_median (2000 when nothing is
+ * selected) in ns, ni, med; row_ptr (M + 1) the offsets of each pair's rows;
+ * rows (cap x 23) the HittedSpots rows [reshape(U,1,9) pos hkl Energy
+ * rot_angle spot_y spot_z spot_area dety detz dis] in (projection,
+ * reflection) order, with pos(2) - RotAxisOffset (:16) and every value
+ * rounded to single precision as MATLAB's concatenation with the single dis
+ * does.  A hit is paired with the nearest spot of its projection (first
+ * minimum) when that distance is below min(sqrt(area), 15).  rows = NULL
+ * returns row_ptr only; GRS_ERR_ARG when the rows exceed cap.  Replaces the
+ * host loop of forward_comp_pair_spots.m (called from
+ * fit_OR_constrain_with_local_comp_cuda.m:89,151). */
+grs_status grs_pair_spots(grs_ctx *ctx, const double *pos, const double *U, int64_t M, int32_t *ns, int32_t *ni,
+                          float *med, int64_t *row_ptr, double *rows, int64_t cap);
+
 /* ---- candidate generation ------------------------------------------------ */
 
 /* ori_local_sampling.m:12-34 for K centres [q0 q1 q2 q3] (used as given,

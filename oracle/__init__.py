@@ -132,7 +132,7 @@ class Stack:
 
 def forward_comp(geo, stack: Stack, U, pos, rot, hkl, events: bool = False):
     """forward_comp.m for one (U, pos).  Returns (Nr_simu, Nr_intersect,
-    dis_median[, events (n, 7): omega idx, hkl idx, dety, detz, dety22, detz22, dis])."""
+    dis_median[, events (n, 8): omega idx, hkl idx, dety, detz, dety22, detz22, dis, lambdahkl])."""
     g = geo_from(geo)
     U = _d(np.asarray(U).reshape(9))
     pos = _d(np.asarray(pos).reshape(3))
@@ -141,7 +141,7 @@ def forward_comp(geo, stack: Stack, U, pos, rot, hkl, events: bool = False):
     ns, ni = _i64(), _i64()
     med = ctypes.c_float()
     cap = rot.size * hkl.shape[0] if events else 0
-    ev = np.zeros((max(cap, 1), 7))
+    ev = np.zeros((max(cap, 1), 8))
     n = lib().oracle_forward_comp(ctypes.byref(g), ctypes.byref(stack.c), U.ctypes.data_as(_dp),
                                   pos.ctypes.data_as(_dp), rot.ctypes.data_as(_dp), rot.size,
                                   hkl.ctypes.data_as(_dp), hkl.shape[0], ctypes.byref(ns), ctypes.byref(ni),

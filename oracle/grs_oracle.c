@@ -78,7 +78,7 @@ static int cmp_float(const void *a, const void *b) {
 
 /* forward_comp.m:4-90 for one (U, pos).  Returns the counts and the median;
  * optionally records every selected event (omega index, hkl index, dety,
- * detz, dety22, detz22, dis) into ev (capacity ev_cap, 7 doubles each). */
+ * detz, dety22, detz22, dis, lambdahkl) into ev (capacity ev_cap, 8 doubles each). */
 int64_t oracle_forward_comp(const oracle_geo *g, const oracle_proj *pj, const double *U, const double *pos_in,
                             const double *rot_angles, int64_t nrot, const double *hkl, int64_t nrhkl,
                             int64_t *Nr_simu_out, int64_t *Nr_intersect_out, float *dis_median_out,
@@ -188,8 +188,9 @@ int64_t oracle_forward_comp(const oracle_geo *g, const oracle_proj *pj, const do
       if ((double)dis <= thr) Nr_intersect++;
       dis_list[ndis++] = dis;
       if (ev && nev < ev_cap) {
-        double *e = ev + 7 * nev;
+        double *e = ev + 8 * nev;
         e[0] = (double)i; e[1] = (double)h; e[2] = dy; e[3] = dz; e[4] = d22y[h]; e[5] = d22z[h]; e[6] = dis;
+        e[7] = lam[h];
       }
       nev++;
     }
