@@ -84,6 +84,11 @@ struct grs_ctx {
   /* experimental spots per projection (CSR) for the spot pairing */
   DevBuf p_fptr, p_fsp, p_flag, p_sid, p_lam, p_dyz, p_dis, p_off, p_cnt, p_rowptr, p_rows;
   int64_t p_nframes = -1, p_n = 0;
+  /* reconstruction volumes (completeness, mask, median distance, Nr_intersect)
+   * and the region-growth buffers */
+  DevBuf v_vol[4], r_rows, r_counts, r_acc_lin, r_acc_C;
+  int64_t v_dim[3] = {0, 0, 0};
+  bool v_set[4] = {false, false, false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
   int64_t launches = 0;
@@ -211,7 +216,8 @@ void grs_destroy(grs_ctx *c) {
                     &c->x_ns2, &c->x_ni2, &c->x_gcount, &c->x_gkey, &c->x_grow, &c->x_fitkey, &c->x_x0,
                     &c->x_fgroup, &c->x_fns, &c->x_fni, &c->x_sidx, &c->x_sC, &c->x_bad, &c->p_fptr,
                     &c->p_fsp, &c->p_flag, &c->p_sid, &c->p_lam, &c->p_dyz, &c->p_dis, &c->p_off, &c->p_cnt,
-                    &c->p_rowptr, &c->p_rows};
+                    &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
+                    &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1782,6 +1788,173 @@ grs_status grs_pair_spots(grs_ctx *c, const double *pos, const double *U, int64_
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+/* ---- region growth with resident volumes -------------------------------------- */
+
+grs_status grs_set_volume(grs_ctx *c, int32_t which, const double *v, int64_t nx, int64_t ny, int64_t nz) {
+  if (!c) return GRS_ERR_ARG;
+  if (which < 0 || which > 3 || !v || nx < 1 || ny < 1 || nz < 1) return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (nx != c->v_dim[0] || ny != c->v_dim[1] || nz != c->v_dim[2]) {
+    for (int k = 0; k < 4; k++) c->v_set[k] = false;
+    c->v_dim[0] = nx;
+    c->v_dim[1] = ny;
+    c->v_dim[2] = nz;
+  }
+  const size_t n = (size_t)(nx * ny * nz);
+  ENSURE(c, c->v_vol[which], sizeof(double) * n);
+  HIPCHK(c, hipMemcpyAsync(c->v_vol[which].p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->v_set[which] = true;
+  return GRS_OK;
+}
+
+grs_status grs_get_volume(grs_ctx *c, int32_t which, double *v, int64_t n) {
+  if (!c) return GRS_ERR_ARG;
+  if (which < 0 || which > 3 || !v) return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (!c->v_set[which]) return fail(c, GRS_ERR_STATE, "volume %d not set", which);
+  if (n != c->v_dim[0] * c->v_dim[1] * c->v_dim[2]) return fail(c, GRS_ERR_ARG, "size mismatch");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(v, c->v_vol[which].p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_grow_region(grs_ctx *c, const int32_t *seed, double indexed_comp, const double *U, double spot_L,
+                           double drop_off, double maxD, double maxDmedian, const double *p1, double *center,
+                           int32_t *replace_center, int64_t *n_indexed, int32_t *indexed, int64_t cap,
+                           int64_t *n_checked) {
+  GRS_NO_CONTRACT
+  if (!c) return GRS_ERR_ARG;
+  if (!seed || !U || !p1 || !center || !replace_center || !n_indexed || cap < 0 || (cap > 0 && !indexed))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  for (int k = 0; k < 4; k++)
+    if (!c->v_set[k]) return fail(c, GRS_ERR_STATE, "volume %d not set (grs_set_volume)", k);
+  const int64_t nx = c->v_dim[0], ny = c->v_dim[1], nz = c->v_dim[2];
+  const int64_t dim[3] = {nx, ny, nz};
+  for (int k = 0; k < 3; k++)
+    if (seed[k] < 1 || seed[k] > dim[k]) return fail(c, GRS_ERR_ARG, "seed outside the volume");
+  /* grow_indexed_region_cuda.m:44-66 */
+  const grs_dgeo &g = c->hgeo;
+  const double boxLd = __builtin_ceil((spot_L * ((g.py + g.pz) / 2.0)) / ((g.Lsd / g.Lss) * p1[4]));
+  if (!(boxLd >= 0.0 && boxLd < 4096.0)) return fail(c, GRS_ERR_ARG, "bounding box half width %g", boxLd);
+  const int64_t boxL = (int64_t)boxLd;
+  int64_t lo[3], hi[3], bdim[3];
+  double cp[3];
+  for (int k = 0; k < 3; k++) {
+    lo[k] = seed[k] - boxL > 0 ? seed[k] - boxL : 1;
+    hi[k] = seed[k] + boxL < dim[k] ? seed[k] + boxL : dim[k];
+    bdim[k] = hi[k] - lo[k] + 1;
+    cp[k] = grs_round((double)(hi[k] - lo[k])) / 2.0 + (double)lo[k];
+  }
+  const int64_t nbox = bdim[0] * bdim[1] * bdim[2];
+  const int64_t seed_lin = (seed[0] - 1) + nx * ((seed[1] - 1) + ny * (seed[2] - 1));
+  /* I(x,y,z) = indexed_comp, also the seed's completeness on return */
+  HIPCHK(c, hipMemcpyAsync(P<double>(c->v_vol[0]) + seed_lin, &indexed_comp, sizeof(double), hipMemcpyHostToDevice,
+                           c->st));
+  ENSURE(c, c->r_rows, sizeof(double) * 6 * nbox);
+  ENSURE(c, c->r_counts, sizeof(uint32_t) * 4);
+  HIPCHK(c, grs_launch_fill_u32(P<uint32_t>(c->r_counts), 4, 0u, c->st));
+  HIPCHK(c, grs_launch_region_list(P<double>(c->v_vol[0]), P<double>(c->v_vol[1]), P<double>(c->v_vol[2]), nx, ny,
+                                   lo, bdim, cp, boxLd, seed_lin, indexed_comp, P<double>(c->r_rows),
+                                   P<uint32_t>(c->r_counts), c->st));
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(cnt, c->r_counts.p, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const int64_t M = cnt[0];
+  if (n_checked) *n_checked = cnt[1];
+  /* cuda_grow_indexed_region on the listed voxels, then the growth rule and
+   * the write-back */
+  double p14[14];
+  for (int k = 0; k < 11; k++) p14[k] = p1[k];
+  p14[11] = indexed_comp;
+  p14[12] = drop_off;
+  p14[13] = maxDmedian;
+  std::vector<int64_t> acc_lin;
+  std::vector<double> acc_C;
+  if (M > 0) {
+    ENSURE(c, c->d_p1, sizeof(double) * 14);
+    ENSURE(c, c->d_U, sizeof(double) * 9);
+    ENSURE(c, c->d_sub, sizeof(double) * 9);
+    ENSURE(c, c->d_valid, sizeof(int32_t));
+    ENSURE(c, c->d_pos, sizeof(double) * 3 * M);
+    ENSURE(c, c->d_ns, sizeof(int32_t) * M);
+    ENSURE(c, c->d_ni, sizeof(int32_t) * M);
+    ENSURE(c, c->d_C, sizeof(double) * M);
+    ENSURE(c, c->d_med, sizeof(float) * M);
+    ENSURE(c, c->r_acc_lin, sizeof(int64_t) * M);
+    ENSURE(c, c->r_acc_C, sizeof(double) * M);
+    HIPCHK(c, hipMemcpyAsync(c->d_p1.p, p14, sizeof(double) * 14, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_U.p, U, sizeof(double) * 9, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_sub_from_U(P<grs_dgeo>(c->d_geo), P<double>(c->d_U), 1, P<double>(c->d_sub),
+                                    P<int32_t>(c->d_valid), c->st));
+    HIPCHK(c, grs_launch_voxel_pos(P<double>(c->r_rows), M, 6, P<double>(c->d_p1), P<double>(c->d_pos), c->st));
+    s = run_eval(c, GRS_MAP_ONE_U, M, M, nullptr, nullptr, GRS_METHOD_FAST, P<int32_t>(c->d_ns), P<int32_t>(c->d_ni),
+                 P<double>(c->d_C), P<float>(c->d_med));
+    if (s != GRS_OK) return s;
+    HIPCHK(c, grs_launch_region_accept(P<double>(c->r_rows), M, P<int32_t>(c->d_ns), P<int32_t>(c->d_ni),
+                                       P<double>(c->d_C), P<float>(c->d_med), P<double>(c->d_p1), nx, ny,
+                                       P<double>(c->v_vol[0]), P<double>(c->v_vol[2]), P<double>(c->v_vol[3]),
+                                       P<int64_t>(c->r_acc_lin), P<double>(c->r_acc_C),
+                                       P<uint32_t>(c->r_counts) + 2, c->st));
+    HIPCHK(c, hipMemcpyAsync(cnt + 2, P<uint32_t>(c->r_counts) + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    acc_lin.resize(cnt[2]);
+    acc_C.resize(cnt[2]);
+    if (cnt[2] > 0) {
+      HIPCHK(c, hipMemcpyAsync(acc_lin.data(), c->r_acc_lin.p, sizeof(int64_t) * cnt[2], hipMemcpyDeviceToHost,
+                               c->st));
+      HIPCHK(c, hipMemcpyAsync(acc_C.data(), c->r_acc_C.p, sizeof(double) * cnt[2], hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+  }
+  /* J = seed and accepted voxels in find() order with their completeness
+   * after the write-back (:130-151) */
+  std::vector<std::pair<int64_t, double>> J;
+  J.reserve(acc_lin.size() + 1);
+  bool seed_acc = false;
+  for (size_t k = 0; k < acc_lin.size(); k++) {
+    J.push_back({acc_lin[k], acc_C[k]});
+    seed_acc = seed_acc || acc_lin[k] == seed_lin;
+  }
+  if (!seed_acc) J.push_back({seed_lin, indexed_comp});
+  std::sort(J.begin(), J.end());
+  const double sx = seed[0], sy = seed[1], sz = seed[2];
+  double sum = indexed_comp;
+  double sp[3] = {indexed_comp * sx, indexed_comp * sy, indexed_comp * sz};
+  const double c0[3] = {sp[0] / sum, sp[1] / sum, sp[2] / sum};
+  if (J.size() > 1) {
+    for (const auto &j : J) {
+      const int64_t x = j.first % nx + 1, y = (j.first / nx) % ny + 1, z = j.first / (nx * ny) + 1;
+      sum = sum + j.second;
+      sp[0] = sp[0] + j.second * (double)x;
+      sp[1] = sp[1] + j.second * (double)y;
+      sp[2] = sp[2] + j.second * (double)z;
+    }
+  }
+  const double c1[3] = {sp[0] / sum, sp[1] / sum, sp[2] / sum};
+  const double d0 = c1[0] - c0[0], d1 = c1[1] - c0[1], d2 = c1[2] - c0[2];
+  const double dist = grs_sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+  const double alt = (maxD / 10000.0) * (double)J.size();
+  const double thr = maxD != maxD ? alt : (alt != alt ? maxD : (maxD > alt ? maxD : alt));
+  if (dist > thr) {
+    for (int k = 0; k < 3; k++) center[k] = grs_round(c1[k]);
+    *replace_center = 1;
+  } else {
+    for (int k = 0; k < 3; k++) center[k] = c0[k];
+    *replace_center = 0;
+  }
+  *n_indexed = (int64_t)J.size();
+  for (int64_t k = 0; k < (int64_t)J.size() && k < cap; k++) {
+    indexed[3 * k] = (int32_t)(J[k].first % nx + 1);
+    indexed[3 * k + 1] = (int32_t)((J[k].first / nx) % ny + 1);
+    indexed[3 * k + 2] = (int32_t)(J[k].first / (nx * ny) + 1);
+  }
   return GRS_OK;
 }
 

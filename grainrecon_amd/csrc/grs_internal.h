@@ -177,4 +177,14 @@ hipError_t grs_launch_pair_rows(const grs_dgeo *d_g, const double *d_hkl, const 
                                 const int64_t *d_sid, const double *d_lam, const int32_t *d_dyz, const float *d_dis,
                                 const int32_t *d_off, const int64_t *d_row_ptr, double *d_rows, hipStream_t st);
 
+/* region growth (grs_region.hip) */
+hipError_t grs_launch_region_list(const double *d_I, const double *d_mask, const double *d_dm, int64_t nx, int64_t ny,
+                                  const int64_t *lo, const int64_t *bdim, const double *cp, double boxL,
+                                  int64_t seed_lin, double seed_comp, double *d_rows, uint32_t *d_counts,
+                                  hipStream_t st);
+hipError_t grs_launch_region_accept(const double *d_rows, int64_t M, const int32_t *d_ns, const int32_t *d_ni,
+                                    const double *d_C, const float *d_med, const double *d_p1, int64_t nx, int64_t ny,
+                                    double *d_vC, double *d_vDM, double *d_vNI, int64_t *d_acc_lin, double *d_acc_C,
+                                    uint32_t *d_count, hipStream_t st);
+
 #endif

@@ -254,6 +254,44 @@ class Engine:
                                            _ptr(med, _fp), rp.ctypes.data_as(i64p), _ptr(rows, _dp), rows.shape[0]))
         return ns, ni, med, [rows[rp[m]:rp[m + 1]].copy() for m in range(M)]
 
+    # -- region growth ------------------------------------------------------------
+    VOLUMES = {"Completeness": 0, "Mask": 1, "Dismedian": 2, "Ninter": 3}
+
+    def set_volume(self, name: str, v) -> None:
+        """Resident volume V(x, y, z) (numpy array indexed [x-1, y-1, z-1])."""
+        v = np.asarray(v, dtype=float)
+        flat = np.ascontiguousarray(v.transpose(2, 1, 0)).reshape(-1)
+        self._chk(self._lib.grs_set_volume(self._h, self.VOLUMES[name], _ptr(flat, _dp), v.shape[0], v.shape[1],
+                                           v.shape[2]))
+        self._vshape = v.shape
+
+    def get_volume(self, name: str) -> np.ndarray:
+        nx, ny, nz = self._vshape
+        flat = np.zeros(nx * ny * nz)
+        self._chk(self._lib.grs_get_volume(self._h, self.VOLUMES[name], _ptr(flat, _dp), flat.size))
+        return flat.reshape(nz, ny, nx).transpose(2, 1, 0).copy()
+
+    def grow_region(self, seed, indexed_comp: float, U, spot_L: float, drop_off: float, maxD: float,
+                    maxDmedian: float, p1) -> dict:
+        """grow_indexed_region_cuda.m on the resident volumes (seed 1-based
+        [x y z]).  Returns center, replace_center, indexed (n, 3) 1-based in
+        find() order, n_checked."""
+        sd = np.asarray(seed, dtype=np.int32).reshape(3)
+        U = _d(np.asarray(U).reshape(9))
+        p1 = _d(np.asarray(p1).reshape(-1)[:11])
+        center = np.zeros(3)
+        rc = ctypes.c_int32()
+        n = ctypes.c_int64()
+        nchk = ctypes.c_int64()
+        cap = int(np.prod(self._vshape))
+        idx = np.zeros((cap, 3), np.int32)
+        self._chk(self._lib.grs_grow_region(self._h, _ptr(sd, _ip), float(indexed_comp), _ptr(U, _dp), float(spot_L),
+                                            float(drop_off), float(maxD), float(maxDmedian), _ptr(p1, _dp),
+                                            _ptr(center, _dp), ctypes.byref(rc), ctypes.byref(n), _ptr(idx, _ip),
+                                            cap, ctypes.byref(nchk)))
+        return dict(center=center, replace_center=int(rc.value), indexed=idx[:n.value].copy(),
+                    n_checked=int(nchk.value))
+
     # -- candidate generation ---------------------------------------------------
     def local_sampling(self, centres, nr: int, gridsize: float = 0.004) -> list:
         """ori_local_sampling.m:12-34 for each centre row [q0 q1 q2 q3] (a

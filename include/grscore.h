@@ -338,6 +338,37 @@ grs_status grs_set_frame_spots(grs_ctx *ctx, const int64_t *frame_ptr, int64_t n
 grs_status grs_pair_spots(grs_ctx *ctx, const double *pos, const double *U, int64_t M, int32_t *ns, int32_t *ni,
                           float *med, int64_t *row_ptr, double *rows, int64_t cap);
 
+/* ---- region growth with resident volumes ---------------------------------------- */
+
+/* Reconstruction volumes kept on the device, nx x ny x nz doubles with x
+ * fastest (MATLAB's V(x,y,z) memory order): which = 0 Completeness, 1 Mask,
+ * 2 Dismedian, 3 Ninter (the DS fields GrainMapping_engine_gpu_comp.m:
+ * 22-32 passes to the growth).  Setting a volume of another size drops the
+ * others.  grs_get_volume copies one back (n = nx*ny*nz). */
+grs_status grs_set_volume(grs_ctx *ctx, int32_t which, const double *v, int64_t nx, int64_t ny, int64_t nz);
+grs_status grs_get_volume(grs_ctx *ctx, int32_t which, double *v, int64_t n);
+
+/* grow_indexed_region_cuda.m:29-151 around the 1-based seed voxel with the
+ * orientation U (row-major) on the resident volumes: Completeness(seed) =
+ * indexed_comp; the voxels of the box of half width
+ * Box_L = ceil(spot_L*mean([pixelysize pixelzsize]) / ((Lsam2det/Lsam2sou)*VoxSize))
+ * within Box_L of its centre point are scored (inside the mask) and grown by
+ * the operator's rule (seed_comp = indexed_comp, drop_off, maxDmedian); the
+ * accepted voxels' Completeness, Dismedian and Ninter are written into the
+ * volumes (the caller's Completeness(ind) = Completeness_out(ind), :232-234).
+ * p1 = [RecVolumePixel(:,1)' simap_data_flag VoxSize tomo_scale.Center'
+ * tomo_scale.Dimension] (11).  Outputs: the completeness-weighted centre
+ * test of :134-151 (center: round(weighted centre) with replace_center = 1,
+ * or the initial centre with 0), n_indexed = the size of the grown region J
+ * (seed included), its voxels (1-based [x y z], find order) in indexed
+ * (cap x 3, the first cap of them), and n_checked = the voxels of the
+ * sphere.  Replaces the wrapper around cuda_grow_indexed_region
+ * (GrainMapping_engine_gpu_comp.m:222-227). */
+grs_status grs_grow_region(grs_ctx *ctx, const int32_t *seed, double indexed_comp, const double *U, double spot_L,
+                           double drop_off, double maxD, double maxDmedian, const double *p1, double *center,
+                           int32_t *replace_center, int64_t *n_indexed, int32_t *indexed, int64_t cap,
+                           int64_t *n_checked);
+
 /* ---- candidate generation ------------------------------------------------ */
 
 /* ori_local_sampling.m:12-34 for K centres [q0 q1 q2 q3] (used as given,
