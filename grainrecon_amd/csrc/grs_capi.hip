@@ -88,6 +88,8 @@ struct grs_ctx {
    * and the region-growth buffers */
   DevBuf v_vol[4], r_rows, r_counts, r_acc_lin, r_acc_C;
   int64_t v_dim[3] = {0, 0, 0};
+  /* G-vector matching score buffers */
+  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out;
   bool v_set[4] = {false, false, false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
@@ -217,7 +219,8 @@ void grs_destroy(grs_ctx *c) {
                     &c->x_fgroup, &c->x_fns, &c->x_fni, &c->x_sidx, &c->x_sC, &c->x_bad, &c->p_fptr,
                     &c->p_fsp, &c->p_flag, &c->p_sid, &c->p_lam, &c->p_dyz, &c->p_dis, &c->p_off, &c->p_cnt,
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
-                    &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C};
+                    &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
+                    &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1955,6 +1958,91 @@ grs_status grs_grow_region(grs_ctx *c, const int32_t *seed, double indexed_comp,
     indexed[3 * k + 1] = (int32_t)((J[k].first / nx) % ny + 1);
     indexed[3 * k + 2] = (int32_t)(J[k].first / (nx * ny) + 1);
   }
+  return GRS_OK;
+}
+
+/* ---- G-vector matching score ---------------------------------------------------- */
+
+grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_t N, const double *fam_sq,
+                          const double *d_possible, int32_t nfam, double *out) {
+  GRS_NO_CONTRACT
+  if (!c) return GRS_ERR_ARG;
+  if (N < 0 || !pos || (N > 0 && (!q || !out)) || nfam < 1 || !fam_sq || !d_possible)
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (c->p_nframes != c->nrot)
+    return fail(c, GRS_ERR_STATE, "experimental spots of %d projections needed (grs_set_frame_spots)", c->nrot);
+  if (N == 0) return GRS_OK;
+  const int32_t F = nfam < 4 ? nfam : 4;
+  const grs_dgeo &g = c->hgeo;
+  /* calc_Gt_match_fun.m:12-16 */
+  const double v[3] = {g.Lsd, g.dety00 - g.RAO, g.detz00};
+  double d_tr[3];
+  for (int r = 0; r < 3; r++) d_tr[r] = (g.R[r] * v[0] + g.R[3 + r] * v[1]) + g.R[6 + r] * v[2];
+  const double Lsd_tr = d_tr[0] / g.R[0];
+  const double dety00_tr = g.R[1] * Lsd_tr - d_tr[1];
+  const double detz00_tr = g.R[2] * Lsd_tr - d_tr[2];
+  const double L_tr = g.Lss + Lsd_tr;
+  std::vector<int32_t> fam((size_t)c->H, -1);
+  for (int32_t h = 0; h < c->H; h++) {
+    const double *x = &c->hkl[3 * (size_t)h];
+    const double s2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+    for (int32_t k = 0; k < F; k++)
+      if (s2 == fam_sq[k]) {
+        fam[h] = k;
+        break;
+      }
+  }
+  double dposs[4] = {0, 0, 0, 0}, glen[4] = {0, 0, 0, 0};
+  for (int32_t k = 0; k < F; k++) {
+    dposs[k] = d_possible[k];
+    glen[k] = (2.0 * GRS_PI) / d_possible[k];
+  }
+  const int64_t nsp = c->p_n;
+  const int64_t cap = (int64_t)c->nrot * c->H;
+  int64_t waves = grs_gt_wave_slots(c->device);
+  if (waves > N) waves = N;
+  const int64_t wslots = ((waves + 3) / 4) * 4;
+  ENSURE(c, c->t_pos, sizeof(double) * 3);
+  ENSURE(c, c->t_proj, sizeof(double) * 18 * (size_t)c->nrot);
+  ENSURE(c, c->t_units, sizeof(double) * 12 * (size_t)(nsp > 0 ? nsp : 1));
+  ENSURE(c, c->t_fam, sizeof(int32_t) * (size_t)c->H);
+  ENSURE(c, c->t_dposs, sizeof(double) * 4);
+  ENSURE(c, c->t_glen, sizeof(double) * 4);
+  ENSURE(c, c->t_scratch, sizeof(double) * (size_t)(wslots * cap));
+  ENSURE(c, c->t_out, sizeof(double) * 4 * (size_t)N);
+  ENSURE(c, c->d_q, sizeof(double) * 4 * N);
+  ENSURE(c, c->d_sub, sizeof(double) * 9 * N);
+  ENSURE(c, c->d_valid, sizeof(int32_t) * N);
+  HIPCHK(c, hipMemcpyAsync(c->t_pos.p, pos, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->t_fam.p, fam.data(), sizeof(int32_t) * fam.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->t_dposs.p, dposs, sizeof(double) * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->t_glen.p, glen, sizeof(double) * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_q.p, q, sizeof(double) * 4 * N, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), P<double>(c->d_q), N, P<double>(c->d_sub),
+                                  P<int32_t>(c->d_valid), c->st));
+  HIPCHK(c, grs_launch_gt_proj(P<grs_dgeo>(c->d_geo), P<double>(c->t_pos), P<double>(c->d_cs), c->nrot, L_tr,
+                               P<double>(c->t_proj), c->st));
+  HIPCHK(c, grs_launch_gt_spots(P<grs_dgeo>(c->d_geo), P<double>(c->t_proj), P<int64_t>(c->p_fptr), c->nrot,
+                                P<double>(c->p_fsp), nsp, Lsd_tr, dety00_tr, detz00_tr, P<double>(c->t_dposs), F,
+                                P<double>(c->t_units), c->st));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  HIPCHK(c, grs_launch_gt_score(P<grs_dgeo>(c->d_geo), P<double>(c->d_hkl), P<int32_t>(c->t_fam),
+                                P<double>(c->t_glen), P<double>(c->d_sub), P<int32_t>(c->d_valid), N,
+                                P<double>(c->t_proj), P<int64_t>(c->p_fptr), P<double>(c->t_units),
+                                P<double>(c->t_scratch), cap, waves, P<double>(c->t_out), c->st));
+  if (c->timing) {
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->ms_total += ms;
+    c->launches += 1;
+  }
+  HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, sizeof(double) * 4 * N, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return GRS_OK;
 }
 

@@ -1,0 +1,299 @@
+/*
+ * grs_gt.hip -- the G-vector matching score ("Gt" indexing method).
+ *
+ * The reference's cuda_forward_Gt has no source in the repository and
+ * ForwardCalc_OR_space_Gt.m:68 calls a calc_Gt_match that is absent; these
+ * kernels follow calc_Gt_match_fun.m:1-152 (with calcExpectSpotNr.m:6-47):
+ *
+ * k_gt_proj   per projection: SamposW = Omega*S*pos (pos not shifted by
+ *             RotAxisOffset, :24), the incoming-beam unit vector and the
+ *             terms of calcExpectSpotNr with the centre at the effective
+ *             detector distance Lsam2det_tr of the tilt (:12-16, :25-31).
+ * k_gt_spots  per experimental spot (Spots{i}(:,9:10)): the diffracted-beam
+ *             unit vector towards its pixel position in the untilted frame
+ *             (:35-48), 2theta, lambda_j = 2 d_j sind(2theta/2) and the unit
+ *             vector of Gt_spot_j = (2 pi/lambda_j)(Kout_tr - Kin) for each of
+ *             the first four families j (:50-59, :88).
+ * k_gt_score  one wavefront per orientation, lanes over reflections: Gt =
+ *             Omega*S*U*B*hkl per projection; the expected-spot count
+ *             (the forward_comp selection); for reflections of the first
+ *             four families (:96-149) the smallest angle acos(n.g)*180/pi to
+ *             the projection's spot unit vectors of the family: the largest
+ *             dot product is found first and the angle taken as the minimum
+ *             over every spot whose dot product is within 1e-12 of it (acos
+ *             is monotone up to its last bit), a dot product above 1 giving
+ *             0 (the real part MATLAB keeps); a match when the angle is
+ *             below 1 degree and |Gt| is within 0.1 of 2 pi/d.  The matched
+ *             angles are listed per wavefront in (projection, reflection)
+ *             order; their median (rank counting across the wave, even count
+ *             a + (b - a)/2) and mean (sequential sum) close the row
+ *             [Nr_match, median, Nr_match/NrSpotExpectAll, mean].
+ */
+#include <hip/hip_runtime.h>
+
+#include "grs_internal.h"
+
+#define GT_PROJ_DOUBLES 18
+
+/* sind(x): reduction to |r| <= 45 degrees by multiples of 90, then
+ * sin or cos of (pi/180)*r with the quadrant's sign */
+__device__ __forceinline__ double gt_sind(double x) {
+  GRS_NO_CONTRACT
+  const double n = grs_round(x / 90.0);
+  const double r = x - n * 90.0;
+  double m = n - 4.0 * __builtin_floor(n / 4.0);
+  const double a = (GRS_PI / 180.0) * r;
+  if (m == 0.0) return grs_sin(a);
+  if (m == 1.0) return grs_cos(a);
+  if (m == 2.0) return -grs_sin(a);
+  return -grs_cos(a);
+}
+
+/* proj[i]: c s SW0 SW1 SW2 u0 u1 u2 ctr1 ctr2 cu1 cu2 L2 k0 tnum ov1 ov2 (17) */
+__global__ void k_gt_proj(const grs_dgeo *g, const double *pos, const double *cs, int32_t nrot, double Ltr,
+                          double *proj) {
+  GRS_NO_CONTRACT
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrot) return;
+  const double c = cs[2 * i], s = cs[2 * i + 1];
+  const double *S = g->S;
+  double OS[9];
+  for (int j = 0; j < 3; j++) {
+    OS[0 * 3 + j] = c * S[0 * 3 + j] - s * S[1 * 3 + j];
+    OS[1 * 3 + j] = s * S[0 * 3 + j] + c * S[1 * 3 + j];
+    OS[2 * 3 + j] = S[2 * 3 + j];
+  }
+  const double p0 = pos[0], p1 = pos[1], p2 = pos[2];
+  const double SW0 = (OS[0] * p0 + OS[1] * p1) + OS[2] * p2;
+  const double SW1 = (OS[3] * p0 + OS[4] * p1) + OS[5] * p2;
+  const double SW2 = (OS[6] * p0 + OS[7] * p1) + OS[8] * p2;
+  const double gp1 = g->Lss + SW0, gp2 = (SW1 - g->P0y) + g->RAO, gp3 = SW2 - g->P0z;
+  const double ng = grs_sqrt((gp1 * gp1 + gp2 * gp2) + gp3 * gp3);
+  const double ctr1 = (gp2 * Ltr) / gp1, ctr2 = (gp3 * Ltr) / gp1;
+  const double alpha = grs_atan(grs_sqrt(gp2 * gp2 + gp3 * gp3) / gp1);
+  const double nc = grs_sqrt((Ltr * Ltr + ctr1 * ctr1) + ctr2 * ctr2);
+  const double *R = g->R;
+  double *o = proj + GT_PROJ_DOUBLES * i;
+  o[0] = c;
+  o[1] = s;
+  o[2] = SW0;
+  o[3] = SW1;
+  o[4] = SW2;
+  o[5] = gp1 / ng;
+  o[6] = gp2 / ng;
+  o[7] = gp3 / ng;
+  o[8] = ctr1;
+  o[9] = ctr2;
+  o[10] = ctr1 / nc;
+  o[11] = ctr2 / nc;
+  o[12] = (g->Lsd - SW0) / grs_cos(alpha);
+  o[13] = g->Lsd - SW0;
+  o[14] = (R[0] * (g->Lsd - SW0) + R[3] * ((g->dety00 - g->RAO) - SW1)) + R[6] * (g->detz00 - SW2);
+  o[15] = (SW1 - g->dety00) + g->RAO;
+  o[16] = SW2 - g->detz00;
+}
+
+/* units[12*s + 3*j + k]: family j's unit vector of spot s */
+__global__ void k_gt_spots(const grs_dgeo *g, const double *proj, const int64_t *fptr, int32_t nrot,
+                           const double *fsp, int64_t nsp, double Lsd_tr, double dety00_tr, double detz00_tr,
+                           const double *dposs, int32_t nfam, double *units) {
+  GRS_NO_CONTRACT
+  const int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx >= nsp) return;
+  int32_t lo = 0, hi = nrot;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (fptr[mid] <= sidx) lo = mid;
+    else hi = mid;
+  }
+  const double *P = proj + GT_PROJ_DOUBLES * lo;
+  const double SW0 = P[2], SW1 = P[3], SW2 = P[4], k0 = P[5], k1 = P[6], k2 = P[7];
+  const double y22 = (g->dety0 - fsp[3 * sidx]) * g->py;
+  const double z22 = (g->detz0 - fsp[3 * sidx + 1]) * g->pz;
+  const double a = Lsd_tr - SW0, b = (y22 - dety00_tr) - SW1, cc = (z22 - detz00_tr) - SW2;
+  const double nk = grs_sqrt((a * a + b * b) + cc * cc);
+  const double o0 = a / nk, o1 = b / nk, o2 = cc / nk;
+  const double tt = (grs_acos((k0 * o0 + k1 * o1) + k2 * o2) * 180.0) / GRS_PI;
+  const double half = gt_sind(tt / 2.0);
+  const double f0 = o0 - k0, f1 = o1 - k1, f2 = o2 - k2;
+  for (int j = 0; j < 4; j++) {
+    double *u = units + 12 * sidx + 3 * j;
+    if (j >= nfam) {
+      u[0] = u[1] = u[2] = 0.0;
+      continue;
+    }
+    const double lam = (2.0 * dposs[j]) * half;
+    const double sc = (2.0 * GRS_PI) / lam;
+    const double g0 = sc * f0, g1 = sc * f1, g2 = sc * f2;
+    const double n = grs_sqrt((g0 * g0 + g1 * g1) + g2 * g2);
+    u[0] = g0 / n;
+    u[1] = g1 / n;
+    u[2] = g2 / n;
+  }
+}
+
+__device__ __forceinline__ uint64_t gt_lanes_below(uint64_t m, int lane) {
+  return lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+}
+
+__global__ __launch_bounds__(256) void k_gt_score(const grs_dgeo *g, const double *hkl, const int32_t *famidx,
+                                                  const double *glen_poss, const double *SUB, const int32_t *valid,
+                                                  int64_t N, const double *proj, const int64_t *fptr,
+                                                  const double *units, double *scratch, int64_t cap, double *out) {
+  GRS_NO_CONTRACT
+  const int32_t H = g->H, nrot = g->nrot;
+  const int lane = threadIdx.x & 63;
+  const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  double *list = scratch + gwave * cap;
+  for (int64_t u = gwave; u < N; u += nwaves) {
+    double *o = out + 4 * u;
+    if (!valid[u]) {
+      if (lane == 0) o[0] = o[1] = o[2] = o[3] = 0.0;
+      continue;
+    }
+    const double *sub = SUB + 9 * u;
+    int32_t nm = 0, expect = 0;
+    for (int32_t i = 0; i < nrot; i++) {
+      const double *P = proj + GT_PROJ_DOUBLES * i;
+      grs_vw v;
+      v.c = P[0];
+      v.s = P[1];
+      v.SW1 = P[3];
+      v.SW2 = P[4];
+      v.u0 = P[5];
+      v.u1 = P[6];
+      v.u2 = P[7];
+      v.ctr1 = P[8];
+      v.ctr2 = P[9];
+      v.cu1 = P[10];
+      v.cu2 = P[11];
+      v.L2 = P[12];
+      v.k0 = P[13];
+      v.tnum = P[14];
+      v.ov1 = P[15];
+      v.ov2 = P[16];
+      const int64_t s0 = fptr[i], s1 = fptr[i + 1];
+      for (int32_t h0 = 0; h0 < H; h0 += 64) {
+        const int32_t h = h0 + lane;
+        bool sel = false, match = false;
+        double amin = 0.0;
+        if (h < H) {
+          double gw[3], Gt[3], Glen;
+          grs_Gw(sub, hkl + 3 * h, gw);
+          grs_Gt(v.c, v.s, gw[0], gw[1], gw[2], Gt);
+          const double beta = grs_event_beta(&v, Gt, &Glen);
+          int32_t dy, dz;
+          sel = grs_event_finish(g, &v, Gt, Glen, beta, &dy, &dz) != 0;
+          const int32_t f = famidx[h];
+          if (f >= 0 && s1 > s0) {
+            const double n0 = Gt[0] / Glen, n1 = Gt[1] / Glen, n2 = Gt[2] / Glen;
+            double best = 0.0;
+            bool any = false;
+            for (int64_t k = s0; k < s1; k++) {
+              const double *w = units + 12 * k + 3 * f;
+              const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
+              if (d == d && (!any || d > best)) {
+                best = d;
+                any = true;
+              }
+            }
+            if (any) {
+              amin = 1e300;
+              const double lim = best - 1e-12;
+              for (int64_t k = s0; k < s1; k++) {
+                const double *w = units + 12 * k + 3 * f;
+                const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
+                if (d >= lim) {
+                  const double a = d > 1.0 ? 0.0 : (grs_acos(d) * 180.0) / GRS_PI;
+                  if (a < amin) amin = a;
+                }
+              }
+              match = amin < 1.0 && grs_fabs(Glen - glen_poss[f]) < 0.1;
+            }
+          }
+        }
+        expect += __popcll(__ballot(sel));
+        const uint64_t mm = __ballot(match);
+        if (match) list[nm + __popcll(gt_lanes_below(mm, lane))] = amin;
+        nm += __popcll(mm);
+      }
+    }
+    __threadfence_block();
+    double med = __builtin_nan(""), mean = __builtin_nan("");
+    if (nm > 0) {
+      /* ranks with ties broken by position: the elements of rank k1 and k2 */
+      const int32_t k1 = (nm & 1) ? (nm >> 1) : (nm >> 1) - 1, k2 = nm >> 1;
+      double v1 = 0.0, v2 = 0.0;
+      int f1 = 0, f2 = 0;
+      for (int32_t j = lane; j < nm; j += 64) {
+        const double x = list[j];
+        int32_t r = 0;
+        for (int32_t t = 0; t < nm; t++) {
+          const double y = list[t];
+          r += (y < x || (y == x && t < j)) ? 1 : 0;
+        }
+        if (r == k1) {
+          v1 = x;
+          f1 = 1;
+        }
+        if (r == k2) {
+          v2 = x;
+          f2 = 1;
+        }
+      }
+      const uint64_t b1 = __ballot(f1), b2 = __ballot(f2);
+      const int l1 = __ffsll((long long)b1) - 1, l2 = __ffsll((long long)b2) - 1;
+      v1 = __shfl(v1, l1);
+      v2 = __shfl(v2, l2);
+      med = (nm & 1) ? v1 : v1 + (v2 - v1) / 2.0;
+      if (lane == 0) {
+        double acc = 0.0;
+        for (int32_t t = 0; t < nm; t++) acc = acc + list[t];
+        mean = acc / (double)nm;
+      }
+    }
+    if (lane == 0) {
+      o[0] = (double)nm;
+      o[1] = med;
+      o[2] = (double)nm / (double)expect;
+      o[3] = mean;
+    }
+    __threadfence_block();
+  }
+}
+
+hipError_t grs_launch_gt_proj(const grs_dgeo *d_g, const double *d_pos, const double *d_cs, int32_t nrot, double Ltr,
+                              double *d_proj, hipStream_t st) {
+  if (nrot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gt_proj, dim3((unsigned)((nrot + 63) / 64)), dim3(64), 0, st, d_g, d_pos, d_cs, nrot, Ltr,
+                     d_proj);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_gt_spots(const grs_dgeo *d_g, const double *d_proj, const int64_t *d_fptr, int32_t nrot,
+                               const double *d_fsp, int64_t nsp, double Lsd_tr, double dety00_tr, double detz00_tr,
+                               const double *d_dposs, int32_t nfam, double *d_units, hipStream_t st) {
+  if (nsp <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gt_spots, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, st, d_g, d_proj, d_fptr, nrot,
+                     d_fsp, nsp, Lsd_tr, dety00_tr, detz00_tr, d_dposs, nfam, d_units);
+  return hipGetLastError();
+}
+
+int64_t grs_gt_wave_slots(int device) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+  return (int64_t)cus * 16;
+}
+
+hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const int32_t *d_famidx,
+                               const double *d_glen_poss, const double *d_SUB, const int32_t *d_valid, int64_t N,
+                               const double *d_proj, const int64_t *d_fptr, const double *d_units, double *d_scratch,
+                               int64_t cap, int64_t waves, double *d_out, hipStream_t st) {
+  if (N <= 0) return hipSuccess;
+  int64_t w = waves < N ? waves : N;
+  const int64_t blocks = (w + 3) / 4;
+  hipLaunchKernelGGL(k_gt_score, dim3((unsigned)blocks), dim3(256), 0, st, d_g, d_hkl, d_famidx, d_glen_poss, d_SUB,
+                     d_valid, N, d_proj, d_fptr, d_units, d_scratch, cap, d_out);
+  return hipGetLastError();
+}

@@ -187,4 +187,16 @@ hipError_t grs_launch_region_accept(const double *d_rows, int64_t M, const int32
                                     double *d_vC, double *d_vDM, double *d_vNI, int64_t *d_acc_lin, double *d_acc_C,
                                     uint32_t *d_count, hipStream_t st);
 
+/* G-vector matching score (grs_gt.hip) */
+hipError_t grs_launch_gt_proj(const grs_dgeo *d_g, const double *d_pos, const double *d_cs, int32_t nrot, double Ltr,
+                              double *d_proj, hipStream_t st);
+hipError_t grs_launch_gt_spots(const grs_dgeo *d_g, const double *d_proj, const int64_t *d_fptr, int32_t nrot,
+                               const double *d_fsp, int64_t nsp, double Lsd_tr, double dety00_tr, double detz00_tr,
+                               const double *d_dposs, int32_t nfam, double *d_units, hipStream_t st);
+int64_t grs_gt_wave_slots(int device);
+hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const int32_t *d_famidx,
+                               const double *d_glen_poss, const double *d_SUB, const int32_t *d_valid, int64_t N,
+                               const double *d_proj, const int64_t *d_fptr, const double *d_units, double *d_scratch,
+                               int64_t cap, int64_t waves, double *d_out, hipStream_t st);
+
 #endif

@@ -369,6 +369,23 @@ grs_status grs_grow_region(grs_ctx *ctx, const int32_t *seed, double indexed_com
                            int32_t *replace_center, int64_t *n_indexed, int32_t *indexed, int64_t cap,
                            int64_t *n_checked);
 
+/* ---- G-vector matching score ("Gt" indexing method) ------------------------------ */
+
+/* The score behind cuda_forward_Gt (ForwardCalc_OR_space_Gt_cuda.m:7-47),
+ * whose CUDA source is not in the reference, with the semantics of
+ * calc_Gt_match_fun.m:1-152: for the seed position pos and N orientation
+ * rows q ([q0 q1 q2 q3], q0 recomputed; |q_vec| > 1 gives a zero row), the
+ * reflections of the first min(nfam, 4) families (fam_sq = hkl_family_square,
+ * d_possible in Angstrom) matched against the experimental spots of each
+ * projection (grs_set_frame_spots, centroids in pixels; projection i is
+ * rotation angle i): a match when the angle between Gt and a spot's
+ * diffraction vector is below 1 degree and |Gt| is within 0.1 of
+ * 2 pi/d_possible.  out (N x 4) = [Nr_match, median matched angle (degrees),
+ * Nr_match/NrSpotExpectAll (calcExpectSpotNr.m with the tilt-corrected
+ * distance), mean matched angle]; NaN statistics when nothing matches. */
+grs_status grs_forward_gt(grs_ctx *ctx, const double *pos, const double *q, int64_t N, const double *fam_sq,
+                          const double *d_possible, int32_t nfam, double *out);
+
 /* ---- candidate generation ------------------------------------------------ */
 
 /* ori_local_sampling.m:12-34 for K centres [q0 q1 q2 q3] (used as given,

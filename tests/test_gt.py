@@ -1,0 +1,97 @@
+"""G-vector matching score (calc_Gt_match_fun.m; cuda_forward_Gt has no
+source in the reference): the oracle pinned by the forward geometry, the GPU
+(grs_forward_gt) against the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+from oracle import gt_match as GM
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return scenes.make_scene(seed=83, ngrains=3, spread_mm=0.05, nrot=37)
+
+
+def _families(s):
+    sq = sorted({int((h * h).sum()) for h in s.hkl.astype(int)})[:4]
+    a = 2.8665
+    return np.array(sq, float), np.array([a / math.sqrt(v) for v in sq])
+
+
+def test_sind_known_values():
+    x = np.array([0.0, 30.0, 45.0, 90.0, 135.0, 180.0, 270.0, 360.0, -90.0, 12.5])
+    y = GM.sind(x)
+    assert y[0] == 0.0 and y[3] == 1.0 and y[5] == 0.0 and y[6] == -1.0 and y[7] == 0.0 and y[8] == -1.0
+    np.testing.assert_allclose(y, np.sin(np.radians(x)), atol=5e-16)
+
+
+def _exact_spots(s, U, pos):
+    """One spot at the unrounded detector position of every selected event."""
+    _, _, _, ev = oracle.forward_comp(s.geo, s.stack, U, pos, s.rot, s.hkl, events=True)
+    frames = [[] for _ in range(s.rot.size)]
+    for om, h, dy, dz, y22, z22, dis, lam in ev:
+        frames[int(om)].append([(-y22) / s.geo.pixelysize + s.geo.dety0, (-z22) / s.geo.pixelzsize + s.geo.detz0,
+                                10.0])
+    return [np.array(f).reshape(-1, 3) for f in frames], ev
+
+
+def test_gt_oracle_inverts_the_forward_geometry(scene):
+    """Spots at the exact positions of an orientation's events give angles of
+    nearly zero for its own reflections (the forward model's spot construction
+    and the Laue condition agree to about 0.002 degrees), and the
+    expected-spot count equals the forward_comp selection (no tilt, no
+    rotation-axis offset)."""
+    s = scene
+    assert s.geo.RotAxisOffset == 0.0 and np.array_equal(np.asarray(s.geo.RotDet), np.eye(3))
+    fam_sq, dposs = _families(s)
+    frames, ev = _exact_spots(s, s.U[0], s.centers[0])
+    q = s.q[0][None, :]
+    out = GM.gt_score(s.geo, s.centers[0], q, s.rot, s.hkl, frames, fam_sq, dposs)
+    ns, ni, med = oracle.forward_comp(s.geo, s.stack, s.U[0], s.centers[0], s.rot, s.hkl)
+    fam_of = {int(v) for v in fam_sq}
+    own = sum(1 for e in ev if int((s.hkl[int(e[1])] ** 2).sum()) in fam_of)
+    assert out[0, 0] >= own > 20
+    assert out[0, 2] == out[0, 0] / ns
+    assert out[0, 1] < 0.01 and out[0, 3] < 0.05
+    # a rotation of 3 degrees moves every spot away: few matches
+    qq = s.q[0].copy()
+    qq[1:] += 0.026
+    far = GM.gt_score(s.geo, s.centers[0], qq[None, :], s.rot, s.hkl, frames, fam_sq, dposs)
+    assert far[0, 0] < 0.2 * out[0, 0]
+
+
+def _frames(s, rng):
+    frames = [[] for _ in range(s.rot.size)]
+    for g in range(len(s.U)):
+        _, _, _, ev = oracle.forward_comp(s.geo, s.stack, s.U[g], s.centers[g], s.rot, s.hkl, events=True)
+        for om, _h, dy, dz, *_ in ev:
+            frames[int(om)].append([dy + rng.uniform(-1, 1), dz + rng.uniform(-1, 1), 20.0])
+    det = int(s.geo.detysize)
+    for f in frames:
+        for _ in range(int(rng.integers(0, 8))):
+            f.append([rng.uniform(1, det), rng.uniform(1, det), 20.0])
+    frames[5] = []
+    return [np.array(f).reshape(-1, 3) for f in frames]
+
+
+@pytest.mark.gpu
+def test_forward_gt_equals_oracle(engine, scene):
+    s = scene
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(31)
+    frames = _frames(s, rng)
+    engine.set_frame_spots(frames)
+    fam_sq, dposs = _families(s)
+    q = np.concatenate([s.q, s.q + rng.normal(scale=0.003, size=(3, 4)) * [0, 1, 1, 1], scenes.random_quats(rng, 40),
+                        [[0.1, 0.9, 0.5, 0.3]]])
+    print(f"forward_gt: {q.shape[0]} orientations, {sum(len(f) for f in frames)} spots", flush=True)
+    got = engine.forward_gt(s.centers[0], q, fam_sq, dposs)
+    ref = GM.gt_score(s.geo, s.centers[0], q, s.rot, s.hkl, frames, fam_sq, dposs)
+    same = (got == ref) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), np.argwhere(~same)[:5]
+    assert got[0, 0] > 20 and np.all(got[-1] == 0.0)
