@@ -314,7 +314,9 @@ def index_leg(eng, q, smp, n_pos, nr_seed, steps, seed=5005, turn_deg=0.05):
     random axis (the completeness peak of this noise-free synthetic stack is
     narrower than the set's spacing: 0.3 degrees off the truth already scores
     at background level); an indexed orientation is counted as found when it
-    lies within 0.1 degrees of its grain's (cubic symmetry)."""
+    lies within 0.1 degrees of its grain's (cubic symmetry), and separately
+    when it lies that close to any grain's (a seed near a grain boundary
+    scores the neighbour's orientation nearly as high)."""
     from grainrecon_amd import synth
 
     rng = np.random.default_rng(seed)
@@ -346,6 +348,8 @@ def index_leg(eng, q, smp, n_pos, nr_seed, steps, seed=5005, turn_deg=0.05):
     sym = synth.cubic_symmetry_quats()
     dots = np.abs(np.einsum("nk,nsk->ns", qfit, synth._qmul(qtrue[:, None, :], sym[None, :, :]))).max(1)
     mis = np.degrees(2 * np.arccos(np.clip(dots, -1, 1)))
+    allq = synth._qmul(smp.q[:, None, :], sym[None, :, :]).reshape(-1, 4)
+    mis_any = np.degrees(2 * np.arccos(np.clip(np.abs(qfit @ allq.T).max(1), -1, 1)))
     N = qset.shape[0]
     evals_dev = steps * n_pos * (N + nr_seed * 730)
     out = {"positions_per_call": n_pos, "calls": steps, "nr_seed": nr_seed, "orientations": N,
@@ -354,11 +358,130 @@ def index_leg(eng, q, smp, n_pos, nr_seed, steps, seed=5005, turn_deg=0.05):
            "device_evals_per_call": int(evals_dev / steps), "fit_evals_mean": round(float(nf.mean()), 1),
            "fitted_C_median": round(float(np.nanmedian(C)), 4), "fitted_C_min": round(float(np.nanmin(C)), 4),
            "found_within_0.1deg": int((mis < 0.1).sum()), "seeds": int(mis.size),
+           "any_grain_within_0.1deg": int((mis_any < 0.1).sum()),
            "misorientation_deg_median": round(float(np.median(mis)), 4), "set_turn_deg": turn_deg}
     log(f"[bench] index: {steps} calls x {n_pos} seeds in {dt:.3f}s -> {out['seeds_per_s']} seeds/s, "
         f"median fitted C {out['fitted_C_median']}, {out['found_within_0.1deg']}/{mis.size} within 0.1 deg, "
         f"graph replays {out['graph_replays']}")
     return out
+
+
+def _grain_com(smp):
+    G = smp.q.shape[0]
+    cnt = np.bincount(smp.grain, minlength=G).astype(float)
+    return np.stack([np.bincount(smp.grain, weights=smp.pos[:, k], minlength=G) for k in range(3)], 1) / cnt[:, None]
+
+
+def gt_leg(eng, cfg, n_orient, cpu, threads, seed=6006):
+    """G-vector matching score (calc_Gt_match_fun.m semantics) at one grain's
+    centre of mass: spot centroids of every grain simulated with the engine
+    (one spot per selected event at its unrounded pixel position), n_orient
+    orientations (the grain's local sampling and random ones)."""
+    from grainrecon_amd import fit as F
+    from grainrecon_amd import synth
+
+    smp, rng = cfg.sample, np.random.default_rng(seed)
+    com = _grain_com(smp)
+    sp = F.simulate_matched_spots(eng, None, com, smp.U)
+    nrot = cfg.rot.size
+    frames = []
+    for i in range(nrot):
+        yz = sp["meas"][sp["omega_idx"] == i]
+        frames.append(np.column_stack([yz, np.full(yz.shape[0], 20.0)]))
+    eng.set_frame_spots(frames)
+    B = np.asarray(cfg.geo.B)
+    sq, dposs = [], []
+    for h in cfg.hkl:
+        v = int(round((h * h).sum()))
+        if v not in sq:
+            sq.append(v)
+            dposs.append(2 * np.pi / np.linalg.norm(B @ h))
+    sq, dposs = np.array(sq[:4], float), np.array(dposs[:4])
+    g = 7
+    qt = synth.fz_reduce_cubic(smp.q[g:g + 1])[0]
+    local = eng.local_sampling(qt, 8, 0.004)[0]
+    q = np.concatenate([local, synth.fz_reduce_cubic(synth.haar_quaternions(rng, n_orient - local.shape[0]))])
+    eng.kernel_time(reset=True)
+    out = eng.forward_gt(com[g], q, sq, dposs)
+    eng.kernel_time(reset=True)
+    t = time.perf_counter()
+    out = eng.forward_gt(com[g], q, sq, dposs)
+    wall = time.perf_counter() - t
+    kms, _ = eng.kernel_time(reset=True)
+    best = int(np.argmax(out[:, 0]))
+    res = {"orientations": int(q.shape[0]), "spots": int(sum(f.shape[0] for f in frames)),
+           "kernel_ms": round(kms, 3), "wall_s": round(wall, 3),
+           "orientations_per_s": round(q.shape[0] / (kms * 1e-3), 1),
+           "best_in_local_sampling": bool(best < local.shape[0]), "best_Nr_match": int(out[best, 0]),
+           "best_median_angle_deg": float(out[best, 1])}
+    if cpu:
+        from oracle import gt_match
+
+        pick = [best, local.shape[0] + 3]
+        t = time.perf_counter()
+        ref = gt_match.gt_score(cfg.geo, com[g], q[pick], cfg.rot, cfg.hkl, frames, sq, dposs)
+        cdt = time.perf_counter() - t
+        same = (out[pick] == ref) | (np.isnan(out[pick]) & np.isnan(ref))
+        res["cpu_baseline"] = {"value": round(len(pick) / cdt, 3), "unit": "orientations/s", "cores": 1,
+                               "kind": "port", "sample": f"{len(pick)} orientations (oracle/gt_match.py, numpy)"}
+        res["parity"] = {"orientations_checked": len(pick), "bit_exact": bool(same.all())}
+    log(f"[bench] gt: {q.shape[0]} orientations x {res['spots']} spots in {kms:.2f} ms of kernel time; best row "
+        f"{best} (Nr_match {res['best_Nr_match']})")
+    return res
+
+
+def grow_leg(eng, cfg, box_l, cpu, threads, seed=7007):
+    """Region growth (grow_indexed_region_cuda.m) around a voxel of a grain
+    with the reconstruction volumes resident on the device, bounding sphere of
+    radius box_l voxels."""
+    smp, geo = cfg.sample, cfg.geo
+    nx, ny, nz = smp.dims
+    rng = np.random.default_rng(seed)
+    vol_c = np.zeros((nx, ny, nz))
+    mask = np.zeros((nx, ny, nz))
+    mask[smp.index[:, 0] - 1, smp.index[:, 1] - 1, smp.index[:, 2] - 1] = 1.0
+    dm = np.full((nx, ny, nz), 10.0)
+    eng.set_volume("Completeness", vol_c)
+    eng.set_volume("Mask", mask)
+    eng.set_volume("Dismedian", dm)
+    eng.set_volume("Ninter", np.zeros((nx, ny, nz)))
+    p1 = np.array([1, 1, 1, 0, smp.vox_mm, 0, 0, 0, nx, ny, nz], float)
+    com = _grain_com(smp)
+    g = 11
+    k = int(np.argmin(((smp.pos - com[g]) ** 2).sum(1)))
+    seed_vox = smp.index[k]
+    spot_l = box_l * ((geo.Lsam2det / geo.Lsam2sou) * smp.vox_mm) / ((geo.pixelysize + geo.pixelzsize) / 2)
+    eng.kernel_time(reset=True)
+    t = time.perf_counter()
+    r = eng.grow_region(seed_vox, 1.0, smp.U[g], spot_l, 0.05, 2.0, 10.0, p1)
+    wall = time.perf_counter() - t
+    kms, _ = eng.kernel_time(reset=True)
+    gid = np.full((nx, ny, nz), -1, np.int32)
+    gid[smp.index[:, 0] - 1, smp.index[:, 1] - 1, smp.index[:, 2] - 1] = smp.grain
+    iv = r["indexed"] - 1
+    n_ok = int((gid[iv[:, 0], iv[:, 1], iv[:, 2]] == smp.grain[k]).sum())
+    res = {"voxels_checked": r["n_checked"], "voxels_grown": int(r["indexed"].shape[0]),
+           "grown_in_own_grain": n_ok, "box_half_width": box_l, "scoring_kernel_ms": round(kms, 3),
+           "wall_s": round(wall, 3), "voxels_per_s": round(r["n_checked"] / wall, 1),
+           "replace_center": r["replace_center"]}
+    if cpu:
+        import oracle
+
+        rows = []
+        for v in r["indexed"][:64].tolist():
+            rows.append([*v, 0.0, 1.0, 10.0])
+        rows = np.array(rows, float)
+        param1 = np.concatenate([p1, [1.0, 0.05, 10.0]])
+        stack = oracle.Stack(eng.get_distance_stack((int(geo.detzsize), int(geo.detysize), cfg.rot.size)))
+        t = time.perf_counter()
+        ref = oracle.grow(geo, stack, rows, smp.U[g], cfg.rot, cfg.hkl, param1, nthreads=threads)
+        cdt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": round(rows.shape[0] / cdt, 2), "unit": "voxels/s", "cores": threads,
+                               "kind": "port", "sample": f"{rows.shape[0]} grown voxels re-scored by oracle.grow"}
+        res["parity"] = {"voxels_checked": int(rows.shape[0]), "all_accepted_by_oracle": bool((ref[:, 6] == 1).all())}
+    log(f"[bench] grow: {r['n_checked']} voxels checked, {r['indexed'].shape[0]} grown ({n_ok} in the seed's "
+        f"grain) in {wall:.3f}s")
+    return res
 
 
 def main():
@@ -381,6 +504,8 @@ def main():
     ap.add_argument("--index-positions", type=int, default=8, help="seed positions per indexing call; 0 = off")
     ap.add_argument("--index-steps", type=int, default=2)
     ap.add_argument("--index-nr-seed", type=int, default=500)
+    ap.add_argument("--gt-orientations", type=int, default=8192, help="orientations of the Gt-score leg; 0 = off")
+    ap.add_argument("--grow-box", type=int, default=24, help="sphere radius (voxels) of the region-growth leg; 0 = off")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
                     help="size of a plain device copy timed (and visible to counter runs) for calibration; 0 = off")
     args = ap.parse_args()
@@ -519,6 +644,11 @@ def main():
         eng.device_free(d)
     if args.index_positions > 0:
         result["index"] = index_leg(eng, q, smp, args.index_positions, args.index_nr_seed, args.index_steps)
+    threads = min(16, os.cpu_count() or 1)
+    if args.gt_orientations > 0:
+        result["gt_score"] = gt_leg(eng, cfg, args.gt_orientations, not args.no_cpu, threads)
+    if args.grow_box > 0:
+        result["grow_region"] = grow_leg(eng, cfg, args.grow_box, not args.no_cpu, threads)
     result["autotune"] = {"median_kernel": eng.tuning(0), "counts_kernel": eng.tuning(1),
                           "counts_grid_kernel": eng.tuning(2)}
     if args.geo_fit_grains > 0:
