@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libgrscore.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("GRS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["grs_kernels.hip", "grs_prep.hip", "grs_fit.hip", "grs_cand.hip", "grs_index.hip", "grs_pair.hip", "grs_region.hip", "grs_gt.hip", "grs_capi.hip"]
+SOURCES = ["grs_kernels.hip", "grs_prep.hip", "grs_fit.hip", "grs_cand.hip", "grs_index.hip", "grs_pair.hip", "grs_region.hip", "grs_gt.hip", "grs_com.hip", "grs_capi.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -90,6 +90,8 @@ struct grs_ctx {
   int64_t v_dim[3] = {0, 0, 0};
   /* G-vector matching score buffers */
   DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out;
+  /* grain centre-of-mass fit buffers */
+  DevBuf m_rows, m_rptr, m_x, m_pfit, m_doff, m_dis;
   bool v_set[4] = {false, false, false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
@@ -220,7 +222,8 @@ void grs_destroy(grs_ctx *c) {
                     &c->p_fsp, &c->p_flag, &c->p_sid, &c->p_lam, &c->p_dyz, &c->p_dis, &c->p_off, &c->p_cnt,
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
-                    &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out};
+                    &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
+                    &c->m_x, &c->m_pfit, &c->m_doff, &c->m_dis};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1958,6 +1961,120 @@ grs_status grs_grow_region(grs_ctx *c, const int32_t *seed, double indexed_comp,
     indexed[3 * k + 1] = (int32_t)((J[k].first / nx) % ny + 1);
     indexed[3 * k + 2] = (int32_t)(J[k].first / (nx * ny) + 1);
   }
+  return GRS_OK;
+}
+
+/* ---- grain centre-of-mass fit ------------------------------------------------------ */
+
+grs_status grs_fit_grain_com(grs_ctx *c, const double *rows, const int64_t *row_ptr, int64_t M, int32_t max_fun_evals,
+                             int32_t max_iter, double tol_fun, double tol_x, double *x_out, double *fval,
+                             int32_t *n_fun_evals, int32_t *n_iter) {
+  GRS_NO_CONTRACT
+  if (!c) return GRS_ERR_ARG;
+  if (M < 0 || !row_ptr || (M > 0 && (!rows || !x_out || !fval || !n_fun_evals || !n_iter)) || max_fun_evals < 1 ||
+      max_iter < 1 || !(tol_fun >= 0.0) || !(tol_x >= 0.0))
+    return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (M == 0) return GRS_OK;
+  if (row_ptr[0] != 0) return fail(c, GRS_ERR_ARG, "row_ptr must start at 0");
+  int64_t maxn = 0;
+  for (int64_t m = 0; m < M; m++) {
+    const int64_t n = row_ptr[m + 1] - row_ptr[m];
+    if (n < 1) return fail(c, GRS_ERR_ARG, "fit %lld has no pair rows", (long long)m);
+    maxn = n > maxn ? n : maxn;
+  }
+  if (M * (GRS_NM_NMAX + 1) > 65535) return fail(c, GRS_ERR_ARG, "too many fits in one call");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  const int64_t total = row_ptr[M];
+  /* fit_grain_COM.m:7,17-23: x0 = [position, u2euler_corr(U)] from the first
+   * row, bounds x0 -/+ 0.1 mm and the angles -/+ 3 degrees within [0, 360] */
+  std::vector<grs_nm> nm((size_t)M);
+  for (int64_t m = 0; m < M; m++) {
+    const double *r = rows + 23 * row_ptr[m];
+    double x0[6], lo[6], hi[6];
+    x0[0] = r[9];
+    x0[1] = r[10];
+    x0[2] = r[11];
+    grs_u2euler_corr(r, x0 + 3);
+    for (int k = 0; k < 3; k++) {
+      lo[k] = x0[k] - 0.1;
+      hi[k] = x0[k] + 0.1;
+    }
+    for (int k = 3; k < 6; k++) {
+      lo[k] = grs_nm_max(x0[k] - 3.0, 0.0);
+      const double u = x0[k] + 3.0;
+      hi[k] = u != u ? 360.0 : (u < 360.0 ? u : 360.0);
+    }
+    nm[m].max_fe = max_fun_evals;
+    nm[m].max_it = max_iter;
+    nm[m].tol_f = tol_fun;
+    nm[m].tol_x = tol_x;
+    nm[m].start(6, x0, lo, hi);
+  }
+  ENSURE(c, c->m_rows, sizeof(double) * 23 * (size_t)total);
+  ENSURE(c, c->m_rptr, sizeof(int64_t) * (size_t)(M + 1));
+  HIPCHK(c, hipMemcpyAsync(c->m_rows.p, rows, sizeof(double) * 23 * (size_t)total, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->m_rptr.p, row_ptr, sizeof(int64_t) * (size_t)(M + 1), hipMemcpyHostToDevice, c->st));
+  std::vector<double> xs, dis, fvals;
+  std::vector<int32_t> pfit;
+  std::vector<int64_t> doff, first;
+  for (;;) {
+    xs.clear();
+    pfit.clear();
+    doff.clear();
+    first.assign((size_t)M + 1, 0);
+    int64_t nd = 0;
+    for (int64_t m = 0; m < M; m++) {
+      first[m] = (int64_t)pfit.size();
+      if (nm[m].done()) continue;
+      for (int p = 0; p < nm[m].npend; p++) {
+        double x[6];
+        nm[m].xform(nm[m].pend[p], x);
+        for (int k = 0; k < 6; k++) xs.push_back(x[k]);
+        pfit.push_back((int32_t)m);
+        doff.push_back(nd);
+        nd += row_ptr[m + 1] - row_ptr[m];
+      }
+    }
+    first[M] = (int64_t)pfit.size();
+    const int64_t np = (int64_t)pfit.size();
+    if (np == 0) break;
+    ENSURE(c, c->m_x, sizeof(double) * 6 * np);
+    ENSURE(c, c->m_pfit, sizeof(int32_t) * np);
+    ENSURE(c, c->m_doff, sizeof(int64_t) * np);
+    ENSURE(c, c->m_dis, sizeof(double) * nd);
+    HIPCHK(c, hipMemcpyAsync(c->m_x.p, xs.data(), sizeof(double) * 6 * np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->m_pfit.p, pfit.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->m_doff.p, doff.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, grs_launch_com_dist(P<grs_dgeo>(c->d_geo), P<double>(c->m_rows), P<int64_t>(c->m_rptr),
+                                  P<double>(c->m_x), P<int32_t>(c->m_pfit), P<int64_t>(c->m_doff), np, maxn,
+                                  P<double>(c->m_dis), c->st));
+    dis.resize((size_t)nd);
+    HIPCHK(c, hipMemcpyAsync(dis.data(), c->m_dis.p, sizeof(double) * nd, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    fvals.resize((size_t)np);
+    for (int64_t p = 0; p < np; p++) {
+      /* mean(Err): the sum in pair order over the count */
+      const int64_t n = row_ptr[pfit[p] + 1] - row_ptr[pfit[p]];
+      double acc = 0.0;
+      for (int64_t i = 0; i < n; i++) acc = acc + dis[(size_t)(doff[p] + i)];
+      fvals[p] = acc / (double)n;
+    }
+    for (int64_t m = 0; m < M; m++)
+      if (first[m + 1] > first[m]) nm[m].feed(&fvals[first[m]]);
+  }
+  for (int64_t m = 0; m < M; m++) {
+    nm[m].result(x_out + 6 * m, fval + m);
+    n_fun_evals[m] = nm[m].nf;
+    n_iter[m] = nm[m].it;
+  }
+  return GRS_OK;
+}
+
+grs_status grs_u2euler(const double *U_colmajor, int64_t M, double *euler_deg) {
+  if (M < 0 || (M > 0 && (!U_colmajor || !euler_deg))) return GRS_ERR_ARG;
+  for (int64_t m = 0; m < M; m++) grs_u2euler_corr(U_colmajor + 9 * m, euler_deg + 3 * m);
   return GRS_OK;
 }
 

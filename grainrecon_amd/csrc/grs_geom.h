@@ -254,6 +254,34 @@ GRS_HD int grs_event_finish(const grs_dgeo *g, const grs_vw *v, const double *Gt
   return grs_event_finish_yz(g, v, Gt, Glen, beta, dety_out, detz_out, (double *)0);
 }
 
+/* The spot pixel of an event without the selection tests
+ * (fit_grain_COM.m:60-83, the same statements as forward_comp.m:33-59):
+ * the unrounded pixel coordinates -dety22/py + dety0 and -detz22/pz + detz0. */
+GRS_HD void grs_event_pixel(const grs_dgeo *g, const grs_vw *v, const double *Gt, double Glen, double beta,
+                            double *ry, double *rz) {
+  GRS_NO_CONTRACT
+  (void)Glen;
+  double theta = beta - g->pio2;
+  double sintth = grs_sin(2.0 * theta);
+  double Gt2 = Gt[1], Gt3 = Gt[2];
+  double nv = grs_sqrt(Gt2 * Gt2 + Gt3 * Gt3);
+  double phix = grs_acos((Gt2 / nv) * v->cu1 + (Gt3 / nv) * v->cu2);
+  double phiy = phix - 2.0 * theta;
+  double diffvec = (v->L2 * sintth) / grs_sin(phiy);
+  double y22 = v->ctr1 + ((diffvec * Gt2) / nv);
+  double z22 = v->ctr2 + ((diffvec * Gt3) / nv);
+  double k0 = v->k0, k1 = y22 - v->SW1, k2 = z22 - v->SW2;
+  double nk = grs_sqrt((k0 * k0 + k1 * k1) + k2 * k2);
+  double K0 = k0 / nk, K1 = k1 / nk, K2 = k2 / nk;
+  const double *R = g->R;
+  double t = v->tnum / ((R[0] * K0 + R[3] * K1) + R[6] * K2);
+  double w0 = t * K0 + (-k0), w1 = t * K1 + v->ov1, w2 = t * K2 + v->ov2;
+  double Y22 = (R[1] * w0 + R[4] * w1) + R[7] * w2;
+  double Z22 = (R[2] * w0 + R[5] * w1) + R[8] * w2;
+  *ry = (-Y22) / g->py + g->dety0;
+  *rz = (-Z22) / g->pz + g->detz0;
+}
+
 /* MATLAB a:d:b, following MathWorks' published description of the colon
  * operator: n = round((b-a)/d) intervals, one fewer when a+n*d overshoots b
  * by more than tol = 2*eps*max(|a|,|b|); the end point snapped to b within

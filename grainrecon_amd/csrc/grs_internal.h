@@ -199,4 +199,9 @@ hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const i
                                const double *d_proj, const int64_t *d_fptr, const double *d_units, double *d_scratch,
                                int64_t cap, int64_t waves, double *d_out, hipStream_t st);
 
+/* fit_grain_COM objective (grs_com.hip) */
+hipError_t grs_launch_com_dist(const grs_dgeo *d_g, const double *d_rows, const int64_t *d_rptr, const double *d_x,
+                               const int32_t *d_pfit, const int64_t *d_doff, int64_t npoints, int64_t maxpairs,
+                               double *d_dis, hipStream_t st);
+
 #endif

@@ -112,4 +112,71 @@ GRS_HD int grs_fit_orientation(const double *x, double *out10, double *U) {
   return 1;
 }
 
+/* euler2u.m:5-16 with the angles in radians, U row-major. */
+GRS_HD void grs_euler2u_rad(double phi1, double PHI, double phi2, double *U) {
+  GRS_NO_CONTRACT
+  const double c1 = grs_cos(phi1), s1 = grs_sin(phi1), c2 = grs_cos(phi2), s2 = grs_sin(phi2);
+  const double cP = grs_cos(PHI), sP = grs_sin(PHI);
+  U[0] = c1 * c2 - (s1 * s2) * cP;
+  U[3] = s1 * c2 + (c1 * s2) * cP;
+  U[6] = s2 * sP;
+  U[1] = -c1 * s2 - (s1 * c2) * cP;
+  U[4] = -s1 * s2 + (c1 * c2) * cP;
+  U[7] = c2 * sP;
+  U[2] = s1 * sP;
+  U[5] = -c1 * sP;
+  U[8] = cP;
+}
+
+/* u2euler_corr.m:1-52 for U given column-major (as reshape(row,3,3) of a
+ * HittedSpots row): the ZXZ angles in degrees of the branch whose euler2u is
+ * closest to U in the sum of absolute differences (first minimum). */
+GRS_HD void grs_u2euler_corr(const double *Ucm, double *out3) {
+  GRS_NO_CONTRACT
+#define GRS_UCM(r, c) Ucm[((c) - 1) * 3 + ((r) - 1)]
+  const double ph = grs_acos(GRS_UCM(3, 3));
+  if (ph < 0.0001) {
+    const double p2 = grs_atan(GRS_UCM(2, 1) / GRS_UCM(1, 1)) / 2.0;
+    out3[0] = (p2 * 180.0) / GRS_PI;
+    out3[1] = (ph * 180.0) / GRS_PI;
+    out3[2] = (p2 * 180.0) / GRS_PI;
+    return;
+  }
+  double a1 = grs_atan2(-GRS_UCM(1, 3), GRS_UCM(2, 3)), a2;
+  double b1 = grs_atan2(GRS_UCM(3, 1), GRS_UCM(3, 2)), b2;
+  if (a1 < 0.0) {
+    a1 = a1 + 2.0 * GRS_PI;
+    a2 = a1 - GRS_PI;
+  } else {
+    a2 = a1 + GRS_PI;
+  }
+  if (b1 < 0.0) {
+    b1 = b1 + 2.0 * GRS_PI;
+    b2 = b1 - GRS_PI;
+  } else {
+    b2 = b1 + GRS_PI;
+  }
+  const double p1s[4] = {a1, a2, a1, a2}, p2s[4] = {b1, b1, b2, b2};
+  double best = 0.0;
+  int bi = 0;
+  for (int k = 0; k < 4; k++) {
+    double Uk[9];
+    grs_euler2u_rad(p1s[k], ph, p2s[k], Uk);
+    double tot = 0.0;
+    for (int c = 1; c <= 3; c++) {
+      double col = 0.0;
+      for (int r = 1; r <= 3; r++) col = col + grs_fabs(Uk[(r - 1) * 3 + (c - 1)] - GRS_UCM(r, c));
+      tot = tot + col;
+    }
+    if (k == 0 || tot < best) {
+      best = tot;
+      bi = k;
+    }
+  }
+  out3[0] = (p1s[bi] * 180.0) / GRS_PI;
+  out3[1] = (ph * 180.0) / GRS_PI;
+  out3[2] = (p2s[bi] * 180.0) / GRS_PI;
+#undef GRS_UCM
+}
+
 #endif /* GRS_ORIENT_H */

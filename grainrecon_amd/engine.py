@@ -63,6 +63,15 @@ def fit_orientations(x):
     return out, U.reshape(M, 3, 3), ok
 
 
+def u2euler(U_colmajor):
+    """u2euler_corr.m for matrices given column-major (M, 9): (M, 3) degrees."""
+    U = _d(np.asarray(U_colmajor).reshape(-1, 9))
+    out = np.zeros((U.shape[0], 3))
+    if L.lib().grs_u2euler(_ptr(U, _dp), U.shape[0], _ptr(out, _dp)) != 0:
+        raise ValueError("grs_u2euler: invalid argument")
+    return out
+
+
 class Engine:
     """FP64 scoring context on one GPU (grs_ctx)."""
 
@@ -253,6 +262,26 @@ class Engine:
         self._chk(self._lib.grs_pair_spots(self._h, _ptr(pos, _dp), _ptr(U, _dp), M, _ptr(ns, _ip), _ptr(ni, _ip),
                                            _ptr(med, _fp), rp.ctypes.data_as(i64p), _ptr(rows, _dp), rows.shape[0]))
         return ns, ni, med, [rows[rp[m]:rp[m + 1]].copy() for m in range(M)]
+
+    # -- grain centre-of-mass fit -----------------------------------------------------
+    def fit_grain_com(self, pair_rows, max_fun_evals: int = 2000, max_iter: int = 500, tol_fun: float = 1e-10,
+                      tol_x: float = 1e-4):
+        """fit_grain_COM.m for a list of per-grain HittedSpots row arrays
+        ((n_m, 23), grs_pair_spots layout).  Returns (FitOutput (M, 6),
+        fval (M,), n_fun_evals, n_iter)."""
+        rows = [np.asarray(r, dtype=float).reshape(-1, 23) for r in pair_rows]
+        M = len(rows)
+        rp = np.zeros(M + 1, np.int64)
+        rp[1:] = np.cumsum([r.shape[0] for r in rows])
+        flat = _d(np.concatenate(rows) if M else np.zeros((1, 23)))
+        x = np.zeros((M, 6))
+        fv = np.zeros(M)
+        nf = np.zeros(M, np.int32)
+        it = np.zeros(M, np.int32)
+        self._chk(self._lib.grs_fit_grain_com(self._h, _ptr(flat, _dp), rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                              M, int(max_fun_evals), int(max_iter), float(tol_fun), float(tol_x),
+                                              _ptr(x, _dp), _ptr(fv, _dp), _ptr(nf, _ip), _ptr(it, _ip)))
+        return x, fv, nf, it
 
     # -- G-vector matching score ---------------------------------------------------
     def forward_gt(self, pos, q, fam_sq, d_possible) -> np.ndarray:

@@ -369,6 +369,26 @@ grs_status grs_grow_region(grs_ctx *ctx, const int32_t *seed, double indexed_com
                            int32_t *replace_center, int64_t *n_indexed, int32_t *indexed, int64_t cap,
                            int64_t *n_checked);
 
+/* ---- grain centre-of-mass fit ------------------------------------------------------ */
+
+/* fit_grain_COM.m:5-29 (fit_grain_COM_only = 0) for M grains: the pair rows
+ * of grain m (rows[23*row_ptr[m] ..], the grs_pair_spots layout without the
+ * leading seed column) give x0 = [position (columns 11:13 of hittedSpots_pair),
+ * u2euler_corr(reshape(columns 2:10, 3, 3))] and the bounds x0 -/+ 0.1 mm and
+ * max(x0-3, 0) .. min(x0+3, 360) degrees; fminsearchbnd (MaxFunEvals,
+ * MaxIter, TolFun, TolX; the reference passes 2000, 500, 1e-10 and the
+ * default 1e-4) minimises fit_grain_COM_fun: the mean over the pairs of the
+ * distance between the rounded simulated pixel of the pair's (hkl, omega)
+ * and its measured centroid.  The fits run together, each step's points
+ * scored in one launch.  x_out (M x 6) = FitOutput, fval, the evaluation and
+ * iteration counts.  At most 7281 grains per call. */
+grs_status grs_fit_grain_com(grs_ctx *ctx, const double *rows, const int64_t *row_ptr, int64_t M,
+                             int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
+                             double *fval, int32_t *n_fun_evals, int32_t *n_iter);
+
+/* u2euler_corr.m for M matrices given column-major (host only). */
+grs_status grs_u2euler(const double *U_colmajor, int64_t M, double *euler_deg);
+
 /* ---- G-vector matching score ("Gt" indexing method) ------------------------------ */
 
 /* The score behind cuda_forward_Gt (ForwardCalc_OR_space_Gt_cuda.m:7-47),
