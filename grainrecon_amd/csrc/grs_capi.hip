@@ -2072,6 +2072,23 @@ grs_status grs_fit_grain_com(grs_ctx *c, const double *rows, const int64_t *row_
   return GRS_OK;
 }
 
+grs_status grs_euler_to_rod_quat(const double *euler_deg, int64_t M, double *rod_quat, int32_t *ok) {
+  if (M < 0 || (M > 0 && (!euler_deg || !rod_quat || !ok))) return GRS_ERR_ARG;
+  for (int64_t m = 0; m < M; m++) {
+    double *o = rod_quat + 7 * m;
+    ok[m] = grs_angle2rod_zxz_deg(euler_deg + 3 * m, o);
+    grs_rod2quat(o, o + 3);
+  }
+  return GRS_OK;
+}
+
+grs_status grs_euler2u_deg(const double *euler_deg, int64_t M, double *U) {
+  if (M < 0 || (M > 0 && (!euler_deg || !U))) return GRS_ERR_ARG;
+  for (int64_t m = 0; m < M; m++) grs_U_from_euler_deg(euler_deg[3 * m], euler_deg[3 * m + 1], euler_deg[3 * m + 2],
+                                                       U + 9 * m);
+  return GRS_OK;
+}
+
 grs_status grs_u2euler(const double *U_colmajor, int64_t M, double *euler_deg) {
   if (M < 0 || (M > 0 && (!U_colmajor || !euler_deg))) return GRS_ERR_ARG;
   for (int64_t m = 0; m < M; m++) grs_u2euler_corr(U_colmajor + 9 * m, euler_deg + 3 * m);

@@ -21,6 +21,47 @@
 
 #include "grs_math.h"
 
+/* angle2rod(a1*pi/180, a2*pi/180, a3*pi/180, 'ZXZ') (angle2rod.m 'zxz' case
+ * and :120-126) for angles in degrees; 0 when acos((tr-1)/2) would be
+ * complex. */
+GRS_HD int grs_angle2rod_zxz_deg(const double *ang, double *rod) {
+  GRS_NO_CONTRACT
+  const double b1 = (ang[0] * GRS_PI) / 180.0, b2 = (ang[1] * GRS_PI) / 180.0, b3 = (ang[2] * GRS_PI) / 180.0;
+  const double cps = grs_cos(b1), sps = grs_sin(b1);
+  const double cth = grs_cos(b2), sth = grs_sin(b2);
+  const double cph = grs_cos(b3), sph = grs_sin(b3);
+  const double tr = (((cph * cps - (sph * cth) * sps) - sph * sps) + (cph * cth) * cps) + cth;
+  const double e31 = sth * sps - sph * sth;
+  const double e23 = cph * sth + sth * cps;
+  const double e12 = ((cph * sps + (sph * cth) * cps) + sph * cps) + (cph * cth) * sps;
+  const double c = (tr - 1.0) / 2.0;
+  rod[0] = rod[1] = rod[2] = 0.0;
+  if (!(c >= -1.0 && c <= 1.0)) return 0;
+  const double th = grs_acos(c);
+  if (th != 0.0) {
+    const double d = 2.0 * grs_sin(th);
+    const double sy = e31 / d, sx = e23 / d, sz = e12 / d;
+    const double t = grs_tan(th / 2.0);
+    rod[0] = t * sx;
+    rod[1] = t * sy;
+    rod[2] = t * sz;
+  }
+  return 1;
+}
+
+/* rod2quat.m: the zero quaternion for a zero vector. */
+GRS_HD void grs_rod2quat(const double *rod, double *qo) {
+  GRS_NO_CONTRACT
+  qo[0] = qo[1] = qo[2] = qo[3] = 0.0;
+  const double nr = grs_sqrt((rod[0] * rod[0] + rod[1] * rod[1]) + rod[2] * rod[2]);
+  if (nr != 0.0) {
+    const double th2 = grs_atan(nr);
+    const double s2 = grs_sin(th2);
+    qo[0] = grs_cos(th2);
+    for (int k = 0; k < 3; k++) qo[k + 1] = (rod[k] * s2) / nr;
+  }
+}
+
 /* out10 = [yaw pitch roll (degrees), rod(3), quat(4)], U row-major 3x3. */
 GRS_HD int grs_fit_orientation(const double *x, double *out10, double *U) {
   GRS_NO_CONTRACT
@@ -65,36 +106,9 @@ GRS_HD int grs_fit_orientation(const double *x, double *out10, double *U) {
   double ang[3] = {(r1 * 180.0) / GRS_PI, (r2 * 180.0) / GRS_PI, (r3 * 180.0) / GRS_PI};
   for (int k = 0; k < 3; k++)
     if (ang[k] < 0.0) ang[k] = ang[k] + 360.0;
-  /* angle2rod(yaw*pi/180, pitch*pi/180, roll*pi/180, 'ZXZ') */
-  const double b1 = (ang[0] * GRS_PI) / 180.0, b2 = (ang[1] * GRS_PI) / 180.0, b3 = (ang[2] * GRS_PI) / 180.0;
-  const double cps = grs_cos(b1), sps = grs_sin(b1);
-  const double cth = grs_cos(b2), sth = grs_sin(b2);
-  const double cph = grs_cos(b3), sph = grs_sin(b3);
-  const double tr = (((cph * cps - (sph * cth) * sps) - sph * sps) + (cph * cth) * cps) + cth;
-  const double e31 = sth * sps - sph * sth;
-  const double e23 = cph * sth + sth * cps;
-  const double e12 = ((cph * sps + (sph * cth) * cps) + sph * cps) + (cph * cth) * sps;
-  const double c = (tr - 1.0) / 2.0;
-  if (!(c >= -1.0 && c <= 1.0)) return 0;
-  const double th = grs_acos(c);
-  double rod[3] = {0.0, 0.0, 0.0};
-  if (th != 0.0) {
-    const double d = 2.0 * grs_sin(th);
-    const double sy = e31 / d, sx = e23 / d, sz = e12 / d;
-    const double t = grs_tan(th / 2.0);
-    rod[0] = t * sx;
-    rod[1] = t * sy;
-    rod[2] = t * sz;
-  }
-  /* rod2quat */
-  double qo[4] = {0.0, 0.0, 0.0, 0.0};
-  const double nr = grs_sqrt((rod[0] * rod[0] + rod[1] * rod[1]) + rod[2] * rod[2]);
-  if (nr != 0.0) {
-    const double th2 = grs_atan(nr);
-    const double s2 = grs_sin(th2);
-    qo[0] = grs_cos(th2);
-    for (int k = 0; k < 3; k++) qo[k + 1] = (rod[k] * s2) / nr;
-  }
+  double rod[3], qo[4];
+  if (!grs_angle2rod_zxz_deg(ang, rod)) return 0;
+  grs_rod2quat(rod, qo);
   /* quaternion2U */
   const double q0 = qo[0], q1 = qo[1], q2 = qo[2], q3 = qo[3];
   U[0] = 2.0 * (q0 * q0 + q1 * q1) - 1.0;

@@ -63,6 +63,27 @@ def fit_orientations(x):
     return out, U.reshape(M, 3, 3), ok
 
 
+def euler_to_rod_quat(euler_deg):
+    """rod = angle2rod(euler*pi/180,'ZXZ'), quat = rod2quat(rod) for (M, 3)
+    degrees: returns ((M, 3) rod, (M, 4) quat, ok (M,))."""
+    e = _d(np.asarray(euler_deg).reshape(-1, 3))
+    out = np.zeros((e.shape[0], 7))
+    ok = np.zeros(e.shape[0], np.int32)
+    if L.lib().grs_euler_to_rod_quat(_ptr(e, _dp), e.shape[0], _ptr(out, _dp), _ptr(ok, _ip)) != 0:
+        raise ValueError("grs_euler_to_rod_quat: invalid argument")
+    return out[:, :3].copy(), out[:, 3:].copy(), ok
+
+
+def euler2u_deg(euler_deg):
+    """euler2u.m of (M, 3) ZXZ angles in degrees with the engine's FP64
+    functions: (M, 3, 3)."""
+    e = _d(np.asarray(euler_deg).reshape(-1, 3))
+    U = np.zeros((e.shape[0], 9))
+    if L.lib().grs_euler2u_deg(_ptr(e, _dp), e.shape[0], _ptr(U, _dp)) != 0:
+        raise ValueError("grs_euler2u_deg: invalid argument")
+    return U.reshape(-1, 3, 3)
+
+
 def u2euler(U_colmajor):
     """u2euler_corr.m for matrices given column-major (M, 9): (M, 3) degrees."""
     U = _d(np.asarray(U_colmajor).reshape(-1, 9))

@@ -1,0 +1,184 @@
+"""The seed -> index -> fit -> grow -> re-centre loop of
+GrainMapping_engine_gpu_comp.m:54-331 on the device stages.
+
+Every numeric step runs in the engine: seed indexing (grs_index_seeds,
+fit_OR_constrain_with_local_comp_cuda), the spot pairing of the fitted
+orientation (grs_pair_spots), the grain centre-of-mass fit
+(grs_fit_grain_com), the re-scoring after a centre update (the counts of
+grs_pair_spots), the region growth on the resident Completeness / Mask /
+Dismedian / Ninter volumes (grs_grow_region).  This module keeps the
+reference's per-seed control flow and the host-side DS volumes (GrainId,
+Rodrigues, IPF001, EulerZXZ, VisitFlag).  Branches the engine does not cover
+raise NotImplementedError: the fit_all mode
+(fit_OR_constrain_with_local_comp_cuda_2.m, entered only through
+check_fit_all) and the CPU connected-voxel growth grow_indexed_region.m
+(used when the spot half length is at most 2 pixels).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .engine import euler2u_deg, euler_to_rod_quat
+from .geometry import quaternion2U
+from .indexing import pos2ind
+
+
+class MappingDS:
+    """DS of GrainMapping_engine_gpu_comp.m:24-34 (volumes indexed
+    [x-1, y-1, z-1]); Completeness, Mask, Dismedian and Ninter are mirrored on
+    the engine for the growth."""
+
+    def __init__(self, mask):
+        mask = np.asarray(mask, dtype=float)
+        nx, ny, nz = mask.shape
+        self.Mask = mask.copy()
+        self.Completeness = np.zeros((nx, ny, nz))
+        self.GrainId = np.zeros((nx, ny, nz))
+        self.Rodrigues = np.zeros((3, nx, ny, nz))
+        self.IPF001 = np.zeros((3, nx, ny, nz))
+        self.EulerZXZ = np.zeros((3, nx, ny, nz))
+        self.Dismedian = np.zeros((nx, ny, nz))
+        self.Ninter = np.zeros((nx, ny, nz))
+        self.VisitFlag = np.zeros((nx, ny, nz))
+
+
+def _upload(eng, ds):
+    for k in ("Completeness", "Mask", "Dismedian", "Ninter"):
+        eng.set_volume(k, getattr(ds, k))
+
+
+def _download(eng, ds):
+    for k in ("Completeness", "Dismedian", "Ninter"):
+        setattr(ds, k, eng.get_volume(k))
+
+
+def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_comp, drop_off, maxD, max_dmedian,
+                         nr_seed, first_grain_id, center, vox_size, rvp_first, tomo_dim, simap=False):
+    """GrainMapping_engine_gpu_comp.m:54-331 with check_fit_all = 0.
+    OR_q: the orientation set (N, 4); ds is updated in place.  Returns
+    (indexing_final (P, 18), pos_list_new (P, 3), Nr_indexed_voxel (P,))."""
+    pos_list = np.asarray(pos_list, dtype=float).reshape(-1, 3)
+    OR_q = np.asarray(OR_q, dtype=float).reshape(-1, 4)
+    P = pos_list.shape[0]
+    dims = ds.Completeness.shape
+    ind_final = np.zeros((P, 18))
+    pos_new = np.zeros((P, 3))
+    nr_vox = np.zeros(P, dtype=np.int64)
+    rcc = np.zeros(P, dtype=np.int64)
+    p1 = np.array([rvp_first[0], rvp_first[1], rvp_first[2], 1.0 if simap else 0.0, vox_size, center[0], center[1],
+                   center[2], tomo_dim[0], tomo_dim[1], tomo_dim[2]], dtype=float)
+    eng.set_orientation_set(OR_q)
+    _upload(eng, ds)
+    temp = None
+
+    def p2i(p):
+        return pos2ind(p, center, vox_size, rvp_first, dims, tomo_dim, simap)
+
+    for i in range(P):
+        gid = i + first_grain_id
+        stop = False
+        pos_ix = pos_list[i].copy()
+        while not stop and rcc[i] <= 3:
+            x, y, z = (v - 1 for v in p2i(pos_ix))
+            c_here = ds.Completeness[x, y, z]
+            if not ((c_here < trust_comp or (rcc[i] > 0 and c_here >= trust_comp)) and ds.VisitFlag[x, y, z] != 1):
+                stop = True
+                pos_new[i] = pos_ix
+                break
+            if rcc[i] == 0:
+                res = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
+            else:
+                # :87-91: the indexing repeated on ori_local_sampling(q, 4) of the last fit
+                local = eng.local_sampling(temp[14:18], 4, 0.004)[0]
+                eng.set_orientation_set(local)
+                try:
+                    res = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
+                finally:
+                    eng.set_orientation_set(OR_q)
+            row = res["indexing"][0].copy()
+            row[0] = i + 1
+            U = quaternion2U(row[14:18])
+            _, _, _, pairs = eng.pair_spots(pos_ix[None, :], U[None, :, :])
+            pairs = pairs[0]
+            ind_final[i] = row
+            if rcc[i] == 0 and pairs.shape[0]:
+                # :140-170: grain centre of mass from the paired spots
+                fit_out, _, _, _ = eng.fit_grain_com([pairs])
+                fo = fit_out[0]
+                d = fo[:3] - ind_final[i, 1:4]
+                if math.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]) / vox_size > maxD:
+                    tx, ty, tz = (v - 1 for v in p2i(fo[:3]))
+                    if ds.Completeness[tx, ty, tz] < ind_final[i, 6] and ds.Mask[tx, ty, tz] == 1:
+                        ind_final[i, 1:4] = fo[:3]
+                        ind_final[i, 8:11] = fo[3:6]
+                        rod, quat, _ = euler_to_rod_quat(fo[3:6][None, :])
+                        ind_final[i, 11:14] = rod[0]
+                        ind_final[i, 14:18] = quat[0]
+                        UU = euler2u_deg(ind_final[i, 8:11][None, :])
+                        ns, ni, med, _ = eng.pair_spots(ind_final[i, 1:4][None, :], UU)
+                        ind_final[i, 4] = ns[0]
+                        ind_final[i, 5] = ni[0]
+                        ind_final[i, 7] = float(med[0])
+                        ind_final[i, 6] = ind_final[i, 5] / ind_final[i, 4] if ns[0] else math.nan
+                        pos_ix = fo[:3].copy()
+                        x, y, z = (v - 1 for v in p2i(pos_ix))
+            Ci, dmed = ind_final[i, 6], ind_final[i, 7]
+            if Ci < min_comp or Ci < ds.Completeness[x, y, z] or dmed > max_dmedian:
+                stop = True
+                pos_new[i] = pos_ix
+                ds.VisitFlag[x, y, z] = 0.8
+                break
+            temp = ind_final[i].copy()
+            centre = np.array([x + 1, y + 1, z + 1], dtype=float)
+            replace = 0
+            if temp[6] >= ds.Completeness[x, y, z]:
+                ds.Completeness[x, y, z] = temp[6]
+                UU = quaternion2U(temp[14:18])
+                if pairs.shape[0]:
+                    a = pairs[:, 19]
+                    n = a.size
+                    mu = sum(float(v) for v in a) / n if n else math.nan
+                    sd = math.sqrt(sum((float(v) - mu) ** 2 for v in a) / (n - 1)) if n > 1 else 0.0
+                    spot_l = math.sqrt((mu + 2 * sd) / math.pi)
+                else:
+                    spot_l = 20.0
+                if spot_l <= 2:
+                    raise NotImplementedError("grow_indexed_region.m (connected-voxel growth for Spot_L <= 2 px)")
+                g = eng.grow_region((x + 1, y + 1, z + 1), temp[6], UU, spot_l, drop_off, maxD, max_dmedian, p1)
+                _download(eng, ds)
+                idx = g["indexed"] - 1
+                nr_vox[i] = idx.shape[0]
+                xi, yi, zi = idx[:, 0], idx[:, 1], idx[:, 2]
+                ds.GrainId[xi, yi, zi] = gid
+                eu = temp[8:11]
+                ds.Rodrigues[:, xi, yi, zi] = temp[11:14][:, None]
+                ds.IPF001[:, xi, yi, zi] = (eu / math.sqrt((eu[0] * eu[0] + eu[1] * eu[1]) + eu[2] * eu[2]))[:, None]
+                ds.EulerZXZ[:, xi, yi, zi] = eu[:, None]
+                ds.VisitFlag[xi, yi, zi] = 0.5
+                ds.VisitFlag[x, y, z] = 1.0
+                centre = g["center"].copy()
+                replace = g["replace_center"]
+                if replace == 1:
+                    cx, cy, cz = (int(v) - 1 for v in centre)
+                    if ds.GrainId[cx, cy, cz] != gid:
+                        # :238-253: the grain's voxel nearest to the new centre
+                        lin = np.flatnonzero(ds.GrainId.reshape(-1, order="F") == gid)
+                        allid = np.stack(np.unravel_index(lin, dims, order="F"), 1) + 1
+                        dd = np.sqrt(((allid - centre) ** 2).sum(1))
+                        centre = allid[int(np.argmin(dd))].astype(float)
+                        dc = centre - np.array([x + 1, y + 1, z + 1])
+                        replace = 1 if math.sqrt((dc[0] * dc[0] + dc[1] * dc[1]) + dc[2] * dc[2]) > maxD else 0
+            if replace == 0:
+                stop = True
+                pos_new[i] = pos_ix
+            else:
+                rcc[i] += 1
+                pn = ((centre + np.asarray(rvp_first, float) - 1) - np.asarray(tomo_dim, float) / 2) * vox_size + \
+                    np.asarray(center, float)
+                if simap:
+                    pn[0], pn[1] = -pn[0], -pn[1]
+                pos_new[i] = pn
+                pos_ix = pn.copy()
+    return ind_final, pos_new, nr_vox

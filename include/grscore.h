@@ -386,6 +386,16 @@ grs_status grs_fit_grain_com(grs_ctx *ctx, const double *rows, const int64_t *ro
                              int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
                              double *fval, int32_t *n_fun_evals, int32_t *n_iter);
 
+/* rod = angle2rod(euler*pi/180, 'ZXZ'), quat = rod2quat(rod) for M ZXZ angle
+ * triples in degrees (GrainMapping_engine_gpu_comp.m:148-149; host only):
+ * rod_quat (M x 7) = [rod(3) quat(4)], ok = 0 where acos((tr-1)/2) would be
+ * complex (rod zero). */
+grs_status grs_euler_to_rod_quat(const double *euler_deg, int64_t M, double *rod_quat, int32_t *ok);
+
+/* euler2u.m for M ZXZ angle triples in degrees converted as (deg*pi)/180
+ * (host only; the engine's FP64 functions): U (M x 9) row-major. */
+grs_status grs_euler2u_deg(const double *euler_deg, int64_t M, double *U);
+
 /* u2euler_corr.m for M matrices given column-major (host only). */
 grs_status grs_u2euler(const double *U_colmajor, int64_t M, double *euler_deg);
 
