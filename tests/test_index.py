@@ -173,3 +173,22 @@ def test_index_seeds_graph_replay(engine):
     for k in ("indexing", "candidate", "x", "seed_idx", "seed_C"):
         assert _same(np.concatenate([a[k], b[k]]), d[k]), k
         assert _same(c[k], a[k]), k
+
+
+@pytest.mark.gpu
+def test_index_seeds_small_set_and_more_fits_than_seeds(engine):
+    """Nr_seed above the set size keeps the whole set (the mapping engine's
+    rule), and Nr_fit above Nr_seed fits every seed."""
+    s, qs, pos = _index_problem()
+    small = qs[[100, 600, 1100, 3, 4]]
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    engine.set_orientation_set(small)
+    got = engine.index_seeds(pos[:2], nr_seed=10, n_fit=7)
+    for p in range(2):
+        ref = IS.index_seed(s.geo, s.stack, pos[p], small, s.rot, s.hkl, nr_seed=10, n_fit=7)
+        assert list(got["seed_idx"][p][:5]) == list(ref["seed_idx"]) and np.all(got["seed_idx"][p][5:] == -1)
+        assert np.all(np.isnan(got["seed_C"][p][5:]))
+        row = ref["indexing"].copy()
+        row[0] = p + 1
+        assert _same(got["indexing"][p], row) and got["candidate"][p] == ref["candidate"]
