@@ -430,6 +430,61 @@ def gt_leg(eng, cfg, n_orient, cpu, threads, seed=6006):
     return res
 
 
+def mapping_leg(eng, cfg, q, n_seeds, nr_seed, seed=8008):
+    """The seed -> index -> pair -> centre-of-mass fit -> grow -> re-centre
+    loop of GrainMapping_engine_gpu_comp.m (grainrecon_amd.mapping) on config
+    2: seeds at random voxels, the 262,144-orientation set plus the grains'
+    orientations turned by 0.05 degrees, spot centroids of all grains with an
+    area of 800 px^2 (a grain-sized spot, so the growth box spans about 25
+    voxels)."""
+    from grainrecon_amd import fit as F
+    from grainrecon_amd import synth
+    from grainrecon_amd.mapping import MappingDS, grain_mapping_engine
+
+    smp, geo = cfg.sample, cfg.geo
+    rng = np.random.default_rng(seed)
+    com = _grain_com(smp)
+    sp = F.simulate_matched_spots(eng, None, com, smp.U)
+    frames = []
+    for i in range(cfg.rot.size):
+        yz = sp["meas"][sp["omega_idx"] == i]
+        frames.append(np.column_stack([yz, np.full(yz.shape[0], 800.0)]))
+    eng.set_frame_spots(frames)
+    G = smp.q.shape[0]
+    ax = rng.normal(size=(G, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    h = np.deg2rad(0.05) / 2
+    turn = np.column_stack([np.full(G, np.cos(h)), np.sin(h) * ax])
+    qset = np.concatenate([q, synth.fz_reduce_cubic(synth._qmul(smp.q, turn))])
+    nx, ny, nz = smp.dims
+    mask = np.zeros((nx, ny, nz))
+    mask[smp.index[:, 0] - 1, smp.index[:, 1] - 1, smp.index[:, 2] - 1] = 1.0
+    ds = MappingDS(mask)
+    picks = smp.pos[rng.choice(smp.pos.shape[0], size=n_seeds, replace=False)]
+    t = time.perf_counter()
+    out, _, nvox = grain_mapping_engine(eng, picks, qset, ds, trust_comp=0.9, min_comp=0.6, drop_off=0.05, maxD=2.0,
+                                        max_dmedian=10.0, nr_seed=nr_seed, first_grain_id=1, center=[0, 0, 0],
+                                        vox_size=smp.vox_mm, rvp_first=[1, 1, 1], tomo_dim=(nx, ny, nz))
+    wall = time.perf_counter() - t
+    gid = np.full((nx, ny, nz), -1, np.int32)
+    gid[smp.index[:, 0] - 1, smp.index[:, 1] - 1, smp.index[:, 2] - 1] = smp.grain
+    grown = ds.GrainId > 0
+    res = {"seeds": n_seeds, "nr_seed": nr_seed, "wall_s": round(wall, 3), "seeds_per_s": round(n_seeds / wall, 2),
+           "seeds_indexed": int((out[:, 6] >= 0.6).sum()), "voxels_assigned": int(grown.sum()),
+           "median_C": round(float(np.nanmedian(out[:, 6])), 4)}
+    # the assigned voxels' true grains: the share in the grain most assigned per seed id
+    same = 0
+    for sid in np.unique(ds.GrainId[grown]):
+        g = gid[ds.GrainId == sid]
+        g = g[g >= 0]
+        if g.size:
+            same += int(np.bincount(g).max())
+    res["voxels_in_majority_grain"] = same
+    log(f"[bench] mapping: {n_seeds} seeds in {wall:.2f}s ({res['seeds_per_s']} seeds/s), "
+        f"{res['voxels_assigned']} voxels assigned, {same} in their seed's majority grain")
+    return res
+
+
 def grow_leg(eng, cfg, box_l, cpu, threads, seed=7007):
     """Region growth (grow_indexed_region_cuda.m) around a voxel of a grain
     with the reconstruction volumes resident on the device, bounding sphere of
@@ -506,6 +561,7 @@ def main():
     ap.add_argument("--index-nr-seed", type=int, default=500)
     ap.add_argument("--gt-orientations", type=int, default=8192, help="orientations of the Gt-score leg; 0 = off")
     ap.add_argument("--grow-box", type=int, default=24, help="sphere radius (voxels) of the region-growth leg; 0 = off")
+    ap.add_argument("--mapping-seeds", type=int, default=6, help="seeds of the mapping-loop leg; 0 = off")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
                     help="size of a plain device copy timed (and visible to counter runs) for calibration; 0 = off")
     args = ap.parse_args()
@@ -649,6 +705,8 @@ def main():
         result["gt_score"] = gt_leg(eng, cfg, args.gt_orientations, not args.no_cpu, threads)
     if args.grow_box > 0:
         result["grow_region"] = grow_leg(eng, cfg, args.grow_box, not args.no_cpu, threads)
+    if args.mapping_seeds > 0:
+        result["mapping"] = mapping_leg(eng, cfg, q, args.mapping_seeds, args.index_nr_seed)
     result["autotune"] = {"median_kernel": eng.tuning(0), "counts_kernel": eng.tuning(1),
                           "counts_grid_kernel": eng.tuning(2)}
     if args.geo_fit_grains > 0:
