@@ -495,7 +495,7 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
   a.out_med = d_med;
   if (method == GRS_METHOD_FAST) {
     if (c->H > 240) return fail(c, GRS_ERR_ARG, "the fast method supports at most 240 reflections (got %d)", c->H);
-    ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)V);
+    ENSURE(c, c->d_vb, sizeof(double) * GRS_VB * (size_t)V);
     HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
                                       P<double>(c->d_vb), c->st));
     int64_t waves = grs_fast_wave_slots(c->device);
@@ -844,7 +844,7 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     HIPCHK(c, hipMemcpyAsync(c->d_roweoff.p, eo.data(), sizeof(int64_t) * (m + 1), hipMemcpyHostToDevice, c->st));
     if (kc > 0)
       HIPCHK(c, hipMemcpyAsync(c->d_candq.p, cand_q + 4 * c0, sizeof(double) * 4 * kc, hipMemcpyHostToDevice, c->st));
-    ENSURE(c, c->d_vb, sizeof(double) * 2 * (size_t)m);
+    ENSURE(c, c->d_vb, sizeof(double) * GRS_VB * (size_t)m);
     ENSURE(c, c->d_bestidx, sizeof(int64_t) * m);
     ENSURE(c, c->d_bestq, sizeof(double) * 4 * m);
     int64_t waves = grs_fast_wave_slots(c->device);
@@ -1430,7 +1430,7 @@ grs_status grs_index_seeds(grs_ctx *c, const double *pos, int64_t n_pos, int32_t
     const int64_t m = n_pos - p0 < chunk ? n_pos - p0 : chunk;
     const int64_t E1 = m * N, E2 = m * Rcap;
     ENSURE(c, c->x_pos, sizeof(double) * 3 * m);
-    ENSURE(c, c->x_vb, sizeof(double) * 2 * m);
+    ENSURE(c, c->x_vb, sizeof(double) * GRS_VB * m);
     ENSURE(c, c->x_key1, sizeof(unsigned long long) * E1);
     ENSURE(c, c->x_ns1, sizeof(int32_t) * E1);
     ENSURE(c, c->x_ni1, sizeof(int32_t) * E1);

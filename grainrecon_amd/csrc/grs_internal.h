@@ -42,7 +42,7 @@ typedef struct grs_eval_args {
   const grs_dgeo *g;       /* device copy of the geometry */
   const double *hkl;       /* H x 3 */
   const double *cs;        /* nrot x 2: cos, sin of omega */
-  const double *vb;        /* V x 2 omega-solve bounds (fast kernel) */
+  const double *vb;        /* V x GRS_VB omega-solve bounds (fast kernel) */
   const double *pos;       /* V x 3 voxel positions in mm */
   const double *SUB;       /* NU x 9 */
   const int32_t *valid;    /* NU: 0 -> orientation row skipped */
@@ -80,9 +80,11 @@ typedef struct grs_eval_args {
 #define GRS_KEY_MAX_EVENTS 23170
 /* largest k of the device top-k (sortrows + first rows) */
 #define GRS_TOPK_MAX 4096
+/* doubles per voxel of the omega-solve bounds (k_voxel_bounds) */
+#define GRS_VB 8
 
 /* Launchers (return hipError_t). */
-/* d_vb: 2 doubles per voxel, the omega-solve bounds */
+/* d_vb: GRS_VB doubles per voxel, the omega-solve bounds */
 hipError_t grs_launch_voxel_bounds(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs,
                                    int32_t nrot, double *d_vb, hipStream_t st);
 hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t row_stride, const double *d_p1,

@@ -109,34 +109,48 @@ __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, d
   u[2] = gp2 * r;
 }
 
-/* Per-voxel bounds of the omega-solve candidate filter:
- *   vb[2v]     >= max over omega of |u - x|, u the unit beam direction at the
- *               voxel (grainpos / norm(grainpos)) and x the unit x vector;
- *   vb[2v + 1] >= max over omega of sin^2(gamma/2), gamma the largest angle
- *               between u and the rays from the voxel to the detector corners
- *               (2 pixels outside the area); 1 (no detector bound) when a
- *               corner is 90 degrees or more off the beam or when the window
- *               admits 2 theta + angle(u, x) >= pi/2 - 1e-3, where the spot
- *               construction of forward_comp.m:40-55 no longer puts the
- *               diffracted ray at 2 theta from u.
- * A spot on the detector lies on a ray at 2 theta from u (the sine-rule
- * triangle of forward_comp.m:40-47 has angle 2 theta at the voxel whenever
- * 0 < phiy < pi), so sin^2(theta) = cos^2(beta) <= vb[2v + 1]. */
+/* Per-voxel bounds of the omega-solve candidate filter (GRS_VB doubles per
+ * voxel).  With u(omega) the unit beam direction at the voxel
+ * (grainpos / norm(grainpos), forward_comp.m:26) and P = S*pos' (RotAxisOffset
+ * taken off pos(2)), the first-order beam direction has
+ * u1 ~ al sin(omega) + be cos(omega) + ga with al = P0/Lss, be = P1/Lss,
+ * ga = (RotAxisOffset - P0y)/Lss; u0 ~ 1 and u2 varies only through the norm.
+ *   vb[0] W  >= max |u0 - 1| + max |u1 - (al sin + be cos + ga)| + hypot(al, be)/2
+ *   vb[1], vb[2]  min and max of u2 (widened by 1e-12)
+ *   vb[3] K  >= max sin^2(gamma/2), gamma the largest angle between u and the
+ *            rays from the voxel to the detector corners (2 pixels outside
+ *            the area); 1 (no detector bound) when a corner is 90 degrees or
+ *            more off the beam or when the window admits
+ *            2 theta + angle(u, x) >= pi/2 - 1e-3, where the spot
+ *            construction of forward_comp.m:40-55 no longer puts the
+ *            diffracted ray at 2 theta from u
+ *   vb[4..6] al, be, ga
+ * All maxima are over the rotation angles the scorer visits, so they bound u
+ * at every event.  A spot on the detector lies on a ray at 2 theta from u
+ * (the sine-rule triangle of forward_comp.m:40-47 has angle 2 theta at the
+ * voxel whenever 0 < phiy < pi), so sin^2(theta) = cos^2(beta) <= K. */
 __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
                                double *vb) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
   double P[3];
   voxel_P(g, pos + 3 * v, P);
-  double d2max = 0.0, kmax = 0.0;
+  const double al = P[0] / g->Lss, be = P[1] / g->Lss, ga = (g->RAO - g->P0y) / g->Lss;
+  double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0;
   bool front = true;
   for (int32_t om = 0; om < nrot; om++) {
     double sw[3], u[3];
-    beam_terms(g, P, cs[2 * om], cs[2 * om + 1], sw, u);
+    const double c = cs[2 * om], s = cs[2 * om + 1];
+    beam_terms(g, P, c, s, sw, u);
     const double u0 = u[0], u1 = u[1], u2 = u[2];
     const double sw0 = sw[0], sw1 = sw[1], sw2 = sw[2];
     const double d2 = (1.0 - u0) * (1.0 - u0) + u1 * u1 + u2 * u2;
     d2max = d2 > d2max ? d2 : d2max;
+    const double a0 = __builtin_fabs(u0 - 1.0), a1 = __builtin_fabs(u1 - ((al * s + be * c) + ga));
+    e0 = a0 > e0 ? a0 : e0;
+    e1 = a1 > e1 ? a1 : e1;
+    u2lo = u2 < u2lo ? u2 : u2lo;
+    u2hi = u2 > u2hi ? u2 : u2hi;
     double cmin = 1.0;
     for (int j = 0; j < 4; j++) {
       const double dx = g->corner[3 * j] - sw0, dy = g->corner[3 * j + 1] - sw1, dz = g->corner[3 * j + 2] - sw2;
@@ -154,8 +168,15 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
     K = kmax * (1.0 + 1e-6) + 1e-9;
     K = K < 1.0 ? K : 1.0;
   }
-  vb[2 * v] = delta;
-  vb[2 * v + 1] = K;
+  double *o = vb + GRS_VB * v;
+  o[0] = ((__builtin_sqrt(al * al + be * be) * 0.5 + e0) + e1) * (1.0 + 1e-9) + 1e-12;
+  o[1] = u2lo - 1e-12;
+  o[2] = u2hi + 1e-12;
+  o[3] = K;
+  o[4] = al;
+  o[5] = be;
+  o[6] = ga;
+  o[7] = 0.0;
 }
 
 __global__ void k_sub_from_q(const grs_dgeo *g, const double *q, int64_t N, double *SUB, int32_t *valid) {
@@ -543,14 +564,12 @@ __device__ __noinline__ double filt_atan2(double y, double x) {
   return y < 0.0 ? -a : a;
 }
 
-/* Omega solve of one reflection.  With g = S*U*B*hkl, |G| = gn and
- * r = |(g0, g1)|, the x component of the unit G at omega is
- * r cos(omega + phi) / gn, phi = atan2(g1, g0); the cos(beta) of the event
- * (forward_comp.m:31) differs from it by at most delta = max |u - x|.  The
- * event can pass the window and reach the detector only if cos(beta) lies in
- * [lo_cb, hi_cb], so omega + phi must lie in the arcs where
- * cos(psi) is in [(lo_cb - delta) gn / r, (hi_cb + delta) gn / r] (the caller
- * folds delta in).  The arcs are widened by 1e-6 rad and the index bounds by
+/* Omega solve of one reflection.  With (g0, g1) the in-plane part of a
+ * vector that rotates with omega, |G| = gn and r = |(g0, g1)|, the term
+ * r cos(omega + phi) / gn, phi = atan2(g1, g0), must lie in [lo_cb, hi_cb]
+ * (the caller folds every other part of cos(beta), forward_comp.m:31, and its
+ * bounds into the window), so omega + phi must lie in the arcs where
+ * cos(psi) is in [lo_cb gn / r, hi_cb gn / r].  The arcs are widened by 1e-6 rad and the index bounds by
  * 1e-6 steps; the resulting ranges of rotation indices [i0, i1] are sorted
  * and disjoint.  Returns their number (at most 4) or -1 when more would be
  * needed (the caller then scans every event). */
@@ -696,10 +715,16 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     /* per-reflection terms; with the omega solve also the candidate ranges */
     const bool solve = g->om_solve != 0;
     if (lane == 0) voxel_P(g, a.pos + 3 * vi, Pw);
-    double delta = 0.0, sK = 1.0;
+    double vW = 0.0, vu2lo = 0.0, vu2hi = 0.0, sK = 1.0, val = 0.0, vbe = 0.0, vga = 0.0;
     if (solve) {
-      delta = a.vb[2 * vi];
-      sK = __builtin_sqrt(a.vb[2 * vi + 1]);
+      const double *b = a.vb + GRS_VB * vi;
+      vW = b[0];
+      vu2lo = b[1];
+      vu2hi = b[2];
+      sK = __builtin_sqrt(b[3]);
+      val = b[4];
+      vbe = b[5];
+      vga = b[6];
     }
     uint32_t R = 0;
     bool overflow = false;
@@ -722,8 +747,12 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
         t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
         if (solve) {
-          /* cos(beta) window: beta and lambda bounds with the phase-A margins,
-           * the detector cone, then widened by delta */
+          /* cos(beta) window: beta and lambda bounds with the phase-A margins
+           * and the detector cone.  cos(beta) = u . G/gn lies within rg W
+           * of r' cos(omega + phi') / gn + c0 + u2 g2 / gn, where
+           * (g0', g1') = (g0 + ga g1, g1 - ga g0) has modulus r' and angle
+           * phi' and u2 spans [vb[1], vb[2]] (the voxel bounds), so the
+           * window moves onto the first term. */
           double lo_cb = cbm;
           double lw = -(lmax + ml) * gn / (4.0 * GRS_PI);
           lo_cb = lw > lo_cb ? lw : lo_cb;
@@ -731,7 +760,14 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
           double hi_cb = dcb;
           double hw = -(lmin - ml) * gn / (4.0 * GRS_PI);
           hi_cb = hw < hi_cb ? hw : hi_cb;
-          nr_h = om_ranges(g, gw[0], gw[1], gn, lo_cb - delta - 1e-12, hi_cb + delta + 1e-12, nrot, ri0, ri1);
+          const double rg = __builtin_sqrt(gw[0] * gw[0] + gw[1] * gw[1]) / gn;
+          const double c0 = (val * gw[0] + vbe * gw[1]) / (2.0 * gn);
+          const double z = gw[2] / gn;
+          const double sa = z * vu2lo, sb = z * vu2hi;
+          const double slo = sa < sb ? sa : sb, shi = sa < sb ? sb : sa;
+          const double w = rg * vW;
+          nr_h = om_ranges(g, gw[0] + vga * gw[1], gw[1] - vga * gw[0], gn, ((lo_cb - c0) - shi) - w - 1e-12,
+                           ((hi_cb - c0) - slo) + w + 1e-12, nrot, ri0, ri1);
         }
       }
       if (solve) {

@@ -49,6 +49,14 @@ def _variant_scene(name):
     s = scenes.make_scene(**kw)
     if name == "offsets":
         s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.02, -0.015, 0.01
+    if name == "sample_tilt":
+        # a tilted sample frame with offsets: the beam direction at a voxel then
+        # depends on every component of S*pos
+        a, b = np.radians(2.0), np.radians(-1.5)
+        Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+        Ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+        s.geo.S = Rx @ Ry
+        s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = -0.03, 0.02, 0.015
     return s
 
 
@@ -73,7 +81,7 @@ def _assert_same(got, ref):
     assert np.array_equal(_bits(med), _bits(ref[:, 3].astype(np.float32)))
 
 
-@pytest.mark.parametrize("variant", ["plain", "tilted", "offsets", "beamstop"])
+@pytest.mark.parametrize("variant", ["plain", "tilted", "offsets", "sample_tilt", "beamstop"])
 @pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
 def test_forward_comp_parity(engine, variant, method):
     s = _variant_scene(variant)
@@ -85,7 +93,7 @@ def test_forward_comp_parity(engine, variant, method):
         ref = oracle.or_space_comp(s.geo, s.stack, pos, q, s.rot, s.hkl)
         got = engine.forward_comp(pos, q, method)
         _assert_same(got, ref)
-        if variant != "offsets":  # that scene was rendered before the offsets were applied
+        if variant not in ("offsets", "sample_tilt"):  # rendered before the geometry was changed
             assert got[2][g] == 1.0  # the grain's own orientation at one of its voxels
         assert got[0][-1] == 0 and got[2][-1] == 0.0
 
