@@ -81,7 +81,7 @@ typedef struct grs_eval_args {
 /* largest k of the device top-k (sortrows + first rows) */
 #define GRS_TOPK_MAX 4096
 /* doubles per voxel of the omega-solve bounds (k_voxel_bounds) */
-#define GRS_VB 8
+#define GRS_VB 12
 
 /* Launchers (return hipError_t). */
 /* d_vb: GRS_VB doubles per voxel, the omega-solve bounds */

@@ -117,18 +117,38 @@ __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, d
  * ga = (RotAxisOffset - P0y)/Lss; u0 ~ 1 and u2 varies only through the norm.
  *   vb[0] W  >= max |u0 - 1| + max |u1 - (al sin + be cos + ga)| + hypot(al, be)/2
  *   vb[1], vb[2]  min and max of u2 (widened by 1e-12)
- *   vb[3] K  >= max sin^2(gamma/2), gamma the largest angle between u and the
- *            rays from the voxel to the detector corners (2 pixels outside
- *            the area); 1 (no detector bound) when a corner is 90 degrees or
+ *   vb[3] K  >= sin^2(theta) = cos^2(beta) of every event that can reach the
+ *            detector; 1 (no bound) when a detector corner is 90 degrees or
  *            more off the beam or when the window admits
- *            2 theta + angle(u, x) >= pi/2 - 1e-3, where the spot
- *            construction of forward_comp.m:40-55 no longer puts the
- *            diffracted ray at 2 theta from u
+ *            2 theta + angle(u, x) >= pi/2 - 1e-3
  *   vb[4..6] al, be, ga
- * All maxima are over the rotation angles the scorer visits, so they bound u
- * at every event.  A spot on the detector lies on a ray at 2 theta from u
- * (the sine-rule triangle of forward_comp.m:40-47 has angle 2 theta at the
- * voxel whenever 0 < phiy < pi), so sin^2(theta) = cos^2(beta) <= K. */
+ *   vb[7], vb[8]  Zt >= and Zb <= the detector rows term of every event
+ *            (below); +-1e300 (no bound) where K is 1
+ *   vb[9], vb[10] sin(eta), cos(eta): eta >= asin |e . u|
+ * All extremes are over the rotation angles the scorer visits, so they hold
+ * at every event.  Corners are the detector corners 2 pixels outside the
+ * pixel area (g->corner, frame of SamposW).
+ *
+ * Spot geometry (forward_comp.m:40-50).  center - grainpos = L2 u
+ * (L2 = (Lsd - SW1)/u0) and dety22/detz22 add diffvec e, e = (0, Gt2, Gt3)
+ * / konst, so in the frame of grainpos the triangle (voxel, center, spot)
+ * has the angle 2 theta at the voxel: phix = acos(e . u) and
+ * phiy = phix - 2 theta are its other angles, and diffvec =
+ * L2 sin(2 theta) / sin(phiy).  K_out (:49) takes the voxel as SamposW and
+ * the spot as (Lsd, dety22, detz22), which differ from that frame by
+ * o = (0, RotAxisOffset - P0y, -P0z):
+ *   K_out ~ L2 u + diffvec e + o.
+ * The tilted-detector spot (:51-55) lies on the ray SamposW + t K_out.
+ * Cone: |L2 u + diffvec e| >= L2 sqrt(1 - a^2), a = e . u, |a| <=
+ * |(u1, u2)|, so K_out is within eps = asin(|o| / (L2 sqrt(1 - a^2))) of
+ * the triangle's side and 2 theta <= gamma + eps, gamma the largest angle
+ * between u and the corner rays (the detector is convex, so the corners
+ * bound it).  K = max sin^2((gamma + eps) / 2).
+ * Rows: the z/x slope of K_out is u2/u0 + (diffvec/L2) e_z / u0 + o_z /
+ * (L2 u0), and a spot on the detector has a slope between the smallest and
+ * largest corner slope (all corners in front of the voxel).  So
+ * (diffvec/L2) e_z lies in [s_min u0 - u2 - o_z/L2, s_max u0 - u2 - o_z/L2],
+ * whose extremes over omega are Zb, Zt. */
 __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
                                double *vb) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,8 +156,10 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
   double P[3];
   voxel_P(g, pos + 3 * v, P);
   const double al = P[0] / g->Lss, be = P[1] / g->Lss, ga = (g->RAO - g->P0y) / g->Lss;
-  double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0;
-  bool front = true;
+  const double oy = g->RAO - g->P0y, oz = -g->P0z;
+  const double on = __builtin_sqrt(oy * oy + oz * oz);
+  double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0, ayz = 0.0, zt = -1e300, zb = 1e300;
+  bool front = true, rows = true;
   for (int32_t om = 0; om < nrot; om++) {
     double sw[3], u[3];
     const double c = cs[2 * om], s = cs[2 * om + 1];
@@ -151,23 +173,45 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
     e1 = a1 > e1 ? a1 : e1;
     u2lo = u2 < u2lo ? u2 : u2lo;
     u2hi = u2 > u2hi ? u2 : u2hi;
-    double cmin = 1.0;
+    const double ay = __builtin_sqrt(u1 * u1 + u2 * u2) * (1.0 + 1e-9) + 1e-12;
+    ayz = ay > ayz ? ay : ayz;
+    double cmin = 1.0, smin = 1e300, smax = -1e300;
     for (int j = 0; j < 4; j++) {
       const double dx = g->corner[3 * j] - sw0, dy = g->corner[3 * j + 1] - sw1, dz = g->corner[3 * j + 2] - sw2;
       const double cj = (u0 * dx + u1 * dy + u2 * dz) / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
       front = front && (cj > 1e-3);
       cmin = cj < cmin ? cj : cmin;
+      rows = rows && (dx > 0.0);
+      const double sl = dz / dx;
+      smin = sl < smin ? sl : smin;
+      smax = sl > smax ? sl : smax;
     }
-    const double k = (1.0 - cmin) * 0.5;
+    const double L2 = (g->Lsd - sw0) / u0;
+    front = front && (L2 > 0.0) && (ay < 1.0);
+    /* sin and cos of gamma + eps, gamma = acos(cmin) */
+    double se = on / (L2 * __builtin_sqrt(1.0 - (ay < 1.0 ? ay * ay : 0.0)));
+    se = se * (1.0 + 1e-9) + 1e-12;
+    front = front && (se < 1.0);
+    se = se < 1.0 ? se : 1.0;
+    const double ce = __builtin_sqrt(1.0 - se * se);
+    const double cm = cmin < 1.0 ? cmin : 1.0;
+    const double sg = __builtin_sqrt(1.0 - cm * cm);
+    const double k = (1.0 - (cm * ce - sg * se)) * 0.5;
     kmax = k > kmax ? k : kmax;
+    const double ozl = oz / L2;
+    const double t1 = (smax * u0 - u2) - ozl, t0 = (smin * u0 - u2) - ozl;
+    zt = t1 > zt ? t1 : zt;
+    zb = t0 < zb ? t0 : zb;
   }
   const double delta = __builtin_sqrt(d2max) * (1.0 + 1e-9) + 1e-12;
   const double alpha = 2.0 * grs_asin(delta * 0.5 < 1.0 ? delta * 0.5 : 1.0) + 1e-9;
   double K = 1.0;
-  if (front && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3) {
+  const bool ok = front && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
+  if (ok) {
     K = kmax * (1.0 + 1e-6) + 1e-9;
     K = K < 1.0 ? K : 1.0;
   }
+  const bool use_rows = ok && rows && zt == zt && zb == zb && ayz < 1.0;
   double *o = vb + GRS_VB * v;
   o[0] = ((__builtin_sqrt(al * al + be * be) * 0.5 + e0) + e1) * (1.0 + 1e-9) + 1e-12;
   o[1] = u2lo - 1e-12;
@@ -176,7 +220,37 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
   o[4] = al;
   o[5] = be;
   o[6] = ga;
-  o[7] = 0.0;
+  o[7] = use_rows ? zt + (__builtin_fabs(zt) * 1e-9 + 1e-12) : 1e300;
+  o[8] = use_rows ? zb - (__builtin_fabs(zb) * 1e-9 + 1e-12) : -1e300;
+  const double sn = ayz < 1.0 ? ayz : 1.0;
+  o[9] = sn;
+  o[10] = __builtin_sqrt(1.0 - sn * sn);
+  o[11] = 0.0;
+}
+
+/* Largest sin(theta) = -cos(beta) of an event that can reach the detector
+ * rows, from the voxel bounds: with F = diffvec / L2 = sin(2 theta) /
+ * sin(phiy) and e_z = Gz / konst, F e_z lies in [Zb, Zt].  For Gz > 0
+ * (Gz < 0 likewise with -Zb): e_z >= Gz/|G|, and phiy = pi/2 - asin(a) -
+ * 2 theta with |asin a| <= eta gives F >= sin(2 theta) / cos(2 theta - eta)
+ * when 2 theta >= eta, so sin(2 theta) (1 - T sin eta) <= T cos eta
+ * cos(2 theta) with T = Zt |G| / Gz, i.e. tan(2 theta) <= T' =
+ * T cos(eta) / (1 - T sin(eta)), or 2 theta < eta.  Then
+ * sin(theta)^2 = T'^2 / (2 q (q + 1)), q = sqrt(1 + T'^2), with T' raised to
+ * tan(eta).  Returns 1 (no bound) where the bound does not apply. */
+__device__ __forceinline__ double rows_sin_theta(double gz, double zt, double zb, double sn, double cn) {
+  if (!(gz != 0.0)) return 1.0;
+  const double Z = gz > 0.0 ? zt : -zb;
+  if (!(Z > 0.0 && Z < 1e299)) return 1.0;
+  const double T = Z / __builtin_fabs(gz);
+  const double den = 1.0 - T * sn;
+  if (!(den > 0.0) || !(cn > 0.0)) return 1.0;
+  double Tp = T * cn / den;
+  const double te = sn / cn;
+  Tp = Tp > te ? Tp : te;
+  const double q = __builtin_sqrt(1.0 + Tp * Tp);
+  const double st = Tp * __builtin_sqrt(1.0 / ((2.0 * q) * (q + 1.0)));
+  return st * (1.0 + 1e-9) + 1e-12;
 }
 
 __global__ void k_sub_from_q(const grs_dgeo *g, const double *q, int64_t N, double *SUB, int32_t *valid) {
@@ -716,6 +790,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     const bool solve = g->om_solve != 0;
     if (lane == 0) voxel_P(g, a.pos + 3 * vi, Pw);
     double vW = 0.0, vu2lo = 0.0, vu2hi = 0.0, sK = 1.0, val = 0.0, vbe = 0.0, vga = 0.0;
+    double vzt = 1e300, vzb = -1e300, vsn = 1.0, vcn = 0.0;
     if (solve) {
       const double *b = a.vb + GRS_VB * vi;
       vW = b[0];
@@ -725,6 +800,10 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       val = b[4];
       vbe = b[5];
       vga = b[6];
+      vzt = b[7];
+      vzb = b[8];
+      vsn = b[9];
+      vcn = b[10];
     }
     uint32_t R = 0;
     bool overflow = false;
@@ -747,8 +826,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
         t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
         if (solve) {
-          /* cos(beta) window: beta and lambda bounds with the phase-A margins
-           * and the detector cone.  cos(beta) = u . G/gn lies within rg W
+          /* cos(beta) window: beta and lambda bounds with the phase-A margins,
+           * the detector cone and the detector rows.  cos(beta) = u . G/gn lies within rg W
            * of r' cos(omega + phi') / gn + c0 + u2 g2 / gn, where
            * (g0', g1') = (g0 + ga g1, g1 - ga g0) has modulus r' and angle
            * phi' and u2 spans [vb[1], vb[2]] (the voxel bounds), so the
@@ -757,12 +836,14 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
           double lw = -(lmax + ml) * gn / (4.0 * GRS_PI);
           lo_cb = lw > lo_cb ? lw : lo_cb;
           lo_cb = -sK > lo_cb ? -sK : lo_cb;
+          const double z = gw[2] / gn;
+          const double st = rows_sin_theta(z, vzt, vzb, vsn, vcn);
+          lo_cb = -st > lo_cb ? -st : lo_cb;
           double hi_cb = dcb;
           double hw = -(lmin - ml) * gn / (4.0 * GRS_PI);
           hi_cb = hw < hi_cb ? hw : hi_cb;
           const double rg = __builtin_sqrt(gw[0] * gw[0] + gw[1] * gw[1]) / gn;
           const double c0 = (val * gw[0] + vbe * gw[1]) / (2.0 * gn);
-          const double z = gw[2] / gn;
           const double sa = z * vu2lo, sb = z * vu2hi;
           const double slo = sa < sb ? sa : sb, shi = sa < sb ? sb : sa;
           const double w = rg * vW;

@@ -49,6 +49,11 @@ def _variant_scene(name):
     s = scenes.make_scene(**kw)
     if name == "offsets":
         s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.02, -0.015, 0.01
+    if name == "beam_offsets":
+        # beam offsets far above the 2-pixel margin of the detector corners: the
+        # ray K_out of forward_comp.m:49 is then visibly off the 2 theta cone
+        # around the beam direction
+        s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.3, -0.3, 0.05
     if name == "sample_tilt":
         # a tilted sample frame with offsets: the beam direction at a voxel then
         # depends on every component of S*pos
@@ -81,7 +86,7 @@ def _assert_same(got, ref):
     assert np.array_equal(_bits(med), _bits(ref[:, 3].astype(np.float32)))
 
 
-@pytest.mark.parametrize("variant", ["plain", "tilted", "offsets", "sample_tilt", "beamstop"])
+@pytest.mark.parametrize("variant", ["plain", "tilted", "offsets", "beam_offsets", "sample_tilt", "beamstop"])
 @pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
 def test_forward_comp_parity(engine, variant, method):
     s = _variant_scene(variant)
@@ -93,7 +98,7 @@ def test_forward_comp_parity(engine, variant, method):
         ref = oracle.or_space_comp(s.geo, s.stack, pos, q, s.rot, s.hkl)
         got = engine.forward_comp(pos, q, method)
         _assert_same(got, ref)
-        if variant not in ("offsets", "sample_tilt"):  # rendered before the geometry was changed
+        if variant not in ("offsets", "beam_offsets", "sample_tilt"):  # rendered before the geometry was changed
             assert got[2][g] == 1.0  # the grain's own orientation at one of its voxels
         assert got[0][-1] == 0 and got[2][-1] == 0.0
 
