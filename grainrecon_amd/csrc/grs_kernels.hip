@@ -124,7 +124,7 @@ __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, d
  *   vb[4..6] al, be, ga
  *   vb[7], vb[8]  Zt >= and Zb <= the detector rows term of every event
  *            (below); +-1e300 (no bound) where K is 1
- *   vb[9], vb[10] sin(eta), cos(eta): eta >= asin |e . u|
+ *   vb[9], vb[10], vb[11] sin(eta), cos(eta), tan(eta): eta >= asin |e . u|
  * All extremes are over the rotation angles the scorer visits, so they hold
  * at every event.  Corners are the detector corners 2 pixels outside the
  * pixel area (g->corner, frame of SamposW).
@@ -225,7 +225,7 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
   const double sn = ayz < 1.0 ? ayz : 1.0;
   o[9] = sn;
   o[10] = __builtin_sqrt(1.0 - sn * sn);
-  o[11] = 0.0;
+  o[11] = o[10] > 0.0 ? sn / o[10] : 1e300;
 }
 
 /* Largest sin(theta) = -cos(beta) of an event that can reach the detector
@@ -238,7 +238,7 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
  * T cos(eta) / (1 - T sin(eta)), or 2 theta < eta.  Then
  * sin(theta)^2 = T'^2 / (2 q (q + 1)), q = sqrt(1 + T'^2), with T' raised to
  * tan(eta).  Returns 1 (no bound) where the bound does not apply. */
-__device__ __forceinline__ double rows_sin_theta(double gz, double zt, double zb, double sn, double cn) {
+__device__ __forceinline__ double rows_sin_theta(double gz, double zt, double zb, double sn, double cn, double te) {
   if (!(gz != 0.0)) return 1.0;
   const double Z = gz > 0.0 ? zt : -zb;
   if (!(Z > 0.0 && Z < 1e299)) return 1.0;
@@ -246,10 +246,9 @@ __device__ __forceinline__ double rows_sin_theta(double gz, double zt, double zb
   const double den = 1.0 - T * sn;
   if (!(den > 0.0) || !(cn > 0.0)) return 1.0;
   double Tp = T * cn / den;
-  const double te = sn / cn;
   Tp = Tp > te ? Tp : te;
   const double q = __builtin_sqrt(1.0 + Tp * Tp);
-  const double st = Tp * __builtin_sqrt(1.0 / ((2.0 * q) * (q + 1.0)));
+  const double st = Tp / __builtin_sqrt((2.0 * q) * (q + 1.0));
   return st * (1.0 + 1e-9) + 1e-12;
 }
 
@@ -648,7 +647,7 @@ __device__ __noinline__ double filt_atan2(double y, double x) {
  * and disjoint.  Returns their number (at most 4) or -1 when more would be
  * needed (the caller then scans every event). */
 __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1, double gn, double lo_cb,
-                                         double hi_cb, int32_t nrot, int32_t *i0, int32_t *i1) {
+                                         double hi_cb, int32_t nrot, double idom, int32_t *i0, int32_t *i1) {
   const double twopi = 2.0 * GRS_PI, eps = 1e-6;
   const double r = __builtin_sqrt(g0 * g0 + g1 * g1);
   if (lo_cb != lo_cb || hi_cb != hi_cb || r != r || !(gn > 0.0)) return -1;
@@ -661,7 +660,8 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
     whole = lo_cb <= 1e-12 && hi_cb >= -1e-12;
     if (!whole) return 0;
   } else {
-    const double a = lo_cb * gn / r, b = hi_cb * gn / r;
+    const double gr = gn / r;
+    const double a = lo_cb * gr, b = hi_cb * gr;
     if (!(a < 1.0) || !(b > -1.0)) return 0;
     const double pa = a <= -1.0 ? GRS_PI : filt_acos(a);
     const double pb = b >= 1.0 ? 0.0 : filt_acos(b);
@@ -696,8 +696,8 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
       const double L = lo[k] + twopi * m, U = hi[k] + twopi * m;
       if (L > g->om_last + 1e-9) break;
       if (U < g->om0 - 1e-9) continue;
-      double f0 = __builtin_ceil((L - g->om0) / g->dom - 1e-6);
-      double f1 = __builtin_floor((U - g->om0) / g->dom + 1e-6);
+      double f0 = __builtin_ceil((L - g->om0) * idom - 1e-6);
+      double f1 = __builtin_floor((U - g->om0) * idom + 1e-6);
       f0 = f0 > 0.0 ? f0 : 0.0;
       f1 = f1 < (double)(nrot - 1) ? f1 : (double)(nrot - 1);
       if (f0 <= f1) {
@@ -762,6 +762,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   const double cbm = g->cos_beta_max - dcb;
   const double lmin = g->lambda_min, lmax = g->lambda_max;
   const double lrel = 1e-9 * lmax;
+  const double inv4pi = 1.0 / (4.0 * GRS_PI);
+  const double idom = g->om_solve ? 1.0 / g->dom : 0.0;
   /* starting (omega, hkl) of each lane in the flattened event space */
   const int32_t om0 = lane / H, h0 = lane - (lane / H) * H;
 
@@ -790,7 +792,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     const bool solve = g->om_solve != 0;
     if (lane == 0) voxel_P(g, a.pos + 3 * vi, Pw);
     double vW = 0.0, vu2lo = 0.0, vu2hi = 0.0, sK = 1.0, val = 0.0, vbe = 0.0, vga = 0.0;
-    double vzt = 1e300, vzb = -1e300, vsn = 1.0, vcn = 0.0;
+    double vzt = 1e300, vzb = -1e300, vsn = 1.0, vcn = 0.0, vte = 0.0;
     if (solve) {
       const double *b = a.vb + GRS_VB * vi;
       vW = b[0];
@@ -804,6 +806,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       vzb = b[8];
       vsn = b[9];
       vcn = b[10];
+      vte = b[11];
     }
     uint32_t R = 0;
     bool overflow = false;
@@ -816,13 +819,14 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         grs_Gw(sub, a.hkl + 3 * h, gw);
         double n2 = (gw[0] * gw[0] + gw[1] * gw[1]) + gw[2] * gw[2];
         double gn = __builtin_sqrt(n2);
+        const double ign = 1.0 / gn;
         double *t = gwt + h * GRS_GWT;
         t[0] = gw[0];
         t[1] = gw[1];
         t[2] = gw[2];
         t[3] = dcb * gn;                              /* keep if dot <  tb0 */
         t[4] = cbm * gn;                              /* keep if dot >  tb1 */
-        double ml = lrel + 1e-12 * (4.0 * GRS_PI) / gn; /* absolute lambda margin */
+        double ml = lrel + (1e-12 * (4.0 * GRS_PI)) * ign; /* absolute lambda margin */
         t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
         t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
         if (solve) {
@@ -833,22 +837,22 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
            * phi' and u2 spans [vb[1], vb[2]] (the voxel bounds), so the
            * window moves onto the first term. */
           double lo_cb = cbm;
-          double lw = -(lmax + ml) * gn / (4.0 * GRS_PI);
+          double lw = -(lmax + ml) * gn * inv4pi;
           lo_cb = lw > lo_cb ? lw : lo_cb;
           lo_cb = -sK > lo_cb ? -sK : lo_cb;
-          const double z = gw[2] / gn;
-          const double st = rows_sin_theta(z, vzt, vzb, vsn, vcn);
+          const double z = gw[2] * ign;
+          const double st = rows_sin_theta(z, vzt, vzb, vsn, vcn, vte);
           lo_cb = -st > lo_cb ? -st : lo_cb;
           double hi_cb = dcb;
-          double hw = -(lmin - ml) * gn / (4.0 * GRS_PI);
+          double hw = -(lmin - ml) * gn * inv4pi;
           hi_cb = hw < hi_cb ? hw : hi_cb;
-          const double rg = __builtin_sqrt(gw[0] * gw[0] + gw[1] * gw[1]) / gn;
-          const double c0 = (val * gw[0] + vbe * gw[1]) / (2.0 * gn);
+          const double rg = __builtin_sqrt(gw[0] * gw[0] + gw[1] * gw[1]) * ign;
+          const double c0 = (val * gw[0] + vbe * gw[1]) * (0.5 * ign);
           const double sa = z * vu2lo, sb = z * vu2hi;
           const double slo = sa < sb ? sa : sb, shi = sa < sb ? sb : sa;
           const double w = rg * vW;
           nr_h = om_ranges(g, gw[0] + vga * gw[1], gw[1] - vga * gw[0], gn, ((lo_cb - c0) - shi) - w - 1e-12,
-                           ((hi_cb - c0) - slo) + w + 1e-12, nrot, ri0, ri1);
+                           ((hi_cb - c0) - slo) + w + 1e-12, nrot, idom, ri0, ri1);
         }
       }
       if (solve) {
@@ -1001,22 +1005,33 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     };
 
     if (solve && !overflow) {
-      /* candidates of the omega solve, 64 per step, straight to phase B1 */
-      uint32_t lo_r = 0;
+      /* candidates of the omega solve, 64 per step, straight to phase B1.
+       * Ranges are non-empty and rcum is their exclusive prefix, so the range
+       * of candidate base + l is rb plus the number of later ranges that
+       * start at or before base + l, where rb is the last range starting
+       * before base (0 at base 0).  Those starts lie in the next 64 ranges;
+       * each marks its position in ring1 (unused on this path) and a ballot
+       * turns the marks into a bit mask. */
+      uint32_t rb = 0;
       for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t idx = base + (uint32_t)lane;
         const bool valid = idx < total;
+        ring1[lane] = 0u;
+        wave_sync();
+        const uint32_t rn = rb + 1u + (uint32_t)lane;
+        if (rn < R) {
+          const uint32_t p = rcum[rn] - base;
+          if (p < 64u) ring1[p] = 1u;
+        }
+        wave_sync();
+        const uint64_t starts = __ballot(ring1[lane] != 0u);
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const uint32_t r = rb + (uint32_t)__popcll(starts & upto);
+        rb += (uint32_t)__popcll(starts);
         uint32_t ev = 0;
         if (valid) {
-          uint32_t lo = lo_r, hi = R;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (rcum[mid] <= idx) lo = mid;
-            else hi = mid;
-          }
-          lo_r = lo;
-          const uint32_t st0 = rstart[lo];
-          ev = (((st0 >> 8) + (idx - rcum[lo])) << 8) | (st0 & 255u);
+          const uint32_t st0 = rstart[r];
+          ev = (((st0 >> 8) + (idx - rcum[r])) << 8) | (st0 & 255u);
         }
         b1_core(valid, ev, total - base < 64u ? total - base : 64u);
       }
