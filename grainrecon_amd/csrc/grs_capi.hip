@@ -503,6 +503,7 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
     ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)waves * stride);
     a.vb = P<double>(c->d_vb);
     a.scratch = P<uint32_t>(c->d_scratch);
+    a.nrot = c->nrot;
     const bool audit = c->audit && map == GRS_MAP_ONE_VOXEL;
     if (audit) a.audit = P<unsigned long long>(c->d_audit);
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));

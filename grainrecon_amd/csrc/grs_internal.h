@@ -62,6 +62,7 @@ typedef struct grs_eval_args {
   int32_t count_keys;      /* counts-only kernels: 1 = hit test on the keys, 0 = on hit_bits */
   int64_t n_u, u_stride;   /* GRS_MAP_GRID */
   unsigned long long *out_key; /* counts-only GRS_MAP_GRID: per-evaluation sort key */
+  int32_t nrot;            /* rotation angles (must equal the geometry's); 0 turns the one-voxel beam table off */
 } grs_eval_args;
 
 /* Audit counters of the fast kernel's resolution stage: events seen by phase

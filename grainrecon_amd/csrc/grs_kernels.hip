@@ -45,6 +45,20 @@
 #define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
 #define GRS_RANGE_DOUBLES(H) (4 * (int64_t)(H) + 1) /* 8H + 2 u32: omega-solve ranges */
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
+#define GRS_BT 7        /* doubles per omega of the beam table: SamposW (3), u (3), 1/u0 */
+
+/* LDS doubles of one wave's region of k_fast */
+__host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
+  return (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4;
+}
+
+/* LDS doubles of the one-voxel beam table behind the wave regions: the
+ * table is kept when the block's dynamic LDS stays within 64 KB */
+__host__ __device__ inline int64_t fast_tab_doubles(int32_t map, int32_t nrot, int32_t H, int wpb) {
+  if (map != GRS_MAP_ONE_VOXEL || nrot <= 0) return 0;
+  const int64_t t = (int64_t)GRS_BT * nrot;
+  return ((int64_t)wpb * fast_wave_doubles(H) + t) * (int64_t)sizeof(double) <= 65536 ? t : 0;
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -531,10 +545,22 @@ __device__ __forceinline__ bool res_round(double r, double *k) {
   return fr >= GRS_RES_TAU && fr <= 1.0 - GRS_RES_TAU;
 }
 
-__device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *P, double gw0,
-                                           double gw1, double gw2, b1_out *o) {
-  double swv[3], uv[3];
-  beam_terms(g, P, c, s, swv, uv);
+__device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *P, const double *bt,
+                                           double gw0, double gw1, double gw2, b1_out *o) {
+  double swv[3], uv[3], iu0;
+  if (bt) {
+    /* the beam table row of this omega: the same values as below */
+    swv[0] = bt[0];
+    swv[1] = bt[1];
+    swv[2] = bt[2];
+    uv[0] = bt[3];
+    uv[1] = bt[4];
+    uv[2] = bt[5];
+    iu0 = bt[6];
+  } else {
+    beam_terms(g, P, c, s, swv, uv);
+    iu0 = fast_rcp(uv[0]);
+  }
   const double u0 = uv[0], u1 = uv[1], u2 = uv[2];
   const double G0 = c * gw0 - s * gw1, G1 = s * gw0 + c * gw1, G2 = gw2;
   const double g2 = __builtin_fma(G0, G0, __builtin_fma(G1, G1, G2 * G2));
@@ -559,7 +585,6 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
    * L u(2:3) / u0, L2 = (Lsd - SW1) / u0 (cos(alpha) = u0), and the t
    * numerator and offsets of forward_comp.m:51-54 */
   const double k0 = g->Lsd - swv[0], sw1 = swv[1], sw2 = swv[2];
-  const double iu0 = fast_rcp(u0);
   const double ctr1 = g->L * u1 * iu0, ctr2 = g->L * u2 * iu0, L2 = k0 * iu0;
   const double *R = g->R;
   const double tnum = __builtin_fma(R[0], k0, __builtin_fma(R[3], (g->dety00 - g->RAO) - sw1, R[6] * (g->detz00 - sw2)));
@@ -735,7 +760,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
-  const int64_t per_wave = (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4;
+  const int64_t per_wave = fast_wave_doubles(H);
   double *gwt = smem + wib * per_wave;
   uint32_t *ring1 = (uint32_t *)(gwt + (int64_t)H * GRS_GWT); /* phase-A survivors (omega << 8 | hkl) */
   uint32_t *ring2 = ring1 + GRS_QCAP;                          /* events for the exact pipeline */
@@ -751,6 +776,30 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   __syncthreads();
   const grs_dgeo *g = &sg;
   const int wpb = (int)(blockDim.x >> 6);
+  /* one voxel for every evaluation: its per-omega beam terms are computed
+   * once per block into LDS behind the per-wave regions (the launch sizes
+   * the allocation with the same rule) */
+  const double *btab = nullptr;
+  if (MAP == GRS_MAP_ONE_VOXEL && a.nrot == g->nrot && fast_tab_doubles(MAP, a.nrot, H, wpb) > 0 &&
+      a.e0 < a.n_eval) {
+    double *bt = smem + (int64_t)wpb * per_wave;
+    double P[3];
+    voxel_P(g, a.pos, P);
+    for (int32_t om = (int32_t)threadIdx.x; om < g->nrot; om += (int32_t)blockDim.x) {
+      double sw[3], u[3];
+      beam_terms(g, P, a.cs[2 * om], a.cs[2 * om + 1], sw, u);
+      double *r = bt + GRS_BT * om;
+      r[0] = sw[0];
+      r[1] = sw[1];
+      r[2] = sw[2];
+      r[3] = u[0];
+      r[4] = u[1];
+      r[5] = u[2];
+      r[6] = fast_rcp(u[0]);
+    }
+    __syncthreads();
+    btab = bt;
+  }
   const int64_t gwave = (int64_t)blockIdx.x * wpb + wib;
   const int64_t nwaves = (int64_t)gridDim.x * wpb;
   uint32_t *wlist = MEDIAN ? a.scratch + gwave * a.scratch_stride : nullptr;
@@ -962,7 +1011,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         b1_out o;
         o.ry = o.rz = 0.0;
         o.dy = o.dz = 0;
-        st = b1_classify(g, a.cs[2 * om], a.cs[2 * om + 1], Pw, t[0], t[1], t[2], &o);
+        st = b1_classify(g, a.cs[2 * om], a.cs[2 * om + 1], Pw, btab ? btab + GRS_BT * om : nullptr, t[0], t[1],
+                         t[2], &o);
         if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
         if (AUDIT) {
           double yz[2] = {0.0, 0.0};
@@ -1334,8 +1384,8 @@ hipError_t grs_launch_sub_from_euler(const grs_dgeo *d_g, const double *d_eu, in
   return hipGetLastError();
 }
 
-static size_t fast_lds(int32_t H, int wpb) {
-  return (size_t)wpb * ((size_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4) * sizeof(double);
+static size_t fast_lds(int32_t H, int wpb, const grs_eval_args *a) {
+  return (size_t)((int64_t)wpb * fast_wave_doubles(H) + fast_tab_doubles(a->map, a->nrot, H, wpb)) * sizeof(double);
 }
 
 /* Resident blocks per CU of one k_fast instantiation at wpb waves per block
@@ -1367,7 +1417,7 @@ static int device_cus() {
 template <bool MEDIAN, int MAP, bool AUDIT = false>
 static hipError_t launch_fast_impl(const grs_eval_args *a, int32_t H, int64_t n_waves, int wpb, hipStream_t st) {
   if (wpb != 1 && wpb != 2 && wpb != 4) return hipErrorInvalidValue;
-  size_t lds = fast_lds(H, wpb);
+  size_t lds = fast_lds(H, wpb, a);
   int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP, AUDIT>(lds, wpb);
   int64_t blocks = n_waves / wpb; /* n_waves is a multiple of 4: no wave beyond the scratch slots */
   if (blocks > resident) blocks = resident;
