@@ -305,6 +305,8 @@ __global__ void k_math_eval(int fn, const double *x, double *y, int64_t n) {
     case 4: r = grs_round(v); break;
     case 5: r = grs_sqrt(v); break;
     case 7: r = grs_asin(v); break;
+    case 8: r = grs_filt_acos(v); break;
+    case 9: r = grs_filt_atan2(grs_sin(v), grs_cos(v)); break;
     default: r = 1.0 / v; break;
   }
   y[i] = r;
@@ -648,27 +650,20 @@ __device__ __noinline__ int exact_event(const grs_dgeo *g, const double *pos, co
   return 1;
 }
 
-/* acos and atan2 for the candidate filter (errors ~1e-16 rad; the filter
- * widens its intervals by 1e-6 rad).  Out of line to keep the register
- * pressure of the main loop low; they run a few times per evaluation. */
-__device__ __noinline__ double filt_acos(double x) { return grs_acos(x); }
-
-__device__ __noinline__ double filt_atan2(double y, double x) {
-  const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
-  double a;
-  if (ax >= ay) a = ax == 0.0 ? 0.0 : grs_atan(ay / ax);
-  else a = GRS_PI / 2.0 - grs_atan(ax / ay);
-  if (x < 0.0) a = GRS_PI - a;
-  return y < 0.0 ? -a : a;
-}
+/* The filter-grade acos and atan2 out of line: inlined, they raise the
+ * register pressure of the whole kernel (spills in the candidate loop). */
+__device__ __noinline__ double filt_acos(double x) { return grs_filt_acos(x); }
+__device__ __noinline__ double filt_atan2(double y, double x) { return grs_filt_atan2(y, x); }
 
 /* Omega solve of one reflection.  With (g0, g1) the in-plane part of a
  * vector that rotates with omega, |G| = gn and r = |(g0, g1)|, the term
  * r cos(omega + phi) / gn, phi = atan2(g1, g0), must lie in [lo_cb, hi_cb]
  * (the caller folds every other part of cos(beta), forward_comp.m:31, and its
  * bounds into the window), so omega + phi must lie in the arcs where
- * cos(psi) is in [lo_cb gn / r, hi_cb gn / r].  The arcs are widened by 1e-6 rad and the index bounds by
- * 1e-6 steps; the resulting ranges of rotation indices [i0, i1] are sorted
+ * cos(psi) is in [lo_cb gn / r, hi_cb gn / r].  The arc ends come from the
+ * filter-grade acos and atan2 of grs_math.h (errors below 5e-12 rad); the
+ * arcs are widened by 1e-6 rad and the index bounds by 1e-6 steps; the
+ * resulting ranges of rotation indices [i0, i1] are sorted
  * and disjoint.  Returns their number (at most 4) or -1 when more would be
  * needed (the caller then scans every event). */
 __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1, double gn, double lo_cb,

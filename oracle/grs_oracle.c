@@ -377,6 +377,8 @@ void oracle_math_eval(int fn, const double *x, double *y, int64_t n) {
       case 2: y[i] = grs_acos(x[i]); break;
       case 3: y[i] = grs_atan(x[i]); break;
       case 7: y[i] = grs_asin(x[i]); break;
+      case 8: y[i] = grs_filt_acos(x[i]); break;
+      case 9: y[i] = grs_filt_atan2(grs_sin(x[i]), grs_cos(x[i])); break;
       default: y[i] = grs_round(x[i]); break;
     }
   }

@@ -32,6 +32,8 @@ def test_math_host_equals_device(engine):
         3: rng.uniform(-10, 10, 20000),
         4: rng.uniform(-5000, 5000, 20000).round(1) + rng.choice([0.0, 0.5, -0.5], 20000),
         7: np.concatenate([rng.uniform(-1, 1, 20000), 1 - rng.random(2000) ** 8, -1 + rng.random(2000) ** 8]),
+        8: np.concatenate([rng.uniform(-1, 1, 20000), 1 - rng.random(2000) ** 8, -1 + rng.random(2000) ** 8]),
+        9: np.concatenate([rng.uniform(-np.pi, np.pi, 20000), np.arange(-8, 9) * (np.pi / 8)]),
     }
     for fn, x in cases.items():
         assert np.array_equal(_bits(engine.math_device(fn, x)), _bits(oracle.math_eval(fn, x))), fn

@@ -112,3 +112,42 @@ def test_asin_near_endpoints_and_branch_points():
     assert y[0] == math.pi / 2 and y[1] == -math.pi / 2
     assert y[2] == 0.0 and math.copysign(1.0, y[2]) < 0
     assert np.isnan(y[3]) and np.isnan(y[4])
+
+
+def _edges():
+    xs = []
+    for c in (-1.0, -0.5, 0.0, 0.5, 1.0):
+        for k in range(1, 60):
+            for s in (-1, 1):
+                x = c + s * 2.0 ** (-k)
+                if -1.0 <= x <= 1.0:
+                    xs.append(x)
+    return xs + [-1.0, -0.5, 0.0, 0.5, 1.0]
+
+
+def test_filter_acos_error_bound():
+    """grs_filt_acos (the omega solve's arc ends, widened there by 1e-6 rad):
+    absolute error below 5e-12 rad over [-1, 1], at the branch points and
+    the ends; NaN outside."""
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([np.linspace(-1.0, 1.0, 200001), rng.uniform(-1, 1, 20000), _edges()])
+    y = oracle.math_eval(8, xs)
+    err = max(abs(float(mp.mpf(float(a)) - mp.acos(mp.mpf(float(x))))) for x, a in zip(xs[::7], y[::7]))
+    assert err < 5e-12
+    assert np.max(np.abs(y - np.arccos(xs))) < 5e-12
+    assert np.isnan(oracle.math_eval(8, np.array([1.0000000000000002, -1.5, np.nan]))).all()
+
+
+def test_filter_atan2_error_bound():
+    """grs_filt_atan2 at (sin x, cos x) returns x within 3e-12 rad over the
+    whole circle, including the axes, the diagonals and the reduction point
+    tan(pi/8) of every octant."""
+    rng = np.random.default_rng(12)
+    special = [k * math.pi / 8 for k in range(-8, 9)]
+    xs = np.concatenate([np.linspace(-math.pi, math.pi, 200001), rng.uniform(-math.pi, math.pi, 20000),
+                         special, [s + d for s in special for d in (-1e-9, 1e-9, -1e-15, 1e-15)]])
+    xs = xs[(xs > -math.pi) & (xs <= math.pi)]
+    y = oracle.math_eval(9, xs)
+    d = np.abs((y - xs + math.pi) % (2 * math.pi) - math.pi)
+    assert d.max() < 3e-12
+    assert (y > -math.pi - 1e-15).all() and (y <= math.pi + 1e-15).all()
