@@ -46,7 +46,7 @@ def flop_per_eval(nrot: int, H: int) -> int:
 # factors a plain copy kernel of known size gives for the same counters;
 # profiles/k_fast_seed_cfg2_counters.md): every random distance-key read
 # pulls a 128-byte line.
-TRAFFIC_BYTES_PER_EVAL = 56_600
+TRAFFIC_BYTES_PER_EVAL = 55_900
 
 
 def log(msg):
@@ -644,7 +644,7 @@ def main():
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "traffic_GB_per_s": round(traffic / (kms / kn * 1e-3) / 1e9, 1) if traffic else None,
                 "achieved_note": ("algorithmic: the full forward_comp.m work of every evaluation; the omega solve and "
-                                  "the resolution stage execute about 13% of it (counter profile)"),
+                                  "the resolution stage execute about 9% of it (counter profile)"),
                 "kernel": "k_fast", "kernel_ms": round(kms / kn, 4), "flop_per_eval": fe,
                 "evals_per_launch": N}
 
