@@ -163,9 +163,29 @@ __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, d
  * largest corner slope (all corners in front of the voxel).  So
  * (diffvec/L2) e_z lies in [s_min u0 - u2 - o_z/L2, s_max u0 - u2 - o_z/L2],
  * whose extremes over omega are Zb, Zt. */
-__global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, const double *cs, int32_t nrot,
-                               double *vb) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ double wave_max_d(double v) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ double wave_min_d(double v) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+/* One wave per voxel (4 voxels per 256-thread block): lanes take the
+ * rotation angles lane, lane + 64, ... and the extremes are reduced across
+ * the wave. */
+__global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V,
+                                                      const double *cs, int32_t nrot, double *vb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (v >= V) return;
   double P[3];
   voxel_P(g, pos + 3 * v, P);
@@ -174,7 +194,7 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
   const double on = __builtin_sqrt(oy * oy + oz * oz);
   double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0, ayz = 0.0, zt = -1e300, zb = 1e300;
   bool front = true, rows = true;
-  for (int32_t om = 0; om < nrot; om++) {
+  for (int32_t om = lane; om < nrot; om += 64) {
     double sw[3], u[3];
     const double c = cs[2 * om], s = cs[2 * om + 1];
     beam_terms(g, P, c, s, sw, u);
@@ -217,15 +237,29 @@ __global__ void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V, 
     zt = t1 > zt ? t1 : zt;
     zb = t0 < zb ? t0 : zb;
   }
+  /* a NaN anywhere turns the bound off: the comparisons above skip NaNs */
+  const bool nan_seen = __ballot(!(d2max == d2max && kmax == kmax && zt == zt && zb == zb)) != 0;
+  front = __ballot(!front) == 0;
+  rows = __ballot(!rows) == 0;
+  d2max = wave_max_d(d2max);
+  kmax = wave_max_d(kmax);
+  e0 = wave_max_d(e0);
+  e1 = wave_max_d(e1);
+  u2lo = wave_min_d(u2lo);
+  u2hi = wave_max_d(u2hi);
+  ayz = wave_max_d(ayz);
+  zt = wave_max_d(zt);
+  zb = wave_min_d(zb);
+  if (lane != 0) return;
   const double delta = __builtin_sqrt(d2max) * (1.0 + 1e-9) + 1e-12;
   const double alpha = 2.0 * grs_asin(delta * 0.5 < 1.0 ? delta * 0.5 : 1.0) + 1e-9;
   double K = 1.0;
-  const bool ok = front && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
+  const bool ok = front && !nan_seen && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
   if (ok) {
     K = kmax * (1.0 + 1e-6) + 1e-9;
     K = K < 1.0 ? K : 1.0;
   }
-  const bool use_rows = ok && rows && zt == zt && zb == zb && ayz < 1.0;
+  const bool use_rows = ok && rows && ayz < 1.0;
   double *o = vb + GRS_VB * v;
   o[0] = ((__builtin_sqrt(al * al + be * be) * 0.5 + e0) + e1) * (1.0 + 1e-9) + 1e-12;
   o[1] = u2lo - 1e-12;
@@ -1354,7 +1388,7 @@ hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t stride,
 hipError_t grs_launch_voxel_bounds(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs,
                                    int32_t nrot, double *d_vb, hipStream_t st) {
   if (V <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 64)), dim3(64), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
+  hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 4)), dim3(256), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
   return hipGetLastError();
 }
 
