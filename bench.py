@@ -58,11 +58,13 @@ def cpu_baseline(cfg, proj_host, pos, q, gpu_out, seconds, threads):
     import oracle
 
     stack = oracle.Stack(proj_host)
-    n0 = 64
+    # the rate of a first sample (threads started, caches warm) sizes the timed one
+    n0 = 1024
+    oracle.or_space_comp(cfg.geo, stack, pos, q[:64], cfg.rot, cfg.hkl, nthreads=threads)
     t = time.perf_counter()
     oracle.or_space_comp(cfg.geo, stack, pos, q[:n0], cfg.rot, cfg.hkl, nthreads=threads)
     rate0 = n0 / (time.perf_counter() - t)
-    n = int(min(q.shape[0], max(256, rate0 * seconds)))
+    n = int(min(q.shape[0], max(n0, rate0 * seconds)))
     t = time.perf_counter()
     ref = oracle.or_space_comp(cfg.geo, stack, pos, q[:n], cfg.rot, cfg.hkl, nthreads=threads)
     dt = time.perf_counter() - t
