@@ -83,13 +83,13 @@ struct grs_ctx {
   std::map<std::vector<uint64_t>, hipGraphExec_t> gi_cache;
   /* experimental spots per projection (CSR) for the spot pairing */
   DevBuf p_fptr, p_fsp, p_flag, p_sid, p_lam, p_dyz, p_dis, p_off, p_cnt, p_rowptr, p_rows;
-  int64_t p_nframes = -1, p_n = 0;
+  int64_t p_nframes = -1, p_n = 0, p_maxframe = 0;
   /* reconstruction volumes (completeness, mask, median distance, Nr_intersect)
    * and the region-growth buffers */
   DevBuf v_vol[4], r_rows, r_counts, r_acc_lin, r_acc_C;
   int64_t v_dim[3] = {0, 0, 0};
   /* G-vector matching score buffers */
-  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out;
+  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_sidx, t_slen;
   /* grain centre-of-mass fit buffers */
   DevBuf m_rows, m_rptr, m_x, m_pfit, m_doff, m_dis;
   bool v_set[4] = {false, false, false, false};
@@ -223,7 +223,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
-                    &c->m_x, &c->m_pfit, &c->m_doff, &c->m_dis};
+                    &c->m_x, &c->m_pfit, &c->m_doff, &c->m_dis, &c->t_skey, &c->t_sidx, &c->t_slen};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1709,6 +1709,9 @@ grs_status grs_set_frame_spots(grs_ctx *c, const int64_t *frame_ptr, int64_t nfr
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->p_nframes = nframes;
   c->p_n = n;
+  c->p_maxframe = 0;
+  for (int64_t i = 0; i < nframes; i++)
+    c->p_maxframe = frame_ptr[i + 1] - frame_ptr[i] > c->p_maxframe ? frame_ptr[i + 1] - frame_ptr[i] : c->p_maxframe;
   return GRS_OK;
 }
 
@@ -2163,11 +2166,24 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
   HIPCHK(c, grs_launch_gt_spots(P<grs_dgeo>(c->d_geo), P<double>(c->t_proj), P<int64_t>(c->p_fptr), c->nrot,
                                 P<double>(c->p_fsp), nsp, Lsd_tr, dety00_tr, detz00_tr, P<double>(c->t_dposs), F,
                                 P<double>(c->t_units), c->st));
+  /* spots sorted by unit-vector z per (projection, family) when every
+   * projection fits the sort's LDS; otherwise every spot is visited */
+  const bool sorted = nsp > 0 && c->p_maxframe <= grs_gt_sort_max();
+  if (sorted) {
+    ENSURE(c, c->t_skey, sizeof(double) * 4 * (size_t)nsp);
+    ENSURE(c, c->t_sidx, sizeof(int32_t) * 4 * (size_t)nsp);
+    ENSURE(c, c->t_slen, sizeof(int32_t) * 4 * (size_t)c->nrot);
+  }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  if (sorted)
+    HIPCHK(c, grs_launch_gt_sort(P<int64_t>(c->p_fptr), c->nrot, P<double>(c->t_units), nsp, c->p_maxframe,
+                                 P<double>(c->t_skey), P<int32_t>(c->t_sidx), P<int32_t>(c->t_slen), c->st));
   HIPCHK(c, grs_launch_gt_score(P<grs_dgeo>(c->d_geo), P<double>(c->d_hkl), P<int32_t>(c->t_fam),
                                 P<double>(c->t_glen), P<double>(c->d_sub), P<int32_t>(c->d_valid), N,
                                 P<double>(c->t_proj), P<int64_t>(c->p_fptr), P<double>(c->t_units),
-                                P<double>(c->t_scratch), cap, waves, P<double>(c->t_out), c->st));
+                                sorted ? P<double>(c->t_skey) : nullptr, sorted ? P<int32_t>(c->t_sidx) : nullptr,
+                                sorted ? P<int32_t>(c->t_slen) : nullptr, nsp, P<double>(c->t_scratch), cap, waves,
+                                P<double>(c->t_out), c->st));
   if (c->timing) {
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     HIPCHK(c, hipEventSynchronize(c->ev1));

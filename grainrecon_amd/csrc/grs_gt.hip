@@ -28,12 +28,23 @@
  *             order; their median (rank counting across the wave, even count
  *             a + (b - a)/2) and mean (sequential sum) close the row
  *             [Nr_match, median, Nr_match/NrSpotExpectAll, mean].
+ * k_gt_sort   per (projection, family), when every projection has at most
+ *             GT_SORT_MAX spots: the spot indices sorted by the z component
+ *             of their unit vectors (NaN last, not counted).  k_gt_score then
+ *             visits only the spots with |z - n_z| <= GT_WIN: a spot within
+ *             1.01 degrees of n has |w - n| = 2 sin(angle/2) <= GT_WIN, so
+ *             the window holds every spot that can give an angle below 1
+ *             degree and every spot within 1e-12 of the best dot product of
+ *             such a spot; the row equals the full scan's (without such a
+ *             spot neither finds a match).
  */
 #include <hip/hip_runtime.h>
 
 #include "grs_internal.h"
 
 #define GT_PROJ_DOUBLES 18
+#define GT_SORT_MAX 4096 /* spots per projection up to which k_gt_sort keeps them in LDS */
+#define GT_WIN 0.0177    /* > 2 sin(1.01 deg / 2) = 0.017627 */
 
 /* sind(x): reduction to |r| <= 45 degrees by multiples of 90, then
  * sin or cos of (pi/180)*r with the quadrant's sign */
@@ -132,6 +143,63 @@ __global__ void k_gt_spots(const grs_dgeo *g, const double *proj, const int64_t 
   }
 }
 
+/* block (i, f): spots of projection i sorted by units[.., f, 2]; n2 = the
+ * power of two >= the largest projection (dynamic LDS n2 * 12 bytes) */
+__global__ __launch_bounds__(256) void k_gt_sort(const int64_t *fptr, const double *units, int64_t nsp, int32_t n2,
+                                                 double *skey, int32_t *sidx, int32_t *slen) {
+  extern __shared__ double gsm[];
+  double *key = gsm;
+  int32_t *idx = (int32_t *)(gsm + n2);
+  const int32_t i = blockIdx.x >> 2, f = blockIdx.x & 3;
+  const int64_t s0 = fptr[i];
+  const int32_t m = (int32_t)(fptr[i + 1] - s0);
+  for (int32_t t = threadIdx.x; t < n2; t += blockDim.x) {
+    double k = __builtin_inf();
+    if (t < m) {
+      const double z = units[12 * (s0 + t) + 3 * f + 2];
+      k = z == z ? z : __builtin_inf();
+    }
+    key[t] = k;
+    idx[t] = t;
+  }
+  __syncthreads();
+  /* bitonic sort, ascending by (key, idx) */
+  for (int32_t size = 2; size <= n2; size <<= 1) {
+    for (int32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int32_t t = threadIdx.x; t < (n2 >> 1); t += blockDim.x) {
+        const int32_t lo = 2 * t - (t & (stride - 1));
+        const int32_t hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const double ka = key[lo], kb = key[hi];
+        const int32_t ia = idx[lo], ib = idx[hi];
+        const bool gt = ka > kb || (ka == kb && ia > ib);
+        if (gt == up) {
+          key[lo] = kb;
+          key[hi] = ka;
+          idx[lo] = ib;
+          idx[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  int32_t valid = 0;
+  for (int32_t t = threadIdx.x; t < m; t += blockDim.x) {
+    skey[(int64_t)f * nsp + s0 + t] = key[t];
+    sidx[(int64_t)f * nsp + s0 + t] = (int32_t)(s0 + idx[t]);
+    valid += key[t] != __builtin_inf() ? 1 : 0;
+  }
+  /* block sum of the finite keys */
+  __shared__ int32_t red[256];
+  red[threadIdx.x] = valid;
+  __syncthreads();
+  for (int32_t o = 128; o > 0; o >>= 1) {
+    if ((int32_t)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) slen[4 * i + f] = red[0];
+}
+
 __device__ __forceinline__ uint64_t gt_lanes_below(uint64_t m, int lane) {
   return lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
 }
@@ -139,7 +207,9 @@ __device__ __forceinline__ uint64_t gt_lanes_below(uint64_t m, int lane) {
 __global__ __launch_bounds__(256) void k_gt_score(const grs_dgeo *g, const double *hkl, const int32_t *famidx,
                                                   const double *glen_poss, const double *SUB, const int32_t *valid,
                                                   int64_t N, const double *proj, const int64_t *fptr,
-                                                  const double *units, double *scratch, int64_t cap, double *out) {
+                                                  const double *units, const double *skey, const int32_t *sidx,
+                                                  const int32_t *slen, int64_t nsp, double *scratch, int64_t cap,
+                                                  double *out) {
   GRS_NO_CONTRACT
   const int32_t H = g->H, nrot = g->nrot;
   const int lane = threadIdx.x & 63;
@@ -188,10 +258,36 @@ __global__ __launch_bounds__(256) void k_gt_score(const grs_dgeo *g, const doubl
           const int32_t f = famidx[h];
           if (f >= 0 && s1 > s0) {
             const double n0 = Gt[0] / Glen, n1 = Gt[1] / Glen, n2 = Gt[2] / Glen;
+            /* spots to visit: all of the projection, or the z window of the
+             * sorted list (entries k0 .. k1 - 1 of the family's order) */
+            int64_t k0 = s0, k1 = s1;
+            const int32_t *ord = nullptr;
+            if (skey) {
+              const double *K = skey + (int64_t)f * nsp + s0;
+              const int32_t mlen = slen[4 * i + f];
+              const double zlo = n2 - GT_WIN, zhi = n2 + GT_WIN;
+              int32_t lo = 0, hi = mlen;
+              while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (K[mid] < zlo) lo = mid + 1;
+                else hi = mid;
+              }
+              int32_t e = lo;
+              hi = mlen;
+              while (e < hi) {
+                const int32_t mid = (e + hi) >> 1;
+                if (K[mid] <= zhi) e = mid + 1;
+                else hi = mid;
+              }
+              ord = sidx + (int64_t)f * nsp + s0;
+              k0 = lo;
+              k1 = e;
+            }
             double best = 0.0;
             bool any = false;
-            for (int64_t k = s0; k < s1; k++) {
-              const double *w = units + 12 * k + 3 * f;
+            for (int64_t k = k0; k < k1; k++) {
+              const int64_t sp = ord ? (int64_t)ord[k] : k;
+              const double *w = units + 12 * sp + 3 * f;
               const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
               if (d == d && (!any || d > best)) {
                 best = d;
@@ -201,8 +297,9 @@ __global__ __launch_bounds__(256) void k_gt_score(const grs_dgeo *g, const doubl
             if (any) {
               amin = 1e300;
               const double lim = best - 1e-12;
-              for (int64_t k = s0; k < s1; k++) {
-                const double *w = units + 12 * k + 3 * f;
+              for (int64_t k = k0; k < k1; k++) {
+                const int64_t sp = ord ? (int64_t)ord[k] : k;
+                const double *w = units + 12 * sp + 3 * f;
                 const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
                 if (d >= lim) {
                   const double a = d > 1.0 ? 0.0 : (grs_acos(d) * 180.0) / GRS_PI;
@@ -286,14 +383,28 @@ int64_t grs_gt_wave_slots(int device) {
   return (int64_t)cus * 16;
 }
 
+hipError_t grs_launch_gt_sort(const int64_t *d_fptr, int32_t nrot, const double *d_units, int64_t nsp,
+                              int64_t max_frame, double *d_skey, int32_t *d_sidx, int32_t *d_slen, hipStream_t st) {
+  if (nrot <= 0) return hipSuccess;
+  if (max_frame > GT_SORT_MAX) return hipErrorInvalidValue;
+  int32_t n2 = 64;
+  while (n2 < max_frame) n2 <<= 1;
+  hipLaunchKernelGGL(k_gt_sort, dim3((unsigned)(4 * nrot)), dim3(256), (size_t)n2 * 12, st, d_fptr, d_units, nsp, n2,
+                     d_skey, d_sidx, d_slen);
+  return hipGetLastError();
+}
+
+int64_t grs_gt_sort_max() { return GT_SORT_MAX; }
+
 hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const int32_t *d_famidx,
                                const double *d_glen_poss, const double *d_SUB, const int32_t *d_valid, int64_t N,
-                               const double *d_proj, const int64_t *d_fptr, const double *d_units, double *d_scratch,
-                               int64_t cap, int64_t waves, double *d_out, hipStream_t st) {
+                               const double *d_proj, const int64_t *d_fptr, const double *d_units,
+                               const double *d_skey, const int32_t *d_sidx, const int32_t *d_slen, int64_t nsp,
+                               double *d_scratch, int64_t cap, int64_t waves, double *d_out, hipStream_t st) {
   if (N <= 0) return hipSuccess;
   int64_t w = waves < N ? waves : N;
   const int64_t blocks = (w + 3) / 4;
   hipLaunchKernelGGL(k_gt_score, dim3((unsigned)blocks), dim3(256), 0, st, d_g, d_hkl, d_famidx, d_glen_poss, d_SUB,
-                     d_valid, N, d_proj, d_fptr, d_units, d_scratch, cap, d_out);
+                     d_valid, N, d_proj, d_fptr, d_units, d_skey, d_sidx, d_slen, nsp, d_scratch, cap, d_out);
   return hipGetLastError();
 }
