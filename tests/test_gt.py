@@ -95,3 +95,35 @@ def test_forward_gt_equals_oracle(engine, scene):
     same = (got == ref) | (np.isnan(got) & np.isnan(ref))
     assert same.all(), np.argwhere(~same)[:5]
     assert got[0, 0] > 20 and np.all(got[-1] == 0.0)
+
+
+@pytest.mark.gpu
+def test_forward_gt_dense_spots_and_full_scan(engine, scene):
+    """Spots crowded around every event (pixel offsets up to 60, i.e. angles
+    spread across the 1-degree threshold) exercise the z-window search of the
+    sorted spots; a projection with more spots than the sort holds (4200
+    extra) switches to the full scan.  Both equal the oracle."""
+    s = scene
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(37)
+    frames = _frames(s, rng)
+    dense = []
+    for f in frames:
+        extra = [[y + rng.uniform(-60, 60), z + rng.uniform(-60, 60), 20.0] for y, z, _ in f for _ in range(6)]
+        dense.append(np.array(list(f) + extra).reshape(-1, 3))
+    fam_sq, dposs = _families(s)
+    q = np.concatenate([s.q, s.q + rng.normal(scale=0.004, size=(3, 4)) * [0, 1, 1, 1], scenes.random_quats(rng, 10)])
+    det = int(s.geo.detysize)
+    crowded = list(dense)
+    crowded[0] = np.vstack([dense[0], np.column_stack([rng.uniform(1, det, 4200), rng.uniform(1, det, 4200),
+                                                       np.full(4200, 20.0)])])
+    for name, fr in (("dense", dense), ("full scan", crowded)):
+        print(f"forward_gt {name}: {q.shape[0]} orientations, {sum(len(f) for f in fr)} spots, "
+              f"largest projection {max(len(f) for f in fr)}", flush=True)
+        engine.set_frame_spots(fr)
+        got = engine.forward_gt(s.centers[1], q, fam_sq, dposs)
+        ref = GM.gt_score(s.geo, s.centers[1], q, s.rot, s.hkl, fr, fam_sq, dposs)
+        same = (got == ref) | (np.isnan(got) & np.isnan(ref))
+        assert same.all(), (name, np.argwhere(~same)[:5])
+        assert got[:3, 0].min() > 10
