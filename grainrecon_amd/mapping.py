@@ -2,17 +2,18 @@
 GrainMapping_engine_gpu_comp.m:54-331 on the device stages.
 
 Every numeric step runs in the engine: seed indexing (grs_index_seeds,
-fit_OR_constrain_with_local_comp_cuda), the spot pairing of the fitted
+fit_OR_constrain_with_local_comp_cuda; with check_fit_all the
+fit_OR_constrain_with_local_comp_cuda_2 variant of
+indexing.index_seed_fit_all for seeds more than 2 voxels from every voxel
+indexed before the call), the spot pairing of the fitted
 orientation (grs_pair_spots), the grain centre-of-mass fit
 (grs_fit_grain_com), the re-scoring after a centre update (the counts of
 grs_pair_spots), the region growth on the resident Completeness / Mask /
 Dismedian / Ninter volumes (grs_grow_region).  This module keeps the
 reference's per-seed control flow and the host-side DS volumes (GrainId,
-Rodrigues, IPF001, EulerZXZ, VisitFlag).  Branches the engine does not cover
-raise NotImplementedError: the fit_all mode
-(fit_OR_constrain_with_local_comp_cuda_2.m, entered only through
-check_fit_all) and the CPU connected-voxel growth grow_indexed_region.m
-(used when the spot half length is at most 2 pixels).
+Rodrigues, IPF001, EulerZXZ, VisitFlag).  The CPU connected-voxel growth
+grow_indexed_region.m (used when the spot half length is at most 2 pixels)
+is not covered and raises NotImplementedError.
 """
 from __future__ import annotations
 
@@ -22,7 +23,7 @@ import numpy as np
 
 from .engine import euler2u_deg, euler_to_rod_quat
 from .geometry import quaternion2U
-from .indexing import pos2ind
+from .indexing import index_seed_fit_all, pos2ind
 
 
 class MappingDS:
@@ -55,9 +56,14 @@ def _download(eng, ds):
 
 
 def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_comp, drop_off, maxD, max_dmedian,
-                         nr_seed, first_grain_id, center, vox_size, rvp_first, tomo_dim, simap=False):
-    """GrainMapping_engine_gpu_comp.m:54-331 with check_fit_all = 0.
-    OR_q: the orientation set (N, 4); ds is updated in place.  Returns
+                         nr_seed, first_grain_id, center, vox_size, rvp_first, tomo_dim, simap=False,
+                         check_fit_all=False):
+    """GrainMapping_engine_gpu_comp.m:54-331.  OR_q: the orientation set
+    (N, 4); ds is updated in place.  With check_fit_all, the voxels with
+    Completeness >= trust_comp at entry (else >= min(1.5 min_comp,
+    0.8 trust_comp)) are listed (:46-53), and a first indexing of a seed more
+    than 2 voxels from all of them uses the fit_all variant (:119-123;
+    MATLAB rejects an empty list there, and so does this function).  Returns
     (indexing_final (P, 18), pos_list_new (P, 3), Nr_indexed_voxel (P,))."""
     pos_list = np.asarray(pos_list, dtype=float).reshape(-1, 3)
     OR_q = np.asarray(OR_q, dtype=float).reshape(-1, 4)
@@ -72,6 +78,13 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
     eng.set_orientation_set(OR_q)
     _upload(eng, ds)
     temp = None
+    ind_gid = None
+    if check_fit_all:
+        flat = ds.Completeness.reshape(-1, order="F")
+        lin = np.flatnonzero(flat >= trust_comp)
+        if lin.size == 0:
+            lin = np.flatnonzero(flat >= min(min_comp * 1.5, trust_comp * 0.8))
+        ind_gid = np.stack(np.unravel_index(lin, dims, order="F"), 1) + 1
 
     def p2i(p):
         return pos2ind(p, center, vox_size, rvp_first, dims, tomo_dim, simap)
@@ -87,7 +100,16 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                 stop = True
                 pos_new[i] = pos_ix
                 break
-            if rcc[i] == 0:
+            fit_all = False
+            if rcc[i] == 0 and check_fit_all:
+                if ind_gid.shape[0] == 0:
+                    raise ValueError("check_fit_all needs indexed voxels (an empty list fails in MATLAB)")
+                d = ind_gid - np.array([x + 1, y + 1, z + 1])
+                fit_all = float(np.sqrt((d * d).sum(1)).min()) > 2
+            if fit_all:
+                # :123-127: fit_OR_constrain_with_local_comp_cuda_2 with fit_all_flag = 1
+                res = {"indexing": index_seed_fit_all(eng, pos_ix, OR_q, nr_seed)["indexing"][None, :]}
+            elif rcc[i] == 0:
                 res = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
             else:
                 # :87-91: the indexing repeated on ori_local_sampling(q, 4) of the last fit
