@@ -11,9 +11,10 @@ orientation (grs_pair_spots), the grain centre-of-mass fit
 grs_pair_spots), the region growth on the resident Completeness / Mask /
 Dismedian / Ninter volumes (grs_grow_region).  This module keeps the
 reference's per-seed control flow and the host-side DS volumes (GrainId,
-Rodrigues, IPF001, EulerZXZ, VisitFlag).  The CPU connected-voxel growth
-grow_indexed_region.m (used when the spot half length is at most 2 pixels)
-is not covered and raises NotImplementedError.
+Rodrigues, IPF001, EulerZXZ, VisitFlag).  When the spot half length is at
+most 2 pixels the reference grows by connected voxels on the CPU
+(grow_indexed_region.m); grow_connected keeps that loop on the host and
+scores each step's new neighbours on the engine.
 """
 from __future__ import annotations
 
@@ -43,6 +44,102 @@ class MappingDS:
         self.Dismedian = np.zeros((nx, ny, nz))
         self.Ninter = np.zeros((nx, ny, nz))
         self.VisitFlag = np.zeros((nx, ny, nz))
+
+
+_NEIGHBOURS = ((1, 1, 0), (1, -1, 0), (1, 0, 1), (1, 0, -1), (1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 1, 1), (0, 0, 1),
+               (0, -1, 1), (0, 1, -1), (0, 0, -1), (0, -1, -1), (-1, 1, 0), (-1, -1, 0), (-1, 0, 1), (-1, 0, -1),
+               (-1, 0, 0))
+
+
+def _mround(v: float) -> float:
+    return math.floor(v + 0.5) if v >= 0 else -math.floor(-v + 0.5)
+
+
+def grow_connected(eng, ds, seed, indexed_comp, U, drop_off, maxD, max_dmedian, p1):
+    """grow_indexed_region.m:1-142, the growth GrainMapping_engine_gpu_comp.m:
+    207-214 uses when the spot half length is at most 2 pixels: from the
+    seed (1-based voxel), the 18 neighbours of the voxel last added that are
+    in the mask and not yet listed are scored with forward_comp on the engine
+    (the scoring of cuda_grow_indexed_region, one launch per step for the
+    neighbours not scored before: a voxel's score depends only on the voxel,
+    U and the stack) and queued when C > I and dis_median <= min(DisMedian,
+    maxDmedian), or |C - I| <= 0.01 and Nr_intersect > Ninter; the queued
+    voxel of largest C / seed_comp (first maximum) joins the region while that
+    ratio exceeds 1 - drop_off and the region is smaller than the mask count
+    minus 1, and leaves the queue by the last entry moving into its slot.
+    The grown voxels' Completeness, Dismedian and Ninter are written into ds
+    (the caller's copy-back, :234-236).  p1: the 11 values [RecVolumePixel(:,1)'
+    simap VoxSize Center' Dimension].  Returns (center, replace_center,
+    indexed (n, 3) 1-based in find() order)."""
+    I = ds.Completeness.copy()
+    mask, DM, NI = ds.Mask, ds.Dismedian, ds.Ninter
+    x, y, z = (int(v) for v in seed)
+    I[x - 1, y - 1, z - 1] = indexed_comp
+    seed_comp = I[x - 1, y - 1, z - 1]
+    J = np.zeros(I.shape, np.int8)
+    upd = {}
+    param1 = np.concatenate([np.asarray(p1, float)[:11], [seed_comp, drop_off, max_dmedian]])
+    n_mask = int((mask > 0).sum())
+    csum = seed_comp
+    cpos = [seed_comp * x, seed_comp * y, seed_comp * z]
+    c0 = [v / csum for v in cpos]
+    queue = []
+    scores = {}
+    reg_size = 0
+    nx, ny, nz = I.shape
+    while True:
+        cand = []
+        for dx, dy, dz in _NEIGHBOURS:
+            xn, yn, zn = x + dx, y + dy, z + dz
+            if 1 <= xn <= nx and 1 <= yn <= ny and 1 <= zn <= nz and J[xn - 1, yn - 1, zn - 1] == 0 and \
+                    mask[xn - 1, yn - 1, zn - 1] > 0:
+                cand.append((xn, yn, zn))
+        new = [c for c in cand if c not in scores]
+        if new:
+            rows = np.array([[c[0], c[1], c[2], 0.0, 1.0, 0.0] for c in new])
+            out = eng.grow_indexed_region(rows, U, param1)
+            for c, o in zip(new, out):
+                scores[c] = (o[3], o[4], o[5])
+        for c in cand:
+            ns, ni, med = scores[c]
+            C = ni / ns if ns > 0 else math.nan
+            Iv, DMv, NIv = I[c[0] - 1, c[1] - 1, c[2] - 1], DM[c[0] - 1, c[1] - 1, c[2] - 1], NI[c[0] - 1, c[1] - 1,
+                                                                                                   c[2] - 1]
+            lim = max_dmedian if math.isnan(DMv) else (DMv if math.isnan(max_dmedian) else min(DMv, max_dmedian))
+            if (C > Iv and med <= lim) or (abs(C - Iv) <= 0.01 and ni > NIv):
+                queue.append((c[0], c[1], c[2], C, med, ni))
+                J[c[0] - 1, c[1] - 1, c[2] - 1] = 1
+        J[x - 1, y - 1, z - 1] = 2
+        if not queue:
+            break
+        ratios = [q[3] / seed_comp for q in queue]
+        best = max(ratios)
+        k = ratios.index(best)
+        reg_size += 1
+        if not (best > 1 - drop_off and reg_size < n_mask - 1):
+            break
+        qx, qy, qz, C, med, ni = queue[k]
+        csum = csum + C
+        cpos = [cpos[0] + C * qx, cpos[1] + C * qy, cpos[2] + C * qz]
+        upd[(qx, qy, qz)] = (C, med, ni)
+        x, y, z = qx, qy, qz
+        queue[k] = queue[-1]
+        queue.pop()
+    lin = np.flatnonzero((J > 1).reshape(-1, order="F"))
+    idx = np.stack(np.unravel_index(lin, J.shape, order="F"), 1) + 1
+    for xi, yi, zi in idx:
+        if (xi, yi, zi) in upd:
+            C, med, ni = upd[(xi, yi, zi)]
+            ds.Completeness[xi - 1, yi - 1, zi - 1] = C
+            DM[xi - 1, yi - 1, zi - 1] = med
+            NI[xi - 1, yi - 1, zi - 1] = ni
+        else:
+            ds.Completeness[xi - 1, yi - 1, zi - 1] = I[xi - 1, yi - 1, zi - 1]
+    c1 = [v / csum for v in cpos]
+    e0, e1, e2 = c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]
+    if math.sqrt((e0 * e0 + e1 * e1) + e2 * e2) > max(maxD, maxD / 10000 * idx.shape[0]):
+        return np.array([_mround(v) for v in c1]), 1, idx
+    return np.array(c0), 0, idx
 
 
 def _upload(eng, ds):
@@ -167,9 +264,14 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                 else:
                     spot_l = 20.0
                 if spot_l <= 2:
-                    raise NotImplementedError("grow_indexed_region.m (connected-voxel growth for Spot_L <= 2 px)")
-                g = eng.grow_region((x + 1, y + 1, z + 1), temp[6], UU, spot_l, drop_off, maxD, max_dmedian, p1)
-                _download(eng, ds)
+                    # :207-214: connected-voxel growth on the host volumes
+                    gc, grep_, gidx = grow_connected(eng, ds, (x + 1, y + 1, z + 1), temp[6], UU, drop_off, maxD,
+                                                     max_dmedian, p1)
+                    _upload(eng, ds)
+                    g = {"center": gc, "replace_center": grep_, "indexed": gidx}
+                else:
+                    g = eng.grow_region((x + 1, y + 1, z + 1), temp[6], UU, spot_l, drop_off, maxD, max_dmedian, p1)
+                    _download(eng, ds)
                 idx = g["indexed"] - 1
                 nr_vox[i] = idx.shape[0]
                 xi, yi, zi = idx[:, 0], idx[:, 1], idx[:, 2]

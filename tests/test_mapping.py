@@ -46,3 +46,37 @@ def test_grain_mapping_engine_end_to_end(engine):
     assert o1[0, 6] > 0.95 and n1[0] > 1 and d1.GrainId[6, 6, 6] == 1
     assert d1.VisitFlag[6, 6, 6] == 1.0
     assert (d1.GrainId == 1).sum() == n1[0]
+
+
+@pytest.mark.gpu
+def test_grain_mapping_engine_small_spots_grows_by_connected_voxels(engine):
+    """Spot areas of 3 px^2 (half length sqrt(3/pi) < 2 px) send the growth to
+    the connected-voxel loop (GrainMapping_engine_gpu_comp.m:207-214): the
+    grain grows from its centre, the engine's volumes equal the host's after
+    the run, and a second run gives the same result."""
+    from grainrecon_amd.mapping import MappingDS, grain_mapping_engine
+    from test_index import _index_problem
+
+    s, qs, _ = _index_problem()
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    frames = [f.copy() for f in _frames(s)]
+    for f in frames:
+        if f.shape[0]:
+            f[:, 2] = 3.0
+    engine.set_frame_spots(frames)
+    n, vox = 14, 0.005
+    kw = dict(trust_comp=0.9, min_comp=0.5, drop_off=0.2, maxD=2.0, max_dmedian=20.0, nr_seed=6, first_grain_id=1,
+              center=s.centers[0], vox_size=vox, rvp_first=[1, 1, 1], tomo_dim=(n, n, n))
+    runs = []
+    for _ in range(2):
+        ds = MappingDS(np.ones((n, n, n)))
+        print("grain_mapping_engine: connected growth, 1 seed", flush=True)
+        out, pos_new, nvox = grain_mapping_engine(engine, s.centers[0][None, :], qs, ds, **kw)
+        for k in ("Completeness", "Dismedian", "Ninter"):
+            np.testing.assert_array_equal(engine.get_volume(k), getattr(ds, k), err_msg=k)
+        runs.append((out, nvox, ds))
+    (o1, n1, d1), (o2, n2, d2) = runs
+    np.testing.assert_array_equal(o1, o2)
+    np.testing.assert_array_equal(d1.GrainId, d2.GrainId)
+    assert o1[0, 6] > 0.95 and n1[0] > 1 and d1.GrainId[6, 6, 6] == 1

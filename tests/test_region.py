@@ -93,3 +93,54 @@ def test_grow_region_equals_oracle(engine):
         assert got["replace_center"] == rc and np.array_equal(got["center"], c)
         for k, v in vols.items():
             np.testing.assert_array_equal(engine.get_volume(k), v, err_msg=k)
+
+
+def _connected_problem():
+    s = scenes.make_scene(seed=83, ngrains=3, spread_mm=0.05, nrot=37)
+    n, vox = 14, 0.005
+    p1 = np.array([1, 1, 1, 0, vox, *s.centers[0], n, n, n], float)  # voxel (7,7,7) sits on grain 0's centre
+    rng = np.random.default_rng(23)
+    vols = {"Completeness": rng.uniform(0, 0.3, (n, n, n)), "Mask": (rng.random((n, n, n)) > 0.05).astype(float),
+            "Dismedian": rng.uniform(0, 5, (n, n, n)), "Ninter": rng.integers(0, 40, (n, n, n)).astype(float)}
+    vols["Dismedian"][rng.random((n, n, n)) < 0.05] = np.nan
+    return s, p1, vols
+
+
+def test_grow_connected_oracle_grows_the_grain():
+    """The connected growth from grain 0's centre stays connected (every grown
+    voxel has an 18-neighbour in the region) and covers the grain's voxels."""
+    from oracle import grow_connected as GC
+
+    s, p1, vols = _connected_problem()
+    c, rc, idx = GC.grow_connected(s.geo, s.stack, vols, (7, 7, 7), 0.95, s.U[0], 0.2, 1.0, 10.0, p1, s.rot, s.hkl)
+    cells = {tuple(v) for v in idx.tolist()}
+    assert (7, 7, 7) in cells and len(cells) > 5
+    offs = [o for o in GC.NEIGHBOURS]
+    for v in cells:
+        if v != (7, 7, 7):
+            assert any((v[0] + a, v[1] + b, v[2] + d) in cells for a, b, d in offs)
+    assert vols["Completeness"][6, 6, 6] == 0.95
+
+
+@pytest.mark.gpu
+def test_grow_connected_equals_oracle(engine):
+    """grow_indexed_region.m on the engine (grainrecon_amd.mapping.grow_connected)
+    against the oracle: the same region in find() order, centre decision and
+    written-back volumes."""
+    from grainrecon_amd.mapping import MappingDS, grow_connected
+    from oracle import grow_connected as GC
+
+    s, p1, vols = _connected_problem()
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    for seed, comp, drop in (((7, 7, 7), 0.95, 0.2), ((8, 6, 7), 0.7, 0.5)):
+        ds = MappingDS(vols["Mask"])
+        ds.Completeness, ds.Dismedian, ds.Ninter = (vols[k].copy() for k in ("Completeness", "Dismedian", "Ninter"))
+        print(f"grow_connected: seed {seed}", flush=True)
+        c, rc, idx = grow_connected(engine, ds, seed, comp, s.U[0], drop, 1.0, 10.0, p1)
+        ref = {k: v.copy() for k, v in vols.items()}
+        c2, rc2, idx2 = GC.grow_connected(s.geo, s.stack, ref, seed, comp, s.U[0], drop, 1.0, 10.0, p1, s.rot, s.hkl)
+        assert idx.tolist() == idx2.tolist() and idx.shape[0] > 3
+        assert rc == rc2 and np.array_equal(c, c2)
+        for k, a in (("Completeness", ds.Completeness), ("Dismedian", ds.Dismedian), ("Ninter", ds.Ninter)):
+            np.testing.assert_array_equal(a, ref[k], err_msg=k)
