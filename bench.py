@@ -47,6 +47,11 @@ def flop_per_eval(nrot: int, H: int) -> int:
 # profiles/k_fast_seed_cfg2_counters.md): every random distance-key read
 # pulls a 128-byte line.
 TRAFFIC_BYTES_PER_EVAL = 55_900
+# FP64 operations the seeding launch executes per evaluation, from the same
+# counter profile ((2 FMA + MUL + ADD + TRANS) F64 wave instructions x 64):
+# the omega solve and the resolution stage skip the rest of the algorithmic
+# count
+EXECUTED_FP64_PER_EVAL = 166_150
 
 
 def log(msg):
@@ -648,6 +653,9 @@ def main():
                 "achieved_note": ("algorithmic: the full forward_comp.m work of every evaluation; the omega solve and "
                                   "the resolution stage execute about 9% of it (counter profile)"),
                 "kernel": "k_fast", "kernel_ms": round(kms / kn, 4), "flop_per_eval": fe,
+                "executed_fp64_per_eval": EXECUTED_FP64_PER_EVAL if args.config == "cfg2" else None,
+                "executed_tflops": (round(N * EXECUTED_FP64_PER_EVAL / (kms / kn * 1e-3) / 1e12, 3)
+                                    if args.config == "cfg2" else None),
                 "evals_per_launch": N}
 
     result = {
