@@ -80,6 +80,44 @@ def cpu_baseline(cfg, proj_host, pos, q, gpu_out, seconds, threads):
     return n / dt, n, dt, exact
 
 
+def host_io_leg(eng, cfg, q, pos, calls, legacy_calls):
+    """The same seeding call with transfers included: grs_forward_comp with
+    host buffers (orientations up, results down every call), and the legacy
+    single-precision entry grs_cuda_forward_comp, which like the reference
+    wrapper (ForwardCalc_OR_space_comp_cuda.m:17-42) receives the geometry and
+    the whole distance stack with every call and uploads them."""
+    from grainrecon_amd import engine as E
+
+    N = q.shape[0]
+    eng.forward_comp(pos, q)
+    t = time.perf_counter()
+    for _ in range(calls):
+        eng.forward_comp(pos, q)
+    dt = time.perf_counter() - t
+    res = {"host_buffers": {"value": round(calls * N / dt, 1), "unit": "evals/s",
+                            "ms_per_call": round(dt / calls * 1e3, 3), "calls": calls,
+                            "bytes_per_call": int(N * (32 + 20))}}
+    log(f"[bench] host buffers: {calls} calls -> {calls * N / dt:.4g} evals/s ({dt / calls * 1e3:.2f} ms per call)")
+    if legacy_calls > 0:
+        geo, nrot, H = cfg.geo, cfg.rot.size, cfg.hkl.shape[0]
+        proj = eng.get_distance_stack((int(geo.detzsize), int(geo.detysize), nrot)).reshape(-1, order="F")
+        args = (np.asarray(pos, float), q.reshape(-1), np.asarray(geo.S).reshape(-1), np.asarray(geo.B).reshape(-1),
+                cfg.hkl.reshape(-1), proj, cfg.rot, np.asarray(geo.RotDet).reshape(-1), geo.legacy_param(H, nrot))
+        E.cuda_forward_comp(*args)
+        t = time.perf_counter()
+        for _ in range(legacy_calls):
+            out = E.cuda_forward_comp(*args)
+        ldt = time.perf_counter() - t
+        res["legacy_f32"] = {"value": round(legacy_calls * N / ldt, 1), "unit": "evals/s",
+                             "ms_per_call": round(ldt / legacy_calls * 1e3, 1), "calls": legacy_calls,
+                             "stack_bytes_per_call": int(proj.nbytes),
+                             "orientations_with_events": int((out.reshape(-1, 4)[:, 0] > 0).sum())}
+        log(f"[bench] legacy f32 entry: {legacy_calls} calls -> {legacy_calls * N / ldt:.4g} evals/s "
+            f"({ldt / legacy_calls * 1e3:.0f} ms per call, {proj.nbytes / 1e9:.2f} GB stack uploaded each)")
+        del proj
+    return res
+
+
 def geofit_leg(eng, cfg, n_grains, nm_grains, seed=4004):
     """Geometry and grain refinement on matched spot centroids (0.5 px
     Gaussian noise, perturbed start): adjoint LM on the GPU and, when
@@ -561,6 +599,8 @@ def main():
     ap.add_argument("--al", type=int, default=1, help="run the FCC Al 2.5 M-voxel leg (1) or not (0)")
     ap.add_argument("--al-steps", type=int, default=2)
     ap.add_argument("--al-rows", type=int, default=16, help="Al voxels checked against the oracle")
+    ap.add_argument("--host-io-calls", type=int, default=3, help="seeding calls with host buffers; 0 = off")
+    ap.add_argument("--legacy-calls", type=int, default=2, help="calls of the legacy f32 entry; 0 = off")
     ap.add_argument("--geo-fit-grains", type=int, default=300, help="grains in the config-4 fit leg; 0 = off")
     ap.add_argument("--geo-fit-nm-grains", type=int, default=20, help="grains in the Nelder-Mead comparison")
     ap.add_argument("--index-positions", type=int, default=8, help="seed positions per indexing call; 0 = off")
@@ -675,6 +715,8 @@ def main():
     }
     if copy_probe:
         result["copy_probe"] = copy_probe
+    if args.host_io_calls > 0:
+        result["with_transfers"] = host_io_leg(eng, cfg, q, seeds[0], args.host_io_calls, args.legacy_calls)
     batched = None
     if args.batched_voxels > 0:
         bres, bbatch, bout, nrg = batched_leg(eng, cfg, args.batched_voxels, args.batched_steps, 1)
