@@ -614,7 +614,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by voxel or "
-              "orientation with no exchange beyond gathering results; see DESIGN.md)", file=sys.stderr)
+              "orientation with no exchange beyond gathering results)", file=sys.stderr)
         sys.exit(2)
 
     from grainrecon_amd import _lib

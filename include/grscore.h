@@ -12,7 +12,7 @@
  *  1. Legacy MEX operators, one per native operator of GrainRecon.  They take
  *     exactly the single-precision row vectors that the MATLAB wrappers build
  *     and return the same single-precision output layout, so a MEX gateway
- *      can forward its mxArrays unchanged.  The inputs are
+ *     can forward its mxArrays unchanged.  The inputs are
  *     promoted to double and scored in FP64.
  *       grs_cuda_forward_comp          replaces cuda_forward_comp
  *           (call sites ForwardCalc_OR_space_comp_cuda.m:32,

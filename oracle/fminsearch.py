@@ -1,5 +1,5 @@
-"""Bounded Nelder-Mead with fminsearchbnd semantics -- TEST INFRASTRUCTURE /
-CPU BASELINE ONLY.
+"""Bounded Nelder-Mead with fminsearchbnd semantics -- TEST INFRASTRUCTURE
+(and the CPU baseline of the geometry fit) ONLY.
 
 The reference fits geometry, grains and orientations with fminsearchbnd
 (L_shift_tilt_fitting.m:195-222, fit_grain_COM.m:25-28,
