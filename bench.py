@@ -739,6 +739,16 @@ def main():
                                   "sample": f"{n} orientations x 1 seed voxel of the same workload "
                                             f"(oracle/grs_oracle.c, gcc -O2, OpenMP)"}
         result["parity"] = {"evals_checked": n, "bit_exact": bool(exact)}
+        # the same oracle on one core, on a smaller sample
+        import oracle
+
+        n1 = 1024
+        t1 = time.perf_counter()
+        oracle.or_space_comp(cfg.geo, oracle.Stack(proj), pos0, qn[:n1], cfg.rot, cfg.hkl, nthreads=1)
+        r1 = n1 / (time.perf_counter() - t1)
+        result["cpu_baseline_1core"] = {"value": round(r1, 2), "unit": "evals/s", "cores": 1, "kind": "port",
+                                        "sample": f"{n1} orientations of the same sample"}
+        log(f"[bench] cpu oracle on 1 core: {r1:.4g} evals/s")
         if batched:
             bbatch, bout, nrg = batched
             sizes = np.diff(bbatch[1])
