@@ -48,6 +48,10 @@ def _variant_scene(name):
         kw.update(tilts=(0.3, -0.2, 0.25), dety00=0.04, detz00=-0.03)
     if name == "beamstop":
         kw.update(beamstop=((110.0, 150.0), (100.0, 160.0)))
+    if name == "off_centre":
+        # detector well off the beam and tilted: the corner rays and slopes
+        # of the candidate bounds are far from symmetric
+        kw.update(tilts=(2.0, -1.5, 3.0), dety00=1.5, detz00=-1.0)
     s = scenes.make_scene(**kw)
     if name == "offsets":
         s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.02, -0.015, 0.01
@@ -88,7 +92,7 @@ def _assert_same(got, ref):
     assert np.array_equal(_bits(med), _bits(ref[:, 3].astype(np.float32)))
 
 
-@pytest.mark.parametrize("variant", ["plain", "tilted", "offsets", "beam_offsets", "sample_tilt", "beamstop"])
+@pytest.mark.parametrize("variant", ["plain", "tilted", "off_centre", "offsets", "beam_offsets", "sample_tilt", "beamstop"])
 @pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
 def test_forward_comp_parity(engine, variant, method):
     s = _variant_scene(variant)
