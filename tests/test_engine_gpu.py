@@ -60,6 +60,10 @@ def _variant_scene(name):
         # ray K_out of forward_comp.m:49 is then visibly off the 2 theta cone
         # around the beam direction
         s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.3, -0.3, 0.05
+    if name == "irregular_rot":
+        # rotation angles not on a uniform grid: no omega solve, every event
+        # goes through the phase-A filter
+        s.rot = s.rot + np.random.default_rng(9).uniform(-1.5, 1.5, s.rot.size)
     if name == "sample_tilt":
         # a tilted sample frame with offsets: the beam direction at a voxel then
         # depends on every component of S*pos
@@ -92,7 +96,7 @@ def _assert_same(got, ref):
     assert np.array_equal(_bits(med), _bits(ref[:, 3].astype(np.float32)))
 
 
-@pytest.mark.parametrize("variant", ["plain", "tilted", "off_centre", "offsets", "beam_offsets", "sample_tilt", "beamstop"])
+@pytest.mark.parametrize("variant", ["plain", "tilted", "off_centre", "offsets", "beam_offsets", "sample_tilt", "irregular_rot", "beamstop"])
 @pytest.mark.parametrize("method", METHODS, ids=["fast", "plain"])
 def test_forward_comp_parity(engine, variant, method):
     s = _variant_scene(variant)
@@ -104,7 +108,7 @@ def test_forward_comp_parity(engine, variant, method):
         ref = oracle.or_space_comp(s.geo, s.stack, pos, q, s.rot, s.hkl)
         got = engine.forward_comp(pos, q, method)
         _assert_same(got, ref)
-        if variant not in ("offsets", "beam_offsets", "sample_tilt"):  # rendered before the geometry was changed
+        if variant not in ("offsets", "beam_offsets", "sample_tilt", "irregular_rot"):  # rendered before the change
             assert got[2][g] == 1.0  # the grain's own orientation at one of its voxels
         assert got[0][-1] == 0 and got[2][-1] == 0.0
 
