@@ -469,7 +469,7 @@ __device__ float wave_median(const uint32_t *list, int32_t n, int lane) {
  * row and the index inside the row's expanded list. */
 template <int MAP>
 __device__ __forceinline__ int eval_setup(const grs_eval_args &a, int64_t e, int64_t *vi, double *sub, int64_t *row,
-                                          int64_t *jloc) {
+                                          int64_t *jloc, int64_t *hint = nullptr) {
   if (MAP == GRS_MAP_CSR || MAP == GRS_MAP_CSR_LIST) {
     const grs_csr &c = a.csr;
     int64_t ge = e;
@@ -477,12 +477,26 @@ __device__ __forceinline__ int eval_setup(const grs_eval_args &a, int64_t e, int
       ge = c.elist[e];
       if (ge < 0) return -1;
     }
-    int64_t lo = 0, hi = c.nrows;
-    while (hi - lo > 1) {
-      int64_t mid = (lo + hi) >> 1;
-      if (c.row_eoff[mid] <= ge) lo = mid;
-      else hi = mid;
+    /* the row of the previous evaluation of this wave, or the next one,
+     * usually holds ge (a wave's evaluations ascend); else a binary search */
+    int64_t lo = -1;
+    if (hint && *hint >= 0) {
+      const int64_t h = *hint;
+      if (c.row_eoff[h] <= ge) {
+        if (h + 1 >= c.nrows || c.row_eoff[h + 1] > ge) lo = h;
+        else if (h + 2 >= c.nrows || c.row_eoff[h + 2] > ge) lo = h + 1;
+      }
     }
+    if (lo < 0) {
+      int64_t hi = c.nrows;
+      lo = 0;
+      while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (c.row_eoff[mid] <= ge) lo = mid;
+        else hi = mid;
+      }
+    }
+    if (hint) *hint = lo;
     int64_t j = ge - c.row_eoff[lo];
     int64_t cand = c.row_ptr[lo] + j / c.S;
     int32_t smp = (int32_t)(j % c.S);
@@ -845,10 +859,11 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   /* starting (omega, hkl) of each lane in the flattened event space */
   const int32_t om0 = lane / H, h0 = lane - (lane / H) * H;
 
+  int64_t row_hint = -1;
   for (int64_t e = a.e0 + gwave; e < a.n_eval; e += nwaves) {
     int64_t vi, row, jloc;
     double sub[9];
-    int ok = eval_setup<MAP>(a, e, &vi, sub, &row, &jloc);
+    int ok = eval_setup<MAP>(a, e, &vi, sub, &row, &jloc, &row_hint);
     if (ok <= 0) {
       if (!MEDIAN) {
         /* a skipped orientation counts as C = NaN for the argmax; in a grid
