@@ -90,6 +90,7 @@ struct grs_ctx {
   int64_t v_dim[3] = {0, 0, 0};
   /* G-vector matching score buffers */
   DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_sidx, t_slen;
+  DevBuf v_lin, v_out; /* grs_get_volume_at */
   /* grain centre-of-mass fit buffers */
   DevBuf m_rows, m_rptr, m_x, m_pfit, m_doff, m_dis;
   bool v_set[4] = {false, false, false, false};
@@ -223,7 +224,8 @@ void grs_destroy(grs_ctx *c) {
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
-                    &c->m_x, &c->m_pfit, &c->m_doff, &c->m_dis, &c->t_skey, &c->t_sidx, &c->t_slen};
+                    &c->m_x, &c->m_pfit, &c->m_doff, &c->m_dis, &c->t_skey, &c->t_sidx, &c->t_slen, &c->v_lin,
+                    &c->v_out};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1828,6 +1830,26 @@ grs_status grs_get_volume(grs_ctx *c, int32_t which, double *v, int64_t n) {
   if (n != c->v_dim[0] * c->v_dim[1] * c->v_dim[2]) return fail(c, GRS_ERR_ARG, "size mismatch");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(v, c->v_vol[which].p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
+grs_status grs_get_volume_at(grs_ctx *c, int32_t which, const int64_t *lin, int64_t n, double *out) {
+  if (!c) return GRS_ERR_ARG;
+  if (which < 0 || which > 3 || n < 0 || (n > 0 && (!lin || !out))) return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (!c->v_set[which]) return fail(c, GRS_ERR_STATE, "volume %d not set", which);
+  const int64_t total = c->v_dim[0] * c->v_dim[1] * c->v_dim[2];
+  for (int64_t i = 0; i < n; i++)
+    if (lin[i] < 0 || lin[i] >= total)
+      return fail(c, GRS_ERR_ARG, "linear index %lld outside the volume (%lld voxels)", (long long)lin[i],
+                  (long long)total);
+  if (n == 0) return GRS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  ENSURE(c, c->v_lin, sizeof(int64_t) * (size_t)n);
+  ENSURE(c, c->v_out, sizeof(double) * (size_t)n);
+  HIPCHK(c, hipMemcpyAsync(c->v_lin.p, lin, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_gather_f64(P<double>(c->v_vol[which]), P<int64_t>(c->v_lin), n, P<double>(c->v_out), c->st));
+  HIPCHK(c, hipMemcpyAsync(out, c->v_out.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return GRS_OK;
 }

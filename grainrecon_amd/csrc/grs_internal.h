@@ -190,6 +190,9 @@ hipError_t grs_launch_region_accept(const double *d_rows, int64_t M, const int32
                                     double *d_vC, double *d_vDM, double *d_vNI, int64_t *d_acc_lin, double *d_acc_C,
                                     uint32_t *d_count, hipStream_t st);
 
+/* out[i] = v[lin[i]] */
+hipError_t grs_launch_gather_f64(const double *d_v, const int64_t *d_lin, int64_t n, double *d_out, hipStream_t st);
+
 /* G-vector matching score (grs_gt.hip) */
 hipError_t grs_launch_gt_proj(const grs_dgeo *d_g, const double *d_pos, const double *d_cs, int32_t nrot, double Ltr,
                               double *d_proj, hipStream_t st);

@@ -90,3 +90,15 @@ hipError_t grs_launch_region_accept(const double *d_rows, int64_t M, const int32
                      d_C, d_med, d_p1, nx, ny, d_vC, d_vDM, d_vNI, d_acc_lin, d_acc_C, d_count);
   return hipGetLastError();
 }
+
+/* out[i] = v[lin[i]] (lin checked by the caller) */
+__global__ void k_gather_f64(const double *v, const int64_t *lin, int64_t n, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v[lin[i]];
+}
+
+hipError_t grs_launch_gather_f64(const double *d_v, const int64_t *d_lin, int64_t n, double *d_out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_v, d_lin, n, d_out);
+  return hipGetLastError();
+}

@@ -335,6 +335,20 @@ class Engine:
         self._chk(self._lib.grs_get_volume(self._h, self.VOLUMES[name], _ptr(flat, _dp), flat.size))
         return flat.reshape(nz, ny, nx).transpose(2, 1, 0).copy()
 
+    def get_volume_at(self, name: str, vox) -> np.ndarray:
+        """V(x, y, z) at the (n, 3) 1-based [x y z] rows of vox, gathered on
+        the device (an index outside the volume raises GrsError)."""
+        nx, ny, nz = self._vshape
+        v = np.asarray(vox, dtype=np.int64).reshape(-1, 3) - 1
+        lin = np.ascontiguousarray((v[:, 0] + nx * (v[:, 1] + ny * v[:, 2])).astype(np.int64))
+        # a component outside its axis could still land inside the volume
+        lin[np.any((v < 0) | (v >= np.array([nx, ny, nz])), axis=1)] = -1
+        out = np.zeros(lin.size)
+        self._chk(self._lib.grs_get_volume_at(self._h, self.VOLUMES[name],
+                                              lin.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lin.size,
+                                              _ptr(out, _dp)))
+        return out
+
     def grow_region(self, seed, indexed_comp: float, U, spot_L: float, drop_off: float, maxD: float,
                     maxDmedian: float, p1) -> dict:
         """grow_indexed_region_cuda.m on the resident volumes (seed 1-based

@@ -147,9 +147,14 @@ def _upload(eng, ds):
         eng.set_volume(k, getattr(ds, k))
 
 
-def _download(eng, ds):
+def _download(eng, ds, vox):
+    """The device growth changes Completeness, Dismedian and Ninter only at
+    the voxels it returns (the seed and the accepted ones): copy those."""
+    if vox.shape[0] == 0:
+        return
+    x, y, z = vox[:, 0] - 1, vox[:, 1] - 1, vox[:, 2] - 1
     for k in ("Completeness", "Dismedian", "Ninter"):
-        setattr(ds, k, eng.get_volume(k))
+        getattr(ds, k)[x, y, z] = eng.get_volume_at(k, vox)
 
 
 def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_comp, drop_off, maxD, max_dmedian,
@@ -271,7 +276,7 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                     g = {"center": gc, "replace_center": grep_, "indexed": gidx}
                 else:
                     g = eng.grow_region((x + 1, y + 1, z + 1), temp[6], UU, spot_l, drop_off, maxD, max_dmedian, p1)
-                    _download(eng, ds)
+                    _download(eng, ds, g["indexed"])
                 idx = g["indexed"] - 1
                 nr_vox[i] = idx.shape[0]
                 xi, yi, zi = idx[:, 0], idx[:, 1], idx[:, 2]

@@ -347,6 +347,11 @@ grs_status grs_pair_spots(grs_ctx *ctx, const double *pos, const double *U, int6
  * others.  grs_get_volume copies one back (n = nx*ny*nz). */
 grs_status grs_set_volume(grs_ctx *ctx, int32_t which, const double *v, int64_t nx, int64_t ny, int64_t nz);
 grs_status grs_get_volume(grs_ctx *ctx, int32_t which, double *v, int64_t n);
+/* out[i] = volume `which` at linear index lin[i] (x fastest, 0-based; every
+ * index checked against nx*ny*nz): a caller mirroring the volumes on the
+ * host fetches only the voxels a growth changed (no reference counterpart;
+ * the reference keeps its volumes on the host). */
+grs_status grs_get_volume_at(grs_ctx *ctx, int32_t which, const int64_t *lin, int64_t n, double *out);
 
 /* grow_indexed_region_cuda.m:29-151 around the 1-based seed voxel with the
  * orientation U (row-major) on the resident volumes: Completeness(seed) =

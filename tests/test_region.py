@@ -70,6 +70,8 @@ def test_region_one_sided_growth_moves_centre(monkeypatch):
 
 @pytest.mark.gpu
 def test_grow_region_equals_oracle(engine):
+    from grainrecon_amd._lib import GrsError
+
     s = scenes.make_scene(seed=83, ngrains=3, spread_mm=0.05, nrot=37)
     engine.set_geometry(s.geo, s.hkl, s.rot)
     engine.set_distance_stack(s.proj)
@@ -93,6 +95,16 @@ def test_grow_region_equals_oracle(engine):
         assert got["replace_center"] == rc and np.array_equal(got["center"], c)
         for k, v in vols.items():
             np.testing.assert_array_equal(engine.get_volume(k), v, err_msg=k)
+        # the device gather at the grown voxels (and a few others, repeated
+        # and out of order) reads the same values
+        vox = np.concatenate([idx, [[1, 1, 1], [n, n, n], [1, n, 2]], idx[::-1]])
+        for k, v in vols.items():
+            want = np.array([v[a - 1, b - 1, c - 1] for a, b, c in vox.tolist()])
+            np.testing.assert_array_equal(engine.get_volume_at(k, vox), want, err_msg=k)
+    assert engine.get_volume_at("Completeness", np.zeros((0, 3), int)).shape == (0,)
+    for bad in ([[n + 1, 1, 1]], [[0, 1, 1]], [[1, 1, n + 1]]):
+        with pytest.raises(GrsError):
+            engine.get_volume_at("Completeness", bad)
 
 
 def _connected_problem():
