@@ -92,7 +92,7 @@ struct grs_ctx {
   DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_sidx, t_slen;
   DevBuf v_lin, v_out; /* grs_get_volume_at */
   /* grain centre-of-mass fit buffers */
-  DevBuf m_rows, m_rptr, m_x, m_pfit, m_doff, m_dis;
+  DevBuf m_rows, m_rptr, m_nm;
   bool v_set[4] = {false, false, false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
@@ -224,7 +224,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
-                    &c->m_x, &c->m_pfit, &c->m_doff, &c->m_dis, &c->t_skey, &c->t_sidx, &c->t_slen, &c->v_lin,
+                    &c->m_nm, &c->t_skey, &c->t_sidx, &c->t_slen, &c->v_lin,
                     &c->v_out};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
@@ -2002,13 +2002,11 @@ grs_status grs_fit_grain_com(grs_ctx *c, const double *rows, const int64_t *row_
     return fail(c, GRS_ERR_ARG, "invalid argument");
   if (M == 0) return GRS_OK;
   if (row_ptr[0] != 0) return fail(c, GRS_ERR_ARG, "row_ptr must start at 0");
-  int64_t maxn = 0;
-  for (int64_t m = 0; m < M; m++) {
-    const int64_t n = row_ptr[m + 1] - row_ptr[m];
-    if (n < 1) return fail(c, GRS_ERR_ARG, "fit %lld has no pair rows", (long long)m);
-    maxn = n > maxn ? n : maxn;
-  }
-  if (M * (GRS_NM_NMAX + 1) > 65535) return fail(c, GRS_ERR_ARG, "too many fits in one call");
+  for (int64_t m = 0; m < M; m++)
+    if (row_ptr[m + 1] - row_ptr[m] < 1) return fail(c, GRS_ERR_ARG, "fit %lld has no pair rows", (long long)m);
+  if (M > 0x7fffffff) return fail(c, GRS_ERR_ARG, "too many fits in one call");
+  /* each iteration feeds at most three requests: this bounds every wave of k_com_nm */
+  if (max_iter > 100000) return fail(c, GRS_ERR_ARG, "max_iter above 100000");
   HIPCHK(c, hipSetDevice(c->device));
   grs_status s = check_ready(c);
   if (s != GRS_OK) return s;
@@ -2040,56 +2038,14 @@ grs_status grs_fit_grain_com(grs_ctx *c, const double *rows, const int64_t *row_
   }
   ENSURE(c, c->m_rows, sizeof(double) * 23 * (size_t)total);
   ENSURE(c, c->m_rptr, sizeof(int64_t) * (size_t)(M + 1));
+  ENSURE(c, c->m_nm, sizeof(grs_nm) * (size_t)M);
   HIPCHK(c, hipMemcpyAsync(c->m_rows.p, rows, sizeof(double) * 23 * (size_t)total, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->m_rptr.p, row_ptr, sizeof(int64_t) * (size_t)(M + 1), hipMemcpyHostToDevice, c->st));
-  std::vector<double> xs, dis, fvals;
-  std::vector<int32_t> pfit;
-  std::vector<int64_t> doff, first;
-  for (;;) {
-    xs.clear();
-    pfit.clear();
-    doff.clear();
-    first.assign((size_t)M + 1, 0);
-    int64_t nd = 0;
-    for (int64_t m = 0; m < M; m++) {
-      first[m] = (int64_t)pfit.size();
-      if (nm[m].done()) continue;
-      for (int p = 0; p < nm[m].npend; p++) {
-        double x[6];
-        nm[m].xform(nm[m].pend[p], x);
-        for (int k = 0; k < 6; k++) xs.push_back(x[k]);
-        pfit.push_back((int32_t)m);
-        doff.push_back(nd);
-        nd += row_ptr[m + 1] - row_ptr[m];
-      }
-    }
-    first[M] = (int64_t)pfit.size();
-    const int64_t np = (int64_t)pfit.size();
-    if (np == 0) break;
-    ENSURE(c, c->m_x, sizeof(double) * 6 * np);
-    ENSURE(c, c->m_pfit, sizeof(int32_t) * np);
-    ENSURE(c, c->m_doff, sizeof(int64_t) * np);
-    ENSURE(c, c->m_dis, sizeof(double) * nd);
-    HIPCHK(c, hipMemcpyAsync(c->m_x.p, xs.data(), sizeof(double) * 6 * np, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->m_pfit.p, pfit.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->m_doff.p, doff.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, grs_launch_com_dist(P<grs_dgeo>(c->d_geo), P<double>(c->m_rows), P<int64_t>(c->m_rptr),
-                                  P<double>(c->m_x), P<int32_t>(c->m_pfit), P<int64_t>(c->m_doff), np, maxn,
-                                  P<double>(c->m_dis), c->st));
-    dis.resize((size_t)nd);
-    HIPCHK(c, hipMemcpyAsync(dis.data(), c->m_dis.p, sizeof(double) * nd, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    fvals.resize((size_t)np);
-    for (int64_t p = 0; p < np; p++) {
-      /* mean(Err): the sum in pair order over the count */
-      const int64_t n = row_ptr[pfit[p] + 1] - row_ptr[pfit[p]];
-      double acc = 0.0;
-      for (int64_t i = 0; i < n; i++) acc = acc + dis[(size_t)(doff[p] + i)];
-      fvals[p] = acc / (double)n;
-    }
-    for (int64_t m = 0; m < M; m++)
-      if (first[m + 1] > first[m]) nm[m].feed(&fvals[first[m]]);
-  }
+  HIPCHK(c, hipMemcpyAsync(c->m_nm.p, nm.data(), sizeof(grs_nm) * (size_t)M, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_com_nm(P<grs_dgeo>(c->d_geo), P<double>(c->m_rows), P<int64_t>(c->m_rptr), M,
+                              P<grs_nm>(c->m_nm), c->st));
+  HIPCHK(c, hipMemcpyAsync(nm.data(), c->m_nm.p, sizeof(grs_nm) * (size_t)M, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
   for (int64_t m = 0; m < M; m++) {
     nm[m].result(x_out + 6 * m, fval + m);
     n_fun_evals[m] = nm[m].nf;

@@ -212,9 +212,10 @@ hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const i
                                const double *d_skey, const int32_t *d_sidx, const int32_t *d_slen, int64_t nsp,
                                double *d_scratch, int64_t cap, int64_t waves, double *d_out, hipStream_t st);
 
-/* fit_grain_COM objective (grs_com.hip) */
-hipError_t grs_launch_com_dist(const grs_dgeo *d_g, const double *d_rows, const int64_t *d_rptr, const double *d_x,
-                               const int32_t *d_pfit, const int64_t *d_doff, int64_t npoints, int64_t maxpairs,
-                               double *d_dis, hipStream_t st);
+/* fit_grain_COM: the M started Nelder-Mead fits run to completion on the
+ * device, one wave each (grs_com.hip) */
+struct grs_nm;
+hipError_t grs_launch_com_nm(const grs_dgeo *d_g, const double *d_rows, const int64_t *d_rptr, int64_t M,
+                             grs_nm *d_nm, hipStream_t st);
 
 #endif

@@ -20,29 +20,38 @@
  * many fits in one device launch) and feeds the values back, so the
  * evaluations of independent fits are batched while each fit follows the
  * sequential method exactly.  sin/asin are the engine's FP64 functions.
+ * Everything is host and device code: a fit can also run the whole method
+ * inside one kernel, with the same operations and so the same results.
  */
 #ifndef GRS_NM_H
 #define GRS_NM_H
 
+#include <hip/hip_runtime.h>
+
 #include <cmath>
-#include <cstring>
 
 #include "grs_math.h"
 
 #define GRS_NM_NMAX 8
+#define GRS_NM_FN __host__ __device__
 
-static inline double grs_nm_eps(double x) {
+/* eps(x) of MATLAB: the spacing of doubles at |x| */
+GRS_NM_FN static inline double grs_nm_eps(double x) {
   if (x != x) return x;
-  x = std::fabs(x);
-  if (std::isinf(x)) return std::nan("");
+  x = fabs(x);
+  if (__builtin_isinf(x)) return __builtin_nan("");
   if (x < 2.2250738585072014e-308) return 4.9406564584124654e-324;
   int e;
-  std::frexp(x, &e);
-  return std::ldexp(1.0, e - 53);
+  frexp(x, &e);
+  return ldexp(1.0, e - 53);
+}
+
+GRS_NM_FN static inline void grs_nm_copy(double *d, const double *s, int n) {
+  for (int k = 0; k < n; k++) d[k] = s[k];
 }
 
 /* MATLAB max of two values: NaN ignored unless both are NaN */
-static inline double grs_nm_max(double a, double b) {
+GRS_NM_FN static inline double grs_nm_max(double a, double b) {
   if (a != a) return b;
   if (b != b) return a;
   return a > b ? a : b;
@@ -62,20 +71,20 @@ struct grs_nm {
   double pend[GRS_NM_NMAX + 1][GRS_NM_NMAX]; /* requested points, transformed (u) space */
 
   /* u -> x (fminsearchbnd.m:263-268) */
-  void xform(const double *u, double *x) const {
+  GRS_NM_FN void xform(const double *u, double *x) const {
     GRS_NO_CONTRACT
     for (int k = 0; k < n; k++) {
       double t = (grs_sin(u[k]) + 1.0) / 2.0;
       t = t * (ub[k] - lb[k]) + lb[k];
-      x[k] = std::fmax(lb[k], std::fmin(ub[k], t));
+      x[k] = fmax(lb[k], fmin(ub[k], t));
     }
   }
 
-  void start(int dims, const double *x0, const double *lo, const double *hi) {
+  GRS_NM_FN void start(int dims, const double *x0, const double *lo, const double *hi) {
     GRS_NO_CONTRACT
     n = dims;
-    std::memcpy(lb, lo, sizeof(double) * n);
-    std::memcpy(ub, hi, sizeof(double) * n);
+    grs_nm_copy(lb, lo, n);
+    grs_nm_copy(ub, hi, n);
     double u0[GRS_NM_NMAX];
     for (int k = 0; k < n; k++) {
       if (x0[k] <= lb[k]) {
@@ -88,44 +97,44 @@ struct grs_nm {
         u0[k] = 2.0 * GRS_PI + grs_asin(z);
       }
     }
-    std::memcpy(pend[0], u0, sizeof(double) * n);
+    grs_nm_copy(pend[0], u0, n);
     for (int j = 0; j < n; j++) {
-      std::memcpy(pend[j + 1], u0, sizeof(double) * n);
+      grs_nm_copy(pend[j + 1], u0, n);
       pend[j + 1][j] = u0[j] != 0.0 ? (1.0 + 0.05) * u0[j] : 0.00025;
     }
     npend = n + 1;
     st = INIT;
   }
 
-  bool done() const { return st == DONE; }
+  GRS_NM_FN bool done() const { return st == DONE; }
 
-  void sort_simplex() {
+  GRS_NM_FN void sort_simplex() {
     /* stable insertion sort, NaN last */
     for (int i = 1; i <= n; i++) {
       double f = fv[i], row[GRS_NM_NMAX];
-      std::memcpy(row, v[i], sizeof(double) * n);
+      grs_nm_copy(row, v[i], n);
       int j = i - 1;
       while (j >= 0 && (f == f) && (fv[j] != fv[j] || fv[j] > f)) {
         fv[j + 1] = fv[j];
-        std::memcpy(v[j + 1], v[j], sizeof(double) * n);
+        grs_nm_copy(v[j + 1], v[j], n);
         j--;
       }
       fv[j + 1] = f;
-      std::memcpy(v[j + 1], row, sizeof(double) * n);
+      grs_nm_copy(v[j + 1], row, n);
     }
   }
 
-  void begin_iteration() {
+  GRS_NM_FN void begin_iteration() {
     GRS_NO_CONTRACT
     if (!(nf < max_fe && it < max_it)) {
       st = DONE;
       npend = 0;
       return;
     }
-    double dfv = std::nan(""), dx = std::nan(""), vmax = std::nan("");
-    for (int j = 1; j <= n; j++) dfv = grs_nm_max(dfv, std::fabs(fv[0] - fv[j]));
+    double dfv = __builtin_nan(""), dx = __builtin_nan(""), vmax = __builtin_nan("");
+    for (int j = 1; j <= n; j++) dfv = grs_nm_max(dfv, fabs(fv[0] - fv[j]));
     for (int j = 1; j <= n; j++)
-      for (int k = 0; k < n; k++) dx = grs_nm_max(dx, std::fabs(v[j][k] - v[0][k]));
+      for (int k = 0; k < n; k++) dx = grs_nm_max(dx, fabs(v[j][k] - v[0][k]));
     for (int k = 0; k < n; k++) vmax = grs_nm_max(vmax, v[0][k]);
     if (dfv <= grs_nm_max(tol_f, 10.0 * grs_nm_eps(fv[0])) && dx <= grs_nm_max(tol_x, 10.0 * grs_nm_eps(vmax))) {
       st = DONE;
@@ -138,34 +147,34 @@ struct grs_nm {
       xbar[k] = acc / (double)n;
     }
     for (int k = 0; k < n; k++) xr[k] = (1.0 + 1.0) * xbar[k] - 1.0 * v[n][k];
-    std::memcpy(pend[0], xr, sizeof(double) * n);
+    grs_nm_copy(pend[0], xr, n);
     npend = 1;
     st = REFLECT;
   }
 
-  void end_iteration() {
+  GRS_NM_FN void end_iteration() {
     sort_simplex();
     it += 1;
     begin_iteration();
   }
 
-  void request_shrink() {
+  GRS_NM_FN void request_shrink() {
     GRS_NO_CONTRACT
     for (int j = 1; j <= n; j++) {
       for (int k = 0; k < n; k++) v[j][k] = v[0][k] + 0.5 * (v[j][k] - v[0][k]);
-      std::memcpy(pend[j - 1], v[j], sizeof(double) * n);
+      grs_nm_copy(pend[j - 1], v[j], n);
     }
     npend = n;
     st = SHRINK;
   }
 
   /* values of the npend requested points, in request order */
-  void feed(const double *f) {
+  GRS_NM_FN void feed(const double *f) {
     GRS_NO_CONTRACT
     switch (st) {
       case INIT:
         for (int j = 0; j <= n; j++) {
-          std::memcpy(v[j], pend[j], sizeof(double) * n);
+          grs_nm_copy(v[j], pend[j], n);
           fv[j] = f[j];
         }
         nf = n + 1;
@@ -183,7 +192,7 @@ struct grs_nm {
           return;
         }
         if (fxr < fv[n - 1]) {
-          std::memcpy(v[n], xr, sizeof(double) * n);
+          grs_nm_copy(v[n], xr, n);
           fv[n] = fxr;
           end_iteration();
           return;
@@ -201,10 +210,10 @@ struct grs_nm {
       case EXPAND:
         nf += 1;
         if (f[0] < fxr) {
-          std::memcpy(v[n], pend[0], sizeof(double) * n);
+          grs_nm_copy(v[n], pend[0], n);
           fv[n] = f[0];
         } else {
-          std::memcpy(v[n], xr, sizeof(double) * n);
+          grs_nm_copy(v[n], xr, n);
           fv[n] = fxr;
         }
         end_iteration();
@@ -212,7 +221,7 @@ struct grs_nm {
       case CONTRACT_OUT:
         nf += 1;
         if (f[0] <= fxr) {
-          std::memcpy(v[n], pend[0], sizeof(double) * n);
+          grs_nm_copy(v[n], pend[0], n);
           fv[n] = f[0];
           end_iteration();
         } else {
@@ -222,7 +231,7 @@ struct grs_nm {
       case CONTRACT_IN:
         nf += 1;
         if (f[0] < fv[n]) {
-          std::memcpy(v[n], pend[0], sizeof(double) * n);
+          grs_nm_copy(v[n], pend[0], n);
           fv[n] = f[0];
           end_iteration();
         } else {
@@ -240,7 +249,7 @@ struct grs_nm {
   }
 
   /* result in the original variables */
-  void result(double *x, double *fval) const {
+  GRS_NM_FN void result(double *x, double *fval) const {
     xform(v[0], x);
     *fval = fv[0];
   }

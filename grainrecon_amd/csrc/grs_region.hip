@@ -93,12 +93,13 @@ hipError_t grs_launch_region_accept(const double *d_rows, int64_t M, const int32
 
 /* out[i] = v[lin[i]] (lin checked by the caller) */
 __global__ void k_gather_f64(const double *v, const int64_t *lin, int64_t n, double *out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = v[lin[i]];
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) out[i] = v[lin[i]];
 }
 
 hipError_t grs_launch_gather_f64(const double *d_v, const int64_t *d_lin, int64_t n, double *d_out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_v, d_lin, n, d_out);
+  const int64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+  hipLaunchKernelGGL(k_gather_f64, dim3((unsigned)blocks), dim3(256), 0, st, d_v, d_lin, n, d_out);
   return hipGetLastError();
 }

@@ -333,7 +333,7 @@ class Engine:
         nx, ny, nz = self._vshape
         flat = np.zeros(nx * ny * nz)
         self._chk(self._lib.grs_get_volume(self._h, self.VOLUMES[name], _ptr(flat, _dp), flat.size))
-        return flat.reshape(nz, ny, nx).transpose(2, 1, 0).copy()
+        return flat.reshape(nz, ny, nx).transpose(2, 1, 0)  # column-major view of the fresh buffer
 
     def get_volume_at(self, name: str, vox) -> np.ndarray:
         """V(x, y, z) at the (n, 3) 1-based [x y z] rows of vox, gathered on

@@ -30,20 +30,22 @@ from .indexing import index_seed_fit_all, pos2ind
 class MappingDS:
     """DS of GrainMapping_engine_gpu_comp.m:24-34 (volumes indexed
     [x-1, y-1, z-1]); Completeness, Mask, Dismedian and Ninter are mirrored on
-    the engine for the growth."""
+    the engine for the growth.  The arrays are column-major like MATLAB's,
+    which is also the engine's layout (x fastest), so a volume goes to the
+    device without a transposing copy."""
 
     def __init__(self, mask):
         mask = np.asarray(mask, dtype=float)
         nx, ny, nz = mask.shape
-        self.Mask = mask.copy()
-        self.Completeness = np.zeros((nx, ny, nz))
-        self.GrainId = np.zeros((nx, ny, nz))
-        self.Rodrigues = np.zeros((3, nx, ny, nz))
-        self.IPF001 = np.zeros((3, nx, ny, nz))
-        self.EulerZXZ = np.zeros((3, nx, ny, nz))
-        self.Dismedian = np.zeros((nx, ny, nz))
-        self.Ninter = np.zeros((nx, ny, nz))
-        self.VisitFlag = np.zeros((nx, ny, nz))
+        self.Mask = np.asfortranarray(mask).copy(order="F")
+        self.Completeness = np.zeros((nx, ny, nz), order="F")
+        self.GrainId = np.zeros((nx, ny, nz), order="F")
+        self.Rodrigues = np.zeros((3, nx, ny, nz), order="F")
+        self.IPF001 = np.zeros((3, nx, ny, nz), order="F")
+        self.EulerZXZ = np.zeros((3, nx, ny, nz), order="F")
+        self.Dismedian = np.zeros((nx, ny, nz), order="F")
+        self.Ninter = np.zeros((nx, ny, nz), order="F")
+        self.VisitFlag = np.zeros((nx, ny, nz), order="F")
 
 
 _NEIGHBOURS = ((1, 1, 0), (1, -1, 0), (1, 0, 1), (1, 0, -1), (1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 1, 1), (0, 0, 1),

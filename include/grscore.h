@@ -384,9 +384,10 @@ grs_status grs_grow_region(grs_ctx *ctx, const int32_t *seed, double indexed_com
  * MaxIter, TolFun, TolX; the reference passes 2000, 500, 1e-10 and the
  * default 1e-4) minimises fit_grain_COM_fun: the mean over the pairs of the
  * distance between the rounded simulated pixel of the pair's (hkl, omega)
- * and its measured centroid.  The fits run together, each step's points
- * scored in one launch.  x_out (M x 6) = FitOutput, fval, the evaluation and
- * iteration counts.  At most 7281 grains per call. */
+ * and its measured centroid.  Each fit runs its whole Nelder-Mead on the
+ * device, one wave per grain, all grains in one launch.  x_out (M x 6) =
+ * FitOutput, fval, the evaluation and iteration counts.  max_iter is at most
+ * 100000 (it bounds a wave's run time). */
 grs_status grs_fit_grain_com(grs_ctx *ctx, const double *rows, const int64_t *row_ptr, int64_t M,
                              int32_t max_fun_evals, int32_t max_iter, double tol_fun, double tol_x, double *x_out,
                              double *fval, int32_t *n_fun_evals, int32_t *n_iter);

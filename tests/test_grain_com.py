@@ -63,10 +63,23 @@ def test_fit_grain_com_equals_oracle(engine, scene):
         rows = _rows(s, g, _frames_at_pixels(s, g), shift)
         if rows.shape[0]:
             sets.append(rows)
-    print(f"fit_grain_com: {len(sets)} grains, {[r.shape[0] for r in sets]} pairs", flush=True)
-    x, fv, nf, it = engine.fit_grain_com(sets)
-    for m, rows in enumerate(sets):
-        rx, rf, rnf, rit = GC.fit_grain_com(s.geo, rows)
-        np.testing.assert_array_equal(x[m], rx)
-        assert fv[m] == rf and nf[m] == rnf and it[m] == rit, m
     assert len(sets) >= 2
+    # a grain with more pairs than a wave has lanes (its rows three times, so
+    # the in-order sum runs over several 64-pair chunks)
+    sets.append(np.concatenate([sets[0]] * max(3, 130 // sets[0].shape[0] + 1)))
+    assert sets[-1].shape[0] > 128
+    print(f"fit_grain_com: {len(sets)} grains, {[r.shape[0] for r in sets]} pairs", flush=True)
+    for lim in ((2000, 500), (40, 500), (2000, 7)):  # defaults, the evaluation limit, the iteration limit
+        x, fv, nf, it = engine.fit_grain_com(sets, max_fun_evals=lim[0], max_iter=lim[1])
+        for m, rows in enumerate(sets):
+            rx, rf, rnf, rit = GC.fit_grain_com(s.geo, rows, max_fun_evals=lim[0], max_iter=lim[1])
+            np.testing.assert_array_equal(x[m], rx)
+            assert fv[m] == rf and nf[m] == rnf and it[m] == rit, (lim, m)
+        if lim[0] == 40:
+            assert np.all(nf >= 40) and np.all(nf <= 40 + 6)
+        if lim[1] == 7:
+            assert np.all(it == 7)
+    from grainrecon_amd._lib import GrsError
+
+    with pytest.raises(GrsError):
+        engine.fit_grain_com(sets, max_iter=100001)
