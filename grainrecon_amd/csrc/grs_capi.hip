@@ -93,6 +93,9 @@ struct grs_ctx {
   DevBuf v_lin, v_out; /* grs_get_volume_at */
   /* grain centre-of-mass fit buffers */
   DevBuf m_rows, m_rptr, m_nm;
+  DevBuf f_in, f_cnt; /* fit_batch: packed per-step inputs and counts */
+  void *h_pin = nullptr; /* pinned staging of fit_batch (hipHostMalloc) */
+  size_t h_pin_cap = 0;
   bool v_set[4] = {false, false, false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
@@ -225,11 +228,12 @@ void grs_destroy(grs_ctx *c) {
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
                     &c->m_nm, &c->t_skey, &c->t_sidx, &c->t_slen, &c->v_lin,
-                    &c->v_out};
+                    &c->v_out, &c->f_in, &c->f_cnt};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (DevBuf *b : bufs) free_buf(*b);
+  if (c->h_pin) hipHostFree(c->h_pin);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->st) hipStreamDestroy(c->st);
@@ -474,7 +478,7 @@ static bool tune_pending(const grs_ctx *c, int kind, int64_t E, int64_t waves) {
 
 static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, const int32_t *d_evox,
                            const int32_t *d_eu, int32_t method, int32_t *d_ns, int32_t *d_ni, double *d_C,
-                           float *d_med) {
+                           float *d_med, bool bounds_ready = false) {
   if (n_eval <= 0) return GRS_OK;
   const int64_t stride = (int64_t)c->nrot * c->H;
   grs_eval_args a;
@@ -497,9 +501,11 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
   a.out_med = d_med;
   if (method == GRS_METHOD_FAST) {
     if (c->H > 240) return fail(c, GRS_ERR_ARG, "the fast method supports at most 240 reflections (got %d)", c->H);
-    ENSURE(c, c->d_vb, sizeof(double) * GRS_VB * (size_t)V);
-    HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
-                                      P<double>(c->d_vb), c->st));
+    if (!bounds_ready) { /* else d_vb holds the bounds of these V positions */
+      ENSURE(c, c->d_vb, sizeof(double) * GRS_VB * (size_t)V);
+      HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), V, P<double>(c->d_cs), c->nrot,
+                                        P<double>(c->d_vb), c->st));
+    }
     int64_t waves = grs_fast_wave_slots(c->device);
     if (waves > n_eval) waves = ((n_eval + 3) / 4) * 4;
     ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)waves * stride);
@@ -1286,57 +1292,75 @@ static grs_status fit_batch(grs_ctx *c, const double *pos, const double *x0, int
   }
   ENSURE(c, c->d_pos, sizeof(double) * 3 * M);
   HIPCHK(c, hipMemcpyAsync(c->d_pos.p, pos, sizeof(double) * 3 * M, hipMemcpyHostToDevice, c->st));
-  std::vector<double> q;
-  std::vector<int32_t> evox, eu, hns, hni;
+  /* the positions stay fixed during the fits: their omega-solve bounds once */
+  ENSURE(c, c->d_vb, sizeof(double) * GRS_VB * (size_t)M);
+  HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), M, P<double>(c->d_cs), c->nrot,
+                                    P<double>(c->d_vb), c->st));
+  /* each step: [q (4E doubles) | eval voxel (E) | eval orientation (E)]
+   * uploaded in one copy from pinned memory, [ns (E) | ni (E)] back in one */
   std::vector<int64_t> first;
   std::vector<double> fvals;
   for (;;) {
-    q.clear();
-    evox.clear();
     first.assign((size_t)M + 1, 0);
+    int64_t E = 0;
     for (int64_t m = 0; m < M; m++) {
-      first[m] = (int64_t)evox.size();
+      first[m] = E;
+      if (!nm[m].done()) E += nm[m].npend;
+    }
+    first[M] = E;
+    if (E == 0) break;
+    const size_t in_bytes = (sizeof(double) * 4 + sizeof(int32_t) * 2) * (size_t)E;
+    const size_t cnt_bytes = sizeof(int32_t) * 2 * (size_t)E;
+    if (c->h_pin_cap < in_bytes + cnt_bytes) {
+      if (c->h_pin) HIPCHK(c, hipHostFree(c->h_pin));
+      c->h_pin = nullptr;
+      c->h_pin_cap = 0;
+      const size_t want = 2 * (in_bytes + cnt_bytes);
+      const hipError_t e = hipHostMalloc(&c->h_pin, want, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        c->h_pin = nullptr;
+        (void)hipGetLastError();
+        return fail(c, GRS_ERR_NOMEM, "hipHostMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+      }
+      c->h_pin_cap = want;
+    }
+    double *hq = static_cast<double *>(c->h_pin);
+    int32_t *hvox = reinterpret_cast<int32_t *>(hq + 4 * E), *heu = hvox + E;
+    int32_t *hcnt = reinterpret_cast<int32_t *>(static_cast<char *>(c->h_pin) + in_bytes);
+    for (int64_t m = 0; m < M; m++) {
       if (nm[m].done()) continue;
       for (int p = 0; p < nm[m].npend; p++) {
+        const int64_t e = first[m] + p;
         double x[3];
         nm[m].xform(nm[m].pend[p], x);
-        q.push_back(0.0); /* q0 is recomputed from the vector part (fit_OR_fun_local.m:9) */
-        q.push_back(x[0]);
-        q.push_back(x[1]);
-        q.push_back(x[2]);
-        evox.push_back((int32_t)m);
+        hq[4 * e] = 0.0; /* q0 is recomputed from the vector part (fit_OR_fun_local.m:9) */
+        hq[4 * e + 1] = x[0];
+        hq[4 * e + 2] = x[1];
+        hq[4 * e + 3] = x[2];
+        hvox[e] = (int32_t)m;
+        heu[e] = (int32_t)e;
       }
     }
-    first[M] = (int64_t)evox.size();
-    const int64_t E = (int64_t)evox.size();
-    if (E == 0) break;
-    eu.resize((size_t)E);
-    for (int64_t e = 0; e < E; e++) eu[e] = (int32_t)e;
-    ENSURE(c, c->d_q, sizeof(double) * 4 * E);
+    ENSURE(c, c->f_in, in_bytes);
+    ENSURE(c, c->f_cnt, cnt_bytes);
     ENSURE(c, c->d_sub, sizeof(double) * 9 * E);
     ENSURE(c, c->d_valid, sizeof(int32_t) * E);
-    ENSURE(c, c->d_evox, sizeof(int32_t) * E);
-    ENSURE(c, c->d_eu_idx, sizeof(int32_t) * E);
-    ENSURE(c, c->d_ns, sizeof(int32_t) * E);
-    ENSURE(c, c->d_ni, sizeof(int32_t) * E);
     ENSURE(c, c->d_C, sizeof(double) * E);
     ENSURE(c, c->d_med, sizeof(float) * E);
-    HIPCHK(c, hipMemcpyAsync(c->d_q.p, q.data(), sizeof(double) * 4 * E, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->d_evox.p, evox.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->d_eu_idx.p, eu.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), P<double>(c->d_q), E, P<double>(c->d_sub),
-                                    P<int32_t>(c->d_valid), c->st));
-    s = run_eval(c, GRS_MAP_PAIRS, E, M, P<int32_t>(c->d_evox), P<int32_t>(c->d_eu_idx), GRS_METHOD_FAST,
-                 P<int32_t>(c->d_ns), P<int32_t>(c->d_ni), P<double>(c->d_C), P<float>(c->d_med));
+    HIPCHK(c, hipMemcpyAsync(c->f_in.p, hq, in_bytes, hipMemcpyHostToDevice, c->st));
+    double *dq = P<double>(c->f_in);
+    int32_t *dvox = reinterpret_cast<int32_t *>(dq + 4 * E), *deu = dvox + E;
+    int32_t *dns = P<int32_t>(c->f_cnt), *dni = dns + E;
+    HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), dq, E, P<double>(c->d_sub), P<int32_t>(c->d_valid),
+                                    c->st));
+    s = run_eval(c, GRS_MAP_PAIRS, E, M, dvox, deu, GRS_METHOD_FAST, dns, dni, P<double>(c->d_C),
+                 P<float>(c->d_med), true);
     if (s != GRS_OK) return s;
-    hns.resize((size_t)E);
-    hni.resize((size_t)E);
-    HIPCHK(c, hipMemcpyAsync(hns.data(), c->d_ns.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(hni.data(), c->d_ni.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hcnt, dns, cnt_bytes, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     fvals.resize((size_t)E);
     for (int64_t e = 0; e < E; e++) /* fit_OR_fun_local.m:98; 0/0 (no event, or a skipped row) is NaN */
-      fvals[e] = 1.0 - (double)hni[e] / (double)hns[e];
+      fvals[e] = 1.0 - (double)hcnt[E + e] / (double)hcnt[e];
     for (int64_t m = 0; m < M; m++)
       if (first[m + 1] > first[m]) nm[m].feed(&fvals[first[m]]);
   }
