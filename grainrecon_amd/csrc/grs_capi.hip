@@ -92,7 +92,7 @@ struct grs_ctx {
   DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_sidx, t_slen;
   DevBuf v_lin, v_out; /* grs_get_volume_at */
   /* grain centre-of-mass fit buffers */
-  DevBuf m_rows, m_rptr, m_nm;
+  DevBuf m_rows, m_rptr, m_nm, m_cs, m_dis;
   DevBuf f_in, f_cnt; /* fit_batch: packed per-step inputs and counts */
   void *h_pin = nullptr; /* pinned staging of fit_batch (hipHostMalloc) */
   size_t h_pin_cap = 0;
@@ -228,7 +228,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
                     &c->m_nm, &c->t_skey, &c->t_sidx, &c->t_slen, &c->v_lin,
-                    &c->v_out, &c->f_in, &c->f_cnt};
+                    &c->v_out, &c->f_in, &c->f_cnt, &c->m_cs, &c->m_dis};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -2026,8 +2026,12 @@ grs_status grs_fit_grain_com(grs_ctx *c, const double *rows, const int64_t *row_
     return fail(c, GRS_ERR_ARG, "invalid argument");
   if (M == 0) return GRS_OK;
   if (row_ptr[0] != 0) return fail(c, GRS_ERR_ARG, "row_ptr must start at 0");
-  for (int64_t m = 0; m < M; m++)
-    if (row_ptr[m + 1] - row_ptr[m] < 1) return fail(c, GRS_ERR_ARG, "fit %lld has no pair rows", (long long)m);
+  for (int64_t m = 0; m < M; m++) {
+    const int64_t n = row_ptr[m + 1] - row_ptr[m];
+    if (n < 1) return fail(c, GRS_ERR_ARG, "fit %lld has no pair rows", (long long)m);
+    /* (point, pair) work items of one step are counted in 32 bits */
+    if (n > 0x7fffffff / (GRS_NM_NMAX + 1)) return fail(c, GRS_ERR_ARG, "fit %lld has too many pair rows", (long long)m);
+  }
   if (M > 0x7fffffff) return fail(c, GRS_ERR_ARG, "too many fits in one call");
   /* each iteration feeds at most three requests: this bounds every wave of k_com_nm */
   if (max_iter > 100000) return fail(c, GRS_ERR_ARG, "max_iter above 100000");
@@ -2063,11 +2067,13 @@ grs_status grs_fit_grain_com(grs_ctx *c, const double *rows, const int64_t *row_
   ENSURE(c, c->m_rows, sizeof(double) * 23 * (size_t)total);
   ENSURE(c, c->m_rptr, sizeof(int64_t) * (size_t)(M + 1));
   ENSURE(c, c->m_nm, sizeof(grs_nm) * (size_t)M);
+  ENSURE(c, c->m_cs, sizeof(double) * 2 * (size_t)total);
+  ENSURE(c, c->m_dis, sizeof(double) * (GRS_NM_NMAX + 1) * (size_t)total);
   HIPCHK(c, hipMemcpyAsync(c->m_rows.p, rows, sizeof(double) * 23 * (size_t)total, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->m_rptr.p, row_ptr, sizeof(int64_t) * (size_t)(M + 1), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->m_nm.p, nm.data(), sizeof(grs_nm) * (size_t)M, hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, grs_launch_com_nm(P<grs_dgeo>(c->d_geo), P<double>(c->m_rows), P<int64_t>(c->m_rptr), M,
-                              P<grs_nm>(c->m_nm), c->st));
+  HIPCHK(c, grs_launch_com_nm(P<grs_dgeo>(c->d_geo), P<double>(c->m_rows), P<int64_t>(c->m_rptr), M, total,
+                              P<double>(c->m_cs), P<double>(c->m_dis), P<grs_nm>(c->m_nm), c->st));
   HIPCHK(c, hipMemcpyAsync(nm.data(), c->m_nm.p, sizeof(grs_nm) * (size_t)M, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   for (int64_t m = 0; m < M; m++) {

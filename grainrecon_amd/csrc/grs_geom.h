@@ -94,15 +94,9 @@ GRS_HD int grs_U_from_q(double q1, double q2, double q3, double *U) {
   return 1;
 }
 
-/* euler2u.m:5-16 with the angles given in degrees and converted as
- * (deg*pi)/180 (grow_unindexed_compete_comp_cuda.m:84). */
-GRS_HD void grs_U_from_euler_deg(double p1d, double Pd, double p2d, double *U) {
+/* euler2u.m:5-16 from the cosines and sines of phi1, PHI and phi2 */
+GRS_HD void grs_U_from_trig(double c1, double s1, double cP, double sP, double c2, double s2, double *U) {
   GRS_NO_CONTRACT
-  const double pi = GRS_PI;
-  double phi1 = (p1d * pi) / 180.0, PHI = (Pd * pi) / 180.0, phi2 = (p2d * pi) / 180.0;
-  double c1 = grs_cos(phi1), s1 = grs_sin(phi1);
-  double c2 = grs_cos(phi2), s2 = grs_sin(phi2);
-  double cP = grs_cos(PHI), sP = grs_sin(PHI);
   U[0] = c1 * c2 - (s1 * s2) * cP;
   U[3] = s1 * c2 + (c1 * s2) * cP;
   U[6] = s2 * sP;
@@ -112,6 +106,18 @@ GRS_HD void grs_U_from_euler_deg(double p1d, double Pd, double p2d, double *U) {
   U[2] = s1 * sP;
   U[5] = -c1 * sP;
   U[8] = cP;
+}
+
+/* euler2u.m:5-16 with the angles given in degrees and converted as
+ * (deg*pi)/180 (grow_unindexed_compete_comp_cuda.m:84) */
+GRS_HD double grs_deg2rad(double d) {
+  GRS_NO_CONTRACT
+  return (d * GRS_PI) / 180.0;
+}
+
+GRS_HD void grs_U_from_euler_deg(double p1d, double Pd, double p2d, double *U) {
+  const double phi1 = grs_deg2rad(p1d), PHI = grs_deg2rad(Pd), phi2 = grs_deg2rad(p2d);
+  grs_U_from_trig(grs_cos(phi1), grs_sin(phi1), grs_cos(PHI), grs_sin(PHI), grs_cos(phi2), grs_sin(phi2), U);
 }
 
 /* SUB = (S*U)*B; Gw(:,h) = SUB*hkl(:,h). */

@@ -213,9 +213,10 @@ hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const i
                                double *d_scratch, int64_t cap, int64_t waves, double *d_out, hipStream_t st);
 
 /* fit_grain_COM: the M started Nelder-Mead fits run to completion on the
- * device, one wave each (grs_com.hip) */
+ * device, one block each (grs_com.hip); d_cs holds 2 doubles and d_dis
+ * GRS_NM_NMAX + 1 doubles per pair row */
 struct grs_nm;
 hipError_t grs_launch_com_nm(const grs_dgeo *d_g, const double *d_rows, const int64_t *d_rptr, int64_t M,
-                             grs_nm *d_nm, hipStream_t st);
+                             int64_t total, double *d_cs, double *d_dis, grs_nm *d_nm, hipStream_t st);
 
 #endif

@@ -70,14 +70,16 @@ struct grs_nm {
   int npend = 0;
   double pend[GRS_NM_NMAX + 1][GRS_NM_NMAX]; /* requested points, transformed (u) space */
 
-  /* u -> x (fminsearchbnd.m:263-268) */
-  GRS_NM_FN void xform(const double *u, double *x) const {
+  /* coordinate k of u -> x (fminsearchbnd.m:263-268) */
+  GRS_NM_FN void xform1(const double *u, int k, double *xk) const {
     GRS_NO_CONTRACT
-    for (int k = 0; k < n; k++) {
-      double t = (grs_sin(u[k]) + 1.0) / 2.0;
-      t = t * (ub[k] - lb[k]) + lb[k];
-      x[k] = fmax(lb[k], fmin(ub[k], t));
-    }
+    double t = (grs_sin(u[k]) + 1.0) / 2.0;
+    t = t * (ub[k] - lb[k]) + lb[k];
+    *xk = fmax(lb[k], fmin(ub[k], t));
+  }
+
+  GRS_NM_FN void xform(const double *u, double *x) const {
+    for (int k = 0; k < n; k++) xform1(u, k, &x[k]);
   }
 
   GRS_NM_FN void start(int dims, const double *x0, const double *lo, const double *hi) {
