@@ -179,14 +179,15 @@ __device__ __forceinline__ double wave_min_d(double v) {
   return v;
 }
 
-/* One wave per voxel (4 voxels per 256-thread block): lanes take the
- * rotation angles lane, lane + 64, ... and the extremes are reduced across
- * the wave. */
+/* One 256-thread block per voxel: thread t takes the rotation angles t,
+ * t + 256, ... (one each for up to 256 angles, which keeps the latency of a
+ * one-voxel call short), and the extremes are reduced across each wave and
+ * then across the four waves; maxima, minima and flags are exact, so the
+ * result does not depend on the partition. */
 __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const double *pos, int64_t V,
                                                       const double *cs, int32_t nrot, double *vb) {
-  const int lane = threadIdx.x & 63;
-  const int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (v >= V) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t v = blockIdx.x; /* one block per voxel, one omega per thread */
   double P[3];
   voxel_P(g, pos + 3 * v, P);
   const double al = P[0] / g->Lss, be = P[1] / g->Lss, ga = (g->RAO - g->P0y) / g->Lss;
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
   const double on = __builtin_sqrt(oy * oy + oz * oz);
   double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0, ayz = 0.0, zt = -1e300, zb = 1e300;
   bool front = true, rows = true;
-  for (int32_t om = lane; om < nrot; om += 64) {
+  for (int32_t om = tid; om < nrot; om += 256) {
     double sw[3], u[3];
     const double c = cs[2 * om], s = cs[2 * om + 1];
     beam_terms(g, P, c, s, sw, u);
@@ -250,11 +251,35 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
   ayz = wave_max_d(ayz);
   zt = wave_max_d(zt);
   zb = wave_min_d(zb);
-  if (lane != 0) return;
+  /* the four waves' maxima, minima and flags (exact, so order-free) */
+  __shared__ double red[4][12];
+  if (lane == 0) {
+    double *r = red[wv];
+    r[0] = d2max, r[1] = kmax, r[2] = e0, r[3] = e1, r[4] = u2lo, r[5] = u2hi, r[6] = ayz, r[7] = zt, r[8] = zb;
+    r[9] = front ? 1.0 : 0.0, r[10] = rows ? 1.0 : 0.0, r[11] = nan_seen ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  bool nan_any = false;
+  for (int k = 0; k < 4; k++) {
+    const double *r = red[k];
+    d2max = r[0] > d2max ? r[0] : d2max;
+    kmax = r[1] > kmax ? r[1] : kmax;
+    e0 = r[2] > e0 ? r[2] : e0;
+    e1 = r[3] > e1 ? r[3] : e1;
+    u2lo = r[4] < u2lo ? r[4] : u2lo;
+    u2hi = r[5] > u2hi ? r[5] : u2hi;
+    ayz = r[6] > ayz ? r[6] : ayz;
+    zt = r[7] > zt ? r[7] : zt;
+    zb = r[8] < zb ? r[8] : zb;
+    front = front && r[9] != 0.0;
+    rows = rows && r[10] != 0.0;
+    nan_any = nan_any || r[11] != 0.0;
+  }
   const double delta = __builtin_sqrt(d2max) * (1.0 + 1e-9) + 1e-12;
   const double alpha = 2.0 * grs_asin(delta * 0.5 < 1.0 ? delta * 0.5 : 1.0) + 1e-9;
   double K = 1.0;
-  const bool ok = front && !nan_seen && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
+  const bool ok = front && !nan_any && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
   if (ok) {
     K = kmax * (1.0 + 1e-6) + 1e-9;
     K = K < 1.0 ? K : 1.0;
@@ -1403,7 +1428,8 @@ hipError_t grs_launch_voxel_pos(const double *d_rows, int64_t M, int32_t stride,
 hipError_t grs_launch_voxel_bounds(const grs_dgeo *d_g, const double *d_pos, int64_t V, const double *d_cs,
                                    int32_t nrot, double *d_vb, hipStream_t st) {
   if (V <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_voxel_bounds, dim3(blocks_for(V, 4)), dim3(256), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
+  if (V > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_voxel_bounds, dim3((unsigned)V), dim3(256), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
   return hipGetLastError();
 }
 
