@@ -89,7 +89,7 @@ struct grs_ctx {
   DevBuf v_vol[4], r_rows, r_counts, r_acc_lin, r_acc_C;
   int64_t v_dim[3] = {0, 0, 0};
   /* G-vector matching score buffers */
-  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_sidx, t_slen;
+  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_su, t_bst, t_slen;
   DevBuf v_lin, v_out; /* grs_get_volume_at */
   /* grain centre-of-mass fit buffers */
   DevBuf m_rows, m_rptr, m_nm, m_cs, m_dis;
@@ -227,7 +227,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
-                    &c->m_nm, &c->t_skey, &c->t_sidx, &c->t_slen, &c->v_lin,
+                    &c->m_nm, &c->t_skey, &c->t_su, &c->t_bst, &c->t_slen, &c->v_lin,
                     &c->v_out, &c->f_in, &c->f_cnt, &c->m_cs, &c->m_dis};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
@@ -2179,18 +2179,21 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
   const bool sorted = nsp > 0 && c->p_maxframe <= grs_gt_sort_max();
   if (sorted) {
     ENSURE(c, c->t_skey, sizeof(double) * 4 * (size_t)nsp);
-    ENSURE(c, c->t_sidx, sizeof(int32_t) * 4 * (size_t)nsp);
+    ENSURE(c, c->t_su, sizeof(double) * 12 * (size_t)nsp);
+    ENSURE(c, c->t_bst, sizeof(int32_t) * 4 * (GRS_GT_NB + 1) * (size_t)c->nrot);
     ENSURE(c, c->t_slen, sizeof(int32_t) * 4 * (size_t)c->nrot);
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
   if (sorted)
     HIPCHK(c, grs_launch_gt_sort(P<int64_t>(c->p_fptr), c->nrot, P<double>(c->t_units), nsp, c->p_maxframe,
-                                 P<double>(c->t_skey), P<int32_t>(c->t_sidx), P<int32_t>(c->t_slen), c->st));
+                                 P<double>(c->t_skey), P<double>(c->t_su), P<int32_t>(c->t_bst),
+                                 P<int32_t>(c->t_slen), c->st));
   HIPCHK(c, grs_launch_gt_score(P<grs_dgeo>(c->d_geo), P<double>(c->d_hkl), P<int32_t>(c->t_fam),
                                 P<double>(c->t_glen), P<double>(c->d_sub), P<int32_t>(c->d_valid), N,
                                 P<double>(c->t_proj), P<int64_t>(c->p_fptr), P<double>(c->t_units),
-                                sorted ? P<double>(c->t_skey) : nullptr, sorted ? P<int32_t>(c->t_sidx) : nullptr,
-                                sorted ? P<int32_t>(c->t_slen) : nullptr, nsp, P<double>(c->t_scratch), cap, waves,
+                                sorted ? P<double>(c->t_skey) : nullptr, sorted ? P<double>(c->t_su) : nullptr,
+                                sorted ? P<int32_t>(c->t_bst) : nullptr, sorted ? P<int32_t>(c->t_slen) : nullptr,
+                                nsp, P<double>(c->t_scratch), cap, waves,
                                 P<double>(c->t_out), c->st));
   if (c->timing) {
     HIPCHK(c, hipEventRecord(c->ev1, c->st));

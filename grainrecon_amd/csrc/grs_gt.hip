@@ -29,9 +29,12 @@
  *             a + (b - a)/2) and mean (sequential sum) close the row
  *             [Nr_match, median, Nr_match/NrSpotExpectAll, mean].
  * k_gt_sort   per (projection, family), when every projection has at most
- *             GT_SORT_MAX spots: the spot indices sorted by the z component
- *             of their unit vectors (NaN last, not counted).  k_gt_score then
- *             visits only the spots with |z - n_z| <= GT_WIN: a spot within
+ *             GT_SORT_MAX spots: the keys (z components of the unit vectors,
+ *             NaN last, not counted) sorted, the unit vectors packed in that
+ *             order, and the start of every z bucket of width 2/GRS_GT_NB.
+ *             k_gt_score then visits only the spots with |z - n_z| <= GT_WIN,
+ *             found from the bucket below n_z - GT_WIN by a short forward
+ *             scan (the same index a binary search gives): a spot within
  *             1.01 degrees of n has |w - n| = 2 sin(angle/2) <= GT_WIN, so
  *             the window holds every spot that can give an angle below 1
  *             degree and every spot within 1e-12 of the best dot product of
@@ -146,7 +149,7 @@ __global__ void k_gt_spots(const grs_dgeo *g, const double *proj, const int64_t 
 /* block (i, f): spots of projection i sorted by units[.., f, 2]; n2 = the
  * power of two >= the largest projection (dynamic LDS n2 * 12 bytes) */
 __global__ __launch_bounds__(256) void k_gt_sort(const int64_t *fptr, const double *units, int64_t nsp, int32_t n2,
-                                                 double *skey, int32_t *sidx, int32_t *slen) {
+                                                 double *skey, double *su, int32_t *bst, int32_t *slen) {
   extern __shared__ double gsm[];
   double *key = gsm;
   int32_t *idx = (int32_t *)(gsm + n2);
@@ -185,9 +188,29 @@ __global__ __launch_bounds__(256) void k_gt_sort(const int64_t *fptr, const doub
   }
   int32_t valid = 0;
   for (int32_t t = threadIdx.x; t < m; t += blockDim.x) {
-    skey[(int64_t)f * nsp + s0 + t] = key[t];
-    sidx[(int64_t)f * nsp + s0 + t] = (int32_t)(s0 + idx[t]);
+    const int64_t o = (int64_t)f * nsp + s0 + t;
+    skey[o] = key[t];
+    const double *w = units + 12 * (s0 + idx[t]) + 3 * f;
+    su[3 * o] = w[0];
+    su[3 * o + 1] = w[1];
+    su[3 * o + 2] = w[2];
     valid += key[t] != __builtin_inf() ? 1 : 0;
+  }
+  /* bucket starts: entry b > 0 is the first key >= -1 + 2b/GRS_GT_NB (the
+   * non-finite keys sort last, so it never passes the finite ones), entry 0
+   * is 0 */
+  for (int32_t b = threadIdx.x; b <= GRS_GT_NB; b += blockDim.x) {
+    int32_t lo = 0;
+    if (b > 0) {
+      const double zb = -1.0 + (2.0 * (double)b) / (double)GRS_GT_NB;
+      int32_t hi = n2;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (key[mid] < zb) lo = mid + 1;
+        else hi = mid;
+      }
+    }
+    bst[(int64_t)(4 * i + f) * (GRS_GT_NB + 1) + b] = lo;
   }
   /* block sum of the finite keys */
   __shared__ int32_t red[256];
@@ -210,9 +233,9 @@ __device__ __forceinline__ uint64_t gt_lanes_below(uint64_t m, int lane) {
 __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const double *hkl, const int32_t *famidx,
                                                   const double *glen_poss, const double *SUB, const int32_t *valid,
                                                   int64_t N, const double *proj, const int64_t *fptr,
-                                                  const double *units, const double *skey, const int32_t *sidx,
-                                                  const int32_t *slen, int64_t nsp, double *scratch, int64_t cap,
-                                                  double *out) {
+                                                  const double *units, const double *skey, const double *su,
+                                                  const int32_t *bst, const int32_t *slen, int64_t nsp,
+                                                  double *scratch, int64_t cap, double *out) {
   GRS_NO_CONTRACT
   const int32_t H = g->H, nrot = g->nrot;
   const int lane = threadIdx.x & 63;
@@ -261,48 +284,55 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
           const int32_t f = famidx[h];
           if (f >= 0 && s1 > s0) {
             const double n0 = Gt[0] / Glen, n1 = Gt[1] / Glen, n2 = Gt[2] / Glen;
-            /* spots to visit: all of the projection, or the z window of the
-             * sorted list (entries k0 .. k1 - 1 of the family's order) */
-            int64_t k0 = s0, k1 = s1;
-            const int32_t *ord = nullptr;
-            if (skey) {
-              const double *K = skey + (int64_t)f * nsp + s0;
-              const int32_t mlen = slen[4 * i + f];
-              const double zlo = n2 - GT_WIN, zhi = n2 + GT_WIN;
-              int32_t lo = 0, hi = mlen;
-              while (lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (K[mid] < zlo) lo = mid + 1;
-                else hi = mid;
-              }
-              int32_t e = lo;
-              hi = mlen;
-              while (e < hi) {
-                const int32_t mid = (e + hi) >> 1;
-                if (K[mid] <= zhi) e = mid + 1;
-                else hi = mid;
-              }
-              ord = sidx + (int64_t)f * nsp + s0;
-              k0 = lo;
-              k1 = e;
-            }
+            /* spots to visit: the z window of the family's sorted list
+             * (packed unit vectors su, entries lo .. e - 1, found from the
+             * bucket start below zlo), or every spot of the projection */
+            const double *W;
+            int64_t k0, k1;
             double best = 0.0;
             bool any = false;
-            for (int64_t k = k0; k < k1; k++) {
-              const int64_t sp = ord ? (int64_t)ord[k] : k;
-              const double *w = units + 12 * sp + 3 * f;
-              const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
-              if (d == d && (!any || d > best)) {
-                best = d;
-                any = true;
+            if (skey) {
+              const int64_t base = (int64_t)f * nsp + s0;
+              const double *K = skey + base;
+              const int32_t mlen = slen[4 * i + f];
+              const double zlo = n2 - GT_WIN, zhi = n2 + GT_WIN;
+              int32_t b = 0;
+              if (zlo == zlo) {
+                const double q = __builtin_floor((zlo + 1.0) * (0.5 * (double)GRS_GT_NB)) - 1.0;
+                b = q < 0.0 ? 0 : (q > (double)GRS_GT_NB ? GRS_GT_NB : (int32_t)q);
+              }
+              int32_t lo = bst[(int64_t)(4 * i + f) * (GRS_GT_NB + 1) + b];
+              while (lo < mlen && K[lo] < zlo) lo++;
+              W = su + 3 * base;
+              int32_t k = lo;
+              for (; k < mlen && K[k] <= zhi; k++) {
+                const double *w = W + 3 * k;
+                const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
+                if (d == d && (!any || d > best)) {
+                  best = d;
+                  any = true;
+                }
+              }
+              k0 = lo;
+              k1 = k;
+            } else {
+              W = nullptr;
+              k0 = s0;
+              k1 = s1;
+              for (int64_t k = k0; k < k1; k++) {
+                const double *w = units + 12 * k + 3 * f;
+                const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
+                if (d == d && (!any || d > best)) {
+                  best = d;
+                  any = true;
+                }
               }
             }
             if (any) {
               amin = 1e300;
               const double lim = best - 1e-12;
               for (int64_t k = k0; k < k1; k++) {
-                const int64_t sp = ord ? (int64_t)ord[k] : k;
-                const double *w = units + 12 * sp + 3 * f;
+                const double *w = W ? W + 3 * k : units + 12 * k + 3 * f;
                 const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
                 if (d >= lim) {
                   const double a = d > 1.0 ? 0.0 : (grs_acos(d) * 180.0) / GRS_PI;
@@ -387,13 +417,14 @@ int64_t grs_gt_wave_slots(int device) {
 }
 
 hipError_t grs_launch_gt_sort(const int64_t *d_fptr, int32_t nrot, const double *d_units, int64_t nsp,
-                              int64_t max_frame, double *d_skey, int32_t *d_sidx, int32_t *d_slen, hipStream_t st) {
+                              int64_t max_frame, double *d_skey, double *d_su, int32_t *d_bst, int32_t *d_slen,
+                              hipStream_t st) {
   if (nrot <= 0) return hipSuccess;
   if (max_frame > GT_SORT_MAX) return hipErrorInvalidValue;
   int32_t n2 = 64;
   while (n2 < max_frame) n2 <<= 1;
   hipLaunchKernelGGL(k_gt_sort, dim3((unsigned)(4 * nrot)), dim3(256), (size_t)n2 * 12, st, d_fptr, d_units, nsp, n2,
-                     d_skey, d_sidx, d_slen);
+                     d_skey, d_su, d_bst, d_slen);
   return hipGetLastError();
 }
 
@@ -402,12 +433,13 @@ int64_t grs_gt_sort_max() { return GT_SORT_MAX; }
 hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const int32_t *d_famidx,
                                const double *d_glen_poss, const double *d_SUB, const int32_t *d_valid, int64_t N,
                                const double *d_proj, const int64_t *d_fptr, const double *d_units,
-                               const double *d_skey, const int32_t *d_sidx, const int32_t *d_slen, int64_t nsp,
-                               double *d_scratch, int64_t cap, int64_t waves, double *d_out, hipStream_t st) {
+                               const double *d_skey, const double *d_su, const int32_t *d_bst,
+                               const int32_t *d_slen, int64_t nsp, double *d_scratch, int64_t cap, int64_t waves,
+                               double *d_out, hipStream_t st) {
   if (N <= 0) return hipSuccess;
   int64_t w = waves < N ? waves : N;
   const int64_t blocks = (w + 3) / 4;
   hipLaunchKernelGGL(k_gt_score, dim3((unsigned)blocks), dim3(256), 0, st, d_g, d_hkl, d_famidx, d_glen_poss, d_SUB,
-                     d_valid, N, d_proj, d_fptr, d_units, d_skey, d_sidx, d_slen, nsp, d_scratch, cap, d_out);
+                     d_valid, N, d_proj, d_fptr, d_units, d_skey, d_su, d_bst, d_slen, nsp, d_scratch, cap, d_out);
   return hipGetLastError();
 }
