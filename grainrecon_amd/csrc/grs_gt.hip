@@ -293,7 +293,6 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
             bool any = false;
             if (skey) {
               const int64_t base = (int64_t)f * nsp + s0;
-              const double *K = skey + base;
               const int32_t mlen = slen[4 * i + f];
               const double zlo = n2 - GT_WIN, zhi = n2 + GT_WIN;
               int32_t b = 0;
@@ -302,10 +301,11 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
                 b = q < 0.0 ? 0 : (q > (double)GRS_GT_NB ? GRS_GT_NB : (int32_t)q);
               }
               int32_t lo = bst[(int64_t)(4 * i + f) * (GRS_GT_NB + 1) + b];
-              while (lo < mlen && K[lo] < zlo) lo++;
               W = su + 3 * base;
+              /* the key of a finite entry is its packed z (W[3k + 2]) */
+              while (lo < mlen && W[3 * lo + 2] < zlo) lo++;
               int32_t k = lo;
-              for (; k < mlen && K[k] <= zhi; k++) {
+              for (; k < mlen && W[3 * k + 2] <= zhi; k++) {
                 const double *w = W + 3 * k;
                 const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
                 if (d == d && (!any || d > best)) {
