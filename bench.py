@@ -475,6 +475,31 @@ def gt_leg(eng, cfg, n_orient, cpu, threads, seed=6006):
     return res
 
 
+def audit_leg(eng, smp, q, n_evals, n_seeds=2, seed=3002):
+    """Resolution-stage audit of the seeding kernel: for n_seeds seed voxels
+    x the first n_evals orientations, the audit launch also runs the exact
+    forward_comp.m pipeline on every event the resolution stage decides and
+    counts candidates, decisions, disagreements and the largest coordinate
+    difference; the counts and C must equal those of the plain launch."""
+    qa = q[:n_evals]
+    pos = smp.pos[np.random.default_rng(seed).choice(smp.pos.shape[0], n_seeds, replace=False)]
+    rows = []
+    for k, p in enumerate(pos):
+        ref = eng.forward_comp(p, qa)
+        eng.set_audit(True)
+        try:
+            out = eng.forward_comp(p, qa)
+            a = dict(eng.audit())
+        finally:
+            eng.set_audit(False)
+        same = all(bool(np.array_equal(x, y, equal_nan=True)) for x, y in zip(out, ref))
+        per = {kk: (v / qa.shape[0] if kk != "max_dev_px" else v) for kk, v in a.items()}
+        per["equal_to_plain_launch"] = same
+        rows.append(per)
+        log(f"[bench] audit seed {k}: {json.dumps(per)}")
+    return {"evals_per_seed": int(qa.shape[0]), "seeds": rows, "note": "per-evaluation averages"}
+
+
 def mapping_leg(eng, cfg, q, n_seeds, nr_seed, seed=8008):
     """The seed -> index -> pair -> centre-of-mass fit -> grow -> re-centre
     loop of GrainMapping_engine_gpu_comp.m (grainrecon_amd.mapping) on config
@@ -609,6 +634,8 @@ def main():
     ap.add_argument("--gt-orientations", type=int, default=8192, help="orientations of the Gt-score leg; 0 = off")
     ap.add_argument("--grow-box", type=int, default=24, help="sphere radius (voxels) of the region-growth leg; 0 = off")
     ap.add_argument("--mapping-seeds", type=int, default=6, help="seeds of the mapping-loop leg; 0 = off")
+    ap.add_argument("--audit-evals", type=int, default=0,
+                    help="orientations per seed of the resolution-stage audit leg; 0 = off")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
                     help="size of a plain device copy timed (and visible to counter runs) for calibration; 0 = off")
     args = ap.parse_args()
@@ -769,6 +796,8 @@ def main():
         result["grow_region"] = grow_leg(eng, cfg, args.grow_box, not args.no_cpu, threads)
     if args.mapping_seeds > 0:
         result["mapping"] = mapping_leg(eng, cfg, q, args.mapping_seeds, args.index_nr_seed)
+    if args.audit_evals > 0:
+        result["audit"] = audit_leg(eng, smp, q, args.audit_evals)
     result["autotune"] = {"median_kernel": eng.tuning(0), "counts_kernel": eng.tuning(1),
                           "counts_grid_kernel": eng.tuning(2)}
     if args.geo_fit_grains > 0:
