@@ -103,6 +103,7 @@ class Engine:
         self._h = h
         self.H = 0
         self.nrot = 0
+        self._vshape = None
 
     def close(self):
         if getattr(self, "_h", None):
@@ -329,8 +330,13 @@ class Engine:
                                            v.shape[2]))
         self._vshape = v.shape
 
+    def _volume_shape(self):
+        if self._vshape is None:
+            raise L.GrsError(L.GRS_ERR_STATE, "no volume set")
+        return self._vshape
+
     def get_volume(self, name: str) -> np.ndarray:
-        nx, ny, nz = self._vshape
+        nx, ny, nz = self._volume_shape()
         flat = np.zeros(nx * ny * nz)
         self._chk(self._lib.grs_get_volume(self._h, self.VOLUMES[name], _ptr(flat, _dp), flat.size))
         return flat.reshape(nz, ny, nx).transpose(2, 1, 0)  # column-major view of the fresh buffer
@@ -338,7 +344,7 @@ class Engine:
     def get_volume_at(self, name: str, vox) -> np.ndarray:
         """V(x, y, z) at the (n, 3) 1-based [x y z] rows of vox, gathered on
         the device (an index outside the volume raises GrsError)."""
-        nx, ny, nz = self._vshape
+        nx, ny, nz = self._volume_shape()
         v = np.asarray(vox, dtype=np.int64).reshape(-1, 3) - 1
         lin = np.ascontiguousarray((v[:, 0] + nx * (v[:, 1] + ny * v[:, 2])).astype(np.int64))
         # a component outside its axis could still land inside the volume
@@ -361,7 +367,7 @@ class Engine:
         rc = ctypes.c_int32()
         n = ctypes.c_int64()
         nchk = ctypes.c_int64()
-        cap = int(np.prod(self._vshape))
+        cap = int(np.prod(self._volume_shape()))
         idx = np.zeros((cap, 3), np.int32)
         self._chk(self._lib.grs_grow_region(self._h, _ptr(sd, _ip), float(indexed_comp), _ptr(U, _dp), float(spot_L),
                                             float(drop_off), float(maxD), float(maxDmedian), _ptr(p1, _dp),
