@@ -88,6 +88,7 @@ _SIGS = {
     "grs_euler2u_deg": (c_i32, [c_dp, c_i64, c_dp]),
     "grs_get_volume": (c_i32, [c_vp, c_i32, c_dp, c_i64]),
     "grs_get_volume_at": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_i64), c_i64, c_dp]),
+    "grs_set_volume_at": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_i64), c_i64, c_dp]),
     "grs_grow_region": (c_i32, [c_vp, c_ip, ctypes.c_double, c_dp, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, c_dp, c_dp, c_ip, ctypes.POINTER(c_i64), c_ip,
                                 c_i64, ctypes.POINTER(c_i64)]),

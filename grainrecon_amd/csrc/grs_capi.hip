@@ -1878,6 +1878,33 @@ grs_status grs_get_volume_at(grs_ctx *c, int32_t which, const int64_t *lin, int6
   return GRS_OK;
 }
 
+grs_status grs_set_volume_at(grs_ctx *c, int32_t which, const int64_t *lin, int64_t n, const double *values) {
+  if (!c) return GRS_ERR_ARG;
+  if (which < 0 || which > 3 || n < 0 || (n > 0 && (!lin || !values))) return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (!c->v_set[which]) return fail(c, GRS_ERR_STATE, "volume %d not set", which);
+  const int64_t total = c->v_dim[0] * c->v_dim[1] * c->v_dim[2];
+  for (int64_t i = 0; i < n; i++)
+    if (lin[i] < 0 || lin[i] >= total)
+      return fail(c, GRS_ERR_ARG, "linear index %lld outside the volume (%lld voxels)", (long long)lin[i],
+                  (long long)total);
+  {
+    /* a repeated index would leave the written value to the hardware's order */
+    std::vector<int64_t> sorted(lin, lin + n);
+    std::sort(sorted.begin(), sorted.end());
+    for (int64_t i = 1; i < n; i++)
+      if (sorted[i] == sorted[i - 1]) return fail(c, GRS_ERR_ARG, "linear index %lld repeated", (long long)sorted[i]);
+  }
+  if (n == 0) return GRS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  ENSURE(c, c->v_lin, sizeof(int64_t) * (size_t)n);
+  ENSURE(c, c->v_out, sizeof(double) * (size_t)n);
+  HIPCHK(c, hipMemcpyAsync(c->v_lin.p, lin, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->v_out.p, values, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_scatter_f64(P<double>(c->v_vol[which]), P<int64_t>(c->v_lin), P<double>(c->v_out), n, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return GRS_OK;
+}
+
 grs_status grs_grow_region(grs_ctx *c, const int32_t *seed, double indexed_comp, const double *U, double spot_L,
                            double drop_off, double maxD, double maxDmedian, const double *p1, double *center,
                            int32_t *replace_center, int64_t *n_indexed, int32_t *indexed, int64_t cap,

@@ -192,6 +192,8 @@ hipError_t grs_launch_region_accept(const double *d_rows, int64_t M, const int32
 
 /* out[i] = v[lin[i]] */
 hipError_t grs_launch_gather_f64(const double *d_v, const int64_t *d_lin, int64_t n, double *d_out, hipStream_t st);
+/* v[lin[i]] = val[i], no index repeated */
+hipError_t grs_launch_scatter_f64(double *d_v, const int64_t *d_lin, const double *d_val, int64_t n, hipStream_t st);
 
 /* G-vector matching score (grs_gt.hip) */
 hipError_t grs_launch_gt_proj(const grs_dgeo *d_g, const double *d_pos, const double *d_cs, int32_t nrot, double Ltr,

@@ -103,3 +103,16 @@ hipError_t grs_launch_gather_f64(const double *d_v, const int64_t *d_lin, int64_
   hipLaunchKernelGGL(k_gather_f64, dim3((unsigned)blocks), dim3(256), 0, st, d_v, d_lin, n, d_out);
   return hipGetLastError();
 }
+
+/* v[lin[i]] = val[i] (lin checked by the caller, no index repeated) */
+__global__ void k_scatter_f64(double *v, const int64_t *lin, const double *val, int64_t n) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) v[lin[i]] = val[i];
+}
+
+hipError_t grs_launch_scatter_f64(double *d_v, const int64_t *d_lin, const double *d_val, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+  hipLaunchKernelGGL(k_scatter_f64, dim3((unsigned)blocks), dim3(256), 0, st, d_v, d_lin, d_val, n);
+  return hipGetLastError();
+}

@@ -341,14 +341,32 @@ class Engine:
         self._chk(self._lib.grs_get_volume(self._h, self.VOLUMES[name], _ptr(flat, _dp), flat.size))
         return flat.reshape(nz, ny, nx).transpose(2, 1, 0)  # column-major view of the fresh buffer
 
-    def get_volume_at(self, name: str, vox) -> np.ndarray:
-        """V(x, y, z) at the (n, 3) 1-based [x y z] rows of vox, gathered on
-        the device (an index outside the volume raises GrsError)."""
+    def _linear(self, vox) -> np.ndarray:
+        """Linear indices (x fastest, 0-based) of (n, 3) 1-based [x y z] rows;
+        a row with a component outside its axis maps to -1, which the C ABI
+        rejects (it could otherwise land inside the volume)."""
         nx, ny, nz = self._volume_shape()
         v = np.asarray(vox, dtype=np.int64).reshape(-1, 3) - 1
         lin = np.ascontiguousarray((v[:, 0] + nx * (v[:, 1] + ny * v[:, 2])).astype(np.int64))
-        # a component outside its axis could still land inside the volume
         lin[np.any((v < 0) | (v >= np.array([nx, ny, nz])), axis=1)] = -1
+        return lin
+
+    def set_volume_at(self, name: str, vox, values) -> None:
+        """V(x, y, z) = values at the (n, 3) 1-based [x y z] rows of vox (the
+        rows distinct; an index outside the volume or repeated raises
+        GrsError)."""
+        lin = self._linear(vox)
+        val = _d(np.asarray(values, dtype=float).reshape(-1))
+        if val.size != lin.size:
+            raise ValueError("one value per voxel row")
+        self._chk(self._lib.grs_set_volume_at(self._h, self.VOLUMES[name],
+                                              lin.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lin.size,
+                                              _ptr(val, _dp)))
+
+    def get_volume_at(self, name: str, vox) -> np.ndarray:
+        """V(x, y, z) at the (n, 3) 1-based [x y z] rows of vox, gathered on
+        the device (an index outside the volume raises GrsError)."""
+        lin = self._linear(vox)
         out = np.zeros(lin.size)
         self._chk(self._lib.grs_get_volume_at(self._h, self.VOLUMES[name],
                                               lin.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lin.size,

@@ -149,6 +149,16 @@ def _upload(eng, ds):
         eng.set_volume(k, getattr(ds, k))
 
 
+def _upload_at(eng, ds, vox):
+    """grow_connected changes Completeness, Dismedian and Ninter of ds only
+    at the voxels it returns: write those to the device copies."""
+    if vox.shape[0] == 0:
+        return
+    x, y, z = vox[:, 0] - 1, vox[:, 1] - 1, vox[:, 2] - 1
+    for k in ("Completeness", "Dismedian", "Ninter"):
+        eng.set_volume_at(k, vox, getattr(ds, k)[x, y, z])
+
+
 def _download(eng, ds, vox):
     """The device growth changes Completeness, Dismedian and Ninter only at
     the voxels it returns (the seed and the accepted ones): copy those."""
@@ -274,7 +284,7 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                     # :207-214: connected-voxel growth on the host volumes
                     gc, grep_, gidx = grow_connected(eng, ds, (x + 1, y + 1, z + 1), temp[6], UU, drop_off, maxD,
                                                      max_dmedian, p1)
-                    _upload(eng, ds)
+                    _upload_at(eng, ds, gidx)
                     g = {"center": gc, "replace_center": grep_, "indexed": gidx}
                 else:
                     g = eng.grow_region((x + 1, y + 1, z + 1), temp[6], UU, spot_l, drop_off, maxD, max_dmedian, p1)

@@ -352,6 +352,11 @@ grs_status grs_get_volume(grs_ctx *ctx, int32_t which, double *v, int64_t n);
  * host fetches only the voxels a growth changed (no reference counterpart;
  * the reference keeps its volumes on the host). */
 grs_status grs_get_volume_at(grs_ctx *ctx, int32_t which, const int64_t *lin, int64_t n, double *out);
+/* volume `which` at linear index lin[i] = values[i] (indices checked and
+ * distinct): the write-back of a growth done on the host, e.g. the
+ * connected-voxel growth (no reference counterpart; the reference keeps its
+ * volumes on the host). */
+grs_status grs_set_volume_at(grs_ctx *ctx, int32_t which, const int64_t *lin, int64_t n, const double *values);
 
 /* grow_indexed_region_cuda.m:29-151 around the 1-based seed voxel with the
  * orientation U (row-major) on the resident volumes: Completeness(seed) =

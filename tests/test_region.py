@@ -105,6 +105,18 @@ def test_grow_region_equals_oracle(engine):
     for bad in ([[n + 1, 1, 1]], [[0, 1, 1]], [[1, 1, n + 1]]):
         with pytest.raises(GrsError):
             engine.get_volume_at("Completeness", bad)
+    # the device scatter writes exactly the listed voxels
+    vox = np.array([[1, 1, 1], [n, n, n], [3, 5, 7], [7, 5, 3]])
+    for k in ("Completeness", "Dismedian", "Ninter"):
+        want = engine.get_volume(k).copy()
+        vals = np.array([0.125, -2.5, 7.0, np.nan])
+        engine.set_volume_at(k, vox, vals)
+        for (a, b, c), w in zip(vox.tolist(), vals):
+            want[a - 1, b - 1, c - 1] = w
+        np.testing.assert_array_equal(engine.get_volume(k), want, err_msg=k)
+    for bad in ([[n + 1, 1, 1]], [[2, 2, 2], [2, 2, 2]]):
+        with pytest.raises(GrsError):
+            engine.set_volume_at("Completeness", bad, np.zeros(len(bad)))
 
 
 def _connected_problem():
