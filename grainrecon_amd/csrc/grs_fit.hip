@@ -5,7 +5,7 @@
  * Residual of one matched spot (pixels, unrounded):
  *   r = (-dety22/py + dety0 - y_meas, -detz22/pz + detz0 - z_meas)
  * with dety22/detz22 from the forward_comp geometry (forward_comp.m:15-55, the
- * convention of dis_calc_spotspair.m:129-165 and fit_grain_COM.m:116-160).
+ * convention of dis_calc_spotspair.m:35-71 and fit_grain_COM.m:49-83).
  * Parameters: the 7 geometry values [Lsam2sou Lsam2det dety00 detz00 tilt_x
  * tilt_y tilt_z] (geo_fitting.m:8-17) and, per grain, the Rodrigues vector
  * (U = quaternion2U(rod2quat(r))) and the centre of mass in mm

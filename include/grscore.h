@@ -27,7 +27,7 @@
  *     The MEX operators receive no beam-centre pixel (dety0, detz0) and no hit
  *     threshold; these entries use dety0 = detysize/2 + 0.5,
  *     detz0 = detzsize/2 + 0.5 (the convention of
- *     revise_all_unindexed_voxel_cuda.m:119-120) and a threshold of 0.5 pixel
+ *     revise_all_unindexed_voxel_cuda.m:40-42) and a threshold of 0.5 pixel
  *     (minEucDis = 0.5*mean([pixelysize pixelzsize]), setup_para.m:34,91,133).
  *
  *  2. Native FP64 entries on a context that keeps the geometry and the
@@ -133,7 +133,7 @@ grs_status grs_grow_indexed_region(grs_ctx *ctx, const double *vox, int64_t M, c
  * find_neighbor_ORs.m:26-29) against the candidate grains id_all (1-based);
  * euler holds 3 angles (degrees) per grain id.  param1 = the first 11 values
  * of the grow param1.  out rows [i phi1 PHI phi2 GrainId C Nr_intersect
- * dis_median] for the first maximum of C (revise_all_unindexed_voxel.m:318).
+ * dis_median] for the first maximum of C (revise_all_unindexed_voxel.m:152).
  * Replaces cuda_forward_comp_compete (grow_unindexed_compete_comp_cuda.m:39). */
 grs_status grs_forward_comp_compete(grs_ctx *ctx, const double *vox, int64_t M, const double *id_all, int64_t n_id,
                                     const double *euler, int64_t n_grains, const double *param1, double *out,
@@ -146,7 +146,7 @@ grs_status grs_forward_comp_compete(grs_ctx *ctx, const double *vox, int64_t M, 
  * Each candidate expands to (nr+1)^3 samples on the grid of
  * ori_local_sampling.m:22-30: q_i = (c_i - (gridsize*nr)/2) + k_i*gridsize,
  * k in [0, nr]^3 with q1 slowest, q0 recomputed.  Per row: the first maximum
- * of C over the expanded list (revise_all_unindexed_voxel.m:318), as its index
+ * of C over the expanded list (revise_all_unindexed_voxel.m:152), as its index
  * j in that list (best, -1 for an empty row), its quaternion (best_q), counts,
  * C and median.  The scoring launch counts hits on a bit-packed mask and
  * reduces a packed 64-bit argmax key per row; only the winners are re-scored

@@ -60,40 +60,64 @@ def _sort(fv, sim):
     return [fv[i] for i in order], [sim[i] for i in order]
 
 
+def initial_simplex(u0):
+    """fminsearch's start: u0 and, for every coordinate j, u0 with u0_j
+    scaled by 1 + 0.05 (usual_delta), or set to 0.00025 (zero_term_delta)
+    when it is 0."""
+    u0 = [float(v) for v in u0]
+    sim = [list(u0)]
+    for j in range(len(u0)):
+        y = list(u0)
+        y[j] = (1.0 + 0.05) * y[j] if y[j] != 0.0 else 0.00025
+        sim.append(y)
+    return sim
+
+
+def bnd_start(x0, lb, ub, asin=math.asin):
+    """fminsearchbnd.m:152-165 (two-sided bounds): the transformed start point."""
+    u0 = []
+    for x, lo, hi in zip(x0, lb, ub):
+        if x <= lo:
+            u0.append(-math.pi / 2)
+        elif x >= hi:
+            u0.append(math.pi / 2)
+        else:
+            z = 2.0 * (x - lo) / (hi - lo) - 1.0
+            u0.append(2.0 * math.pi + asin(max(-1.0, min(1.0, z))))
+    return u0
+
+
+def bnd_xform(u, lb, ub, sin=math.sin):
+    """fminsearchbnd.m:263-268: u -> x for two-sided bounds."""
+    out = []
+    for k in range(len(u)):
+        t = (sin(u[k]) + 1.0) / 2.0
+        t = t * (ub[k] - lb[k]) + lb[k]
+        out.append(max(lb[k], min(ub[k], t)))
+    return np.array(out)
+
+
 def fminsearchbnd(fun, x0, lb, ub, max_fun_evals=2000, max_iter=500, tol_fun=1e-10, tol_x=1e-4,
                   sin=math.sin, asin=math.asin):
     """Returns (x, fval, n_fun_evals, n_iter); every bound must be finite."""
     x0 = [float(v) for v in np.asarray(x0, float).reshape(-1)]
     lb = [float(v) for v in np.asarray(lb, float).reshape(-1)]
     ub = [float(v) for v in np.asarray(ub, float).reshape(-1)]
-    n = len(x0)
-
-    def xform(u):
-        out = []
-        for k in range(n):
-            t = (sin(u[k]) + 1.0) / 2.0
-            t = t * (ub[k] - lb[k]) + lb[k]
-            out.append(max(lb[k], min(ub[k], t)))
-        return np.array(out)
-
-    u0 = []
-    for k in range(n):
-        if x0[k] <= lb[k]:
-            u0.append(-math.pi / 2)
-        elif x0[k] >= ub[k]:
-            u0.append(math.pi / 2)
-        else:
-            z = 2.0 * (x0[k] - lb[k]) / (ub[k] - lb[k]) - 1.0
-            u0.append(2.0 * math.pi + asin(max(-1.0, min(1.0, z))))
 
     def f(u):
-        return float(fun(xform(u)))
+        return float(fun(bnd_xform(u, lb, ub, sin)))
 
-    sim = [list(u0)]
-    for j in range(n):
-        y = list(u0)
-        y[j] = (1.0 + 0.05) * y[j] if y[j] != 0.0 else 0.00025
-        sim.append(y)
+    u, fval, nf, it, _, _ = nelder_mead(f, initial_simplex(bnd_start(x0, lb, ub, asin)), max_fun_evals, max_iter,
+                                        tol_fun, tol_x)
+    return bnd_xform(u, lb, ub, sin), fval, nf, it
+
+
+def nelder_mead(f, sim, max_fun_evals=2000, max_iter=500, tol_fun=1e-10, tol_x=1e-4):
+    """fminsearch's iteration from the simplex `sim` (n+1 vertices, evaluated
+    in order).  Returns (best vertex, its value, evaluations, iterations,
+    final simplex, final values)."""
+    sim = [[float(c) for c in v] for v in sim]
+    n = len(sim) - 1
     fv = [f(v) for v in sim]
     fv, sim = _sort(fv, sim)
     nf, it = n + 1, 1
@@ -146,4 +170,4 @@ def fminsearchbnd(fun, x0, lb, ub, max_fun_evals=2000, max_iter=500, tol_fun=1e-
             nf += n
         fv, sim = _sort(fv, sim)
         it += 1
-    return xform(sim[0]), fv[0], nf, it
+    return np.array(sim[0]), fv[0], nf, it, sim, fv

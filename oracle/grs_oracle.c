@@ -11,7 +11,7 @@
  *                                  -> oracle_or_space_comp()
  *   grow_indexed_region_parallel.m:61-124 + ind2pos.m:6-10 (growth rule)
  *                                  -> oracle_grow()
- *   revise_all_unindexed_voxel.m:303-319 + euler2u.m (argmax over candidates)
+ *   revise_all_unindexed_voxel.m:138-153 + euler2u.m (argmax over candidates)
  *                                  -> oracle_compete()
  * Like the vectorised MATLAB, every quantity is computed for every (omega,
  * hkl) event before the selection mask is applied; the median is taken by
@@ -308,7 +308,7 @@ void oracle_grow(const oracle_geo *g, const oracle_proj *pj, const double *vox, 
   }
 }
 
-/* Compete (revise_all_unindexed_voxel.m:303-319): for each voxel row
+/* Compete (revise_all_unindexed_voxel.m:138-153): for each voxel row
  * [x y z start end] (1-based inclusive into id_all), score every candidate
  * grain's orientation (Euler, degrees, 3 per grain id) and keep the first
  * maximum of C (NaN never wins).  out M x 8 [i phi1 PHI phi2 gid C Nr_intersect dis_median]. */

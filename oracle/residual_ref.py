@@ -6,7 +6,7 @@ The residual of one matched spot is the difference between the simulated and
 the measured centroid, in pixels, before rounding:
     r = (-dety22/pixelysize + dety0 - y_meas, -detz22/pixelzsize + detz0 - z_meas)
 with dety22/detz22 from the forward_comp geometry (forward_comp.m:15-55, the
-convention of dis_calc_spotspair.m:129-165 and fit_grain_COM.m:116-160).
+convention of dis_calc_spotspair.m:35-71 and fit_grain_COM.m:49-83).
 Parameters: the 7 geometry values x = [Lsam2sou Lsam2det dety00 detz00 tilt_x
 tilt_y tilt_z] (geo_fitting.m:8-17, tilts in degrees through get_det_R.m) and
 per grain its Rodrigues vector r (U = quaternion2U(rod2quat(r)), the path of
