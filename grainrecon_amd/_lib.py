@@ -39,6 +39,7 @@ _SIGS = {
     "grs_set_distance_stack": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_i64, c_i32]),
     "grs_forward_comp": (c_i32, [c_vp, c_dp, c_dp, c_i64, c_ip, c_ip, c_dp, c_fp, c_i32]),
     "grs_forward_comp_device": (c_i32, [c_vp, c_dp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32]),
+    "grs_forward_comp_device_noalloc": (c_i32, [c_vp, c_dp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "grs_grow_indexed_region": (c_i32, [c_vp, c_dp, c_i64, c_dp, c_dp, c_dp, c_i32]),
     "grs_forward_comp_compete": (c_i32, [c_vp, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64, c_dp, c_dp, c_i32]),
     "grs_compete_csr": (c_i32, [c_vp, c_dp, c_i64, ctypes.POINTER(c_i64), c_dp, c_i64, c_i32, ctypes.c_double,

@@ -96,6 +96,10 @@ struct grs_ctx {
   DevBuf f_in, f_cnt; /* fit_batch: packed per-step inputs and counts */
   void *h_pin = nullptr; /* pinned staging of fit_batch (hipHostMalloc) */
   size_t h_pin_cap = 0;
+  /* device address of this GPU's allocation-free scratch (nullptr if the
+   * lookup failed; noalloc_err then holds the reason) */
+  unsigned char *noalloc = nullptr;
+  char noalloc_err[160] = {0};
   bool v_set[4] = {false, false, false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_total = 0.0;
@@ -170,6 +174,38 @@ static void hit_key_for(double thr, uint32_t *key, int32_t *has) {
   *key = k;
 }
 
+/* Working memory of grs_forward_comp_device_noalloc: one copy per GPU (the
+ * code object's data), held until the process ends.  In host code this name
+ * is not a device address; the address of the current device's copy comes
+ * from hipGetSymbolAddress (noalloc_lookup) and only that is passed on. */
+static __device__ __align__(256) unsigned char scratch[GRS_NOALLOC_SCRATCH_BYTES];
+
+#define GRS_NOALLOC_MAX_DEVICES 64
+static std::mutex g_noalloc_mu;
+static unsigned char *g_noalloc_ptr[GRS_NOALLOC_MAX_DEVICES];
+
+/* Device address of `scratch` on `device`, looked up once per device with
+ * that device current (the caller has made it current). */
+static unsigned char *noalloc_lookup(int device, char *err, size_t errlen) {
+  if (device < 0 || device >= GRS_NOALLOC_MAX_DEVICES) {
+    snprintf(err, errlen, "device %d is beyond the %d devices the allocation-free entry supports", device,
+             GRS_NOALLOC_MAX_DEVICES);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      snprintf(err, errlen, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_noalloc_ptr[device] = static_cast<unsigned char *>(p);
+  }
+  return g_noalloc_ptr[device];
+}
+
 static double nanmin2(double a, double b) {
   if (a != a) return b;
   if (b != b) return a;
@@ -202,6 +238,9 @@ grs_status grs_create(int32_t device, grs_ctx **out) {
     delete c;
     return GRS_ERR_DEVICE;
   }
+  /* the allocation-free entry's scratch on this device (a failed lookup
+   * only disables that entry, which then reports the reason) */
+  c->noalloc = noalloc_lookup(device, c->noalloc_err, sizeof(c->noalloc_err));
   *out = c;
   return GRS_OK;
 }
@@ -596,6 +635,98 @@ grs_status grs_forward_comp(grs_ctx *c, const double *pos, const double *q, int6
 grs_status grs_forward_comp_device(grs_ctx *c, const double *pos, const double *d_q, int64_t N, int32_t *d_ns,
                                    int32_t *d_ni, double *d_C, float *d_med, int32_t method) {
   return forward_comp_impl(c, pos, d_q, N, true, d_ns, d_ni, d_C, d_med, true, method);
+}
+
+/* Layout of the allocation-free scratch (byte offsets, 256-aligned):
+ *   0      voxel position (3 doubles)
+ *   256    omega-solve bounds of the voxel (GRS_VB doubles)
+ *   512    S*U*B of a chunk (9 doubles per orientation)
+ *   ...    valid flags of a chunk (int32 per orientation)
+ *   keys   per-wave median key lists, nrot*H uint32 each
+ * Every region is written before it is read within one call (position and
+ * bounds first, then per chunk the S*U*B rows, then the kernel, whose waves
+ * fill their key lists before the median reads them), so nothing has to be
+ * cleared between calls. */
+static const int64_t kNaPos = 0, kNaVb = 256, kNaSub = 512;
+static const int64_t kNaValid = kNaSub + 72 * GRS_NOALLOC_CHUNK;
+static const int64_t kNaKeys = ((kNaValid + 4 * GRS_NOALLOC_CHUNK) + 255) & ~(int64_t)255;
+static_assert(GRS_VB * 8 <= kNaSub - kNaVb, "voxel bounds overflow their slot");
+static_assert(kNaKeys < GRS_NOALLOC_SCRATCH_BYTES, "scratch too small for the chunk buffers");
+
+grs_status grs_forward_comp_device_noalloc(grs_ctx *c, const double *pos, const double *d_q, int64_t N,
+                                           int32_t *d_ns, int32_t *d_ni, double *d_C, float *d_med) {
+  if (!c) return GRS_ERR_ARG;
+  if (N < 0 || !pos || (N > 0 && (!d_q || !d_ns || !d_ni || !d_C || !d_med))) return fail(c, GRS_ERR_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  grs_status s = check_ready(c);
+  if (s != GRS_OK) return s;
+  if (!c->noalloc) return fail(c, GRS_ERR_DEVICE, "allocation-free scratch unavailable: %s", c->noalloc_err);
+  if (c->H > 240) return fail(c, GRS_ERR_ARG, "the fast method supports at most 240 reflections (got %d)", c->H);
+  const int64_t stride = (int64_t)c->nrot * c->H;
+  int64_t wmax = (GRS_NOALLOC_SCRATCH_BYTES - kNaKeys) / (4 * stride);
+  const int64_t slots = grs_fast_wave_slots(c->device);
+  if (wmax > slots) wmax = slots;
+  wmax &= ~(int64_t)3;
+  if (wmax < 4)
+    return fail(c, GRS_ERR_ARG,
+                "nrot*nrhkl = %lld events per evaluation: the %lld-byte allocation-free scratch holds fewer than 4 "
+                "waves' key lists (at most %lld events)",
+                (long long)stride, (long long)GRS_NOALLOC_SCRATCH_BYTES,
+                (long long)((GRS_NOALLOC_SCRATCH_BYTES - kNaKeys) / 16));
+  if (N == 0) return GRS_OK;
+  unsigned char *base = c->noalloc;
+  double *s_pos = reinterpret_cast<double *>(base + kNaPos);
+  double *s_vb = reinterpret_cast<double *>(base + kNaVb);
+  double *s_sub = reinterpret_cast<double *>(base + kNaSub);
+  int32_t *s_valid = reinterpret_cast<int32_t *>(base + kNaValid);
+  uint32_t *s_keys = reinterpret_cast<uint32_t *>(base + kNaKeys);
+  /* the launch configuration the autotuner chose for this class, if any
+   * (no tuning here: it records events) */
+  int32_t wpb = 4, keys = 0;
+  auto it = c->tunes.find(tune_key(c, 0));
+  if (it != c->tunes.end() && it->second.done) {
+    wpb = it->second.wpb;
+    keys = it->second.keys;
+  }
+  HIPCHK(c, hipMemcpyAsync(s_pos, pos, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), s_pos, 1, P<double>(c->d_cs), c->nrot, s_vb, c->st));
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  for (int64_t n0 = 0; n0 < N; n0 += GRS_NOALLOC_CHUNK) {
+    const int64_t n = N - n0 < GRS_NOALLOC_CHUNK ? N - n0 : GRS_NOALLOC_CHUNK;
+    HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), d_q + 4 * n0, n, s_sub, s_valid, c->st));
+    grs_eval_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = P<grs_dgeo>(c->d_geo);
+    a.hkl = P<double>(c->d_hkl);
+    a.cs = P<double>(c->d_cs);
+    a.pos = s_pos;
+    a.SUB = s_sub;
+    a.valid = s_valid;
+    a.keys = P<uint32_t>(c->d_keys);
+    a.n_eval = n;
+    a.map = GRS_MAP_ONE_VOXEL;
+    a.scratch_stride = stride;
+    a.scratch = s_keys;
+    a.vb = s_vb;
+    a.nrot = c->nrot;
+    a.count_keys = keys;
+    a.out_ns = d_ns + n0;
+    a.out_ni = d_ni + n0;
+    a.out_C = d_C + n0;
+    a.out_med = d_med + n0;
+    int64_t waves = wmax;
+    if (waves > n) waves = ((n + 3) / 4) * 4;
+    HIPCHK(c, grs_launch_fast(&a, c->H, waves, wpb, c->st));
+  }
+  if (c->timing) {
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->ms_total += ms;
+    c->launches += 1;
+  }
+  return GRS_OK;
 }
 
 /* voxel rows per device chunk (bounds the per-(voxel, omega) table) */

@@ -161,6 +161,15 @@ class Engine:
                                                     ctypes.c_void_p(d_ns), ctypes.c_void_p(d_ni),
                                                     ctypes.c_void_p(d_C), ctypes.c_void_p(d_med), int(method)))
 
+    def forward_comp_device_noalloc(self, pos, d_q: int, N: int, d_ns: int, d_ni: int, d_C: int,
+                                    d_med: int) -> None:
+        """grs_forward_comp_device_noalloc: the fast method with no device
+        allocation (working memory in the library's static per-GPU scratch)."""
+        pos = _d(np.asarray(pos).reshape(3))
+        self._chk(self._lib.grs_forward_comp_device_noalloc(self._h, _ptr(pos, _dp), ctypes.c_void_p(d_q), int(N),
+                                                            ctypes.c_void_p(d_ns), ctypes.c_void_p(d_ni),
+                                                            ctypes.c_void_p(d_C), ctypes.c_void_p(d_med)))
+
     def grow_indexed_region(self, vox6, U, param1, method: int = L.METHOD_FAST) -> np.ndarray:
         vox = _d(np.asarray(vox6).reshape(-1, 6))
         U = _d(np.asarray(U).reshape(9))

@@ -116,6 +116,28 @@ grs_status grs_forward_comp_device(grs_ctx *ctx, const double *pos, const double
                                    int32_t *d_nr_simu, int32_t *d_nr_intersect, double *d_completeness,
                                    float *d_dis_median, int32_t method);
 
+/* Allocation-free form of grs_forward_comp_device (fast method).  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The call makes no hipMalloc/hipFree and no allocating
+ * library call; its working memory (voxel position and bounds, S*U*B of a
+ * chunk of GRS_NOALLOC_CHUNK orientations, per-wave median key lists) lives in
+ * one static device array of GRS_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends.  The context must already hold the geometry and the
+ * distance stack (grs_set_geometry / grs_set_distance_stack allocate).  Any N
+ * is accepted (chunks of GRS_NOALLOC_CHUNK); geometries whose nrot*nrhkl
+ * events per evaluation leave room for fewer than 4 waves' key lists
+ * (nrot*nrhkl > ~3.9e6) are rejected with GRS_ERR_ARG.  The key lists bound
+ * the number of resident waves (about 1000 at 181 omega x 86 reflections), so
+ * it runs slower than grs_forward_comp_device on large N; results are
+ * identical.  All calls on one GPU share the array: calls that overlap on one
+ * GPU (two contexts, two host threads) race. */
+#define GRS_NOALLOC_SCRATCH_BYTES ((int64_t)64 << 20)
+#define GRS_NOALLOC_CHUNK ((int64_t)16384)
+grs_status grs_forward_comp_device_noalloc(grs_ctx *ctx, const double *pos, const double *d_q, int64_t N,
+                                           int32_t *d_nr_simu, int32_t *d_nr_intersect, double *d_completeness,
+                                           float *d_dis_median);
+
 /* One orientation U (row-major) against M voxels; vox rows are
  * [x y z C mask DisMedian] with 1-based voxel indices, param1 the 14 values
  * [RecVolumePixel(:,1)' simap_data_flag VoxSize Center' Dimension seed_comp
