@@ -84,6 +84,9 @@ def lib():
         L.oracle_compete.restype = None
         L.oracle_compete.argtypes = [ctypes.POINTER(OracleGeo), ctypes.POINTER(OracleProj), _dp, _i64, _dp, _dp,
                                      _dp, _i64, _dp, _i64, _dp, _dp, ctypes.c_int]
+        L.oracle_margin_audit.restype = None
+        L.oracle_margin_audit.argtypes = [ctypes.POINTER(OracleGeo), ctypes.POINTER(OracleProj), _dp, _dp, _i64,
+                                          _dp, _i64, _dp, _i64, _dp, ctypes.c_int]
         L.oracle_math_eval.restype = None
         L.oracle_math_eval.argtypes = [ctypes.c_int, _dp, _dp, _i64]
         L.oracle_set_libm.restype = None
@@ -193,6 +196,31 @@ def compete(geo, stack: Stack, vox5, id_all, euler, rot, hkl, param1, nthreads: 
                          hkl.ctypes.data_as(_dp), hkl.shape[0], p1.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
                          int(nthreads))
     return out
+
+
+AUDIT_FIELDS = ("events", "events_differing", "orientations_differing", "max_dbeta_rad", "max_dlambda_rel",
+                "max_dpixel_px", "min_margin_beta_low_rad", "min_margin_beta_high_rad", "min_margin_lambda_min_rel",
+                "min_margin_lambda_max_rel", "min_margin_rounding_px", "events_in_window",
+                "min_ratio_beta_low", "min_ratio_beta_high", "min_ratio_lambda_min", "min_ratio_lambda_max",
+                "min_ratio_rounding")
+
+
+def margin_audit(geo, stack: Stack, pos, q, rot, hkl, nthreads: int = 8) -> dict:
+    """Every event of one voxel x N orientation rows computed with the
+    project's FP64 math and with the C library's: the decisions and counts
+    that differ, the largest differences of beta, lambda and the raw pixel
+    coordinates, and the smallest margin of every continuous decision
+    (oracle_margin_audit in grs_oracle.c)."""
+    g = geo_from(geo)
+    pos = _d(np.asarray(pos).reshape(3))
+    q = _d(np.asarray(q).reshape(-1, 4))
+    rot = _d(np.asarray(rot).reshape(-1))
+    hkl = _d(np.asarray(hkl).reshape(-1, 3))
+    out = np.zeros(len(AUDIT_FIELDS))
+    lib().oracle_margin_audit(ctypes.byref(g), ctypes.byref(stack.c), pos.ctypes.data_as(_dp), q.ctypes.data_as(_dp),
+                              q.shape[0], rot.ctypes.data_as(_dp), rot.size, hkl.ctypes.data_as(_dp), hkl.shape[0],
+                              out.ctypes.data_as(_dp), int(nthreads))
+    return dict(zip(AUDIT_FIELDS, out.tolist()))
 
 
 def math_eval(fn: int, x) -> np.ndarray:

@@ -44,10 +44,13 @@ static int g_use_libm = 0;
 
 void oracle_set_libm(int on) { g_use_libm = on; }
 
-static double o_acos(double x) { return g_use_libm ? acos(x) : grs_acos(x); }
-static double o_sin(double x) { return g_use_libm ? sin(x) : grs_sin(x); }
-static double o_cos(double x) { return g_use_libm ? cos(x) : grs_cos(x); }
-static double o_atan(double x) { return g_use_libm ? atan(x) : grs_atan(x); }
+/* lm != 0: the C library's functions, else the project's */
+static double m_acos(int lm, double x) { return lm ? acos(x) : grs_acos(x); }
+static double m_sin(int lm, double x) { return lm ? sin(x) : grs_sin(x); }
+static double m_cos(int lm, double x) { return lm ? cos(x) : grs_cos(x); }
+static double m_atan(int lm, double x) { return lm ? atan(x) : grs_atan(x); }
+static double o_sin(double x) { return m_sin(g_use_libm, x); }
+static double o_cos(double x) { return m_cos(g_use_libm, x); }
 
 /* Geometry and selection parameters, the argument list of forward_comp.m:4-6. */
 typedef struct {
@@ -76,6 +79,87 @@ static int cmp_float(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
+/* Gw = S*U*B*hkl (forward_comp.m:15-16), evaluated ((S*U)*B)*hkl; Gw[h*3 + r] */
+static void oracle_Gw(const oracle_geo *g, const double *U, const double *hkl, int64_t H, double *Gw) {
+  double SU[9], SUB[9];
+  matmul3(g->S, U, SU);
+  matmul3(SU, g->B, SUB);
+  for (int64_t h = 0; h < H; h++) {
+    const double *x = hkl + 3 * h;
+    for (int r = 0; r < 3; r++) Gw[h * 3 + r] = (SUB[r * 3 + 0] * x[0] + SUB[r * 3 + 1] * x[1]) + SUB[r * 3 + 2] * x[2];
+  }
+}
+
+/* Every event of one rotation angle (forward_comp.m:19-59) for the shifted
+ * position pos (:17), with the math of mode lm: beta, lambdahkl, the
+ * detector coordinates dety22/detz22 (mm) and the unrounded pixel
+ * coordinates -dety22/pixelysize + dety0 (and z), H each. */
+static void omega_events(const oracle_geo *g, const double *pos, double rot_deg, const double *Gw, int64_t H, int lm,
+                         double *beta, double *lam, double *d22y, double *d22z, double *yraw, double *zraw) {
+  const double pi = GRS_PI;
+  const double L = g->Lsam2sou + g->Lsam2det;
+  const double Lss = g->Lsam2sou, Lsd = g->Lsam2det, RAO = g->RotAxisOffset;
+  const double *R = g->RotDet;
+  const double pio2 = pi / 2.0;
+  double omega = (rot_deg * pi) / 180.0; /* :19 */
+  double c = m_cos(lm, omega), s = m_sin(lm, omega);
+  /* Omega = [c -s 0; s c 0; 0 0 1] (:20); Omega*S without the structural zeros */
+  double OS[9];
+  for (int j = 0; j < 3; j++) {
+    OS[0 * 3 + j] = c * g->S[0 * 3 + j] - s * g->S[1 * 3 + j];
+    OS[1 * 3 + j] = s * g->S[0 * 3 + j] + c * g->S[1 * 3 + j];
+    OS[2 * 3 + j] = g->S[2 * 3 + j];
+  }
+  double SW[3];
+  for (int r = 0; r < 3; r++) SW[r] = (OS[r * 3 + 0] * pos[0] + OS[r * 3 + 1] * pos[1]) + OS[r * 3 + 2] * pos[2]; /* :22 */
+  double gp1 = Lss + SW[0];
+  double gp2 = (SW[1] - g->P0y) + RAO;
+  double gp3 = SW[2] - g->P0z;
+  double center[3] = {L, (gp2 * L) / gp1, (gp3 * L) / gp1};    /* :23-24 */
+  double alpha = m_atan(lm, sqrt(gp2 * gp2 + gp3 * gp3) / gp1); /* :25 */
+  double ng = sqrt((gp1 * gp1 + gp2 * gp2) + gp3 * gp3);        /* norm(grainpos) :31 */
+  double u[3] = {gp1 / ng, gp2 / ng, gp3 / ng};
+  double nc = sqrt((center[0] * center[0] + center[1] * center[1]) + center[2] * center[2]);
+  double cu[3] = {center[0] / nc, center[1] / nc, center[2] / nc};
+  double L2 = (Lsd - SW[0]) / m_cos(lm, alpha); /* :42 */
+  double tnum = (R[0] * (Lsd - SW[0]) + R[3] * ((g->dety00 - RAO) - SW[1])) + R[6] * (g->detz00 - SW[2]); /* :51 */
+  double ov[3] = {SW[0] - Lsd, (SW[1] - g->dety00) + RAO, SW[2] - g->detz00};                            /* :54 */
+
+  for (int64_t h = 0; h < H; h++) {
+    const double *gw = Gw + 3 * h;
+    double Gt1 = c * gw[0] - s * gw[1]; /* Gt = Omega*Gw (:28) */
+    double Gt2 = s * gw[0] + c * gw[1];
+    double Gt3 = gw[2];
+    double Glen = sqrt((Gt1 * Gt1 + Gt2 * Gt2) + Gt3 * Gt3);                     /* :30 */
+    double cb = (u[0] * (Gt1 / Glen) + u[1] * (Gt2 / Glen)) + u[2] * (Gt3 / Glen); /* :31 */
+    double b = m_acos(lm, cb);
+    double theta = b - pio2;                     /* :33 */
+    double sintth = m_sin(lm, 2.0 * theta);      /* :34 */
+    double d = ((1.0 / Glen) * 2.0) * pi;        /* :36 */
+    double lambdahkl = (2.0 * d) * m_sin(lm, theta); /* :37 */
+    double nv = sqrt(Gt2 * Gt2 + Gt3 * Gt3);      /* |v1|, v1 = [0 Gt2 Gt3] (:29,40) */
+    double phix = m_acos(lm, (Gt2 / nv) * cu[1] + (Gt3 / nv) * cu[2]); /* :40 */
+    double phiy = phix - 2.0 * theta;            /* :41 */
+    double diffvec = (L2 * sintth) / m_sin(lm, phiy); /* :43 */
+    double konst = sqrt(Gt2 * Gt2 + Gt3 * Gt3);   /* :44 */
+    double y22 = center[1] + ((diffvec * Gt2) / konst); /* :46 */
+    double z22 = center[2] + ((diffvec * Gt3) / konst); /* :47 */
+    double k0 = Lsd - SW[0], k1 = y22 - SW[1], k2 = z22 - SW[2]; /* :49 */
+    double nk = sqrt((k0 * k0 + k1 * k1) + k2 * k2);             /* :50 */
+    double K0 = k0 / nk, K1 = k1 / nk, K2 = k2 / nk;
+    double t = tnum / ((R[0] * K0 + R[3] * K1) + R[6] * K2);     /* :51-52 */
+    double w0 = t * K0 + ov[0], w1 = t * K1 + ov[1], w2 = t * K2 + ov[2];
+    double Y22 = (R[1] * w0 + R[4] * w1) + R[7] * w2; /* :54 */
+    double Z22 = (R[2] * w0 + R[5] * w1) + R[8] * w2; /* :55 */
+    yraw[h] = (-Y22) / g->pixelysize + g->dety0; /* :58, before round */
+    zraw[h] = (-Z22) / g->pixelzsize + g->detz0; /* :59 */
+    beta[h] = b;
+    lam[h] = lambdahkl;
+    d22y[h] = Y22;
+    d22z[h] = Z22;
+  }
+}
+
 /* forward_comp.m:4-90 for one (U, pos).  Returns the counts and the median;
  * optionally records every selected event (omega index, hkl index, dety,
  * detz, dety22, detz22, dis, lambdahkl) into ev (capacity ev_cap, 8 doubles each). */
@@ -84,9 +168,7 @@ int64_t oracle_forward_comp(const oracle_geo *g, const oracle_proj *pj, const do
                             int64_t *Nr_simu_out, int64_t *Nr_intersect_out, float *dis_median_out,
                             double *ev, int64_t ev_cap) {
   const double pi = GRS_PI;
-  const double L = g->Lsam2sou + g->Lsam2det;
-  const double Lss = g->Lsam2sou, Lsd = g->Lsam2det, RAO = g->RotAxisOffset;
-  const double *R = g->RotDet;
+  const double RAO = g->RotAxisOffset;
   int64_t Nr_simu = 0, Nr_intersect = 0, nev = 0;
   int64_t H = nrhkl;
   double *buf = (double *)malloc(sizeof(double) * 3 * H * 8);
@@ -100,76 +182,17 @@ int64_t oracle_forward_comp(const oracle_geo *g, const oracle_proj *pj, const do
   float *dis_list = (float *)malloc(sizeof(float) * (nrot * H + 1));
   int64_t ndis = 0;
 
-  /* Gw = S*U*B*hkl  (forward_comp.m:15-16), evaluated ((S*U)*B)*hkl */
-  double SU[9], SUB[9];
-  matmul3(g->S, U, SU);
-  matmul3(SU, g->B, SUB);
-  for (int64_t h = 0; h < H; h++) {
-    const double *x = hkl + 3 * h;
-    for (int r = 0; r < 3; r++) Gw[h * 3 + r] = (SUB[r * 3 + 0] * x[0] + SUB[r * 3 + 1] * x[1]) + SUB[r * 3 + 2] * x[2];
-  }
+  oracle_Gw(g, U, hkl, H, Gw);
   double pos[3] = {pos_in[0], pos_in[1] - RAO, pos_in[2]}; /* :17 */
   const double thr = g->minEucDis / ((g->pixelysize + g->pixelzsize) / 2.0);
   const double beta_max = ((90.0 + g->thetamax * 4.0) / 180.0) * pi;
   const double pio2 = pi / 2.0;
 
   for (int64_t i = 0; i < nrot; i++) {
-    double omega = (rot_angles[i] * pi) / 180.0; /* :19 */
-    double c = o_cos(omega), s = o_sin(omega);
-    /* Omega = [c -s 0; s c 0; 0 0 1] (:20); Omega*S without the structural zeros */
-    double OS[9];
-    for (int j = 0; j < 3; j++) {
-      OS[0 * 3 + j] = c * g->S[0 * 3 + j] - s * g->S[1 * 3 + j];
-      OS[1 * 3 + j] = s * g->S[0 * 3 + j] + c * g->S[1 * 3 + j];
-      OS[2 * 3 + j] = g->S[2 * 3 + j];
-    }
-    double SW[3];
-    for (int r = 0; r < 3; r++) SW[r] = (OS[r * 3 + 0] * pos[0] + OS[r * 3 + 1] * pos[1]) + OS[r * 3 + 2] * pos[2]; /* :22 */
-    double gp1 = Lss + SW[0];
-    double gp2 = (SW[1] - g->P0y) + RAO;
-    double gp3 = SW[2] - g->P0z;
-    double center[3] = {L, (gp2 * L) / gp1, (gp3 * L) / gp1};  /* :23-24 */
-    double alpha = o_atan(sqrt(gp2 * gp2 + gp3 * gp3) / gp1);  /* :25 */
-    double ng = sqrt((gp1 * gp1 + gp2 * gp2) + gp3 * gp3);     /* norm(grainpos) :31 */
-    double u[3] = {gp1 / ng, gp2 / ng, gp3 / ng};
-    double nc = sqrt((center[0] * center[0] + center[1] * center[1]) + center[2] * center[2]);
-    double cu[3] = {center[0] / nc, center[1] / nc, center[2] / nc};
-    double L2 = (Lsd - SW[0]) / o_cos(alpha); /* :42 */
-    double tnum = (R[0] * (Lsd - SW[0]) + R[3] * ((g->dety00 - RAO) - SW[1])) + R[6] * (g->detz00 - SW[2]); /* :51 */
-    double ov[3] = {SW[0] - Lsd, (SW[1] - g->dety00) + RAO, SW[2] - g->detz00};                            /* :54 */
-
+    omega_events(g, pos, rot_angles[i], Gw, H, g_use_libm, beta, lam, d22y, d22z, dety, detz);
     for (int64_t h = 0; h < H; h++) {
-      const double *gw = Gw + 3 * h;
-      double Gt1 = c * gw[0] - s * gw[1]; /* Gt = Omega*Gw (:28) */
-      double Gt2 = s * gw[0] + c * gw[1];
-      double Gt3 = gw[2];
-      double Glen = sqrt((Gt1 * Gt1 + Gt2 * Gt2) + Gt3 * Gt3);                     /* :30 */
-      double cb = (u[0] * (Gt1 / Glen) + u[1] * (Gt2 / Glen)) + u[2] * (Gt3 / Glen); /* :31 */
-      double b = o_acos(cb);
-      double theta = b - pio2;                 /* :33 */
-      double sintth = o_sin(2.0 * theta);      /* :34 */
-      double d = ((1.0 / Glen) * 2.0) * pi;    /* :36 */
-      double lambdahkl = (2.0 * d) * o_sin(theta); /* :37 */
-      double nv = sqrt(Gt2 * Gt2 + Gt3 * Gt3);  /* |v1|, v1 = [0 Gt2 Gt3] (:29,40) */
-      double phix = o_acos((Gt2 / nv) * cu[1] + (Gt3 / nv) * cu[2]); /* :40 */
-      double phiy = phix - 2.0 * theta;        /* :41 */
-      double diffvec = (L2 * sintth) / o_sin(phiy); /* :43 */
-      double konst = sqrt(Gt2 * Gt2 + Gt3 * Gt3);   /* :44 */
-      double y22 = center[1] + ((diffvec * Gt2) / konst); /* :46 */
-      double z22 = center[2] + ((diffvec * Gt3) / konst); /* :47 */
-      double k0 = Lsd - SW[0], k1 = y22 - SW[1], k2 = z22 - SW[2]; /* :49 */
-      double nk = sqrt((k0 * k0 + k1 * k1) + k2 * k2);             /* :50 */
-      double K0 = k0 / nk, K1 = k1 / nk, K2 = k2 / nk;
-      double t = tnum / ((R[0] * K0 + R[3] * K1) + R[6] * K2);     /* :51-52 */
-      double w0 = t * K0 + ov[0], w1 = t * K1 + ov[1], w2 = t * K2 + ov[2];
-      double Y22 = (R[1] * w0 + R[4] * w1) + R[7] * w2; /* :54 */
-      double Z22 = (R[2] * w0 + R[5] * w1) + R[8] * w2; /* :55 */
-      dety[h] = grs_round((-Y22) / g->pixelysize + g->dety0); /* :58 */
-      detz[h] = grs_round((-Z22) / g->pixelzsize + g->detz0); /* :59 */
-      beta[h] = b;
-      lam[h] = lambdahkl;
-      d22y[h] = Y22;
-      d22z[h] = Z22;
+      dety[h] = grs_round(dety[h]); /* :58 */
+      detz[h] = grs_round(detz[h]); /* :59 */
     }
     /* select3 (:61-65) and the hit loop (:66-82) */
     for (int64_t h = 0; h < H; h++) {
@@ -266,6 +289,161 @@ void oracle_or_space_comp(const oracle_geo *g, const oracle_proj *pj, const doub
     o[2] = (double)ni / (double)ns;
     o[3] = (double)med;
   }
+}
+
+static float median_sorted(float *v, int64_t n) {
+  if (n <= 0) return 2000.0f;
+  qsort(v, (size_t)n, sizeof(float), cmp_float);
+  if (n & 1) return v[n / 2];
+  float a = v[n / 2 - 1], b = v[n / 2];
+  return a + (b - a) / 2.0f;
+}
+
+static double dmin(double a, double b) { return b < a ? b : a; }
+static double dmax(double a, double b) { return b > a ? b : a; }
+static double ulp_of(double x) { x = fabs(x); return nextafter(x, INFINITY) - x; }
+/* margin / max(|difference|, ulp(x)) */
+static double sratio(double margin, double diff, double x) { return margin / dmax(fabs(diff), ulp_of(x)); }
+
+/* Decision-margin audit of forward_comp.m's continuous decisions for one
+ * voxel and N orientation rows (as oracle_or_space_comp): every event is
+ * computed with the project's math (p) and with the C library's (c).
+ * out[0]  events examined
+ * out[1]  events whose selection or pixel differs between p and c
+ * out[2]  orientations whose Nr_simu, Nr_intersect or median differ
+ * out[3]  max |beta_p - beta_c| (rad)
+ * out[4]  max |lam_p - lam_c| / lam_p
+ * out[5]  max |raw pixel coordinate p - c| (px), events passing the windows
+ * out[6]  min |beta - pi/2| (rad) over events whose other tests pass
+ * out[7]  min |beta - beta_max| (rad), same condition
+ * out[8]  min |lam - lambda_min| / lambda_min, same condition
+ * out[9]  min |lam - lambda_max| / lambda_max, same condition
+ * out[10] min distance of a raw pixel coordinate from a half-integer (px)
+ *         over events passing the windows with both raw coordinates within
+ *         1.5 px of the detector area (a changed rounding could change the
+ *         pixel, the bounds or the beamstop test)
+ * out[11] events passing the window tests (p)
+ * out[12..16] per-event ratios margin / max(|p - c|, 1 ulp) of the same five
+ *         decisions (beta low, beta high, lambda_min, lambda_max, rounding):
+ *         how many times the difference two < 1-ulp libraries make at that
+ *         event fits into its own margin (the smallest over the events) */
+void oracle_margin_audit(const oracle_geo *g, const oracle_proj *pj, const double *pos_in, const double *q, int64_t N,
+                         const double *rot_angles, int64_t nrot, const double *hkl, int64_t H, double *out,
+                         int nthreads) {
+  const double pi = GRS_PI;
+  const double pio2 = pi / 2.0;
+  const double beta_max = ((90.0 + g->thetamax * 4.0) / 180.0) * pi;
+  const double thr = g->minEucDis / ((g->pixelysize + g->pixelzsize) / 2.0);
+  const double pos[3] = {pos_in[0], pos_in[1] - g->RotAxisOffset, pos_in[2]};
+  double n_ev = 0, n_ev_diff = 0, n_or_diff = 0, n_win = 0;
+  double mx_b = 0, mx_l = 0, mx_px = 0;
+  double mn_b0 = INFINITY, mn_b1 = INFINITY, mn_l0 = INFINITY, mn_l1 = INFINITY, mn_rnd = INFINITY;
+  double rr_b0 = INFINITY, rr_b1 = INFINITY, rr_l0 = INFINITY, rr_l1 = INFINITY, rr_rnd = INFINITY;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) \
+    reduction(+ : n_ev, n_ev_diff, n_or_diff, n_win) reduction(max : mx_b, mx_l, mx_px) \
+    reduction(min : mn_b0, mn_b1, mn_l0, mn_l1, mn_rnd, rr_b0, rr_b1, rr_l0, rr_l1, rr_rnd)
+  for (int64_t i = 0; i < N; i++) {
+    double q1 = q[4 * i + 1], q2 = q[4 * i + 2], q3 = q[4 * i + 3];
+    double ss = (q1 * q1 + q2 * q2) + q3 * q3;
+    if (!(ss <= 1.0)) continue;
+    double U[9];
+    quat_to_U(sqrt(1.0 - ss), q1, q2, q3, U);
+    double *buf = (double *)malloc(sizeof(double) * H * 15);
+    double *Gw = buf, *a = buf + 3 * H; /* a: 2 modes x 6 arrays of H */
+    float *dl[2];
+    dl[0] = (float *)malloc(sizeof(float) * (nrot * H + 1));
+    dl[1] = (float *)malloc(sizeof(float) * (nrot * H + 1));
+    int64_t ns[2] = {0, 0}, ni[2] = {0, 0};
+    oracle_Gw(g, U, hkl, H, Gw);
+    for (int64_t om = 0; om < nrot; om++) {
+      for (int lm = 0; lm < 2; lm++) {
+        double *b = a + lm * 6 * H;
+        omega_events(g, pos, rot_angles[om], Gw, H, lm, b, b + H, b + 2 * H, b + 3 * H, b + 4 * H, b + 5 * H);
+      }
+      for (int64_t h = 0; h < H; h++) {
+        int sel[2], win[2];
+        double dy[2], dz[2];
+        for (int lm = 0; lm < 2; lm++) {
+          const double *b = a + lm * 6 * H;
+          double be = b[h], la = b[H + h];
+          dy[lm] = grs_round(b[4 * H + h]);
+          dz[lm] = grs_round(b[5 * H + h]);
+          win[lm] = be > pio2 && be < beta_max && la > g->lambda_min && la < g->lambda_max;
+          int ondet = dy[lm] >= 1.0 && dy[lm] <= g->detysize && dz[lm] >= 1.0 && dz[lm] <= g->detzsize &&
+                      !(dy[lm] >= g->BeamStopY[0] && dy[lm] <= g->BeamStopY[1] && dz[lm] >= g->BeamStopZ[0] &&
+                        dz[lm] <= g->BeamStopZ[1]);
+          sel[lm] = win[lm] && ondet;
+          if (sel[lm]) {
+            ns[lm]++;
+            int64_t iz = (int64_t)dz[lm] - 1, iy = (int64_t)dy[lm] - 1;
+            float dis = (iz < pj->nz && iy < pj->ny && om < pj->nrot) ? pj->proj[iz + iy * pj->nz + om * pj->nz * pj->ny]
+                                                                      : NAN;
+            if ((double)dis <= thr) ni[lm]++;
+            dl[lm][ns[lm] - 1] = dis;
+          }
+        }
+        const double *p = a, *c = a + 6 * H;
+        n_ev += 1;
+        if (sel[0] != sel[1] || (sel[0] && (dy[0] != dy[1] || dz[0] != dz[1]))) n_ev_diff += 1;
+        mx_b = dmax(mx_b, fabs(p[h] - c[h]));
+        if (p[H + h] == p[H + h] && p[H + h] != 0.0) mx_l = dmax(mx_l, fabs(p[H + h] - c[H + h]) / fabs(p[H + h]));
+        double be = p[h], la = p[H + h];
+        int ondet = dy[0] >= 1.0 && dy[0] <= g->detysize && dz[0] >= 1.0 && dz[0] <= g->detzsize &&
+                    !(dy[0] >= g->BeamStopY[0] && dy[0] <= g->BeamStopY[1] && dz[0] >= g->BeamStopZ[0] &&
+                      dz[0] <= g->BeamStopZ[1]);
+        int t0 = be > pio2, t1 = be < beta_max, t2 = la > g->lambda_min, t3 = la < g->lambda_max;
+        const double db = p[h] - c[h], dl_ = p[H + h] - c[H + h];
+        if (t1 && t2 && t3 && ondet) {
+          mn_b0 = dmin(mn_b0, fabs(be - pio2));
+          rr_b0 = dmin(rr_b0, sratio(fabs(be - pio2), db, be));
+        }
+        if (t0 && t2 && t3 && ondet) {
+          mn_b1 = dmin(mn_b1, fabs(be - beta_max));
+          rr_b1 = dmin(rr_b1, sratio(fabs(be - beta_max), db, be));
+        }
+        if (t0 && t1 && t3 && ondet) {
+          mn_l0 = dmin(mn_l0, fabs(la - g->lambda_min) / g->lambda_min);
+          rr_l0 = dmin(rr_l0, sratio(fabs(la - g->lambda_min), dl_, la));
+        }
+        if (t0 && t1 && t2 && ondet) {
+          mn_l1 = dmin(mn_l1, fabs(la - g->lambda_max) / g->lambda_max);
+          rr_l1 = dmin(rr_l1, sratio(fabs(la - g->lambda_max), dl_, la));
+        }
+        if (win[0]) {
+          n_win += 1;
+          double y = p[4 * H + h], z = p[5 * H + h];
+          mx_px = dmax(mx_px, dmax(fabs(y - c[4 * H + h]), fabs(z - c[5 * H + h])));
+          if (y >= -1.0 && y <= g->detysize + 2.0 && z >= -1.0 && z <= g->detzsize + 2.0) {
+            double ry = fabs(y - (floor(y) + 0.5)), rz = fabs(z - (floor(z) + 0.5));
+            mn_rnd = dmin(mn_rnd, dmin(ry, rz));
+            rr_rnd = dmin(rr_rnd, dmin(sratio(ry, y - c[4 * H + h], y), sratio(rz, z - c[5 * H + h], z)));
+          }
+        }
+      }
+    }
+    float m0 = median_sorted(dl[0], ns[0]), m1 = median_sorted(dl[1], ns[1]);
+    if (ns[0] != ns[1] || ni[0] != ni[1] || !(m0 == m1 || (m0 != m0 && m1 != m1))) n_or_diff += 1;
+    free(dl[0]);
+    free(dl[1]);
+    free(buf);
+  }
+  out[0] = n_ev;
+  out[1] = n_ev_diff;
+  out[2] = n_or_diff;
+  out[3] = mx_b;
+  out[4] = mx_l;
+  out[5] = mx_px;
+  out[6] = mn_b0;
+  out[7] = mn_b1;
+  out[8] = mn_l0;
+  out[9] = mn_l1;
+  out[10] = mn_rnd;
+  out[11] = n_win;
+  out[12] = rr_b0;
+  out[13] = rr_b1;
+  out[14] = rr_l0;
+  out[15] = rr_l1;
+  out[16] = rr_rnd;
 }
 
 /* Voxel index -> lab position (ind2pos.m:6-10, grow_indexed_region_parallel.m:70-74).
