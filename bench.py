@@ -532,7 +532,7 @@ def mapping_leg(eng, cfg, q, n_seeds, nr_seed, seed=8008):
     ds = MappingDS(mask)
     picks = smp.pos[rng.choice(smp.pos.shape[0], size=n_seeds, replace=False)]
     t = time.perf_counter()
-    out, _, nvox = grain_mapping_engine(eng, picks, qset, ds, trust_comp=0.9, min_comp=0.6, drop_off=0.05, maxD=2.0,
+    out, _, nvox, _ = grain_mapping_engine(eng, picks, qset, ds, trust_comp=0.9, min_comp=0.6, drop_off=0.05, maxD=2.0,
                                         max_dmedian=10.0, nr_seed=nr_seed, first_grain_id=1, center=[0, 0, 0],
                                         vox_size=smp.vox_mm, rvp_first=[1, 1, 1], tomo_dim=(nx, ny, nz))
     wall = time.perf_counter() - t

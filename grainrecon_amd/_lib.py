@@ -29,6 +29,8 @@ GRS_ERR_STATE = -3
 GRS_ERR_NOMEM = -4
 METHOD_FAST = 0
 METHOD_PLAIN = 1
+GT_MATCH = 0
+GT_MATCH_FUN = 1
 
 _SIGS = {
     "grs_version": (ctypes.c_char_p, []),
@@ -81,7 +83,7 @@ _SIGS = {
     "grs_fit_orientations": (c_i32, [c_dp, c_i64, c_dp, c_dp, ctypes.POINTER(c_i32)]),
     "grs_set_frame_spots": (c_i32, [c_vp, ctypes.POINTER(c_i64), c_i64, c_dp, c_i64]),
     "grs_set_volume": (c_i32, [c_vp, c_i32, c_dp, c_i64, c_i64, c_i64]),
-    "grs_forward_gt": (c_i32, [c_vp, c_dp, c_dp, c_i64, c_dp, c_dp, c_i32, c_dp]),
+    "grs_forward_gt": (c_i32, [c_vp, c_dp, c_dp, c_i64, c_dp, c_dp, c_i32, c_i32, c_dp]),
     "grs_fit_grain_com": (c_i32, [c_vp, c_dp, ctypes.POINTER(c_i64), c_i64, c_i32, c_i32, ctypes.c_double,
                                   ctypes.c_double, c_dp, c_dp, c_ip, c_ip]),
     "grs_u2euler": (c_i32, [c_dp, c_i64, c_dp]),

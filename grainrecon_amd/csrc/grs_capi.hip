@@ -89,7 +89,7 @@ struct grs_ctx {
   DevBuf v_vol[4], r_rows, r_counts, r_acc_lin, r_acc_C;
   int64_t v_dim[3] = {0, 0, 0};
   /* G-vector matching score buffers */
-  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_su, t_bst, t_slen;
+  DevBuf t_pos, t_proj, t_units, t_fam, t_dposs, t_glen, t_scratch, t_out, t_skey, t_su, t_bst, t_slen, t_sorig;
   DevBuf v_lin, v_out; /* grs_get_volume_at */
   /* grain centre-of-mass fit buffers */
   DevBuf m_rows, m_rptr, m_nm, m_cs, m_dis;
@@ -266,7 +266,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->p_rowptr, &c->p_rows, &c->v_vol[0], &c->v_vol[1], &c->v_vol[2], &c->v_vol[3],
                     &c->r_rows, &c->r_counts, &c->r_acc_lin, &c->r_acc_C, &c->t_pos, &c->t_proj, &c->t_units,
                     &c->t_fam, &c->t_dposs, &c->t_glen, &c->t_scratch, &c->t_out, &c->m_rows, &c->m_rptr,
-                    &c->m_nm, &c->t_skey, &c->t_su, &c->t_bst, &c->t_slen, &c->v_lin,
+                    &c->m_nm, &c->t_skey, &c->t_su, &c->t_bst, &c->t_slen, &c->t_sorig, &c->v_lin,
                     &c->v_out, &c->f_in, &c->f_cnt, &c->m_cs, &c->m_dis};
   if (c->g_exec) hipGraphExecDestroy(c->g_exec);
   for (auto &kv : c->gi_cache) hipGraphExecDestroy(kv.second);
@@ -2268,11 +2268,13 @@ grs_status grs_u2euler(const double *U_colmajor, int64_t M, double *euler_deg) {
 /* ---- G-vector matching score ---------------------------------------------------- */
 
 grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_t N, const double *fam_sq,
-                          const double *d_possible, int32_t nfam, double *out) {
+                          const double *d_possible, int32_t nfam, int32_t semantics, double *out) {
   GRS_NO_CONTRACT
   if (!c) return GRS_ERR_ARG;
   if (N < 0 || !pos || (N > 0 && (!q || !out)) || nfam < 1 || !fam_sq || !d_possible)
     return fail(c, GRS_ERR_ARG, "invalid argument");
+  if (semantics != GRS_GT_MATCH && semantics != GRS_GT_MATCH_FUN)
+    return fail(c, GRS_ERR_ARG, "unknown Gt semantics %d", semantics);
   HIPCHK(c, hipSetDevice(c->device));
   grs_status s = check_ready(c);
   if (s != GRS_OK) return s;
@@ -2281,8 +2283,11 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
   if (N == 0) return GRS_OK;
   const int32_t F = nfam < 4 ? nfam : 4;
   const grs_dgeo &g = c->hgeo;
-  /* calc_Gt_match_fun.m:12-16 */
+  /* ForwardCalc_OR_space_Gt.m:129-133 = calc_Gt_match_fun.m:12-16 */
   const double v[3] = {g.Lsd, g.dety00 - g.RAO, g.detz00};
+  /* calc_Gt_match shifts the position once, pos(:,2) - RotAxisOffset
+   * (ForwardCalc_OR_space_Gt.m:135); calc_Gt_match_fun does not */
+  const double pos_eff[3] = {pos[0], semantics == GRS_GT_MATCH ? pos[1] - g.RAO : pos[1], pos[2]};
   double d_tr[3];
   for (int r = 0; r < 3; r++) d_tr[r] = (g.R[r] * v[0] + g.R[3 + r] * v[1]) + g.R[6 + r] * v[2];
   const double Lsd_tr = d_tr[0] / g.R[0];
@@ -2320,7 +2325,7 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
   ENSURE(c, c->d_q, sizeof(double) * 4 * N);
   ENSURE(c, c->d_sub, sizeof(double) * 9 * N);
   ENSURE(c, c->d_valid, sizeof(int32_t) * N);
-  HIPCHK(c, hipMemcpyAsync(c->t_pos.p, pos, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->t_pos.p, pos_eff, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->t_fam.p, fam.data(), sizeof(int32_t) * fam.size(), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->t_dposs.p, dposs, sizeof(double) * 4, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->t_glen.p, glen, sizeof(double) * 4, hipMemcpyHostToDevice, c->st));
@@ -2338,20 +2343,22 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
   if (sorted) {
     ENSURE(c, c->t_skey, sizeof(double) * 4 * (size_t)nsp);
     ENSURE(c, c->t_su, sizeof(double) * 12 * (size_t)nsp);
+    ENSURE(c, c->t_sorig, sizeof(int32_t) * 4 * (size_t)nsp);
     ENSURE(c, c->t_bst, sizeof(int32_t) * 4 * (GRS_GT_NB + 1) * (size_t)c->nrot);
     ENSURE(c, c->t_slen, sizeof(int32_t) * 4 * (size_t)c->nrot);
   }
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
   if (sorted)
     HIPCHK(c, grs_launch_gt_sort(P<int64_t>(c->p_fptr), c->nrot, P<double>(c->t_units), nsp, c->p_maxframe,
-                                 P<double>(c->t_skey), P<double>(c->t_su), P<int32_t>(c->t_bst),
-                                 P<int32_t>(c->t_slen), c->st));
+                                 P<double>(c->t_skey), P<double>(c->t_su), P<int32_t>(c->t_sorig),
+                                 P<int32_t>(c->t_bst), P<int32_t>(c->t_slen), c->st));
   HIPCHK(c, grs_launch_gt_score(P<grs_dgeo>(c->d_geo), P<double>(c->d_hkl), P<int32_t>(c->t_fam),
                                 P<double>(c->t_glen), P<double>(c->d_sub), P<int32_t>(c->d_valid), N,
                                 P<double>(c->t_proj), P<int64_t>(c->p_fptr), P<double>(c->t_units),
                                 sorted ? P<double>(c->t_skey) : nullptr, sorted ? P<double>(c->t_su) : nullptr,
-                                sorted ? P<int32_t>(c->t_bst) : nullptr, sorted ? P<int32_t>(c->t_slen) : nullptr,
-                                nsp, P<double>(c->t_scratch), cap, waves,
+                                sorted ? P<int32_t>(c->t_sorig) : nullptr, sorted ? P<int32_t>(c->t_bst) : nullptr,
+                                sorted ? P<int32_t>(c->t_slen) : nullptr, nsp, semantics == GRS_GT_MATCH ? 1 : 0,
+                                P<double>(c->t_scratch), cap, waves,
                                 P<double>(c->t_out), c->st));
   if (c->timing) {
     HIPCHK(c, hipEventRecord(c->ev1, c->st));

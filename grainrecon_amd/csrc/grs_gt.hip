@@ -149,7 +149,8 @@ __global__ void k_gt_spots(const grs_dgeo *g, const double *proj, const int64_t 
 /* block (i, f): spots of projection i sorted by units[.., f, 2]; n2 = the
  * power of two >= the largest projection (dynamic LDS n2 * 12 bytes) */
 __global__ __launch_bounds__(256) void k_gt_sort(const int64_t *fptr, const double *units, int64_t nsp, int32_t n2,
-                                                 double *skey, double *su, int32_t *bst, int32_t *slen) {
+                                                 double *skey, double *su, int32_t *sorig, int32_t *bst,
+                                                 int32_t *slen) {
   extern __shared__ double gsm[];
   double *key = gsm;
   int32_t *idx = (int32_t *)(gsm + n2);
@@ -194,6 +195,7 @@ __global__ __launch_bounds__(256) void k_gt_sort(const int64_t *fptr, const doub
     su[3 * o] = w[0];
     su[3 * o + 1] = w[1];
     su[3 * o + 2] = w[2];
+    sorig[o] = idx[t];
     valid += key[t] != __builtin_inf() ? 1 : 0;
   }
   /* bucket starts: entry b > 0 is the first key >= -1 + 2b/GRS_GT_NB (the
@@ -234,8 +236,9 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
                                                   const double *glen_poss, const double *SUB, const int32_t *valid,
                                                   int64_t N, const double *proj, const int64_t *fptr,
                                                   const double *units, const double *skey, const double *su,
-                                                  const int32_t *bst, const int32_t *slen, int64_t nsp,
-                                                  double *scratch, int64_t cap, double *out) {
+                                                  const int32_t *sorig, const int32_t *bst, const int32_t *slen,
+                                                  int64_t nsp, int32_t cost_mode, double *scratch, int64_t cap,
+                                                  double *out) {
   GRS_NO_CONTRACT
   const int32_t H = g->H, nrot = g->nrot;
   const int lane = threadIdx.x & 63;
@@ -250,6 +253,9 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
     }
     const double *sub = SUB + 9 * u;
     int32_t nm = 0, expect = 0;
+    /* calc_Gt_match's cost sums: sum over the matches, in their order, of
+     * (n_k - w_k)^2 per component k (ForwardCalc_OR_space_Gt.m:195-197) */
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
     for (int32_t i = 0; i < nrot; i++) {
       const double *P = proj + GT_PROJ_DOUBLES * i;
       grs_vw v;
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
       for (int32_t h0 = 0; h0 < H; h0 += 64) {
         const int32_t h = h0 + lane;
         bool sel = false, match = false;
-        double amin = 0.0;
+        double amin = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
         if (h < H) {
           double gw[3], Gt[3], Glen;
           grs_Gw(sub, hkl + 3 * h, gw);
@@ -329,17 +335,33 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
               }
             }
             if (any) {
+              /* the smallest angle and, as MATLAB's min, the first spot (in
+               * the projection's order) that has it: the full scan visits
+               * spots in that order, the window carries their indices */
               amin = 1e300;
+              int64_t aidx = INT64_MAX;
+              const double *wbest = nullptr;
               const double lim = best - 1e-12;
               for (int64_t k = k0; k < k1; k++) {
                 const double *w = W ? W + 3 * k : units + 12 * k + 3 * f;
                 const double d = (n0 * w[0] + n1 * w[1]) + n2 * w[2];
                 if (d >= lim) {
                   const double a = d > 1.0 ? 0.0 : (grs_acos(d) * 180.0) / GRS_PI;
-                  if (a < amin) amin = a;
+                  const int64_t oi = W ? (int64_t)sorig[(int64_t)f * nsp + s0 + k] : k;
+                  if (a < amin || (a == amin && oi < aidx)) {
+                    amin = a;
+                    aidx = oi;
+                    wbest = w;
+                  }
                 }
               }
               match = amin < 1.0 && grs_fabs(Glen - glen_poss[f]) < 0.1;
+              if (match && cost_mode) {
+                const double x0 = n0 - wbest[0], x1 = n1 - wbest[1], x2 = n2 - wbest[2];
+                e0 = x0 * x0;
+                e1 = x1 * x1;
+                e2 = x2 * x2;
+              }
             }
           }
         }
@@ -347,6 +369,16 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
         const uint64_t mm = __ballot(match);
         if (match) list[nm + __popcll(gt_lanes_below(mm, lane))] = amin;
         nm += __popcll(mm);
+        if (cost_mode) {
+          /* the matches of this batch in lane order: every lane adds them
+           * in sequence (the wave-uniform loop keeps the shuffles whole) */
+          for (uint64_t m = mm; m; m &= m - 1) {
+            const int l = __ffsll((long long)m) - 1;
+            c0 = c0 + __shfl(e0, l);
+            c1 = c1 + __shfl(e1, l);
+            c2 = c2 + __shfl(e2, l);
+          }
+        }
       }
     }
     __threadfence_block();
@@ -385,9 +417,16 @@ __global__ __launch_bounds__(256, 4) void k_gt_score(const grs_dgeo *g, const do
     }
     if (lane == 0) {
       o[0] = (double)nm;
-      o[1] = med;
       o[2] = (double)nm / (double)expect;
-      o[3] = mean;
+      if (cost_mode) {
+        /* calc_Gt_match: [Nr_match Gt_matched_cost Nr_match/NrSpotExpectAll
+         * mean angle], cost 100 without a match (:198, :203) */
+        o[1] = nm > 0 ? ((c0 + c1) + c2) / (double)nm : 100.0;
+        o[3] = mean;
+      } else {
+        o[1] = med;
+        o[3] = mean;
+      }
     }
     __threadfence_block();
   }
@@ -417,14 +456,14 @@ int64_t grs_gt_wave_slots(int device) {
 }
 
 hipError_t grs_launch_gt_sort(const int64_t *d_fptr, int32_t nrot, const double *d_units, int64_t nsp,
-                              int64_t max_frame, double *d_skey, double *d_su, int32_t *d_bst, int32_t *d_slen,
-                              hipStream_t st) {
+                              int64_t max_frame, double *d_skey, double *d_su, int32_t *d_sorig, int32_t *d_bst,
+                              int32_t *d_slen, hipStream_t st) {
   if (nrot <= 0) return hipSuccess;
   if (max_frame > GT_SORT_MAX) return hipErrorInvalidValue;
   int32_t n2 = 64;
   while (n2 < max_frame) n2 <<= 1;
   hipLaunchKernelGGL(k_gt_sort, dim3((unsigned)(4 * nrot)), dim3(256), (size_t)n2 * 12, st, d_fptr, d_units, nsp, n2,
-                     d_skey, d_su, d_bst, d_slen);
+                     d_skey, d_su, d_sorig, d_bst, d_slen);
   return hipGetLastError();
 }
 
@@ -433,13 +472,14 @@ int64_t grs_gt_sort_max() { return GT_SORT_MAX; }
 hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const int32_t *d_famidx,
                                const double *d_glen_poss, const double *d_SUB, const int32_t *d_valid, int64_t N,
                                const double *d_proj, const int64_t *d_fptr, const double *d_units,
-                               const double *d_skey, const double *d_su, const int32_t *d_bst,
-                               const int32_t *d_slen, int64_t nsp, double *d_scratch, int64_t cap, int64_t waves,
-                               double *d_out, hipStream_t st) {
+                               const double *d_skey, const double *d_su, const int32_t *d_sorig,
+                               const int32_t *d_bst, const int32_t *d_slen, int64_t nsp, int32_t cost_mode,
+                               double *d_scratch, int64_t cap, int64_t waves, double *d_out, hipStream_t st) {
   if (N <= 0) return hipSuccess;
   int64_t w = waves < N ? waves : N;
   const int64_t blocks = (w + 3) / 4;
   hipLaunchKernelGGL(k_gt_score, dim3((unsigned)blocks), dim3(256), 0, st, d_g, d_hkl, d_famidx, d_glen_poss, d_SUB,
-                     d_valid, N, d_proj, d_fptr, d_units, d_skey, d_su, d_bst, d_slen, nsp, d_scratch, cap, d_out);
+                     d_valid, N, d_proj, d_fptr, d_units, d_skey, d_su, d_sorig, d_bst, d_slen, nsp, cost_mode, d_scratch,
+                     cap, d_out);
   return hipGetLastError();
 }

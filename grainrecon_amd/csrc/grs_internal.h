@@ -208,17 +208,20 @@ int64_t grs_gt_wave_slots(int device);
  * packed in that order (3 doubles), GRS_GT_NB + 1 bucket starts, the count
  * of finite keys */
 #define GRS_GT_NB 256
+/* d_sorig: per sorted entry, the spot's index within its projection */
 hipError_t grs_launch_gt_sort(const int64_t *d_fptr, int32_t nrot, const double *d_units, int64_t nsp,
-                              int64_t max_frame, double *d_skey, double *d_su, int32_t *d_bst, int32_t *d_slen,
-                              hipStream_t st);
+                              int64_t max_frame, double *d_skey, double *d_su, int32_t *d_sorig, int32_t *d_bst,
+                              int32_t *d_slen, hipStream_t st);
 int64_t grs_gt_sort_max();
-/* d_skey == nullptr: every spot of a projection is visited */
+/* d_skey == nullptr: every spot of a projection is visited.  cost_mode != 0:
+ * rows [Nr_match, matched cost, ratio, mean] (calc_Gt_match), else
+ * [Nr_match, median angle, ratio, mean] (calc_Gt_match_fun) */
 hipError_t grs_launch_gt_score(const grs_dgeo *d_g, const double *d_hkl, const int32_t *d_famidx,
                                const double *d_glen_poss, const double *d_SUB, const int32_t *d_valid, int64_t N,
                                const double *d_proj, const int64_t *d_fptr, const double *d_units,
-                               const double *d_skey, const double *d_su, const int32_t *d_bst,
-                               const int32_t *d_slen, int64_t nsp, double *d_scratch, int64_t cap, int64_t waves,
-                               double *d_out, hipStream_t st);
+                               const double *d_skey, const double *d_su, const int32_t *d_sorig,
+                               const int32_t *d_bst, const int32_t *d_slen, int64_t nsp, int32_t cost_mode,
+                               double *d_scratch, int64_t cap, int64_t waves, double *d_out, hipStream_t st);
 
 /* fit_grain_COM: the M started Nelder-Mead fits run to completion on the
  * device, one block each (grs_com.hip); d_cs holds 2 doubles and d_dis

@@ -315,17 +315,21 @@ class Engine:
         return x, fv, nf, it
 
     # -- G-vector matching score ---------------------------------------------------
-    def forward_gt(self, pos, q, fam_sq, d_possible) -> np.ndarray:
-        """calc_Gt_match_fun.m score of every orientation row q (N, 4) at pos
-        against the spots of grs_set_frame_spots: (N, 4) [Nr_match, median
-        angle, Nr_match/NrSpotExpectAll, mean angle]."""
+    def forward_gt(self, pos, q, fam_sq, d_possible, semantics: int = L.GT_MATCH) -> np.ndarray:
+        """G-vector matching score of every orientation row q (N, 4) at pos
+        against the spots of grs_set_frame_spots, (N, 4):
+        GT_MATCH (calc_Gt_match, ForwardCalc_OR_space_Gt.m:117-205) gives
+        [Nr_match, Gt_matched_cost, Nr_match/NrSpotExpectAll, mean angle];
+        GT_MATCH_FUN (calc_Gt_match_fun.m) gives [Nr_match, median angle,
+        Nr_match/NrSpotExpectAll, mean angle]."""
         pos = _d(np.asarray(pos).reshape(3))
         q = _d(np.asarray(q).reshape(-1, 4))
         fs = _d(np.asarray(fam_sq, dtype=float).reshape(-1))
         dp = _d(np.asarray(d_possible, dtype=float).reshape(-1))
         out = np.zeros((q.shape[0], 4))
         self._chk(self._lib.grs_forward_gt(self._h, _ptr(pos, _dp), _ptr(q, _dp), q.shape[0], _ptr(fs, _dp),
-                                           _ptr(dp, _dp), int(min(fs.size, dp.size)), _ptr(out, _dp)))
+                                           _ptr(dp, _dp), int(min(fs.size, dp.size)), int(semantics),
+                                           _ptr(out, _dp)))
         return out
 
     # -- region growth ------------------------------------------------------------

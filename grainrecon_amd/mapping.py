@@ -177,8 +177,16 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
     Completeness >= trust_comp at entry (else >= min(1.5 min_comp,
     0.8 trust_comp)) are listed (:46-53), and a first indexing of a seed more
     than 2 voxels from all of them uses the fit_all variant (:119-123;
-    MATLAB rejects an empty list there, and so does this function).  Returns
-    (indexing_final (P, 18), pos_list_new (P, 3), Nr_indexed_voxel (P,))."""
+    MATLAB rejects an empty list there, and so does this function).
+    After every seed its SeedIndexing row is recorded (:293-299:
+    indexing_final(i,1:11), Nr_indexed_voxel, the growth of the GrainId > 0
+    count during the seed, the candidate, the fit_all flag).  Past seed 200
+    (:318-330): without check_fit_all, when fewer than 5% of the indexed
+    seeds among the last 181 added GrainId voxels, check_fit_all turns on
+    with the voxels of GrainId > 0 at entry as the list; with check_fit_all,
+    when fewer than 5% of the indexed seeds among the last 51 did, the loop
+    ends (the remaining rows stay zero).  Returns (indexing_final (P, 18),
+    pos_list_new (P, 3), Nr_indexed_voxel (P,), SeedIndexing (P, 15))."""
     pos_list = np.asarray(pos_list, dtype=float).reshape(-1, 3)
     OR_q = np.asarray(OR_q, dtype=float).reshape(-1, 4)
     P = pos_list.shape[0]
@@ -187,6 +195,11 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
     pos_new = np.zeros((P, 3))
     nr_vox = np.zeros(P, dtype=np.int64)
     rcc = np.zeros(P, dtype=np.int64)
+    seed_ix = np.zeros((P, 15))
+    cand_id = np.zeros(P)
+    fa_track = np.zeros(P)
+    gid_entry = ds.GrainId.reshape(-1, order="F") > 0
+    n_gid_prev = int(np.count_nonzero(gid_entry))
     p1 = np.array([rvp_first[0], rvp_first[1], rvp_first[2], 1.0 if simap else 0.0, vox_size, center[0], center[1],
                    center[2], tomo_dim[0], tomo_dim[1], tomo_dim[2]], dtype=float)
     eng.set_orientation_set(OR_q)
@@ -222,7 +235,8 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                 fit_all = float(np.sqrt((d * d).sum(1)).min()) > 2
             if fit_all:
                 # :123-127: fit_OR_constrain_with_local_comp_cuda_2 with fit_all_flag = 1
-                res = {"indexing": index_seed_fit_all(eng, pos_ix, OR_q, nr_seed)["indexing"][None, :]}
+                fa = index_seed_fit_all(eng, pos_ix, OR_q, nr_seed)
+                res = {"indexing": fa["indexing"][None, :], "candidate": fa["candidate"]}
             elif rcc[i] == 0:
                 res = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
             else:
@@ -235,6 +249,9 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                     eng.set_orientation_set(OR_q)
             row = res["indexing"][0].copy()
             row[0] = i + 1
+            if rcc[i] == 0:
+                cand_id[i] = float(np.asarray(res["candidate"]).reshape(-1)[0])
+                fa_track[i] = 1.0 if fit_all else 0.0
             U = quaternion2U(row[14:18])
             _, _, _, pairs = eng.pair_spots(pos_ix[None, :], U[None, :, :])
             pairs = pairs[0]
@@ -322,4 +339,21 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                     pn[0], pn[1] = -pn[0], -pn[1]
                 pos_new[i] = pn
                 pos_ix = pn.copy()
-    return ind_final, pos_new, nr_vox
+        # :293-299: the seed's record; column 13 = voxels that gained a grain id
+        n_gid = int(np.count_nonzero(ds.GrainId > 0))
+        seed_ix[i] = np.concatenate([ind_final[i, :11], [nr_vox[i], n_gid - n_gid_prev, cand_id[i], fa_track[i]]])
+        n_gid_prev = n_gid
+        k = i + 1  # MATLAB's seed number
+        if not check_fit_all and k > 200:
+            w = seed_ix[k - 181:k]  # SeedIndexing(i-180:i, :)
+            n_ix = int(np.count_nonzero(w[:, 0] > 0))
+            if n_ix and np.count_nonzero(w[:, 12] > 0) / n_ix < 0.05:
+                check_fit_all = True
+                lin = np.flatnonzero(gid_entry)
+                ind_gid = np.stack(np.unravel_index(lin, dims, order="F"), 1) + 1
+        if check_fit_all and k > 200:
+            w = seed_ix[k - 51:k]  # SeedIndexing(i-50:i, :)
+            n_ix = int(np.count_nonzero(w[:, 0] > 0))
+            if n_ix and np.count_nonzero(w[:, 12] > 0) / n_ix < 0.05:
+                break
+    return ind_final, pos_new, nr_vox, seed_ix

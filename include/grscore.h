@@ -435,19 +435,33 @@ grs_status grs_u2euler(const double *U_colmajor, int64_t M, double *euler_deg);
 /* ---- G-vector matching score ("Gt" indexing method) ------------------------------ */
 
 /* The score behind cuda_forward_Gt (ForwardCalc_OR_space_Gt_cuda.m:7-47),
- * whose CUDA source is not in the reference, with the semantics of
- * calc_Gt_match_fun.m:1-152: for the seed position pos and N orientation
- * rows q ([q0 q1 q2 q3], q0 recomputed; |q_vec| > 1 gives a zero row), the
- * reflections of the first min(nfam, 4) families (fam_sq = hkl_family_square,
- * d_possible in Angstrom) matched against the experimental spots of each
- * projection (grs_set_frame_spots, centroids in pixels; projection i is
- * rotation angle i): a match when the angle between Gt and a spot's
- * diffraction vector is below 1 degree and |Gt| is within 0.1 of
- * 2 pi/d_possible.  out (N x 4) = [Nr_match, median matched angle (degrees),
- * Nr_match/NrSpotExpectAll (calcExpectSpotNr.m with the tilt-corrected
- * distance), mean matched angle]; NaN statistics when nothing matches. */
+ * whose CUDA source is not in the reference: for the seed position pos and N
+ * orientation rows q ([q0 q1 q2 q3], q0 recomputed; |q_vec| > 1 gives a zero
+ * row), the reflections of the first min(nfam, 4) families (fam_sq =
+ * hkl_family_square, d_possible in Angstrom) matched against the experimental
+ * spots of each projection (grs_set_frame_spots, centroids in pixels;
+ * projection i is rotation angle i): a match when the angle between Gt and
+ * the nearest spot's diffraction vector (the first such spot of the
+ * projection on a tie) is below 1 degree and |Gt| is within 0.1 of
+ * 2 pi/d_possible.  Nr_match/NrSpotExpectAll uses calcExpectSpotNr.m with
+ * the tilt-corrected distance.  semantics selects the reference function:
+ *   GRS_GT_MATCH      calc_Gt_match (ForwardCalc_OR_space_Gt.m:117-205, the
+ *                     CPU path whose Output(:,2:4) the MEX returns, :72):
+ *                     pos(:,2) shifted by -RotAxisOffset (:135); out (N x 4) =
+ *                     [Nr_match, Gt_matched_cost, Nr_match/NrSpotExpectAll,
+ *                     mean matched angle (degrees)], Gt_matched_cost = the
+ *                     mean over the matches of |n - w|^2 between the unit Gt
+ *                     and the matched spot's unit vector (:195-198), 100 with
+ *                     no match (:203);
+ *   GRS_GT_MATCH_FUN  calc_Gt_match_fun.m:1-152 (used by
+ *                     fit_OR_constrain_with_local_Gt.m:48,105): pos not
+ *                     shifted; out = [Nr_match, median matched angle,
+ *                     Nr_match/NrSpotExpectAll, mean matched angle].
+ * Angle statistics are NaN when nothing matches. */
+#define GRS_GT_MATCH 0
+#define GRS_GT_MATCH_FUN 1
 grs_status grs_forward_gt(grs_ctx *ctx, const double *pos, const double *q, int64_t N, const double *fam_sq,
-                          const double *d_possible, int32_t nfam, double *out);
+                          const double *d_possible, int32_t nfam, int32_t semantics, double *out);
 
 /* ---- candidate generation ------------------------------------------------ */
 

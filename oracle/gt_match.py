@@ -1,13 +1,15 @@
 """G-vector matching score ("Gt" indexing method) -- TEST INFRASTRUCTURE ONLY
 (checker of grs_forward_gt).
 
-The CUDA source of cuda_forward_Gt is not in the reference and
-ForwardCalc_OR_space_Gt.m:68 calls a calc_Gt_match that is not there either;
-the semantics restated here are those of calc_Gt_match_fun.m:1-152, the one
-definition the reference holds:
-  detector tilt folded into an effective distance and offsets (:12-16);
-  per projection i (frame i <-> rot_angles(i)): SamposW = Omega*S*pos' (pos
-  not shifted by RotAxisOffset), Kin = grainpos/|grainpos|, for every spot
+The CUDA source of cuda_forward_Gt is not in the reference.  Its CPU path,
+ForwardCalc_OR_space_Gt.m:6-113, calls calc_Gt_match (the same file,
+:117-205), and fit_OR_constrain_with_local_Gt.m:48,105 call a sibling,
+calc_Gt_match_fun.m:1-152.  Both are restated (semantics="match" / "fun"):
+  detector tilt folded into an effective distance and offsets (:129-133 /
+  :12-16);
+  calc_Gt_match only: pos(:,2) shifted by -RotAxisOffset (:135);
+  per projection i (frame i <-> rot_angles(i)): SamposW = Omega*S*pos',
+  Kin = grainpos/|grainpos|, for every spot
   (Spots{i}(:,9:10)) Kout_tr from the pixel position (:35-48),
   2theta = acosd(Kin.Kout_tr), lambda_j = 2 d_j sind(2theta/2),
   Gt_spot_j = (2 pi/lambda_j)(Kout_tr - Kin), normalised (:50-59, :88);
@@ -15,15 +17,24 @@ definition the reference holds:
   effective distance (calcExpectSpotNr.m:6-47);
   for every reflection of the first four families (:96-149) the smallest
   angle acos(n.g)*180/pi to the spots' unit vectors of its family; a match
-  when it is below 1 degree and |Gt| is within 0.1 of 2 pi/d (:100).
-  Output per orientation: [Nr_match, median(angles), Nr_match/NrSpotExpectAll,
-  mean(angles)] (:70-78; ForwardCalc_OR_space_Gt.m:72).
+  when it is below 1 degree and |Gt| is within 0.1 of 2 pi/d (:100); the
+  matched spot is the first spot of the projection with that angle (min's
+  index, ForwardCalc_OR_space_Gt.m:225).
+  Output per orientation:
+    "match": [Nr_match, Gt_matched_cost, Nr_match/NrSpotExpectAll,
+      mean(angles)], Gt_matched_cost = (sum_k (n_k - w_k)^2 summed over the
+      matches per component k, the three sums added in order) / Nr_match
+      with n the unit Gt and w the matched spot's unit vector (:195-198),
+      100 without a match (:203) (ForwardCalc_OR_space_Gt.m:72);
+    "fun": [Nr_match, median(angles), Nr_match/NrSpotExpectAll,
+      mean(angles)] (calc_Gt_match_fun.m:70-78).
 Choices the reference leaves open (parity unpinned at the last bit):
   sind(x) = sin((pi/180)*x) after reduction to |x| <= 45 by multiples of 90
   (the documented behaviour of MATLAB's sind); acos of a dot product above 1
   is taken as its real part 0; min ignores NaN; the median of an even count
-  is a + (b - a)/2; sums and norms of 3-vectors are sequential; the mean is
-  a sequential sum over the matches in (projection, reflection) order.
+  is a + (b - a)/2; sums and norms of 3-vectors are sequential; the mean and
+  the cost sums are sequential sums over the matches in (projection,
+  reflection) order.
 """
 from __future__ import annotations
 
@@ -86,8 +97,12 @@ def _mm(A, B):
     return C
 
 
-def gt_score(geo, pos, q_rows, rot, hkl, frames, fam_sq, d_possible):
-    """frames: per projection (n_i, >=2) arrays [spot y, spot z, ...]."""
+def gt_score(geo, pos, q_rows, rot, hkl, frames, fam_sq, d_possible, semantics="match", details=False):
+    """frames: per projection (n_i, >=2) arrays [spot y, spot z, ...];
+    semantics "match" (calc_Gt_match) or "fun" (calc_Gt_match_fun).  With
+    details, also the matched angles of every orientation, in match order."""
+    if semantics not in ("match", "fun"):
+        raise ValueError(semantics)
     S, B, R = np.asarray(geo.S, float), np.asarray(geo.B, float), np.asarray(geo.RotDet, float)
     hkl = np.asarray(hkl, float).reshape(-1, 3)
     H = hkl.shape[0]
@@ -111,7 +126,9 @@ def gt_score(geo, pos, q_rows, rot, hkl, frames, fam_sq, d_possible):
             if s2 == f:
                 fam[h] = k
                 break
-    p = np.asarray(pos, float)
+    p = np.array(pos, float).reshape(3)
+    if semantics == "match":
+        p[1] = p[1] - RAO  # ForwardCalc_OR_space_Gt.m:135
     # per projection: spot unit vectors and the per-projection terms
     proj = []
     for i, om_deg in enumerate(np.asarray(rot, float)):
@@ -149,13 +166,16 @@ def gt_score(geo, pos, q_rows, rot, hkl, frames, fam_sq, d_possible):
         proj.append(dict(c=c, s=s, SW=SW, kin=kin, units=units, ctr1=ctr1, ctr2=ctr2, cu1=ctr1 / nc, cu2=ctr2 / nc,
                          L2=(Lsd - SW[0]) / float(_cos(alpha)[0]), n=dety.size))
     out = np.zeros((len(q_rows), 4))
+    matched = []
     for u, q in enumerate(np.asarray(q_rows, float).reshape(-1, 4)):
         if not ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]) <= 1.0:
+            matched.append(np.zeros(0))
             continue
         SUB = _mm(_mm(S, _U(q)), B)
         Gw = np.array([[(SUB[r, 0] * hkl[h, 0] + SUB[r, 1] * hkl[h, 1]) + SUB[r, 2] * hkl[h, 2] for r in range(3)]
                        for h in range(H)])
         angles = []
+        cost = [0.0, 0.0, 0.0]
         expect = 0
         for i, P in enumerate(proj):
             c, s = P["c"], P["s"]
@@ -173,9 +193,13 @@ def gt_score(geo, pos, q_rows, rot, hkl, frames, fam_sq, d_possible):
                 ang = np.where(np.isnan(dots), np.nan, ang)
                 if np.all(np.isnan(ang)):
                     continue
-                amin = float(np.nanmin(ang))
+                j = int(np.nanargmin(ang))  # first index of the minimum
+                amin = float(ang[j])
                 if amin < 1.0 and abs(glen[h] - glen_possible[k]) < 0.1:
                     angles.append(amin)
+                    for r in range(3):
+                        e = nv[r] - float(un[r][j])
+                        cost[r] = cost[r] + e * e
         n = len(angles)
         if n:
             srt = sorted(angles)
@@ -186,8 +210,13 @@ def gt_score(geo, pos, q_rows, rot, hkl, frames, fam_sq, d_possible):
             mean = acc / n
         else:
             med = mean = math.nan
-        out[u] = [n, med, n / expect if expect else math.nan, mean]
-    return out
+        ratio = n / expect if expect else math.nan
+        matched.append(np.array(angles))
+        if semantics == "match":
+            out[u] = [n, ((cost[0] + cost[1]) + cost[2]) / n if n else 100.0, ratio, mean]
+        else:
+            out[u] = [n, med, ratio, mean]
+    return (out, matched) if details else out
 
 
 def _expect(geo, P, Gt, glen, beta_max):

@@ -104,7 +104,25 @@ def fit_orientation(x):
         r3 = atan2(a31, a32)
     ang = [(r * 180.0) / PI for r in (r1, r2, r3)]
     ang = [a + 360.0 if a < 0.0 else a for a in ang]
-    b1, b2, b3 = [(a * PI) / 180.0 for a in ang]
+    rq = angle_to_rod_quat(ang)
+    if rq is None:
+        return None
+    rod, qo = rq
+    q0, q1, q2, q3 = qo
+    U = np.array([
+        [2.0 * (q0 * q0 + q1 * q1) - 1.0, 2.0 * (q1 * q2 - q0 * q3), 2.0 * (q1 * q3 + q0 * q2)],
+        [2.0 * (q1 * q2 + q0 * q3), 2.0 * (q0 * q0 + q2 * q2) - 1.0, 2.0 * (q2 * q3 - q0 * q1)],
+        [2.0 * (q1 * q3 - q0 * q2), 2.0 * (q2 * q3 + q0 * q1), 2.0 * (q0 * q0 + q3 * q3) - 1.0],
+    ])
+    return np.array(ang + rod + qo), U
+
+
+def angle_to_rod_quat(ang):
+    """rod = angle2rod(ang*pi/180, 'ZXZ') and quat = rod2quat(rod) for ZXZ
+    angles in degrees (fit_OR_constrain_with_local_comp_cuda.m:139-140,
+    GrainMapping_engine_gpu_comp.m:151-152).  Returns (rod (3), quat (4))
+    as lists, or None where acos((trace-1)/2) would be complex."""
+    b1, b2, b3 = [(float(a) * PI) / 180.0 for a in ang]
     cps, sps = cos(b1), sin(b1)
     cth, sth = cos(b2), sin(b2)
     cph, sph = cos(b3), sin(b3)
@@ -128,10 +146,4 @@ def fit_orientation(x):
         th2 = atan(nr)
         s2 = sin(th2)
         qo = [cos(th2)] + [(r * s2) / nr for r in rod]
-    q0, q1, q2, q3 = qo
-    U = np.array([
-        [2.0 * (q0 * q0 + q1 * q1) - 1.0, 2.0 * (q1 * q2 - q0 * q3), 2.0 * (q1 * q3 + q0 * q2)],
-        [2.0 * (q1 * q2 + q0 * q3), 2.0 * (q0 * q0 + q2 * q2) - 1.0, 2.0 * (q2 * q3 - q0 * q1)],
-        [2.0 * (q1 * q3 - q0 * q2), 2.0 * (q2 * q3 + q0 * q1), 2.0 * (q0 * q0 + q3 * q3) - 1.0],
-    ])
-    return np.array(ang + rod + qo), U
+    return rod, qo

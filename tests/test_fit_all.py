@@ -87,5 +87,5 @@ def test_mapping_takes_fit_all_for_seeds_away_from_indexed_voxels(engine):
         ds = MappingDS(np.ones((n, n, n)))
         ds.Completeness[marked[0] - 1, marked[1] - 1, marked[2] - 1] = 0.95
         print(f"mapping with check_fit_all, indexed voxel at {marked}", flush=True)
-        out, _, _ = grain_mapping_engine(engine, pos, q, ds, **kw)
+        out, _, _, _ = grain_mapping_engine(engine, pos, q, ds, **kw)
         assert _same(out[0, 1:], (fa if far else pl)[1:])
