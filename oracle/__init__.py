@@ -87,6 +87,9 @@ def lib():
         L.oracle_margin_audit.restype = None
         L.oracle_margin_audit.argtypes = [ctypes.POINTER(OracleGeo), ctypes.POINTER(OracleProj), _dp, _dp, _i64,
                                           _dp, _i64, _dp, _i64, _dp, ctypes.c_int]
+        L.oracle_geo_fitting.restype = ctypes.c_double
+        L.oracle_geo_fitting.argtypes = [_dp, _dp, _dp, _i64, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
         L.oracle_math_eval.restype = None
         L.oracle_math_eval.argtypes = [ctypes.c_int, _dp, _dp, _i64]
         L.oracle_set_libm.restype = None

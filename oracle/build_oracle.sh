@@ -6,6 +6,6 @@
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 gcc -O2 -fPIC -shared -fopenmp -ffp-contract=off -fno-fast-math -std=c11 \
-    -o "$HERE/liboracle.so.tmp" "$HERE/grs_oracle.c" "$HERE/grs_candidates.c" -lm
+    -o "$HERE/liboracle.so.tmp" "$HERE/grs_oracle.c" "$HERE/grs_candidates.c" "$HERE/grs_geofit.c" -lm
 mv "$HERE/liboracle.so.tmp" "$HERE/liboracle.so"
 echo "$HERE/liboracle.so"
