@@ -2757,6 +2757,7 @@ grs_status grs_get_audit(grs_ctx *c, int64_t *counts, double *max_dev) {
   HIPCHK(c, hipMemcpyAsync(w, c->d_audit.p, sizeof(w), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   for (int k = 0; k < 5; k++) counts[k] = (int64_t)w[k];
+  counts[5] = (int64_t)w[6];
   std::memcpy(max_dev, &w[5], sizeof(double));
   return GRS_OK;
 }

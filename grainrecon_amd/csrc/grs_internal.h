@@ -67,9 +67,11 @@ typedef struct grs_eval_args {
 
 /* Audit counters of the fast kernel's resolution stage: events seen by phase
  * B1, resolved as selected, resolved as not selected, sent to the exact
- * pipeline, disagreements with the exact pipeline, and the largest
- * |coordinate difference| (pixels, as the bits of a non-negative double). */
-#define GRS_AUDIT_WORDS 6
+ * pipeline, disagreements with the exact pipeline, the largest
+ * |coordinate difference| (pixels, as the bits of a non-negative double), and
+ * events that passed the angle and wavelength windows into the spot
+ * construction. */
+#define GRS_AUDIT_WORDS 7
 
 /* Packed per-row argmax key of the counts-only kernel: larger is better.
  * ((floor(C*2^30) + 1) << 33) | (2^33-1-j) for a scored candidate j with

@@ -45,7 +45,7 @@
 #define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
 #define GRS_RANGE_DOUBLES(H) (4 * (int64_t)(H) + 1) /* 8H + 2 u32: omega-solve ranges */
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
-#define GRS_BT 7        /* doubles per omega of the beam table: SamposW (3), u (3), 1/u0 */
+#define GRS_BT 9        /* doubles per omega of the beam table: SamposW (3), u (3), 1/u0, cos, sin */
 
 /* LDS doubles of one wave's region of k_fast */
 __host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
@@ -610,6 +610,7 @@ __device__ __forceinline__ double fast_rsq(double x) {
 struct b1_out {
   double ry, rz;
   int32_t dy, dz;
+  int32_t win; /* cleared by the caller; 1 once the event enters the spot construction */
 };
 
 __device__ __forceinline__ bool res_round(double r, double *k) {
@@ -649,6 +650,7 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
   /* inside by less than the margins: the exact pipeline decides */
   if (!(cb < -1e-12 && cb > g->cos_beta_max + 1e-12 && lam > g->lambda_min + lm && lam < g->lambda_max - lm))
     return 2;
+  o->win = 1;
   const double omc = __builtin_fma(-cb, cb, 1.0);
   const double cth = omc * fast_rsq(omc);
   const double s2t = -2.0 * cb * cth;
@@ -864,6 +866,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       r[4] = u[1];
       r[5] = u[2];
       r[6] = fast_rcp(u[0]);
+      r[7] = a.cs[2 * om];
+      r[8] = a.cs[2 * om + 1];
     }
     __syncthreads();
     btab = bt;
@@ -1072,7 +1076,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     auto b1_core = [&](bool valid, uint32_t ev, uint32_t n) {
       int st = 0;
       uint32_t idx = 0;
-      bool bad = false;
+      bool bad = false, inwin = false;
       double dev = 0.0;
       if (valid) {
         int32_t om = (int32_t)(ev >> 8), h = (int32_t)(ev & 255u);
@@ -1080,9 +1084,12 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         b1_out o;
         o.ry = o.rz = 0.0;
         o.dy = o.dz = 0;
-        st = b1_classify(g, a.cs[2 * om], a.cs[2 * om + 1], Pw, btab ? btab + GRS_BT * om : nullptr, t[0], t[1],
-                         t[2], &o);
+        o.win = 0;
+        const double *brow = btab ? btab + GRS_BT * om : nullptr;
+        const double cc = brow ? brow[7] : a.cs[2 * om], ss = brow ? brow[8] : a.cs[2 * om + 1];
+        st = b1_classify(g, cc, ss, Pw, brow, t[0], t[1], t[2], &o);
         if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
+        inwin = o.win != 0;
         if (AUDIT) {
           double yz[2] = {0.0, 0.0};
           uint32_t eidx = 0;
@@ -1094,7 +1101,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       if (AUDIT) {
         /* whole-wave reductions, outside the divergent block */
         uint64_t mb = __ballot(bad), m1 = __ballot(st == 1), m0 = __ballot(valid && st == 0);
-        uint64_t m2 = __ballot(st == 2);
+        uint64_t m2 = __ballot(st == 2), mw = __ballot(inwin);
         for (int off = 32; off >= 1; off >>= 1) dev = __builtin_fmax(dev, __shfl_xor(dev, off, 64));
         if (lane == 0) {
           atomicAdd(a.audit + 0, (unsigned long long)n);
@@ -1103,6 +1110,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
           atomicAdd(a.audit + 3, (unsigned long long)__popcll(m2));
           atomicAdd(a.audit + 4, (unsigned long long)__popcll(mb));
           atomicMax(a.audit + 5, (unsigned long long)__double_as_longlong(dev));
+          atomicAdd(a.audit + 6, (unsigned long long)__popcll(mw));
         }
       }
       uint64_t m2 = __ballot(st == 2);

@@ -543,12 +543,13 @@ class Engine:
     def audit(self) -> dict:
         """Audit counters since set_audit(True): events examined by the
         resolution stage, resolved selected / not selected, sent to the exact
-        pipeline, disagreements, and the largest coordinate difference (px)."""
-        counts = (ctypes.c_int64 * 5)()
+        pipeline, disagreements, events that entered the spot construction,
+        and the largest coordinate difference (px)."""
+        counts = (ctypes.c_int64 * 6)()
         dev = ctypes.c_double()
         self._chk(self._lib.grs_get_audit(self._h, counts, ctypes.byref(dev)))
         return {"examined": counts[0], "resolved_selected": counts[1], "resolved_rejected": counts[2],
-                "exact": counts[3], "mismatched": counts[4], "max_dev_px": dev.value}
+                "exact": counts[3], "mismatched": counts[4], "in_window": counts[5], "max_dev_px": dev.value}
 
     def math_device(self, fn: int, x) -> np.ndarray:
         x = _d(np.asarray(x).reshape(-1))

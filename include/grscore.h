@@ -529,8 +529,9 @@ grs_status grs_graph_stats(grs_ctx *ctx, int64_t *captures, int64_t *replays);
  * also run through that pipeline and compared.  enable != 0 clears and arms
  * the counters.  No reference counterpart (a diagnostic of this engine). */
 grs_status grs_set_audit(grs_ctx *ctx, int32_t enable);
-/* counts[5]: events examined, resolved as selected, resolved as not selected,
- * sent to the exact pipeline, disagreements with it; max_dev: the largest
+/* counts[6]: events examined, resolved as selected, resolved as not selected,
+ * sent to the exact pipeline, disagreements with it, events that passed the
+ * angle and wavelength windows into the spot construction; max_dev: the largest
  * difference (pixels) between the resolved and the exact unrounded spot
  * coordinates of the selected events. */
 grs_status grs_get_audit(grs_ctx *ctx, int64_t *counts, double *max_dev);
