@@ -11,10 +11,16 @@ the 1 degree set), orientations and outputs resident in HBM.
 value      evals/s (voxel x orientation) of the fast (batched) kernel over the
            timed steps; the plain thread-per-evaluation kernel is timed on
            the same workload and reported beside it.
-roofline   algorithmic FP64 operations per evaluation (counted from
-           forward_comp.m:28-59, each + - * / sqrt acos sin = 1):
-           Nomega*(118*H + 124) + 90 + 15*H, times the evaluations of one launch,
-           over the launch's HIP-event duration on the engine's stream.
+roofline   FP64 vector bound: the FP64 operations of the pruned algorithm the
+           seeding kernel runs (per evaluation F_SETUP + F_SOLVE*H + F_CAND*
+           candidates + F_SPOT*window passes + F_EXACT*exact-pipeline events,
+           the per-unit counts taken from the kernel source and the units
+           counted by an audit launch of the same workload), times the
+           evaluations of one launch, over the launch's HIP-event duration on
+           the engine's stream, against the 78.6 TFLOP/s FP64 peak; beside it
+           the forward_comp.m-equivalent rate (Nomega*(118*H + 124) + 90 + 15*H
+           per evaluation) and, from the committed counter profile of the same
+           build, executed FP64 operations and HBM traffic.
 cpu_baseline  the C oracle (FP64 restatement of forward_comp.m, OpenMP) on a
            bounded sample of the same workload, which is also checked bit for
            bit against the GPU results.
@@ -38,24 +44,122 @@ PEAK_FP64_TFLOPS = 78.6
 
 
 def flop_per_eval(nrot: int, H: int) -> int:
+    """The full forward_comp.m work of one evaluation (every event through
+    the whole pipeline): the work the reference's algorithm does."""
     return nrot * (118 * H + 124) + 90 + 15 * H
 
 
-# HBM bytes per evaluation of the config-2 seeding launch, from the counter
-# profile of k_fast (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB per dispatch, the
-# factors a plain copy kernel of known size gives for the same counters;
-# profiles/k_fast_seed_cfg2_counters.md): every random distance-key read
-# pulls a 128-byte line.
-TRAFFIC_BYTES_PER_EVAL = 55_900
-# FP64 operations the seeding launch executes per evaluation, from the same
-# counter profile ((2 FMA + MUL + ADD + TRANS) F64 wave instructions x 64):
-# the omega solve and the resolution stage skip the rest of the algorithmic
-# count
-EXECUTED_FP64_PER_EVAL = 166_150
+# FP64 operations of the algorithm k_fast runs, per unit, counted from the
+# kernel source (grs_kernels.hip; + - * / sqrt rsq rcp acos atan2 = 1 each, a
+# fused multiply-add = 2; comparisons and integer work are not counted):
+F_SETUP = 136    # per evaluation: q0, U(q), S*U*B (k_sub_from_q)
+F_SOLVE = 127    # per (evaluation, reflection): G, |G|, thresholds, the cos(beta) window, om_ranges
+F_CAND = 27      # per candidate event: Gt, |Gt|, cos(beta), lambda and the window tests (b1_classify)
+F_SPOT = 90      # per candidate inside the windows: spot, tilted-detector intersection, pixel
+F_EXACT = 242    # per event sent to the exact pipeline: per-(voxel, omega) terms 124 + event 118
+
+
+def pruned_flop_per_eval(H, cand, in_window, exact):
+    """FP64 operations per evaluation of the pruned algorithm, from the
+    average candidates, window passes and exact-pipeline events per
+    evaluation (the audit counters of the same workload)."""
+    return F_SETUP + F_SOLVE * H + F_CAND * cand + F_SPOT * in_window + F_EXACT * exact
+
+
+# the translation unit of k_fast: its counters change only with these files
+KFAST_SOURCES = ("grs_geom.h", "grs_internal.h", "grs_kernels.hip", "grs_math.h")
+
+
+def kernel_sources_sha():
+    """sha256 (first 16 hex digits) of the sources k_fast is compiled from:
+    names a build so that a counter profile can be matched to it."""
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "grainrecon_amd", "csrc")
+    for name in KFAST_SOURCES:
+        h.update(name.encode())
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def counter_profile(name="k_fast_seed_cfg2_counters.md"):
+    """The machine-readable line of a committed counter profile
+    (`counters_json: {...}`), or None.  Its sources_sha names the build it was
+    measured on."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("counters_json:"):
+            return json.loads(line.split(":", 1)[1])
+    return None
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def roofline_leg(eng, cfg, q, seeds, kernel_ms, N, config, n_audit):
+    """Roofline of the seeding launch: the FP64 work of the pruned algorithm
+    k_fast runs (pruned_flop_per_eval, with its units per evaluation counted
+    by the audit launch on n_audit orientations at two of the timed seed
+    voxels) over the launch time and the FP64 vector peak.  The executed FP64
+    instructions and the HBM traffic come from the committed counter profile
+    when its sources_sha names this build, else they are null."""
+    H = cfg.hkl.shape[0]
+    qa = q[:n_audit]
+    tot = {"examined": 0, "in_window": 0, "exact": 0, "mismatched": 0}
+    sel = 0
+    for p in seeds:
+        eng.set_audit(True)
+        try:
+            ns, _, _, _ = eng.forward_comp(p, qa)
+            a = eng.audit()
+        finally:
+            eng.set_audit(False)
+        for k in tot:
+            tot[k] += a[k]
+        sel += int(ns.sum())
+    E = qa.shape[0] * len(seeds)
+    units = {"reflections": H, "candidates": tot["examined"] / E, "in_window": tot["in_window"] / E,
+             "exact": tot["exact"] / E, "selected": sel / E}
+    fp = pruned_flop_per_eval(H, units["candidates"], units["in_window"], units["exact"])
+    t = kernel_ms * 1e-3
+    achieved = N * fp / t / 1e12
+    gather = 4.0 * units["selected"] * N
+    out = {"bound": "fp64-valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+           "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+           "achieved_note": ("FP64 operations of the pruned algorithm k_fast runs (omega solve per reflection, "
+                             "window tests per candidate, spot construction per candidate inside the windows, exact "
+                             "pipeline per undecided event), counted from the kernel source, per launch, over the "
+                             "launch time"),
+           "kernel": "k_fast", "kernel_ms": round(kernel_ms, 4), "evals_per_launch": N,
+           "flop_per_eval": round(fp, 1), "units_per_eval": {k: round(v, 3) for k, v in units.items()},
+           "audit_sample": f"{len(seeds)} timed seed voxels x {qa.shape[0]} orientations, "
+                           f"{tot['mismatched']} disagreements with the exact pipeline",
+           "gather_bytes_per_launch": int(gather), "gather_GB_per_s": round(gather / t / 1e9, 1),
+           "forward_comp_equivalent_tflops": round(N * flop_per_eval(cfg.rot.size, H) / t / 1e12, 3),
+           "forward_comp_flop_per_eval": flop_per_eval(cfg.rot.size, H)}
+    prof = counter_profile()
+    sha = kernel_sources_sha()
+    out["build_sources_sha"] = sha
+    if prof and config == "cfg2" and prof.get("sources_sha") == sha:
+        ex = prof["fp64_ops_per_eval"]
+        out["executed_fp64_per_eval"] = ex
+        out["executed_tflops"] = round(N * ex / t / 1e12, 3)
+        out["executed_frac"] = round(N * ex / t / 1e12 / PEAK_FP64_TFLOPS, 4)
+        out["traffic"] = int(round(prof["hbm_bytes_per_eval"] * N))
+        out["traffic_GB_per_s"] = round(out["traffic"] / t / 1e9, 1)
+        out["traffic_over_gather"] = round(out["traffic"] / gather, 2)
+        out["counter_profile"] = "profiles/k_fast_seed_cfg2_counters.md (this build)"
+    else:
+        out["counter_profile"] = ("no counter profile of this build (profiles/k_fast_seed_cfg2_counters.md names "
+                                  f"sources {prof.get('sources_sha') if prof else None})")
+    log(f"[bench] roofline: {fp:.0f} FP64 ops per evaluation (pruned algorithm; units {units}) -> "
+        f"{achieved:.3f} TFLOP/s = {achieved / PEAK_FP64_TFLOPS:.4f} of FP64 peak; counters: {out['counter_profile']}")
+    return out
 
 
 def cpu_baseline(cfg, proj_host, pos, q, gpu_out, seconds, threads):
@@ -162,44 +266,36 @@ def geofit_leg(eng, cfg, n_grains, nm_grains, seed=4004):
 
     if nm_grains <= 0:
         return out
-    # CPU baseline: the reference's derivative-free procedure
-    import torch
-
-    from oracle import residual_ref
-    from oracle.fminsearch import fminsearchbnd
+    # CPU baseline: the reference's procedure, FitAllOnce of L_shift_tilt_fitting.m:23-50
+    # (GrainMapping_fit_geo.m:190-210, iter_fit = 1): bounded Nelder-Mead on geo_fitting.m's
+    # mean rounded-pixel distance, re-centred while the result lies within 0.1 of a bound
+    from oracle.geo_fit import GeoObjective, fit_all_once
 
     Gs = min(nm_grains, G)
     sel = sp["grain"] < Gs
     sub = F.SpotFit(eng, hkl[sel], omega[sel], sp["meas"][sel], sp["grain"][sel], Gs)
-    # L_shift_tilt_fitting.m:23-50 bounds: +-0.5 mm, +-50 px, +-0.5 deg around the start
-    half = np.array([0.5, 0.5, 50 * px, 50 * px, 0.5, 0.5, 0.5])
-    lb, ub = g7_0 - half, g7_0 + half
-    spots = dict(hkl=sub.hkl, omega=sub.omega, meas=sub.meas, grain=sub.grain)
-    held = torch.tensor(gr6_0[:Gs], dtype=torch.float64)
-
-    def obj(x):
-        with torch.no_grad():
-            r = residual_ref.residual(geo, spots, torch.tensor(x, dtype=torch.float64), held)
-        return 0.5 * float((r * r).sum())
-
     threads = min(16, os.cpu_count() or 1)
-    torch.set_num_threads(threads)
+    obj = GeoObjective(geo, sub.hkl, sub.omega, sub.meas, sub.grain, gr6_0[:Gs], nthreads=threads)
     t = time.perf_counter()
-    x_nm, f_nm, nfe, nit = fminsearchbnd(obj, g7_0, lb, ub)
+    x_nm, f_nm, nfe, rounds = fit_all_once(obj, g7_0, px)
     nm_t = time.perf_counter() - t
-    nm_rms = float(np.sqrt(2 * f_nm / sub.n))
+    nm_rms = obj.rms(x_nm)
     lm = F.levenberg_marquardt(sub, g7_0, gr6_0[:Gs], max_iter=100, fix_grains=True)
-    out["fminsearchbnd_subset"] = {
+    lm_rounded = obj(lm.g7)
+    out["fminsearchbnd_reference"] = {
         "grains": Gs, "spots": int(sub.n), "unknowns": 7,
-        "cpu": {"rms_px": round(nm_rms, 4), "mean_rounded_px": round(rounded(sub, sub.residual(x_nm, gr6_0[:Gs])), 4),
-                "seconds": round(nm_t, 2), "fun_evals": nfe, "iterations": nit, "threads": threads},
-        "gpu_lm": {"rms_px": round(lm.rms, 4), "mean_rounded_px": round(rounded(sub, sub.residual(lm.g7, gr6_0[:Gs])), 4),
-                   "seconds": round(lm.seconds, 3), "iterations": lm.iterations},
-        "rms_gap_px": round(nm_rms - lm.rms, 4),
+        "cpu": {"rms_px": round(nm_rms, 4), "mean_rounded_px": round(f_nm, 4), "seconds": round(nm_t, 2),
+                "fun_evals": nfe, "rounds": rounds, "threads": threads,
+                "procedure": "FitAllOnce (L_shift_tilt_fitting.m:23-50) on geo_fitting.m, grains held"},
+        "gpu_lm": {"rms_px": round(lm.rms, 4), "mean_rounded_px": round(lm_rounded, 4),
+                   "seconds": round(lm.seconds, 3), "iterations": lm.iterations,
+                   "procedure": "adjoint Levenberg-Marquardt on the unrounded residual, grains held"},
+        "rms_gap_px": round(nm_rms - lm.rms, 4), "mean_rounded_gap_px": round(f_nm - lm_rounded, 4),
         "speedup": round(nm_t / max(lm.seconds, 1e-9), 1),
     }
-    log(f"[bench] geometry fit on {Gs} grains ({sub.n} spots): fminsearchbnd (CPU, {threads} threads) "
-        f"{nm_rms:.4f} px in {nm_t:.1f}s ({nfe} evaluations); adjoint LM (GPU) {lm.rms:.4f} px in {lm.seconds:.3f}s")
+    log(f"[bench] geometry fit on {Gs} grains ({sub.n} spots): FitAllOnce fminsearchbnd (CPU, {threads} threads) "
+        f"{nm_rms:.4f} px RMS / {f_nm:.4f} px rounded mean in {nm_t:.1f}s ({nfe} evaluations, {rounds} rounds); "
+        f"adjoint LM (GPU) {lm.rms:.4f} px RMS / {lm_rounded:.4f} px rounded mean in {lm.seconds:.3f}s")
     return out
 
 
@@ -314,14 +410,37 @@ def al_leg(eng, steps, warmup, cpu_rows, threads, fit_grains):
     rate = evals / (kms * 1e-3)
     fe = flop_per_eval(cfg.rot.size, cfg.hkl.shape[0])
     best, bq, ns, ni, C, med = out
+    # units of the pruned algorithm on this workload: the audit launch (one-voxel
+    # kernel) over 8 of the voxels x the 1000 grain orientations; the counts-only
+    # kernel runs the same omega solve, window tests and spot construction
+    rng_a = np.random.default_rng(3105)
+    tot = {"examined": 0, "in_window": 0, "exact": 0, "mismatched": 0}
+    for r in rng_a.choice(M, size=8, replace=False):
+        eng.set_audit(True)
+        try:
+            eng.forward_comp(pos[r], smp.q)
+            a = eng.audit()
+        finally:
+            eng.set_audit(False)
+        for k in tot:
+            tot[k] += a[k]
+    E = 8 * smp.q.shape[0]
+    H = cfg.hkl.shape[0]
+    units = {"reflections": H, "candidates": tot["examined"] / E, "in_window": tot["in_window"] / E,
+             "exact": tot["exact"] / E}
+    fp = pruned_flop_per_eval(H, units["candidates"], units["in_window"], units["exact"])
     own = smp.grain[order]
     winner_grain = ids[(row_ptr[:-1] + best).clip(0, ids.size - 1)] - 1
     res = {"voxels": int(M), "subvolumes": np.bincount(slab).tolist(), "grains": int(smp.q.shape[0]),
            "omega": int(cfg.rot.size), "hkl": int(cfg.hkl.shape[0]), "candidates": int(row_ptr[-1]),
            "evals_per_launch": int(row_ptr[-1]), "launches": kn, "kernel_ms_per_launch": round(kms / kn, 3),
            "value": round(rate, 1), "unit": "evals/s", "wall_voxels_per_s": round(M * steps / wall, 1),
-           "fp64_algorithmic_tflops": round(rate * fe / 1e12, 3),
-           "fp64_frac": round(rate * fe / 1e12 / PEAK_FP64_TFLOPS, 4), "flop_per_eval": fe,
+           "roofline": {"bound": "fp64-valu", "achieved": round(rate * fp / 1e12, 3), "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(rate * fp / 1e12 / PEAK_FP64_TFLOPS, 4),
+                        "flop_per_eval": round(fp, 1), "units_per_eval": {k: round(v, 3) for k, v in units.items()},
+                        "audit_sample": f"8 voxels x {smp.q.shape[0]} grain orientations, "
+                                        f"{tot['mismatched']} disagreements with the exact pipeline"},
+           "forward_comp_equivalent_tflops": round(rate * fe / 1e12, 3), "forward_comp_flop_per_eval": fe,
            "own_grain_wins": round(float(np.mean(winner_grain == own)), 4),
            "setup_s": {"sample": round(t_gen, 2), "render_and_edt": round(t_prep, 2), "neighbour_lists": round(t_nb, 2)}}
     log(f"[bench] Al: {M} voxels, {row_ptr[-1]} candidates in one launch of {kms / kn:.1f} ms -> {rate:.4g} evals/s; "
@@ -397,17 +516,42 @@ def index_leg(eng, q, smp, n_pos, nr_seed, steps, seed=5005, turn_deg=0.05):
     mis_any = np.degrees(2 * np.arccos(np.clip(np.abs(qfit @ allq.T).max(1), -1, 1)))
     N = qset.shape[0]
     evals_dev = steps * n_pos * (N + nr_seed * 730)
+    # why a seed misses its own grain: the completeness of its own grain's
+    # orientation at the seed position against the fitted one, and how far the
+    # seed lies from its grain's boundary (the distance to the nearest Voronoi
+    # bisector plane)
+    pos_s = smp.pos[idx[n_pos:]]
+    own = smp.grain[idx[n_pos:]]
+    own_C = np.array([float(eng.forward_comp(p, smp.q[g][None, :])[2][0]) for p, g in zip(pos_s, own)])
+    s1 = smp.seeds[own]
+    d_b = np.full(pos_s.shape[0], np.inf)
+    for k in range(pos_s.shape[0]):
+        s2 = np.delete(smp.seeds, own[k], axis=0)
+        n2 = np.linalg.norm(s2 - s1[k], axis=1)
+        d_b[k] = float(np.min((np.sum((pos_s[k] - s2) ** 2, 1) - np.sum((pos_s[k] - s1[k]) ** 2)) / (2 * n2)))
+    miss = mis >= 0.1
+    interior = d_b > 0.02
     out = {"positions_per_call": n_pos, "calls": steps, "nr_seed": nr_seed, "orientations": N,
            "ms_per_call": round(dt / steps * 1e3, 2), "seeds_per_s": round(steps * n_pos / dt, 2),
            "first_call_s": round(first, 3), "graph_replays": int(r1 - r0), "graph_captures": int(c1 - c0),
            "device_evals_per_call": int(evals_dev / steps), "fit_evals_mean": round(float(nf.mean()), 1),
            "fitted_C_median": round(float(np.nanmedian(C)), 4), "fitted_C_min": round(float(np.nanmin(C)), 4),
-           "found_within_0.1deg": int((mis < 0.1).sum()), "seeds": int(mis.size),
+           "seeds": int(mis.size), "own_grain_within_0.1deg": int((~miss).sum()),
+           "interior_seeds": int(interior.sum()),
+           "interior_own_grain_within_0.1deg": int((~miss & interior).sum()),
+           "misses_with_fitted_C_ge_own_grain_C": int((miss & (C >= own_C)).sum()),
+           "misses_with_fitted_C_below_own_grain_C": int((miss & (C < own_C)).sum()),
            "any_grain_within_0.1deg": int((mis_any < 0.1).sum()),
-           "misorientation_deg_median": round(float(np.median(mis)), 4), "set_turn_deg": turn_deg}
+           "boundary_distance_mm_of_misses": [round(float(v), 4) for v in d_b[miss]],
+           "misorientation_deg_median": round(float(np.median(mis)), 4), "set_turn_deg": turn_deg,
+           "note": ("the seed voxels are random; a seed within a spot's tolerance of a neighbour grain sees that "
+                    "grain's orientation score as high as its own (the masks are grain-sized blobs), and the fit "
+                    "keeps whichever it reaches; interior seeds lie more than 20 um from every grain boundary")}
     log(f"[bench] index: {steps} calls x {n_pos} seeds in {dt:.3f}s -> {out['seeds_per_s']} seeds/s, "
-        f"median fitted C {out['fitted_C_median']}, {out['found_within_0.1deg']}/{mis.size} within 0.1 deg, "
-        f"graph replays {out['graph_replays']}")
+        f"median fitted C {out['fitted_C_median']}, own grain within 0.1 deg: {out['own_grain_within_0.1deg']}/"
+        f"{mis.size} (interior {out['interior_own_grain_within_0.1deg']}/{out['interior_seeds']}), misses with "
+        f"fitted C >= own grain's C: {out['misses_with_fitted_C_ge_own_grain_C']}, graph replays "
+        f"{out['graph_replays']}")
     return out
 
 
@@ -418,10 +562,11 @@ def _grain_com(smp):
 
 
 def gt_leg(eng, cfg, n_orient, cpu, threads, seed=6006):
-    """G-vector matching score (calc_Gt_match_fun.m semantics) at one grain's
-    centre of mass: spot centroids of every grain simulated with the engine
-    (one spot per selected event at its unrounded pixel position), n_orient
-    orientations (the grain's local sampling and random ones)."""
+    """G-vector matching score (calc_Gt_match, ForwardCalc_OR_space_Gt.m:117-205)
+    at one grain's centre of mass: spot centroids of every grain simulated
+    with the engine (one spot per selected event at its unrounded pixel
+    position), n_orient orientations (the grain's local sampling and random
+    ones); the calc_Gt_match_fun.m semantics timed beside it."""
     from grainrecon_amd import fit as F
     from grainrecon_amd import synth
 
@@ -453,23 +598,34 @@ def gt_leg(eng, cfg, n_orient, cpu, threads, seed=6006):
     out = eng.forward_gt(com[g], q, sq, dposs)
     wall = time.perf_counter() - t
     kms, _ = eng.kernel_time(reset=True)
+    from grainrecon_amd import _lib
+
+    t = time.perf_counter()
+    out_fun = eng.forward_gt(com[g], q, sq, dposs, _lib.GT_MATCH_FUN)
+    wall_fun = time.perf_counter() - t
+    kms_fun, _ = eng.kernel_time(reset=True)
     best = int(np.argmax(out[:, 0]))
     res = {"orientations": int(q.shape[0]), "spots": int(sum(f.shape[0] for f in frames)),
            "kernel_ms": round(kms, 3), "wall_s": round(wall, 3),
            "orientations_per_s": round(q.shape[0] / (kms * 1e-3), 1),
            "best_in_local_sampling": bool(best < local.shape[0]), "best_Nr_match": int(out[best, 0]),
-           "best_median_angle_deg": float(out[best, 1])}
+           "best_matched_cost": float(out[best, 1]), "best_mean_angle_deg": float(out[best, 3]),
+           "calc_Gt_match_fun": {"kernel_ms": round(kms_fun, 3), "wall_s": round(wall_fun, 3),
+                                 "best_median_angle_deg": float(out_fun[best, 1])}}
     if cpu:
         from oracle import gt_match
 
         pick = [best, local.shape[0] + 3]
         t = time.perf_counter()
-        ref = gt_match.gt_score(cfg.geo, com[g], q[pick], cfg.rot, cfg.hkl, frames, sq, dposs)
+        ref = gt_match.gt_score(cfg.geo, com[g], q[pick], cfg.rot, cfg.hkl, frames, sq, dposs, semantics="match")
         cdt = time.perf_counter() - t
+        ref_fun = gt_match.gt_score(cfg.geo, com[g], q[pick], cfg.rot, cfg.hkl, frames, sq, dposs, semantics="fun")
         same = (out[pick] == ref) | (np.isnan(out[pick]) & np.isnan(ref))
+        same_fun = (out_fun[pick] == ref_fun) | (np.isnan(out_fun[pick]) & np.isnan(ref_fun))
         res["cpu_baseline"] = {"value": round(len(pick) / cdt, 3), "unit": "orientations/s", "cores": 1,
                                "kind": "port", "sample": f"{len(pick)} orientations (oracle/gt_match.py, numpy)"}
-        res["parity"] = {"orientations_checked": len(pick), "bit_exact": bool(same.all())}
+        res["parity"] = {"orientations_checked": len(pick), "bit_exact": bool(same.all()),
+                         "calc_Gt_match_fun_bit_exact": bool(same_fun.all())}
     log(f"[bench] gt: {q.shape[0]} orientations x {res['spots']} spots in {kms:.2f} ms of kernel time; best row "
         f"{best} (Nr_match {res['best_Nr_match']})")
     return res
@@ -500,13 +656,14 @@ def audit_leg(eng, smp, q, n_evals, n_seeds=2, seed=3002):
     return {"evals_per_seed": int(qa.shape[0]), "seeds": rows, "note": "per-evaluation averages"}
 
 
-def mapping_leg(eng, cfg, q, n_seeds, nr_seed, seed=8008):
+def mapping_leg(eng, cfg, q, n_seeds, nr_seed, seed=8008, refine=("nelder-mead", "adjoint")):
     """The seed -> index -> pair -> centre-of-mass fit -> grow -> re-centre
-    loop of GrainMapping_engine_gpu_comp.m (grainrecon_amd.mapping) on config
-    2: seeds at random voxels, the 262,144-orientation set plus the grains'
-    orientations turned by 0.05 degrees, spot centroids of all grains with an
-    area of 800 px^2 (a grain-sized spot, so the growth box spans about 25
-    voxels)."""
+    loop of GrainMapping_engine_gpu_comp.m (grainrecon_amd.mapping) on the
+    Fe 1,000-grain sample: seeds at random voxels, the 262,144-orientation set
+    plus the grains' orientations turned by 0.05 degrees, spot centroids of
+    all grains with an area of 800 px^2 (a grain-sized spot, so the growth box
+    spans about 25 voxels).  Run once per refine mode from the same empty
+    volumes: the reference's Nelder-Mead fits, then the adjoint LM fits."""
     from grainrecon_amd import fit as F
     from grainrecon_amd import synth
     from grainrecon_amd.mapping import MappingDS, grain_mapping_engine
@@ -529,29 +686,49 @@ def mapping_leg(eng, cfg, q, n_seeds, nr_seed, seed=8008):
     nx, ny, nz = smp.dims
     mask = np.zeros((nx, ny, nz))
     mask[smp.index[:, 0] - 1, smp.index[:, 1] - 1, smp.index[:, 2] - 1] = 1.0
-    ds = MappingDS(mask)
     picks = smp.pos[rng.choice(smp.pos.shape[0], size=n_seeds, replace=False)]
-    t = time.perf_counter()
-    out, _, nvox, _ = grain_mapping_engine(eng, picks, qset, ds, trust_comp=0.9, min_comp=0.6, drop_off=0.05, maxD=2.0,
-                                        max_dmedian=10.0, nr_seed=nr_seed, first_grain_id=1, center=[0, 0, 0],
-                                        vox_size=smp.vox_mm, rvp_first=[1, 1, 1], tomo_dim=(nx, ny, nz))
-    wall = time.perf_counter() - t
     gid = np.full((nx, ny, nz), -1, np.int32)
     gid[smp.index[:, 0] - 1, smp.index[:, 1] - 1, smp.index[:, 2] - 1] = smp.grain
-    grown = ds.GrainId > 0
-    res = {"seeds": n_seeds, "nr_seed": nr_seed, "wall_s": round(wall, 3), "seeds_per_s": round(n_seeds / wall, 2),
-           "seeds_indexed": int((out[:, 6] >= 0.6).sum()), "voxels_assigned": int(grown.sum()),
-           "median_C": round(float(np.nanmedian(out[:, 6])), 4)}
-    # the assigned voxels' true grains: the share in the grain most assigned per seed id
-    same = 0
-    for sid in np.unique(ds.GrainId[grown]):
-        g = gid[ds.GrainId == sid]
-        g = g[g >= 0]
-        if g.size:
-            same += int(np.bincount(g).max())
-    res["voxels_in_majority_grain"] = same
-    log(f"[bench] mapping: {n_seeds} seeds in {wall:.2f}s ({res['seeds_per_s']} seeds/s), "
-        f"{res['voxels_assigned']} voxels assigned, {same} in their seed's majority grain")
+    geo7 = np.array([geo.Lsam2sou, geo.Lsam2det, geo.dety00, geo.detz00, 0.0, 0.0, 0.0])
+    sym = synth.cubic_symmetry_quats()
+    res = {"seeds": n_seeds, "nr_seed": nr_seed}
+    for mode in refine:
+        ds = MappingDS(mask)
+        fit_log = []
+        t = time.perf_counter()
+        out, _, nvox, _ = grain_mapping_engine(eng, picks, qset, ds, trust_comp=0.9, min_comp=0.6, drop_off=0.05,
+                                            maxD=2.0, max_dmedian=10.0, nr_seed=nr_seed, first_grain_id=1,
+                                            center=[0, 0, 0], vox_size=smp.vox_mm, rvp_first=[1, 1, 1],
+                                            tomo_dim=(nx, ny, nz), refine=mode, geo7=geo7, fit_log=fit_log)
+        wall = time.perf_counter() - t
+        grown = ds.GrainId > 0
+        r = {"wall_s": round(wall, 3), "seeds_per_s": round(n_seeds / wall, 2),
+             "seeds_indexed": int((out[:, 6] >= 0.6).sum()), "voxels_assigned": int(grown.sum()),
+             "median_C": round(float(np.nanmedian(out[:, 6])), 4)}
+        # the assigned voxels' true grains: the share in the grain most assigned per seed id
+        same = 0
+        for sid in np.unique(ds.GrainId[grown]):
+            g = gid[ds.GrainId == sid]
+            g = g[g >= 0]
+            if g.size:
+                same += int(np.bincount(g).max())
+        r["voxels_in_majority_grain"] = same
+        # orientation error of the indexed rows against the true grain at the row's position
+        err = []
+        for row in out[out[:, 6] >= 0.6]:
+            vx = np.rint((row[1:4] - smp.pos[0]) / smp.vox_mm).astype(int) + (smp.index[0] - 1)
+            if np.all(vx >= 0) and np.all(vx < smp.dims) and gid[tuple(vx)] >= 0:
+                qt = smp.q[gid[tuple(vx)]]
+                d = np.abs(synth._qmul(np.repeat(qt[None, :], sym.shape[0], 0), sym) @ row[14:18]).max()
+                err.append(np.degrees(2 * np.arccos(min(1.0, d))))
+        r["orientation_error_deg_median"] = round(float(np.median(err)), 6) if err else None
+        if fit_log:
+            r["lm_fits"] = len(fit_log)
+            r["lm_rms_px_median"] = float(np.nanmedian([e["rms_px"] for e in fit_log]))
+        res[mode] = r
+        log(f"[bench] mapping ({mode}): {n_seeds} seeds in {wall:.2f}s ({r['seeds_per_s']} seeds/s), "
+            f"{r['voxels_assigned']} voxels assigned, {same} in their seed's majority grain, median orientation "
+            f"error {r['orientation_error_deg_median']} deg")
     return res
 
 
@@ -627,13 +804,16 @@ def main():
     ap.add_argument("--host-io-calls", type=int, default=3, help="seeding calls with host buffers; 0 = off")
     ap.add_argument("--legacy-calls", type=int, default=2, help="calls of the legacy f32 entry; 0 = off")
     ap.add_argument("--geo-fit-grains", type=int, default=300, help="grains in the config-4 fit leg; 0 = off")
-    ap.add_argument("--geo-fit-nm-grains", type=int, default=20, help="grains in the Nelder-Mead comparison")
+    ap.add_argument("--geo-fit-nm-grains", type=int, default=300,
+                    help="grains in the comparison with the reference FitAllOnce Nelder-Mead procedure; 0 = off")
     ap.add_argument("--index-positions", type=int, default=8, help="seed positions per indexing call; 0 = off")
-    ap.add_argument("--index-steps", type=int, default=2)
+    ap.add_argument("--index-steps", type=int, default=4)
     ap.add_argument("--index-nr-seed", type=int, default=500)
     ap.add_argument("--gt-orientations", type=int, default=8192, help="orientations of the Gt-score leg; 0 = off")
     ap.add_argument("--grow-box", type=int, default=24, help="sphere radius (voxels) of the region-growth leg; 0 = off")
     ap.add_argument("--mapping-seeds", type=int, default=6, help="seeds of the mapping-loop leg; 0 = off")
+    ap.add_argument("--roofline-audit-evals", type=int, default=32768,
+                    help="orientations per seed of the audit launch that counts the roofline's units")
     ap.add_argument("--audit-evals", type=int, default=0,
                     help="orientations per seed of the resolution-stage audit leg; 0 = off")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
@@ -710,20 +890,8 @@ def main():
         f"kernel {pms / max(pn, 1):.3f} ms/launch")
 
     nrot, H = cfg.rot.size, cfg.hkl.shape[0]
-    fe = flop_per_eval(nrot, H)
-    achieved = N * fe / (kms / kn * 1e-3) / 1e12
-    traffic = TRAFFIC_BYTES_PER_EVAL * N if args.config == "cfg2" else None
-    roofline = {"bound": "mfma", "compute": "fp64 vector ALU (no GEMM structure on this path)",
-                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "traffic_GB_per_s": round(traffic / (kms / kn * 1e-3) / 1e9, 1) if traffic else None,
-                "achieved_note": ("algorithmic: the full forward_comp.m work of every evaluation; the omega solve and "
-                                  "the resolution stage execute about 9% of it (counter profile)"),
-                "kernel": "k_fast", "kernel_ms": round(kms / kn, 4), "flop_per_eval": fe,
-                "executed_fp64_per_eval": EXECUTED_FP64_PER_EVAL if args.config == "cfg2" else None,
-                "executed_tflops": (round(N * EXECUTED_FP64_PER_EVAL / (kms / kn * 1e-3) / 1e12, 3)
-                                    if args.config == "cfg2" else None),
-                "evals_per_launch": N}
+    roofline = roofline_leg(eng, cfg, q, seeds[args.warmup:args.warmup + 2], kms / kn, N, args.config,
+                            args.roofline_audit_evals)
 
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
@@ -779,7 +947,10 @@ def main():
         if batched:
             bbatch, bout, nrg = batched
             sizes = np.diff(bbatch[1])
-            rows = [int(r) for r in np.argsort(sizes, kind="stable")[:3]]
+            # the longest row and five rows drawn at random
+            longest = int(np.argmax(sizes))
+            rest = np.delete(np.arange(sizes.size), longest)
+            rows = [longest] + [int(r) for r in np.random.default_rng(3003).choice(rest, size=5, replace=False)]
             t = time.perf_counter()
             ok = check_batched_rows(cfg, proj, bbatch, bout, nrg, rows, threads)
             result["batched"]["parity"] = {"rows_checked": rows, "evals_checked": int(sizes[rows].sum()) * 343,

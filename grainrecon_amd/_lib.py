@@ -48,6 +48,7 @@ _SIGS = {
                                 ctypes.POINTER(c_i64), c_dp, c_ip, c_ip, c_dp, c_fp]),
     "grs_set_spots": (c_i32, [c_vp, c_i64, c_dp, c_dp, c_dp, c_ip, c_i64]),
     "grs_spot_residual": (c_i32, [c_vp, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]),
+    "grs_spot_normal": (c_i32, [c_vp, c_dp, c_dp, ctypes.c_double, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]),
     "grs_cuda_forward_comp": (c_i32, [c_fp, c_fp, c_i64, c_fp, c_fp, c_fp, c_i64, c_fp, c_i64, c_fp, c_i64, c_fp,
                                       c_fp, c_i64, c_fp]),
     "grs_cuda_grow_indexed_region": (c_i32, [c_fp, c_i64, c_fp, c_fp, c_fp, c_fp, c_i64, c_fp, c_i64, c_fp, c_i64,

@@ -53,7 +53,7 @@ struct grs_ctx {
   DevBuf c_cen, c_grid, c_gn, c_out, c_n, c_gid, c_vox, c_ids, c_nout;
   bool bits_valid = false; /* hit mask matches the current stack and threshold */
   /* resident matched-spot set of the fits */
-  DevBuf f_hkl, f_om, f_meas, f_grain, f_goff, f_gr6, f_up, f_res, f_contrib, f_jac, f_out;
+  DevBuf f_hkl, f_om, f_meas, f_grain, f_goff, f_gr6, f_up, f_res, f_contrib, f_jac, f_out, f_ne;
   int64_t f_n = 0, f_G = 0;
   int64_t mnz = 0, mny = 0, mnrot = 0; /* shape of d_mask */
   bool timing = false;
@@ -255,7 +255,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->d_bad, &c->d_x, &c->d_y, &c->d_mask, &c->d_g1, &c->d_edt_v, &c->d_edt_z,
                     &c->d_grain, &c->d_bits, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest,
                     &c->d_elist, &c->f_hkl, &c->f_om, &c->f_meas, &c->f_grain, &c->f_goff, &c->f_gr6,
-                    &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out, &c->c_cen, &c->c_grid,
+                    &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out, &c->f_ne, &c->c_cen, &c->c_grid,
                     &c->c_gn, &c->c_out, &c->c_n, &c->c_gid, &c->c_vox, &c->c_ids, &c->c_nout, &c->d_audit,
                     &c->d_bestidx, &c->d_bestq, &c->o_q, &c->o_sub, &c->o_valid, &c->x_pos, &c->x_vb,
                     &c->x_key1, &c->x_ns1, &c->x_ni1, &c->x_top, &c->x_cen, &c->x_grid, &c->x_gn, &c->x_ls,
@@ -1162,6 +1162,33 @@ static void det_rot_and_derivs(const double *t, double *R, double dR[3][9]) {
   mm(T, X, dR[2]);
 }
 
+/* the constants of the residual kernels (grs_fitc in grs_fit.hip) for geometry geo7 */
+struct host_fitc {
+  double S[9], B[9];
+  double P0y, P0z, RAO, py, pz, dety0, detz0;
+  double Lss, Lsd, dy00, dz00;
+  double R[9];
+  double dR[3][9];
+};
+static_assert(sizeof(host_fitc) == 8 * (18 + 7 + 4 + 9 + 27), "layout");
+
+static void fill_fitc(const grs_ctx *c, const double *geo7, host_fitc *f) {
+  std::memcpy(f->S, c->hgeo.S, sizeof(f->S));
+  std::memcpy(f->B, c->hgeo.B, sizeof(f->B));
+  f->P0y = c->hgeo.P0y;
+  f->P0z = c->hgeo.P0z;
+  f->RAO = c->hgeo.RAO;
+  f->py = c->hgeo.py;
+  f->pz = c->hgeo.pz;
+  f->dety0 = c->hgeo.dety0;
+  f->detz0 = c->hgeo.detz0;
+  f->Lss = geo7[0];
+  f->Lsd = geo7[1];
+  f->dy00 = geo7[2];
+  f->dz00 = geo7[3];
+  det_rot_and_derivs(geo7 + 4, f->R, f->dR);
+}
+
 grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grains6, double *resid,
                              const double *upstream, double *grad7, double *grad6, double *jac) {
   if (!c) return GRS_ERR_ARG;
@@ -1170,30 +1197,9 @@ grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grain
   if (!geo7 || (G > 0 && !grains6) || (n > 0 && !resid)) return fail(c, GRS_ERR_ARG, "null argument");
   if ((grad7 == nullptr) != (grad6 == nullptr)) return fail(c, GRS_ERR_ARG, "grad7 and grad6 go together");
   HIPCHK(c, hipSetDevice(c->device));
-  std::vector<char> fc(grs_fitc_size());
-  struct {
-    double S[9], B[9];
-    double P0y, P0z, RAO, py, pz, dety0, detz0;
-    double Lss, Lsd, dy00, dz00;
-    double R[9];
-    double dR[3][9];
-  } f;
-  static_assert(sizeof(f) == 8 * (18 + 7 + 4 + 9 + 27), "layout");
-  if (grs_fitc_size() != sizeof(f)) return fail(c, GRS_ERR_ARG, "internal layout mismatch");
-  std::memcpy(f.S, c->hgeo.S, sizeof(f.S));
-  std::memcpy(f.B, c->hgeo.B, sizeof(f.B));
-  f.P0y = c->hgeo.P0y;
-  f.P0z = c->hgeo.P0z;
-  f.RAO = c->hgeo.RAO;
-  f.py = c->hgeo.py;
-  f.pz = c->hgeo.pz;
-  f.dety0 = c->hgeo.dety0;
-  f.detz0 = c->hgeo.detz0;
-  f.Lss = geo7[0];
-  f.Lsd = geo7[1];
-  f.dy00 = geo7[2];
-  f.dz00 = geo7[3];
-  det_rot_and_derivs(geo7 + 4, f.R, f.dR);
+  if (grs_fitc_size() != sizeof(host_fitc)) return fail(c, GRS_ERR_ARG, "internal layout mismatch");
+  host_fitc f;
+  fill_fitc(c, geo7, &f);
   if (n == 0) {
     if (grad7) {
       for (int k = 0; k < 7; k++) grad7[k] = 0.0;
@@ -1241,6 +1247,59 @@ grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grain
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->ms_total += ms;
     c->launches += 1;
+  }
+  return GRS_OK;
+}
+
+grs_status grs_spot_normal(grs_ctx *c, const double *geo7, const double *grains6, double delta, double *Agg,
+                           double *bg, double *Akk, double *Agk, double *bk, double *sums) {
+  if (!c) return GRS_ERR_ARG;
+  if (!c->geo_set) return fail(c, GRS_ERR_STATE, "geometry not set");
+  const int64_t n = c->f_n, G = c->f_G;
+  if (!geo7 || !Agg || !bg || !sums || (G > 0 && (!grains6 || !Akk || !Agk || !bk)))
+    return fail(c, GRS_ERR_ARG, "null argument");
+  if (!(delta >= 0.0) || !std::isfinite(delta)) return fail(c, GRS_ERR_ARG, "delta must be finite and >= 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (grs_fitc_size() != sizeof(host_fitc)) return fail(c, GRS_ERR_ARG, "internal layout mismatch");
+  host_fitc f;
+  fill_fitc(c, geo7, &f);
+  double out37[37] = {0};
+  std::vector<double> pg((size_t)69 * (G > 0 ? G : 1), 0.0);
+  if (n > 0 && G > 0) {
+    ENSURE(c, c->f_gr6, sizeof(double) * 6 * G);
+    ENSURE(c, c->f_ne, sizeof(double) * (69 * G + 37 * G + 37));
+    HIPCHK(c, hipMemcpyAsync(c->f_gr6.p, grains6, sizeof(double) * 6 * G, hipMemcpyHostToDevice, c->st));
+    double *d_pg = P<double>(c->f_ne), *d_part = d_pg + 69 * G, *d_out = d_part + 37 * G;
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, grs_launch_spot_normal(&f, G, P<double>(c->f_hkl), P<double>(c->f_om), P<double>(c->f_meas),
+                                     P<int64_t>(c->f_goff), P<double>(c->f_gr6), delta, d_pg, d_part, d_out, c->st));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipMemcpyAsync(pg.data(), d_pg, sizeof(double) * 69 * G, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(out37, d_out, sizeof(double) * 37, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (c->timing) {
+      float ms = 0.0f;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->ms_total += ms;
+      c->launches += 1;
+    }
+  }
+  /* expand the upper triangles */
+  int k = 69;
+  for (int a = 0; a < 7; a++)
+    for (int b = a; b < 7; b++, k++) Agg[7 * a + b] = Agg[7 * b + a] = out37[k - 69];
+  for (int a = 0; a < 7; a++) bg[a] = out37[28 + a];
+  sums[0] = out37[35];
+  sums[1] = out37[36];
+  for (int64_t g = 0; g < G; g++) {
+    const double *src = pg.data() + 69 * g;
+    double *A = Akk + 36 * g;
+    int m = 0;
+    for (int a = 0; a < 6; a++)
+      for (int b = a; b < 6; b++, m++) A[6 * a + b] = A[6 * b + a] = src[m];
+    for (int a = 0; a < 7; a++)
+      for (int b = 0; b < 6; b++, m++) Agk[42 * g + 6 * a + b] = src[m];
+    for (int b = 0; b < 6; b++, m++) bk[6 * g + b] = src[m];
   }
   return GRS_OK;
 }

@@ -301,6 +301,115 @@ __global__ __launch_bounds__(256) void k_geo_reduce(const double *contrib, int64
   if (tid < 7) out7[tid] = part[tid][0];
 }
 
+/* Gauss-Newton normal equations of the block-arrow system, one block per grain
+ * (spots sorted by grain).  With the extended rows E = [J(13) r] of a spot's
+ * two residual components, scaled by sqrt(w), every output is
+ * sum over the grain's spots of E_y[p] E_y[q] + E_z[p] E_z[q] for a column pair
+ * (p, q): the grain block Akk (upper triangle, 21), the coupling Agk (7 x 6),
+ * bk (6), and the grain's share of Agg (upper triangle, 28) and bg (7).
+ * w = 1 / sqrt(|r|^2 + delta^2) and rho = sqrt(|r|^2 + delta^2) - delta
+ * (Charbonnier) when delta > 0, else w = 1 and rho = |r|^2 / 2.  The NE_T
+ * threads compute NE_T spots' rows into LDS, then thread o adds output o over
+ * them in spot order, so the sums are reproducible. */
+#define GRS_NE_T 128
+#define GRS_NE_PAIRS 104
+#define GRS_NE_ROW 30 /* sqrt(w) E_y (14), sqrt(w) E_z (14), rho, |r|^2 */
+
+__device__ __forceinline__ void ne_pair(int o, int *p, int *q) {
+  int k = 0;
+  for (int a = 0; a < 6; a++)
+    for (int b = a; b < 6; b++, k++)
+      if (k == o) { *p = 7 + a; *q = 7 + b; return; }
+  for (int a = 0; a < 7; a++)
+    for (int b = 0; b < 6; b++, k++)
+      if (k == o) { *p = a; *q = 7 + b; return; }
+  for (int b = 0; b < 6; b++, k++)
+    if (k == o) { *p = 7 + b; *q = 13; return; }
+  for (int a = 0; a < 7; a++)
+    for (int b = a; b < 7; b++, k++)
+      if (k == o) { *p = a; *q = b; return; }
+  for (int a = 0; a < 7; a++, k++)
+    if (k == o) { *p = a; *q = 13; return; }
+  *p = 0;
+  *q = 0;
+}
+
+__global__ __launch_bounds__(GRS_NE_T) void k_spot_normal(grs_fitc f, const double *hkl, const double *omega,
+                                                          const double *meas, const int64_t *goff, const double *gr6,
+                                                          double delta, double *pergrain, double *partial) {
+  __shared__ double rows[GRS_NE_T][GRS_NE_ROW];
+  const int64_t g = blockIdx.x;
+  const int t = threadIdx.x;
+  int p = 0, q = 0;
+  if (t < GRS_NE_PAIRS) ne_pair(t, &p, &q);
+  const double *par = gr6 + 6 * g;
+  double acc = 0.0;
+  for (int64_t base = goff[g]; base < goff[g + 1]; base += GRS_NE_T) {
+    const int64_t i = base + t;
+    const int cnt = goff[g + 1] - base < GRS_NE_T ? (int)(goff[g + 1] - base) : GRS_NE_T;
+    if (i < goff[g + 1]) {
+      const double om = omega[i] * (GRS_PI / 180.0);
+      const double c = grs_cos(om), s = grs_sin(om);
+      SpotFwd v;
+      spot_forward(f, par, par + 3, hkl + 3 * i, c, s, v);
+      const double ry = -v.Y / f.py + f.dety0 - meas[2 * i];
+      const double rz = -v.Z / f.pz + f.detz0 - meas[2 * i + 1];
+      double jy[13], jz[13];
+      spot_reverse(f, par, hkl + 3 * i, c, s, v, 1.0, 0.0, jy);
+      spot_reverse(f, par, hkl + 3 * i, c, s, v, 0.0, 1.0, jz);
+      const double r2 = ry * ry + rz * rz;
+      double sw = 1.0, rho = 0.5 * r2;
+      if (delta > 0.0) {
+        const double h = __builtin_sqrt(r2 + delta * delta);
+        sw = 1.0 / __builtin_sqrt(h);
+        rho = h - delta;
+      }
+      double *row = rows[t];
+      for (int k = 0; k < 13; k++) {
+        row[k] = sw * jy[k];
+        row[14 + k] = sw * jz[k];
+      }
+      row[13] = sw * ry;
+      row[27] = sw * rz;
+      row[28] = rho;
+      row[29] = r2;
+    }
+    __syncthreads();
+    if (t < GRS_NE_PAIRS) {
+      for (int j = 0; j < cnt; j++) acc += rows[j][p] * rows[j][q] + rows[j][14 + p] * rows[j][14 + q];
+    } else if (t < GRS_NE_PAIRS + 2) {
+      const int col = t == GRS_NE_PAIRS ? 28 : 29;
+      for (int j = 0; j < cnt; j++) acc += rows[j][col];
+    }
+    __syncthreads();
+  }
+  /* outputs 0..68: Akk, Agk, bk of the grain; 69..103 and the two sums: its share of Agg, bg, rho, |r|^2 */
+  if (t < 69)
+    pergrain[69 * g + t] = acc;
+  else if (t < GRS_NE_PAIRS + 2)
+    partial[37 * g + (t - 69)] = acc;
+}
+
+/* the grains' shares of Agg, bg and the two sums added in grain order */
+__global__ void k_normal_sum(const double *partial, int64_t G, double *out37) {
+  const int j = threadIdx.x;
+  if (j >= 37) return;
+  double acc = 0.0;
+  for (int64_t g = 0; g < G; g++) acc += partial[37 * g + j];
+  out37[j] = acc;
+}
+
+hipError_t grs_launch_spot_normal(const void *fitc, int64_t G, const double *hkl, const double *omega,
+                                  const double *meas, const int64_t *goff, const double *gr6, double delta,
+                                  double *pergrain, double *partial, double *out37, hipStream_t st) {
+  if (G <= 0) return hipSuccess;
+  const grs_fitc &f = *static_cast<const grs_fitc *>(fitc);
+  hipLaunchKernelGGL(k_spot_normal, dim3((unsigned)G), dim3(GRS_NE_T), 0, st, f, hkl, omega, meas, goff, gr6, delta,
+                     pergrain, partial);
+  hipLaunchKernelGGL(k_normal_sum, dim3(1), dim3(64), 0, st, partial, G, out37);
+  return hipGetLastError();
+}
+
 hipError_t grs_launch_spot_residual(const void *fitc, int64_t n, const double *hkl, const double *omega,
                                     const double *meas, const int32_t *grain, const double *gr6, const double *up,
                                     double *res, double *contrib, double *jac, hipStream_t st) {

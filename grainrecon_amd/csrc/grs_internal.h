@@ -143,6 +143,9 @@ hipError_t grs_launch_spot_residual(const void *fitc, int64_t n, const double *h
                                     double *res, double *contrib, double *jac, hipStream_t st);
 hipError_t grs_launch_grad_reduce(const double *contrib, int64_t n, const int64_t *goff, int64_t G, double *out7,
                                   double *out6, hipStream_t st);
+hipError_t grs_launch_spot_normal(const void *fitc, int64_t G, const double *hkl, const double *omega,
+                                  const double *meas, const int64_t *goff, const double *gr6, double delta,
+                                  double *pergrain, double *partial, double *out37, hipStream_t st);
 size_t grs_fitc_size(void);
 
 /* Wave slots the fast kernel keeps resident (its scratch is sized from this). */

@@ -71,6 +71,19 @@ class SpotFit:
         r, _, _, J = self._call(g7, gr6, jac=True)
         return r, J
 
+    def normal(self, g7, gr6, delta: float = 0.0) -> dict:
+        """Normal equations of the (weighted) residual reduced on the device
+        (grs_spot_normal): Agg (7, 7), bg (7,), Akk (G, 6, 6), Agk (G, 7, 6),
+        bk (G, 6), cost (sum rho) and sumsq (sum |r|^2)."""
+        g7 = _d(np.asarray(g7, float).reshape(7))
+        gr6 = _d(np.asarray(gr6, float).reshape(self.G, 6))
+        Agg, bg, sums = np.zeros((7, 7)), np.zeros(7), np.zeros(2)
+        Akk, Agk, bk = np.zeros((self.G, 6, 6)), np.zeros((self.G, 7, 6)), np.zeros((self.G, 6))
+        self.eng._chk(self.eng._lib.grs_spot_normal(
+            self.eng._h, _ptr(g7, _dp), _ptr(gr6, _dp), float(delta), _ptr(Agg, _dp), _ptr(bg, _dp),
+            _ptr(Akk, _dp), _ptr(Agk, _dp), _ptr(bk, _dp), _ptr(sums, _dp)))
+        return dict(Agg=Agg, bg=bg, Akk=Akk, Agk=Agk, bk=bk, cost=float(sums[0]), sumsq=float(sums[1]))
+
 
 def rms_px(r: np.ndarray) -> float:
     """Root mean square Euclidean distance (pixels) of residual pairs."""
@@ -95,70 +108,106 @@ class FitResult:
     history: list
 
 
+def host_normal(fit, g7, gr6, delta: float = 0.0) -> dict:
+    """The normal equations of grs_spot_normal computed on the host from the
+    Jacobian rows (fit.jacobian): for fit objects without a device reduction
+    and as its check."""
+    r, J = fit.jacobian(g7, gr6)
+    G, goff = fit.G, fit.goff
+    s2 = np.sum(r * r, axis=1)
+    if delta > 0.0:
+        h = np.sqrt(s2 + delta * delta)
+        cost = float(np.sum(h - delta))
+        sw = 1.0 / np.sqrt(h)
+        r = r * sw[:, None]
+        J = J * sw[:, None, None]
+    else:
+        cost = 0.5 * float(np.sum(s2))
+    Jg = J[:, :, :7].reshape(-1, 7)
+    Jk = J[:, :, 7:]
+    Agg = Jg.T @ Jg
+    bg = Jg.T @ r.reshape(-1)
+    Akk = np.zeros((G, 6, 6))
+    Agk = np.zeros((G, 7, 6))
+    bk = np.zeros((G, 6))
+    nz = goff[:-1] < goff[1:]
+    if fit.n:
+        Akk[nz] = np.add.reduceat(np.einsum("nai,naj->nij", Jk, Jk), goff[:-1][nz], axis=0)
+        Agk[nz] = np.add.reduceat(np.einsum("nai,naj->nij", J[:, :, :7], Jk), goff[:-1][nz], axis=0)
+        bk[nz] = np.add.reduceat(np.einsum("nai,na->ni", Jk, r), goff[:-1][nz], axis=0)
+    return dict(Agg=Agg, bg=bg, Akk=Akk, Agk=Agk, bk=bk, cost=cost, sumsq=float(np.sum(s2)))
+
+
 def levenberg_marquardt(fit: SpotFit, g7, gr6, max_iter: int = 50, tol: float = 1e-10, lam: float = 1e-3,
-                        fix_geometry: bool = False, fix_grains: bool = False) -> FitResult:
+                        fix_geometry: bool = False, fix_grains: bool = False,
+                        fix_position: bool = False, delta: float = 0.0) -> FitResult:
     """Joint refinement of geometry and grains (block-arrow LM); either block
     can be held fixed (fix_grains=True is the reference's geometry-only fit,
-    L_shift_tilt_fitting.m:23-50)."""
+    L_shift_tilt_fitting.m:23-50).  fix_position holds the grain centres and
+    fits the orientations only (the free parameters of
+    fit_OR_constrain_with_local_comp_cuda.m:117-126).  delta > 0 minimises the
+    Charbonnier sum of sqrt(|r|^2 + delta^2) - delta (the mean Euclidean
+    distance the reference's objectives average, smoothed at delta pixels;
+    robust to mispaired spots) by iteratively reweighted Gauss-Newton steps;
+    delta = 0 is least squares.  The normal equations come from the device
+    (SpotFit.normal, grs_spot_normal) when the fit object has them."""
     t0 = time.perf_counter()
     g7 = np.array(g7, float)
     gr6 = np.array(gr6, float).reshape(fit.G, 6)
-    G, goff = fit.G, fit.goff
-    r, J = fit.jacobian(g7, gr6)
-    cost = 0.5 * float(np.sum(r * r))
-    hist = [rms_px(r)]
+    normal = getattr(fit, "normal", None)
+
+    def ne_at(a7, a6):
+        return normal(a7, a6, delta) if normal is not None else host_normal(fit, a7, a6, delta)
+
+    def rms(ne):
+        return float(np.sqrt(ne["sumsq"] / max(fit.n, 1)))
+
+    ne = ne_at(g7, gr6)
+    hist = [rms(ne)]
     it = 0
+    eye6 = np.eye(6)[None]
     for it in range(1, max_iter + 1):
-        Jg = J[:, :, :7].reshape(-1, 7)             # (2n, 7)
-        Jk = J[:, :, 7:]                            # (n, 2, 6)
-        rr = r.reshape(-1)
-        Agg = Jg.T @ Jg
-        bg = Jg.T @ rr
-        # per-spot outer products, summed per grain
-        Akk_s = np.einsum("nai,naj->nij", Jk, Jk)
-        Agk_s = np.einsum("nai,naj->nij", J[:, :, :7], Jk)
-        bk_s = np.einsum("nai,na->ni", Jk, r)
-        nz = goff[:-1] < goff[1:]
-        Akk = np.zeros((G, 6, 6))
-        Agk = np.zeros((G, 7, 6))
-        bk = np.zeros((G, 6))
-        if fit.n:
-            Akk[nz] = np.add.reduceat(Akk_s, goff[:-1][nz], axis=0)
-            Agk[nz] = np.add.reduceat(Agk_s, goff[:-1][nz], axis=0)
-            bk[nz] = np.add.reduceat(bk_s, goff[:-1][nz], axis=0)
+        Agg, bg = ne["Agg"], ne["bg"]
+        Akk, Agk, bk = ne["Akk"].copy(), ne["Agk"].copy(), ne["bk"].copy()
+        if fix_position:
+            # decouple the centre block: its step solves I dk = 0
+            Akk[:, 3:, :] = 0.0
+            Akk[:, :, 3:] = 0.0
+            Akk[:, 3:, 3:] = np.eye(3)
+            Agk[:, :, 3:] = 0.0
+            bk[:, 3:] = 0.0
         while True:
             Dg = np.diag(np.diag(Agg)) * lam + np.eye(7) * 1e-30
-            Ak = Akk + lam * np.einsum("gii->gi", Akk)[:, :, None] * np.eye(6)[None] + 1e-30 * np.eye(6)[None]
-            Ak_inv = np.linalg.inv(Ak)
+            Ak = Akk + lam * np.einsum("gii->gi", Akk)[:, :, None] * eye6 + 1e-30 * eye6
             if fix_geometry:
                 dg = np.zeros(7)
             elif fix_grains:
                 dg = np.linalg.solve(Agg + Dg, -bg)
             else:
+                Ak_inv = np.linalg.inv(Ak)
                 Sg = Agg + Dg - np.einsum("gij,gjk,glk->il", Agk, Ak_inv, Agk)
                 rhs = -(bg - np.einsum("gij,gjk,gk->i", Agk, Ak_inv, bk))
                 dg = np.linalg.solve(Sg, rhs)
             if fix_grains:
                 dk = np.zeros_like(gr6)
             else:
-                dk = -np.einsum("gij,gj->gi", Ak_inv, bk + np.einsum("gji,j->gi", Agk, dg))
+                dk = -np.linalg.solve(Ak, (bk + np.einsum("gji,j->gi", Agk, dg))[:, :, None])[:, :, 0]
             g7n, gr6n = g7 + dg, gr6 + dk
-            rn, Jn = fit.jacobian(g7n, gr6n)
-            costn = 0.5 * float(np.sum(rn * rn))
-            if np.isfinite(costn) and costn <= cost:
+            nen = ne_at(g7n, gr6n)
+            if np.isfinite(nen["cost"]) and nen["cost"] <= ne["cost"]:
                 lam = max(lam / 10.0, 1e-12)
                 break
             lam *= 10.0
             if lam > 1e12:
                 break
-        if not (np.isfinite(costn) and costn <= cost):
+        if not (np.isfinite(nen["cost"]) and nen["cost"] <= ne["cost"]):
             break
-        rel = (cost - costn) / max(cost, 1e-300)
-        g7, gr6, r, J, cost = g7n, gr6n, rn, Jn, costn
-        hist.append(rms_px(r))
+        rel = (ne["cost"] - nen["cost"]) / max(ne["cost"], 1e-300)
+        g7, gr6, ne = g7n, gr6n, nen
+        hist.append(rms(ne))
         if rel < tol:
             break
-    return FitResult(g7=g7, gr6=gr6, rms=rms_px(r), iterations=it, seconds=time.perf_counter() - t0, history=hist)
+    return FitResult(g7=g7, gr6=gr6, rms=rms(ne), iterations=it, seconds=time.perf_counter() - t0, history=hist)
 
 
 def torch_residual():

@@ -22,9 +22,106 @@ import math
 
 import numpy as np
 
-from .engine import euler2u_deg, euler_to_rod_quat
+from .engine import euler2u_deg, euler_to_rod_quat, fit_orientations
+from .fit import SpotFit, levenberg_marquardt
 from .geometry import quaternion2U
 from .indexing import index_seed_fit_all, pos2ind
+
+REFINE_NELDER_MEAD = "nelder-mead"
+REFINE_ADJOINT = "adjoint"
+
+
+# smoothing length (pixels) of the Charbonnier loss of the adjoint fits: the
+# loss is the Euclidean distance the reference's objectives average
+# (fit_grain_COM.m:84-86), quadratic within about this distance
+LM_DELTA_PX = 0.5
+
+
+def _lm_grain(eng, geo7, rows, rod, pos, fix_position):
+    """Levenberg-Marquardt on the unrounded centroid residual of one grain's
+    paired spots (grs_spot_normal: the hand-written adjoint Jacobian reduced
+    on the device), geometry held, Charbonnier loss (LM_DELTA_PX) so that a
+    spot paired with a neighbour grain's spot weighs like one distance, not
+    its square: rows are grs_pair_spots rows (hkl 12:15, rotation angle 16,
+    spot centroid 17:19).  Returns ([rod(3) pos(3)], FitResult)."""
+    n = rows.shape[0]
+    fit = SpotFit(eng, rows[:, 12:15], rows[:, 16], rows[:, 17:19], np.zeros(n, np.int32), 1)
+    gr6 = np.concatenate([np.asarray(rod, float), np.asarray(pos, float)])[None, :]
+    r = levenberg_marquardt(fit, geo7, gr6, fix_geometry=True, fix_position=fix_position, delta=LM_DELTA_PX)
+    return r.gr6[0], r
+
+
+def _rod_to_x(rod):
+    """The vector part of the unit quaternion (q0 > 0) of a Rodrigues vector."""
+    rod = np.asarray(rod, float)
+    return rod / math.sqrt(1.0 + float(rod @ rod))
+
+
+# at least this many paired spots (2 residuals each) before a 3- or
+# 6-parameter least-squares fit is attempted
+_LM_MIN_SPOTS = 4
+
+
+def index_seed_adjoint(eng, geo7, pos, nr_seed, seed_number):
+    """fit_OR_constrain_with_local_comp_cuda.m with its fminsearchbnd of
+    1 - completeness over the quaternion's vector part (:115-126) replaced:
+    the local sample of largest completeness (the reference's start, x0 of
+    grs_index_seeds, whose Nelder-Mead is cut to its first simplex here) is
+    paired with the spots (forward_comp_pair_spots.m), its Rodrigues vector is
+    fitted to those spots' centroids by Levenberg-Marquardt with the position
+    held (once more if the fitted orientation pairs other spots), and the
+    orientation and verification counts follow as in :128-153.  Returns (indexing row (18,), candidate, fit info dict)."""
+    res = eng.index_seeds(pos[None, :], nr_seed=nr_seed, max_fun_evals=1, max_iter=1)
+    x0 = res["x0"][0]
+    q0 = math.sqrt(max(0.0, 1.0 - float(x0 @ x0)))
+    U0 = quaternion2U(np.concatenate([[q0], x0]))
+    _, _, _, pr = eng.pair_spots(pos[None, :], U0[None, :, :])
+    rows = pr[0]
+    info = {"n_pairs": int(rows.shape[0]), "lm_iterations": 0, "rms_px": math.nan}
+    x = x0
+    if rows.shape[0] >= _LM_MIN_SPOTS and q0 > 0.0:
+        rod = x0 / q0
+        for _ in range(2):
+            g, r = _lm_grain(eng, geo7, rows, rod, pos, fix_position=True)
+            info.update(lm_iterations=info["lm_iterations"] + r.iterations, rms_px=r.rms)
+            rod = g[:3]
+            # the pairs of the fitted orientation; one more fit if they changed
+            _, _, _, pr = eng.pair_spots(pos[None, :], quaternion2U(np.concatenate([[1.0], rod]) /
+                                                                     math.sqrt(1.0 + float(rod @ rod)))[None])
+            if pr[0].shape[0] == rows.shape[0] and np.array_equal(pr[0][:, 17:19], rows[:, 17:19]):
+                break
+            rows = pr[0]
+            info["n_pairs"] = int(rows.shape[0])
+            if rows.shape[0] < _LM_MIN_SPOTS:
+                break
+        x = _rod_to_x(rod)
+    o10, U, _ = fit_orientations(x[None, :])
+    ns, ni, med, _ = eng.pair_spots(pos[None, :], U)
+    row = np.zeros(18)
+    row[0] = seed_number
+    row[1:4] = pos
+    row[4], row[5] = ns[0], ni[0]
+    row[6] = ni[0] / ns[0] if ns[0] else math.nan
+    row[7] = float(med[0])
+    row[8:18] = o10[0]
+    return row, float(np.asarray(res["candidate"]).reshape(-1)[0]), info
+
+
+def fit_grain_com_adjoint(eng, geo7, pairs, rod):
+    """fit_grain_COM.m:5-29 with its fminsearchbnd of the mean rounded pixel
+    distance over [position, Euler angles] replaced by Levenberg-Marquardt on
+    the unrounded centroid residual over [Rodrigues vector, position], from
+    the same start (the pair rows' position, :17, and the indexed
+    orientation).  Returns FitOutput (6,) = [position, Euler ZXZ degrees] and
+    the LM result (None below the spot minimum, where the start is
+    returned)."""
+    pos0 = pairs[0, 9:12]
+    if pairs.shape[0] < _LM_MIN_SPOTS:
+        o10, _, _ = fit_orientations(_rod_to_x(rod)[None, :])
+        return np.concatenate([pos0, o10[0, :3]]), None
+    g, r = _lm_grain(eng, geo7, pairs, rod, pos0, fix_position=False)
+    o10, _, _ = fit_orientations(_rod_to_x(g[:3])[None, :])
+    return np.concatenate([g[3:], o10[0, :3]]), r
 
 
 class MappingDS:
@@ -171,7 +268,7 @@ def _download(eng, ds, vox):
 
 def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_comp, drop_off, maxD, max_dmedian,
                          nr_seed, first_grain_id, center, vox_size, rvp_first, tomo_dim, simap=False,
-                         check_fit_all=False):
+                         check_fit_all=False, refine=REFINE_NELDER_MEAD, geo7=None, fit_log=None):
     """GrainMapping_engine_gpu_comp.m:54-331.  OR_q: the orientation set
     (N, 4); ds is updated in place.  With check_fit_all, the voxels with
     Completeness >= trust_comp at entry (else >= min(1.5 min_comp,
@@ -186,7 +283,24 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
     with the voxels of GrainId > 0 at entry as the list; with check_fit_all,
     when fewer than 5% of the indexed seeds among the last 51 did, the loop
     ends (the remaining rows stay zero).  Returns (indexing_final (P, 18),
-    pos_list_new (P, 3), Nr_indexed_voxel (P,), SeedIndexing (P, 15))."""
+    pos_list_new (P, 3), Nr_indexed_voxel (P,), SeedIndexing (P, 15)).
+
+    refine selects the fits at the fit_OR_constrain_with_local_comp_cuda
+    (:87-91, :128-130) and fit_grain_COM (:140-170) call sites:
+    REFINE_NELDER_MEAD runs the reference's fminsearchbnd on the engine;
+    REFINE_ADJOINT (opt-in, needs geo7 = [Lsam2sou Lsam2det dety00 detz00
+    tilt_x tilt_y tilt_z] of the engine's geometry) runs index_seed_adjoint
+    and fit_grain_com_adjoint instead: Levenberg-Marquardt on the paired
+    spots' centroid residual with the hand-written adjoint Jacobian.  The
+    fit_all variant keeps its Nelder-Mead in both modes.  fit_log, a list,
+    receives one dict per adjoint fit (site, seed, spots, iterations, rms)."""
+    if refine not in (REFINE_NELDER_MEAD, REFINE_ADJOINT):
+        raise ValueError(f"refine must be {REFINE_NELDER_MEAD!r} or {REFINE_ADJOINT!r}")
+    adjoint = refine == REFINE_ADJOINT
+    if adjoint:
+        if geo7 is None:
+            raise ValueError("refine='adjoint' needs geo7, the geometry's 7 fit values")
+        geo7 = np.asarray(geo7, float).reshape(7)
     pos_list = np.asarray(pos_list, dtype=float).reshape(-1, 3)
     OR_q = np.asarray(OR_q, dtype=float).reshape(-1, 4)
     P = pos_list.shape[0]
@@ -233,24 +347,32 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                     raise ValueError("check_fit_all needs indexed voxels (an empty list fails in MATLAB)")
                 d = ind_gid - np.array([x + 1, y + 1, z + 1])
                 fit_all = float(np.sqrt((d * d).sum(1)).min()) > 2
+            def index_here():
+                if adjoint:
+                    row_a, cand_a, inf = index_seed_adjoint(eng, geo7, pos_ix, nr_seed, i + 1)
+                    if fit_log is not None:
+                        fit_log.append(dict(site="fit_OR_constrain", seed=i + 1, **inf))
+                    return row_a, cand_a
+                r = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
+                return r["indexing"][0].copy(), float(np.asarray(r["candidate"]).reshape(-1)[0])
+
             if fit_all:
                 # :123-127: fit_OR_constrain_with_local_comp_cuda_2 with fit_all_flag = 1
                 fa = index_seed_fit_all(eng, pos_ix, OR_q, nr_seed)
-                res = {"indexing": fa["indexing"][None, :], "candidate": fa["candidate"]}
+                row, cand = fa["indexing"].copy(), float(np.asarray(fa["candidate"]).reshape(-1)[0])
             elif rcc[i] == 0:
-                res = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
+                row, cand = index_here()
             else:
                 # :87-91: the indexing repeated on ori_local_sampling(q, 4) of the last fit
                 local = eng.local_sampling(temp[14:18], 4, 0.004)[0]
                 eng.set_orientation_set(local)
                 try:
-                    res = eng.index_seeds(pos_ix[None, :], nr_seed=nr_seed)
+                    row, cand = index_here()
                 finally:
                     eng.set_orientation_set(OR_q)
-            row = res["indexing"][0].copy()
             row[0] = i + 1
             if rcc[i] == 0:
-                cand_id[i] = float(np.asarray(res["candidate"]).reshape(-1)[0])
+                cand_id[i] = cand
                 fa_track[i] = 1.0 if fit_all else 0.0
             U = quaternion2U(row[14:18])
             _, _, _, pairs = eng.pair_spots(pos_ix[None, :], U[None, :, :])
@@ -258,8 +380,15 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
             ind_final[i] = row
             if rcc[i] == 0 and pairs.shape[0]:
                 # :140-170: grain centre of mass from the paired spots
-                fit_out, _, _, _ = eng.fit_grain_com([pairs])
-                fo = fit_out[0]
+                if adjoint:
+                    fo, lm = fit_grain_com_adjoint(eng, geo7, pairs, row[11:14])
+                    if fit_log is not None:
+                        fit_log.append(dict(site="fit_grain_COM", seed=i + 1, n_pairs=int(pairs.shape[0]),
+                                            lm_iterations=lm.iterations if lm else 0,
+                                            rms_px=lm.rms if lm else math.nan))
+                else:
+                    fit_out, _, _, _ = eng.fit_grain_com([pairs])
+                    fo = fit_out[0]
                 d = fo[:3] - ind_final[i, 1:4]
                 if math.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]) / vox_size > maxD:
                     tx, ty, tz = (v - 1 for v in p2i(fo[:3]))

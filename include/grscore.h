@@ -203,6 +203,20 @@ grs_status grs_set_spots(grs_ctx *ctx, int64_t n, const double *hkl, const doubl
 grs_status grs_spot_residual(grs_ctx *ctx, const double *geo7, const double *grains6, double *resid,
                              const double *upstream, double *grad7, double *grad6, double *jac);
 
+/* Gauss-Newton normal equations of the same residual for a Levenberg-Marquardt
+ * step over geo7 and grains6, reduced on the device in a fixed order: Agg
+ * (7 x 7) = sum w Jg^T Jg and bg (7) = sum w Jg^T r over all spots; per grain
+ * g, Akk (G x 6 x 6) = sum w Jk^T Jk, Agk (G x 7 x 6) = sum w Jg^T Jk and bk
+ * (G x 6) = sum w Jk^T r over its spots (row-major blocks); sums = [sum rho,
+ * sum |r|^2].  delta > 0 weighs a spot by w = 1/sqrt(|r|^2 + delta^2) and
+ * rho = sqrt(|r|^2 + delta^2) - delta (a smooth stand-in for the Euclidean
+ * distance that geo_fitting.m:70-78 and fit_grain_COM.m:84-86 average);
+ * delta = 0 gives least squares (w = 1, rho = |r|^2 / 2).  Replaces, with
+ * the adjoint, the objective evaluations of fminsearchbnd in
+ * L_shift_tilt_fitting.m:23-50 and fit_grain_COM.m:5-29. */
+grs_status grs_spot_normal(grs_ctx *ctx, const double *geo7, const double *grains6, double delta, double *Agg,
+                           double *bg, double *Akk, double *Agk, double *bk, double *sums);
+
 /* ---- legacy single-precision MEX operators (stateless) ------------------ */
 
 /* cuda_forward_comp(pos, q, S, B, hkl, proj, rot, RotDet, param):

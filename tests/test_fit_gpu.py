@@ -121,3 +121,22 @@ def test_geometry_only_fit_with_grains_held(problem):
     assert np.array_equal(res.gr6, problem["gr6"])
     assert res.rms < 1e-6
     assert np.abs(res.g7 - problem["g7"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("delta", [0.0, 0.5])
+def test_device_normal_equations_equal_the_host_reduction(problem, delta):
+    """grs_spot_normal (per-grain blocks and the shared 7 x 7 system reduced on
+    the device) against the same sums formed on the host from the Jacobian
+    rows of grs_spot_residual, least squares and Charbonnier weights."""
+    rng = np.random.default_rng(8)
+    g7, gr6 = _perturbed(problem, rng, 0.3)
+    fit = problem["fit"]
+    dev = fit.normal(g7, gr6, delta)
+    host = F.host_normal(fit, g7, gr6, delta)
+    for k in ("Agg", "bg", "Akk", "Agk", "bk"):
+        scale = max(np.abs(host[k]).max(), 1e-300)
+        assert np.abs(dev[k] - host[k]).max() <= 1e-12 * scale, k
+    for k in ("cost", "sumsq"):
+        assert abs(dev[k] - host[k]) <= 1e-12 * abs(host[k]), k
+    np.testing.assert_array_equal(dev["Agg"], dev["Agg"].T)
+    np.testing.assert_array_equal(dev["Akk"], np.transpose(dev["Akk"], (0, 2, 1)))
