@@ -531,6 +531,18 @@ def index_leg(eng, q, smp, n_pos, nr_seed, steps, seed=5005, turn_deg=0.05):
         d_b[k] = float(np.min((np.sum((pos_s[k] - s2) ** 2, 1) - np.sum((pos_s[k] - s1[k]) ** 2)) / (2 * n2)))
     miss = mis >= 0.1
     interior = d_b > 0.02
+    # what the procedure sees of the own grain: the best completeness on the
+    # local-sampling grid (nr 8, step 0.004) around the own grain's entry of the
+    # set, against the completeness of the start the fit of the chosen group
+    # began from (the best grid sample of the winning seed)
+    x0 = np.concatenate([r["x0"] for r in res])
+    own_grid_best = np.full(mis.size, np.nan)
+    start_C = np.full(mis.size, np.nan)
+    for k in np.flatnonzero(miss):
+        ls = eng.local_sampling(qset[q.shape[0] + own[k]][None, :], 8, 0.004)[0]
+        own_grid_best[k] = float(np.nanmax(eng.forward_comp(pos_s[k], ls)[2]))
+        v = x0[k]
+        start_C[k] = float(eng.forward_comp(pos_s[k], np.concatenate([[np.sqrt(max(0.0, 1 - v @ v))], v])[None, :])[2][0])
     out = {"positions_per_call": n_pos, "calls": steps, "nr_seed": nr_seed, "orientations": N,
            "ms_per_call": round(dt / steps * 1e3, 2), "seeds_per_s": round(steps * n_pos / dt, 2),
            "first_call_s": round(first, 3), "graph_replays": int(r1 - r0), "graph_captures": int(c1 - c0),
@@ -541,12 +553,18 @@ def index_leg(eng, q, smp, n_pos, nr_seed, steps, seed=5005, turn_deg=0.05):
            "interior_own_grain_within_0.1deg": int((~miss & interior).sum()),
            "misses_with_fitted_C_ge_own_grain_C": int((miss & (C >= own_C)).sum()),
            "misses_with_fitted_C_below_own_grain_C": int((miss & (C < own_C)).sum()),
+           "misses_whose_own_grid_best_C_is_below_the_chosen_start_C": int((miss & (own_grid_best < start_C)).sum()),
+           "misses_own_grid_best_C": [round(float(v), 4) for v in own_grid_best[miss]],
+           "misses_chosen_start_C": [round(float(v), 4) for v in start_C[miss]],
            "any_grain_within_0.1deg": int((mis_any < 0.1).sum()),
            "boundary_distance_mm_of_misses": [round(float(v), 4) for v in d_b[miss]],
            "misorientation_deg_median": round(float(np.median(mis)), 4), "set_turn_deg": turn_deg,
-           "note": ("the seed voxels are random; a seed within a spot's tolerance of a neighbour grain sees that "
-                    "grain's orientation score as high as its own (the masks are grain-sized blobs), and the fit "
-                    "keeps whichever it reaches; interior seeds lie more than 20 um from every grain boundary")}
+           "note": ("the completeness peak of this noise-free stack is narrower than the local-sampling step "
+                    "(0.004, about 0.46 deg): a miss is a seed whose own grain's best grid sample scores below the "
+                    "start the procedure chose (a neighbour grain's sample nearer its own peak), so the fit of "
+                    "fit_OR_constrain_with_local_comp_cuda.m starts on the neighbour; own_grain C is the true "
+                    "orientation's, which no grid sample reaches; interior seeds lie more than 20 um from every "
+                    "grain boundary")}
     log(f"[bench] index: {steps} calls x {n_pos} seeds in {dt:.3f}s -> {out['seeds_per_s']} seeds/s, "
         f"median fitted C {out['fitted_C_median']}, own grain within 0.1 deg: {out['own_grain_within_0.1deg']}/"
         f"{mis.size} (interior {out['interior_own_grain_within_0.1deg']}/{out['interior_seeds']}), misses with "

@@ -31,6 +31,10 @@ METHOD_FAST = 0
 METHOD_PLAIN = 1
 GT_MATCH = 0
 GT_MATCH_FUN = 1
+MASK_BITS = 0
+MASK_KEYS = 1
+MASK_TILED = 2
+TUNE_MAX = 18
 
 _SIGS = {
     "grs_version": (ctypes.c_char_p, []),
@@ -72,7 +76,9 @@ _SIGS = {
     "grs_set_audit": (c_i32, [c_vp, c_i32]),
     "grs_set_graphs": (c_i32, [c_vp, c_i32]),
     "grs_set_autotune": (c_i32, [c_vp, c_i32]),
-    "grs_get_tuning": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), c_dp]),
+    "grs_get_tuning": (c_i32, [c_vp, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
+                               ctypes.POINTER(c_i32), c_dp]),
+    "grs_set_launch_config": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32]),
     "grs_graph_stats": (c_i32, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     "grs_fit_or_local": (c_i32, [c_vp, c_dp, c_dp, c_i64, ctypes.c_double, c_i32, c_i32, ctypes.c_double,
                                  ctypes.c_double, c_dp, c_dp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),

@@ -30,10 +30,17 @@ struct DevBuf {
 
 }  // namespace
 
+/* a launch configuration of k_fast: waves per block, hit-test source of the
+ * counts-only kinds (GRS_MASK_*) and the block-to-evaluation mapping */
+struct tune_cfg {
+  int32_t wpb, mask, xcd;
+};
+
 struct grs_tune_state {
   bool done = false;
-  int32_t wpb = 4, keys = 0, ncfg = 0;
-  double ms_per_eval[6] = {0, 0, 0, 0, 0, 0};
+  tune_cfg cfg = {4, GRS_MASK_BITS, 0};
+  int32_t ncfg = 0;
+  double ms_per_eval[GRS_TUNE_MAX] = {0};
 };
 
 struct grs_ctx {
@@ -49,9 +56,10 @@ struct grs_ctx {
   DevBuf d_q, d_sub, d_valid, d_pos, d_vb, d_scratch, d_ns, d_ni, d_C, d_med;
   DevBuf d_rows, d_p1, d_U, d_eu, d_rowstart, d_best, d_evox, d_eu_idx, d_bad, d_x, d_y;
   DevBuf d_mask, d_g1, d_edt_v, d_edt_z, d_grain;
-  DevBuf d_bits, d_rowptr, d_roweoff, d_candq, d_rowbest, d_elist;
+  DevBuf d_bits, d_tbits, d_rowptr, d_roweoff, d_candq, d_rowbest, d_elist;
   DevBuf c_cen, c_grid, c_gn, c_out, c_n, c_gid, c_vox, c_ids, c_nout;
   bool bits_valid = false; /* hit mask matches the current stack and threshold */
+  bool tbits_ok = false;    /* the tiled hit mask exists (its bit numbers fit 32 bits) */
   /* resident matched-spot set of the fits */
   DevBuf f_hkl, f_om, f_meas, f_grain, f_goff, f_gr6, f_up, f_res, f_contrib, f_jac, f_out, f_ne;
   int64_t f_n = 0, f_G = 0;
@@ -101,10 +109,47 @@ struct grs_ctx {
   unsigned char *noalloc = nullptr;
   char noalloc_err[160] = {0};
   bool v_set[4] = {false, false, false, false};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  /* HIP-event timing of the scoring launches (grs_set_timing): one event pair
+   * per timed region, read back only when the totals are asked for or the
+   * pool is full, so that timing never makes a call wait for its kernels */
+  std::vector<hipEvent_t> tpairs; /* begin, end of pair k at 2k, 2k+1 */
+  int64_t t_pending = 0;          /* pairs recorded and not yet read */
   double ms_total = 0.0;
   int64_t launches = 0;
 };
+
+/* event pairs created with the context (no call creates events later, the
+ * allocation-free entry included) */
+static const int64_t kTimingPairs = 64;
+
+/* adds the elapsed times of the recorded pairs to the totals */
+static hipError_t timing_harvest(grs_ctx *c) {
+  for (int64_t k = 0; k < c->t_pending; k++) {
+    hipError_t e = hipEventSynchronize(c->tpairs[2 * k + 1]);
+    if (e != hipSuccess) return e;
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, c->tpairs[2 * k], c->tpairs[2 * k + 1]);
+    if (e != hipSuccess) return e;
+    c->ms_total += ms;
+    c->launches += 1;
+  }
+  c->t_pending = 0;
+  return hipSuccess;
+}
+
+static hipError_t timing_begin(grs_ctx *c) {
+  if (c->t_pending == kTimingPairs) {
+    hipError_t e = timing_harvest(c);
+    if (e != hipSuccess) return e;
+  }
+  return hipEventRecord(c->tpairs[2 * c->t_pending], c->st);
+}
+
+static hipError_t timing_end(grs_ctx *c) {
+  hipError_t e = hipEventRecord(c->tpairs[2 * c->t_pending + 1], c->st);
+  if (e == hipSuccess) c->t_pending += 1;
+  return e;
+}
 
 static grs_status fail(grs_ctx *c, grs_status code, const char *fmt, ...) {
   if (c) {
@@ -233,10 +278,15 @@ grs_status grs_create(int32_t device, grs_ctx **out) {
     delete c;
     return GRS_ERR_DEVICE;
   }
-  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-    hipStreamDestroy(c->st);
-    delete c;
-    return GRS_ERR_DEVICE;
+  for (int64_t k = 0; k < 2 * kTimingPairs; k++) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) {
+      for (hipEvent_t e : c->tpairs) hipEventDestroy(e);
+      hipStreamDestroy(c->st);
+      delete c;
+      return GRS_ERR_DEVICE;
+    }
+    c->tpairs.push_back(ev);
   }
   /* the allocation-free entry's scratch on this device (a failed lookup
    * only disables that entry, which then reports the reason) */
@@ -253,7 +303,7 @@ void grs_destroy(grs_ctx *c) {
                     &c->d_pos, &c->d_vb, &c->d_scratch, &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_rows,
                     &c->d_p1, &c->d_U, &c->d_eu, &c->d_rowstart, &c->d_best, &c->d_evox, &c->d_eu_idx,
                     &c->d_bad, &c->d_x, &c->d_y, &c->d_mask, &c->d_g1, &c->d_edt_v, &c->d_edt_z,
-                    &c->d_grain, &c->d_bits, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest,
+                    &c->d_grain, &c->d_bits, &c->d_tbits, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest,
                     &c->d_elist, &c->f_hkl, &c->f_om, &c->f_meas, &c->f_grain, &c->f_goff, &c->f_gr6,
                     &c->f_up, &c->f_res, &c->f_contrib, &c->f_jac, &c->f_out, &c->f_ne, &c->c_cen, &c->c_grid,
                     &c->c_gn, &c->c_out, &c->c_n, &c->c_gid, &c->c_vox, &c->c_ids, &c->c_nout, &c->d_audit,
@@ -273,8 +323,7 @@ void grs_destroy(grs_ctx *c) {
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (DevBuf *b : bufs) free_buf(*b);
   if (c->h_pin) hipHostFree(c->h_pin);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->tpairs) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
 }
@@ -426,20 +475,43 @@ static grs_status check_ready(grs_ctx *c) {
  * d_pos holds V voxel positions; d_sub/d_valid the orientations. */
 /* ---- launch autotuning ------------------------------------------------------ */
 
-/* configurations: waves per block x hit-test source (counts-only kind only) */
-static const int32_t k_cfg_wpb[6] = {4, 2, 1, 4, 2, 1};
-static const int32_t k_cfg_keys[6] = {0, 0, 0, 1, 1, 1};
+/* configurations: waves per block x block-to-evaluation mapping, and for
+ * the counts-only kinds x hit-test source (bit mask, keys, tiled bit mask) */
+static int tune_configs(const grs_ctx *c, int kind, tune_cfg *out) {
+  static const int32_t wpbs[3] = {4, 2, 1};
+  int n = 0;
+  for (int32_t x = 0; x < 2; x++)
+    for (int32_t m = 0; m < (kind >= 1 ? 3 : 1); m++) {
+      if (m == GRS_MASK_TILED && !c->tbits_ok) continue;
+      for (int w = 0; w < 3; w++) out[n++] = tune_cfg{wpbs[w], m, x};
+    }
+  return n;
+}
+
+static void apply_cfg(grs_eval_args *a, const tune_cfg &k) {
+  a->count_keys = k.mask == GRS_MASK_KEYS;
+  a->bits_tiled = k.mask == GRS_MASK_TILED;
+  a->xcd_map = k.xcd;
+}
 
 /* evaluations per tuning part: enough to keep every resident wave busy
- * (3 waves per SIMD, 4 SIMDs per CU) for several evaluations */
+ * (3 waves per SIMD, 4 SIMDs per CU) for four evaluations */
 static int64_t tune_part_min(const grs_ctx *c) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
-  return (int64_t)cus * 12 * 8;
+  return (int64_t)cus * 12 * 4;
 }
 
 static uint64_t tune_key(const grs_ctx *c, int kind) {
   return (uint64_t)kind | ((uint64_t)c->H << 4) | ((uint64_t)c->nrot << 16);
+}
+
+/* the configuration a (kind, H, nrot) class runs with, encoded for graph keys */
+static uint64_t tune_code(const grs_ctx *c, int kind) {
+  auto it = c->tunes.find(tune_key(c, kind));
+  if (it == c->tunes.end() || !it->second.done) return 0;
+  const tune_cfg &k = it->second.cfg;
+  return 1 + (uint64_t)k.wpb * 16 + (uint64_t)k.mask * 2 + (uint64_t)k.xcd;
 }
 
 /* Launches k_fast over evaluations [0, n_eval) of a (kind 0: with median,
@@ -452,14 +524,17 @@ static uint64_t tune_key(const grs_ctx *c, int kind) {
  * its own evaluations, so the results do not depend on the configuration. */
 static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t waves) {
   grs_tune_state &t = c->tunes[tune_key(c, kind)];
-  const int ncfg = kind >= 1 ? 6 : 3;
+  tune_cfg cfgs[GRS_TUNE_MAX];
+  const int ncfg = tune_configs(c, kind, cfgs);
   const int64_t E = a.n_eval;
   auto launch = [&](const grs_eval_args &x, int32_t wpb) -> hipError_t {
     return kind >= 1 ? grs_launch_fast_counts(&x, c->H, waves, wpb, c->st) : grs_launch_fast(&x, c->H, waves, wpb, c->st);
   };
   if (!c->autotune || t.done || E < (int64_t)(2 * ncfg + 1) * tune_part_min(c)) {
-    a.count_keys = t.done ? t.keys : 0;
-    HIPCHK(c, launch(a, t.done ? t.wpb : 4));
+    tune_cfg k = t.done ? t.cfg : tune_cfg{4, GRS_MASK_BITS, 0};
+    if (k.mask == GRS_MASK_TILED && !c->tbits_ok) k.mask = GRS_MASK_BITS;
+    apply_cfg(&a, k);
+    HIPCHK(c, launch(a, k.wpb));
     return GRS_OK;
   }
   /* part 0 warms caches with the default configuration and is not timed;
@@ -471,9 +546,9 @@ static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t wa
     c->tev.push_back(e);
   }
   const int64_t part = E / nparts;
-  double ms[6] = {0, 0, 0, 0, 0, 0};
-  int64_t cnt[6] = {0, 0, 0, 0, 0, 0};
-  int cfg_of[13];
+  double ms[GRS_TUNE_MAX] = {0};
+  int64_t cnt[GRS_TUNE_MAX] = {0};
+  int cfg_of[2 * GRS_TUNE_MAX + 1];
   for (int p = 0; p < nparts; p++) {
     const int q = p - 1;
     const int k = p == 0 ? 0 : (q < ncfg ? q : 2 * ncfg - 1 - q);
@@ -481,9 +556,9 @@ static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t wa
     grs_eval_args x = a;
     x.e0 = (int64_t)p * part;
     x.n_eval = p == nparts - 1 ? E : (int64_t)(p + 1) * part;
-    x.count_keys = k_cfg_keys[k];
+    apply_cfg(&x, cfgs[k]);
     HIPCHK(c, hipEventRecord(c->tev[p], c->st));
-    HIPCHK(c, launch(x, k_cfg_wpb[k]));
+    HIPCHK(c, launch(x, cfgs[k].wpb));
   }
   HIPCHK(c, hipEventRecord(c->tev[nparts], c->st));
   HIPCHK(c, hipEventSynchronize(c->tev[nparts]));
@@ -495,12 +570,12 @@ static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t wa
     cnt[cfg_of[p]] += n;
   }
   int best = 0;
+  for (int k = 0; k < GRS_TUNE_MAX; k++) t.ms_per_eval[k] = 0.0;
   for (int k = 0; k < ncfg; k++) {
     t.ms_per_eval[k] = ms[k] / (double)(cnt[k] > 0 ? cnt[k] : 1);
     if (t.ms_per_eval[k] < t.ms_per_eval[best]) best = k;
   }
-  t.wpb = k_cfg_wpb[best];
-  t.keys = k_cfg_keys[best];
+  t.cfg = cfgs[best];
   t.ncfg = ncfg;
   t.done = true;
   return GRS_OK;
@@ -510,7 +585,8 @@ static bool tune_pending(const grs_ctx *c, int kind, int64_t E, int64_t waves) {
   if (!c->autotune) return false;
   auto it = c->tunes.find(tune_key(c, kind));
   if (it != c->tunes.end() && it->second.done) return false;
-  const int ncfg = kind >= 1 ? 6 : 3;
+  tune_cfg cfgs[GRS_TUNE_MAX];
+  const int ncfg = tune_configs(c, kind, cfgs);
   (void)waves;
   return E >= (int64_t)(2 * ncfg + 1) * tune_part_min(c);
 }
@@ -553,30 +629,23 @@ static grs_status run_eval(grs_ctx *c, int32_t map, int64_t n_eval, int64_t V, c
     a.nrot = c->nrot;
     const bool audit = c->audit && map == GRS_MAP_ONE_VOXEL;
     if (audit) a.audit = P<unsigned long long>(c->d_audit);
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    if (c->timing) HIPCHK(c, timing_begin(c));
     if (audit) {
       HIPCHK(c, grs_launch_fast_audit(&a, c->H, waves, c->st));
     } else {
       grs_status ts = launch_tuned(c, a, 0, waves);
       if (ts != GRS_OK) return ts;
     }
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    if (c->timing) HIPCHK(c, timing_end(c));
   } else if (method == GRS_METHOD_PLAIN) {
     int64_t T = n_eval < 32768 ? n_eval : 32768;
     ENSURE(c, c->d_scratch, sizeof(uint32_t) * (size_t)T * stride);
     a.scratch = P<uint32_t>(c->d_scratch);
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    if (c->timing) HIPCHK(c, timing_begin(c));
     HIPCHK(c, grs_launch_plain(&a, T, c->st));
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    if (c->timing) HIPCHK(c, timing_end(c));
   } else {
     return fail(c, GRS_ERR_ARG, "unknown method %d", method);
-  }
-  if (c->timing) {
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->ms_total += ms;
-    c->launches += 1;
   }
   return GRS_OK;
 }
@@ -682,15 +751,12 @@ grs_status grs_forward_comp_device_noalloc(grs_ctx *c, const double *pos, const 
   uint32_t *s_keys = reinterpret_cast<uint32_t *>(base + kNaKeys);
   /* the launch configuration the autotuner chose for this class, if any
    * (no tuning here: it records events) */
-  int32_t wpb = 4, keys = 0;
+  tune_cfg kc = {4, GRS_MASK_BITS, 0};
   auto it = c->tunes.find(tune_key(c, 0));
-  if (it != c->tunes.end() && it->second.done) {
-    wpb = it->second.wpb;
-    keys = it->second.keys;
-  }
+  if (it != c->tunes.end() && it->second.done) kc = it->second.cfg;
   HIPCHK(c, hipMemcpyAsync(s_pos, pos, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), s_pos, 1, P<double>(c->d_cs), c->nrot, s_vb, c->st));
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  if (c->timing) HIPCHK(c, timing_begin(c));
   for (int64_t n0 = 0; n0 < N; n0 += GRS_NOALLOC_CHUNK) {
     const int64_t n = N - n0 < GRS_NOALLOC_CHUNK ? N - n0 : GRS_NOALLOC_CHUNK;
     HIPCHK(c, grs_launch_sub_from_q(P<grs_dgeo>(c->d_geo), d_q + 4 * n0, n, s_sub, s_valid, c->st));
@@ -709,23 +775,16 @@ grs_status grs_forward_comp_device_noalloc(grs_ctx *c, const double *pos, const 
     a.scratch = s_keys;
     a.vb = s_vb;
     a.nrot = c->nrot;
-    a.count_keys = keys;
+    apply_cfg(&a, kc);
     a.out_ns = d_ns + n0;
     a.out_ni = d_ni + n0;
     a.out_C = d_C + n0;
     a.out_med = d_med + n0;
     int64_t waves = wmax;
     if (waves > n) waves = ((n + 3) / 4) * 4;
-    HIPCHK(c, grs_launch_fast(&a, c->H, waves, wpb, c->st));
+    HIPCHK(c, grs_launch_fast(&a, c->H, waves, kc.wpb, c->st));
   }
-  if (c->timing) {
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->ms_total += ms;
-    c->launches += 1;
-  }
+  if (c->timing) HIPCHK(c, timing_end(c));
   return GRS_OK;
 }
 
@@ -931,6 +990,14 @@ static grs_status ensure_bits(grs_ctx *c) {
   ENSURE(c, c->d_bits, sizeof(uint32_t) * (size_t)((n + 31) / 32));
   HIPCHK(c, grs_launch_pack_hits(P<uint32_t>(c->d_keys), n, c->hgeo.hit_key, c->hgeo.has_hit, P<uint32_t>(c->d_bits),
                                  c->st));
+  /* the same mask in 32 x 32 pixel tiles, when its bit numbers fit 32 bits */
+  const int64_t tw = ((c->snz + 31) / 32) * ((c->sny + 31) / 32) * c->snrot * 32;
+  c->tbits_ok = tw * 32 <= (int64_t)UINT32_MAX + 1;
+  if (c->tbits_ok) {
+    ENSURE(c, c->d_tbits, sizeof(uint32_t) * (size_t)tw);
+    HIPCHK(c, grs_launch_pack_hits_tiled(P<uint32_t>(c->d_keys), c->snz, c->sny, c->snrot, c->hgeo.hit_key,
+                                         c->hgeo.has_hit, P<uint32_t>(c->d_tbits), c->st));
+  }
   c->bits_valid = true;
   return GRS_OK;
 }
@@ -1000,6 +1067,7 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
     a.pos = P<double>(c->d_pos);
     a.keys = P<uint32_t>(c->d_keys);
     a.hit_bits = P<uint32_t>(c->d_bits);
+    a.hit_tbits = P<uint32_t>(c->d_tbits);
     a.scratch_stride = stride;
     a.scratch = P<uint32_t>(c->d_scratch);
     a.csr.row_ptr = P<int64_t>(c->d_rowptr);
@@ -1040,7 +1108,7 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
                                       P<double>(c->d_bestq), c->st));
       return GRS_OK;
     };
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    if (c->timing) HIPCHK(c, timing_begin(c));
     /* a chunk that first tunes a launch configuration runs directly: the
      * tuning times launches, which a captured graph cannot */
     const bool tuning = tune_pending(c, 1, E, w1) || tune_pending(c, 0, m, w2);
@@ -1048,13 +1116,11 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
       double gs = gridsize;
       uint64_t gbits;
       std::memcpy(&gbits, &gs, sizeof(gbits));
-      const grs_tune_state &t1 = c->tunes[tune_key(c, 1)], &t0 = c->tunes[tune_key(c, 0)];
       std::vector<uint64_t> key = {(uint64_t)m, (uint64_t)E, (uint64_t)S, (uint64_t)nr, gbits, (uint64_t)c->H,
                                    (uint64_t)c->nrot, (uint64_t)w1, (uint64_t)w2, (uint64_t)stride,
-                                   (uint64_t)(t1.done ? t1.wpb * 2 + t1.keys : 0), (uint64_t)(t0.done ? t0.wpb : 0),
-                                   (uint64_t)c->autotune};
+                                   tune_code(c, 1), tune_code(c, 0), (uint64_t)c->autotune};
       for (const DevBuf *bp : {&c->d_geo, &c->d_hkl, &c->d_cs, &c->d_vb, &c->d_pos, &c->d_keys, &c->d_bits,
-                               &c->d_scratch, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest, &c->d_elist,
+                               &c->d_tbits, &c->d_scratch, &c->d_rowptr, &c->d_roweoff, &c->d_candq, &c->d_rowbest, &c->d_elist,
                                &c->d_ns, &c->d_ni, &c->d_C, &c->d_med, &c->d_bestidx, &c->d_bestq})
         key.push_back((uint64_t)(uintptr_t)bp->p);
       if (!c->g_exec || key != c->g_key) {
@@ -1087,14 +1153,7 @@ grs_status grs_compete_csr(grs_ctx *c, const double *pos, int64_t M, const int64
       grs_status es = enqueue();
       if (es != GRS_OK) return es;
     }
-    if (c->timing) {
-      HIPCHK(c, hipEventRecord(c->ev1, c->st));
-      HIPCHK(c, hipEventSynchronize(c->ev1));
-      float ms = 0.0f;
-      HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      c->ms_total += ms;
-      c->launches += 1;
-    }
+    if (c->timing) HIPCHK(c, timing_end(c));
     HIPCHK(c, hipMemcpyAsync(best + r0, c->d_bestidx.p, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(best_q + 4 * r0, c->d_bestq.p, sizeof(double) * 4 * m, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(ns + r0, c->d_ns.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->st));
@@ -1224,7 +1283,7 @@ grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grain
     ENSURE(c, c->f_jac, sizeof(double) * 26 * n);
     d_jac = P<double>(c->f_jac);
   }
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  if (c->timing) HIPCHK(c, timing_begin(c));
   HIPCHK(c, grs_launch_spot_residual(&f, n, P<double>(c->f_hkl), P<double>(c->f_om), P<double>(c->f_meas),
                                      P<int32_t>(c->f_grain), P<double>(c->f_gr6), d_up, P<double>(c->f_res),
                                      d_contrib, d_jac, c->st));
@@ -1233,7 +1292,7 @@ grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grain
     HIPCHK(c, grs_launch_grad_reduce(d_contrib, n, P<int64_t>(c->f_goff), G, P<double>(c->f_out),
                                      P<double>(c->f_out) + 7, c->st));
   }
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  if (c->timing) HIPCHK(c, timing_end(c));
   HIPCHK(c, hipMemcpyAsync(resid, c->f_res.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->st));
   if (grad7) {
     HIPCHK(c, hipMemcpyAsync(grad7, c->f_out.p, sizeof(double) * 7, hipMemcpyDeviceToHost, c->st));
@@ -1242,12 +1301,6 @@ grs_status grs_spot_residual(grs_ctx *c, const double *geo7, const double *grain
   }
   if (jac) HIPCHK(c, hipMemcpyAsync(jac, c->f_jac.p, sizeof(double) * 26 * n, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  if (c->timing) {
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->ms_total += ms;
-    c->launches += 1;
-  }
   return GRS_OK;
 }
 
@@ -1270,19 +1323,13 @@ grs_status grs_spot_normal(grs_ctx *c, const double *geo7, const double *grains6
     ENSURE(c, c->f_ne, sizeof(double) * (69 * G + 37 * G + 37));
     HIPCHK(c, hipMemcpyAsync(c->f_gr6.p, grains6, sizeof(double) * 6 * G, hipMemcpyHostToDevice, c->st));
     double *d_pg = P<double>(c->f_ne), *d_part = d_pg + 69 * G, *d_out = d_part + 37 * G;
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    if (c->timing) HIPCHK(c, timing_begin(c));
     HIPCHK(c, grs_launch_spot_normal(&f, G, P<double>(c->f_hkl), P<double>(c->f_om), P<double>(c->f_meas),
                                      P<int64_t>(c->f_goff), P<double>(c->f_gr6), delta, d_pg, d_part, d_out, c->st));
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    if (c->timing) HIPCHK(c, timing_end(c));
     HIPCHK(c, hipMemcpyAsync(pg.data(), d_pg, sizeof(double) * 69 * G, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(out37, d_out, sizeof(double) * 37, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    if (c->timing) {
-      float ms = 0.0f;
-      HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      c->ms_total += ms;
-      c->launches += 1;
-    }
   }
   /* expand the upper triangles */
   int k = 69;
@@ -1690,6 +1737,7 @@ grs_status grs_index_seeds(grs_ctx *c, const double *pos, int64_t n_pos, int32_t
     a.pos = P<double>(c->x_pos);
     a.keys = P<uint32_t>(c->d_keys);
     a.hit_bits = P<uint32_t>(c->d_bits);
+    a.hit_tbits = P<uint32_t>(c->d_tbits);
     a.scratch_stride = stride;
     a.scratch = P<uint32_t>(c->d_scratch);
     a.map = GRS_MAP_GRID;
@@ -1759,12 +1807,11 @@ grs_status grs_index_seeds(grs_ctx *c, const double *pos, int64_t n_pos, int32_t
       double gs = gridsize;
       uint64_t gbits;
       std::memcpy(&gbits, &gs, sizeof(gbits));
-      const grs_tune_state &t2 = c->tunes[tune_key(c, 2)];
       std::vector<uint64_t> key = {(uint64_t)m, (uint64_t)N, (uint64_t)K, (uint64_t)nr, gbits, (uint64_t)nfit,
                                    (uint64_t)c->H, (uint64_t)c->nrot, (uint64_t)w1, (uint64_t)w2, (uint64_t)stride,
-                                   (uint64_t)(t2.done ? t2.wpb * 2 + t2.keys : 0), (uint64_t)c->autotune};
+                                   tune_code(c, 2), (uint64_t)c->autotune};
       for (const DevBuf *bp :
-           {&c->d_geo, &c->d_hkl, &c->d_cs, &c->d_keys, &c->d_bits, &c->d_scratch, &c->o_q, &c->o_sub, &c->o_valid,
+           {&c->d_geo, &c->d_hkl, &c->d_cs, &c->d_keys, &c->d_bits, &c->d_tbits, &c->d_scratch, &c->o_q, &c->o_sub, &c->o_valid,
             &c->x_pos, &c->x_vb, &c->x_key1, &c->x_ns1, &c->x_ni1, &c->x_top, &c->x_cen, &c->x_grid, &c->x_gn,
             &c->x_ls, &c->x_lsn, &c->x_owner, &c->x_Rmax, &c->x_q2, &c->x_sub2, &c->x_valid2, &c->x_key2, &c->x_ns2,
             &c->x_ni2, &c->x_gcount, &c->x_gkey, &c->x_grow, &c->x_fitkey, &c->x_x0, &c->x_fgroup, &c->x_fns,
@@ -2406,7 +2453,7 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
     ENSURE(c, c->t_bst, sizeof(int32_t) * 4 * (GRS_GT_NB + 1) * (size_t)c->nrot);
     ENSURE(c, c->t_slen, sizeof(int32_t) * 4 * (size_t)c->nrot);
   }
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  if (c->timing) HIPCHK(c, timing_begin(c));
   if (sorted)
     HIPCHK(c, grs_launch_gt_sort(P<int64_t>(c->p_fptr), c->nrot, P<double>(c->t_units), nsp, c->p_maxframe,
                                  P<double>(c->t_skey), P<double>(c->t_su), P<int32_t>(c->t_sorig),
@@ -2419,14 +2466,7 @@ grs_status grs_forward_gt(grs_ctx *c, const double *pos, const double *q, int64_
                                 sorted ? P<int32_t>(c->t_slen) : nullptr, nsp, semantics == GRS_GT_MATCH ? 1 : 0,
                                 P<double>(c->t_scratch), cap, waves,
                                 P<double>(c->t_out), c->st));
-  if (c->timing) {
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->ms_total += ms;
-    c->launches += 1;
-  }
+  if (c->timing) HIPCHK(c, timing_end(c));
   HIPCHK(c, hipMemcpyAsync(out, c->t_out.p, sizeof(double) * 4 * N, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return GRS_OK;
@@ -2736,16 +2776,10 @@ grs_status grs_copy_to_device(grs_ctx *c, void *d_dst, const void *src, int64_t 
 grs_status grs_device_copy(grs_ctx *c, void *d_dst, const void *d_src, int64_t bytes) {
   if (!c || !d_dst || !d_src || bytes < 0 || bytes % 4) return GRS_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  if (c->timing) HIPCHK(c, timing_begin(c));
   HIPCHK(c, grs_launch_copy_u32(static_cast<const uint32_t *>(d_src), static_cast<uint32_t *>(d_dst), bytes / 4, c->st));
-  if (c->timing) HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  if (c->timing) HIPCHK(c, timing_end(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  if (c->timing) {
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->ms_total += ms;
-    c->launches += 1;
-  }
   return GRS_OK;
 }
 
@@ -2771,16 +2805,36 @@ grs_status grs_set_autotune(grs_ctx *c, int32_t enable) {
   return GRS_OK;
 }
 
-grs_status grs_get_tuning(grs_ctx *c, int32_t kind, int32_t *waves_per_block, int32_t *hit_keys, int32_t *n_configs,
-                          double *ms_per_eval) {
-  if (!c || kind < 0 || kind > 2 || !waves_per_block || !hit_keys || !n_configs || !ms_per_eval)
+grs_status grs_get_tuning(grs_ctx *c, int32_t kind, int32_t *waves_per_block, int32_t *mask_layout, int32_t *xcd_map,
+                          int32_t *n_configs, double *ms_per_eval) {
+  if (!c || kind < 0 || kind > 2 || !waves_per_block || !mask_layout || !xcd_map || !n_configs || !ms_per_eval)
     return GRS_ERR_ARG;
   auto it = c->tunes.find(tune_key(c, kind));
   if (it == c->tunes.end() || !it->second.done) return fail(c, GRS_ERR_STATE, "not tuned yet");
-  *waves_per_block = it->second.wpb;
-  *hit_keys = it->second.keys;
+  *waves_per_block = it->second.cfg.wpb;
+  *mask_layout = it->second.cfg.mask;
+  *xcd_map = it->second.cfg.xcd;
   *n_configs = it->second.ncfg;
-  for (int k = 0; k < 6; k++) ms_per_eval[k] = it->second.ms_per_eval[k];
+  for (int k = 0; k < GRS_TUNE_MAX; k++) ms_per_eval[k] = it->second.ms_per_eval[k];
+  return GRS_OK;
+}
+
+grs_status grs_set_launch_config(grs_ctx *c, int32_t kind, int32_t waves_per_block, int32_t mask_layout,
+                                 int32_t xcd_map) {
+  if (!c || kind < 0 || kind > 2) return GRS_ERR_ARG;
+  const uint64_t key = tune_key(c, kind);
+  if (waves_per_block == 0) {
+    c->tunes.erase(key);
+    return GRS_OK;
+  }
+  if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4) || mask_layout < 0 || mask_layout > 2 ||
+      (kind == 0 && mask_layout != GRS_MASK_BITS) || xcd_map < 0 || xcd_map > 1)
+    return fail(c, GRS_ERR_ARG, "invalid launch configuration");
+  grs_tune_state t;
+  t.done = true;
+  t.cfg = tune_cfg{waves_per_block, mask_layout, xcd_map};
+  t.ncfg = 0;
+  c->tunes[key] = t;
   return GRS_OK;
 }
 
@@ -2823,6 +2877,8 @@ grs_status grs_get_audit(grs_ctx *c, int64_t *counts, double *max_dev) {
 
 grs_status grs_kernel_time(grs_ctx *c, double *ms, int64_t *launches, int32_t reset) {
   if (!c) return GRS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, timing_harvest(c));
   if (ms) *ms = c->ms_total;
   if (launches) *launches = c->launches;
   if (reset) {

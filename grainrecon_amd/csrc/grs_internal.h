@@ -63,7 +63,19 @@ typedef struct grs_eval_args {
   int64_t n_u, u_stride;   /* GRS_MAP_GRID */
   unsigned long long *out_key; /* counts-only GRS_MAP_GRID: per-evaluation sort key */
   int32_t nrot;            /* rotation angles (must equal the geometry's); 0 turns the one-voxel beam table off */
+  const uint32_t *hit_tbits; /* counts-only kernels: the hit mask in 32 x 32 pixel tiles (grs_tiled_bit) */
+  int32_t bits_tiled;      /* counts-only kernels with count_keys = 0: 1 = test hit_tbits, 0 = hit_bits */
+  int32_t xcd_map;         /* 1 = consecutive waves of the evaluation order on one XCD (grid a multiple of 8 blocks) */
 } grs_eval_args;
+
+/* Bit of pixel (dz, dy) (1-based) of projection om in the tiled hit mask:
+ * 32 x 32 pixel tiles of 128 bytes, z fastest inside a 32-bit word (one tile
+ * column), the tile's 32 words for 32 consecutive dy, tiles z-fastest, then
+ * dy, then omega; ntz = ceil(nz / 32), nty = ceil(ny / 32). */
+__host__ __device__ inline uint32_t grs_tiled_bit(uint32_t om, uint32_t dy, uint32_t dz, uint32_t ntz, uint32_t nty) {
+  const uint32_t y0 = dy - 1u, z0 = dz - 1u;
+  return ((((om * nty + (y0 >> 5)) * ntz + (z0 >> 5)) << 10) | ((y0 & 31u) << 5)) | (z0 & 31u);
+}
 
 /* Audit counters of the fast kernel's resolution stage: events seen by phase
  * B1, resolved as selected, resolved as not selected, sent to the exact
@@ -111,6 +123,8 @@ hipError_t grs_launch_winner_out(const int64_t *d_elist, const int64_t *d_row_eo
                                  double *d_best_q, hipStream_t st);
 hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_key, int32_t has_hit,
                                 uint32_t *d_bits, hipStream_t st);
+hipError_t grs_launch_pack_hits_tiled(const uint32_t *d_keys, int64_t nz, int64_t ny, int64_t nrot, uint32_t hit_key,
+                                      int32_t has_hit, uint32_t *d_tbits, hipStream_t st);
 hipError_t grs_launch_plain(const grs_eval_args *a, int64_t n_threads, hipStream_t st);
 hipError_t grs_launch_compete_reduce(const int64_t *d_row_start, int64_t M, const int32_t *d_ns, const int32_t *d_ni,
                                      const double *d_C, const float *d_med, int64_t *d_best, hipStream_t st);

@@ -872,11 +872,22 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     __syncthreads();
     btab = bt;
   }
-  const int64_t gwave = (int64_t)blockIdx.x * wpb + wib;
+  /* blocks are dealt to the 8 XCDs in turn (block b on XCD b mod 8): with
+   * xcd_map, block b takes the place b / 8 + (b mod 8) * gridDim.x / 8 in
+   * the evaluation order, so that each XCD's L2 sees a contiguous run of
+   * evaluations (neighbouring samples of a CSR row share detector tiles) */
+  int64_t vblock = blockIdx.x;
+  if (a.xcd_map && (gridDim.x & 7u) == 0u) vblock = (int64_t)(blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const int64_t gwave = vblock * wpb + wib;
   const int64_t nwaves = (int64_t)gridDim.x * wpb;
   uint32_t *wlist = MEDIAN ? a.scratch + gwave * a.scratch_stride : nullptr;
   const int32_t nrot = g->nrot;
   const int64_t nz = g->nz, nzny = g->nz * g->ny;
+  const uint32_t nz32 = (uint32_t)nz, nzny32 = (uint32_t)nzny; /* the stack holds fewer than 2^32 entries */
+  /* counts-only on the tiled mask: the selected events carry tiled bit numbers */
+  const bool tiled = !MEDIAN && a.bits_tiled && !a.count_keys;
+  const uint32_t ntz = (uint32_t)((nz + 31) >> 5), nty = (uint32_t)((g->ny + 31) >> 5);
+  const uint32_t *hbits = tiled ? a.hit_tbits : a.hit_bits;
   const double m4pi = -4.0 * GRS_PI;
   /* phase A margins: see the header comment */
   const double dcb = 1e-12;
@@ -1044,7 +1055,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         p_sel = true;
         if (MEDIAN) p_key = a.keys[idx];
         else if (a.count_keys) p_hit = g->has_hit && a.keys[idx] <= g->hit_key;
-        else p_hit = (a.hit_bits[idx >> 5] >> (idx & 31u)) & 1u;
+        else p_hit = (hbits[idx >> 5] >> (idx & 31u)) & 1u;
       }
       head3 += n;
     };
@@ -1067,6 +1078,10 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       if ((uint32_t)lane < n) {
         double yz[2];
         sel = exact_event(a.g, vpos, a.cs, gwt, ring2[(head2 + lane) & (GRS_QCAP - 1)], &idx, yz) != 0;
+        if (sel && tiled) {
+          const uint32_t omx = idx / nzny32, rem = idx - omx * nzny32, y0 = rem / nz32;
+          idx = grs_tiled_bit(omx, y0 + 1u, rem - y0 * nz32 + 1u, ntz, nty);
+        }
       }
       head2 += n;
       push3(sel, idx);
@@ -1088,7 +1103,9 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         const double *brow = btab ? btab + GRS_BT * om : nullptr;
         const double cc = brow ? brow[7] : a.cs[2 * om], ss = brow ? brow[8] : a.cs[2 * om + 1];
         st = b1_classify(g, cc, ss, Pw, brow, t[0], t[1], t[2], &o);
-        if (st == 1) idx = (uint32_t)((int64_t)(o.dz - 1) + (int64_t)(o.dy - 1) * nz + (int64_t)om * nzny);
+        if (st == 1)
+          idx = tiled ? grs_tiled_bit((uint32_t)om, (uint32_t)o.dy, (uint32_t)o.dz, ntz, nty)
+                      : (uint32_t)(o.dz - 1) + (uint32_t)(o.dy - 1) * nz32 + (uint32_t)om * nzny32;
         inwin = o.win != 0;
         if (AUDIT) {
           double yz[2] = {0.0, 0.0};
@@ -1307,6 +1324,28 @@ __global__ void k_pack_hits(const uint32_t *keys, int64_t n, uint32_t hit_key, i
     }
   }
   bits[w] = b;
+}
+
+/* the same mask in 32 x 32 pixel tiles (grs_tiled_bit): thread per 32-bit
+ * word, word = tile * 32 + (dy - 1) mod 32, bit = (dz - 1) mod 32; pixels
+ * outside the stack are 0 */
+__global__ void k_pack_hits_tiled(const uint32_t *keys, int64_t nz, int64_t ny, int64_t nw, uint32_t ntz,
+                                  uint32_t nty, uint32_t hit_key, int32_t has_hit, uint32_t *tbits) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const int64_t tile = w >> 5;
+  const int64_t tz = tile % ntz, rest = tile / ntz;
+  const int64_t ty = rest % nty, om = rest / nty;
+  const int64_t y = ty * 32 + (w & 31);
+  uint32_t b = 0;
+  if (has_hit && y < ny) {
+    const uint32_t *col = keys + (om * ny + y) * nz;
+    for (int k = 0; k < 32; k++) {
+      const int64_t z = tz * 32 + k;
+      if (z < nz && col[z] <= hit_key) b |= 1u << k;
+    }
+  }
+  tbits[w] = b;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1535,6 +1574,16 @@ hipError_t grs_launch_pack_hits(const uint32_t *d_keys, int64_t n, uint32_t hit_
   int64_t nw = (n + 31) >> 5;
   if (nw <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_pack_hits, dim3(blocks_for(nw, 256)), dim3(256), 0, st, d_keys, n, hit_key, has_hit, d_bits);
+  return hipGetLastError();
+}
+
+hipError_t grs_launch_pack_hits_tiled(const uint32_t *d_keys, int64_t nz, int64_t ny, int64_t nrot, uint32_t hit_key,
+                                      int32_t has_hit, uint32_t *d_tbits, hipStream_t st) {
+  const int64_t ntz = (nz + 31) >> 5, nty = (ny + 31) >> 5;
+  const int64_t nw = ntz * nty * nrot * 32;
+  if (nw <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_hits_tiled, dim3(blocks_for(nw, 256)), dim3(256), 0, st, d_keys, nz, ny, nw,
+                     (uint32_t)ntz, (uint32_t)nty, hit_key, has_hit, d_tbits);
   return hipGetLastError();
 }
 

@@ -513,17 +513,28 @@ class Engine:
         """Launch autotuning of the scoring kernel (waves per block, hit-test source)."""
         self._chk(self._lib.grs_set_autotune(self._h, 1 if on else 0))
 
+    MASK_NAMES = {L.MASK_BITS: "bits", L.MASK_KEYS: "keys", L.MASK_TILED: "bits_tiled32"}
+
     def tuning(self, kind: int):
         """Chosen configuration of kind 0 (with median), 1 (counts-only CSR) or
-        2 (counts-only grid, seed indexing):
-        dict with waves_per_block, hit_keys and ms_per_eval of every candidate;
-        None before the first large enough call."""
-        w, k, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        ms = (ctypes.c_double * 6)()
-        st = self._lib.grs_get_tuning(self._h, int(kind), ctypes.byref(w), ctypes.byref(k), ctypes.byref(n), ms)
+        2 (counts-only grid, seed indexing): dict with waves_per_block,
+        mask_layout ("bits", "keys", "bits_tiled32"), xcd_map and ms_per_eval
+        of every candidate; None before the first large enough call."""
+        w, m, x, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = (ctypes.c_double * L.TUNE_MAX)()
+        st = self._lib.grs_get_tuning(self._h, int(kind), ctypes.byref(w), ctypes.byref(m), ctypes.byref(x),
+                                      ctypes.byref(n), ms)
         if st != 0:
             return None
-        return {"waves_per_block": w.value, "hit_keys": bool(k.value), "ms_per_eval": list(ms)[:n.value]}
+        return {"waves_per_block": w.value, "mask_layout": self.MASK_NAMES[m.value], "xcd_map": bool(x.value),
+                "ms_per_eval": list(ms)[:n.value]}
+
+    def set_launch_config(self, kind: int, waves_per_block: int, mask_layout: int = L.MASK_BITS,
+                          xcd_map: bool = False) -> None:
+        """Pins kind's launch configuration (grs_set_launch_config);
+        waves_per_block = 0 forgets it."""
+        self._chk(self._lib.grs_set_launch_config(self._h, int(kind), int(waves_per_block), int(mask_layout),
+                                                  int(bool(xcd_map))))
 
     def set_graphs(self, on: bool = True) -> None:
         """HIP-graph capture and replay of the batched compete and seed-indexing

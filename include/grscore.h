@@ -506,7 +506,9 @@ grs_status grs_find_neighbor_ors(grs_ctx *ctx, const int32_t *grain_id, int64_t 
 
 /* ---- measurement and diagnostics ---------------------------------------- */
 
-/* Enables HIP-event timing of the scoring kernel launches on ctx's stream. */
+/* Enables HIP-event timing of the scoring kernel launches on ctx's stream.
+ * The events are read only by grs_kernel_time (or when 64 timed launches are
+ * pending), so a timed call does not wait for its kernels. */
 grs_status grs_set_timing(grs_ctx *ctx, int32_t enable);
 /* Accumulated scoring-kernel time (ms) and launch count since the last reset;
  * reset != 0 clears them after reading. */
@@ -516,16 +518,34 @@ grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, in
  * winners), kind 1, the counts-only CSR kernel of grs_compete_csr, or kind 2,
  * the counts-only grid kernel of grs_index_seeds -- for the current geometry
  * (reflection and rotation counts) that is large enough splits its
- * evaluations among the configurations (waves per block 4, 2, 1; for kinds 1
- * and 2 also the hit test on the bit mask or on the distance keys), times
- * them and keeps the fastest.  Results never depend on it. */
+ * evaluations among the configurations, times them and keeps the fastest:
+ * waves per block (4, 2, 1) x the block-to-evaluation mapping (blocks in
+ * launch order, or each XCD given a contiguous run of evaluations), and for
+ * kinds 1 and 2 x the hit-test source (GRS_MASK_BITS: the bit mask, 1 bit per
+ * pixel in stack order; GRS_MASK_KEYS: the distance keys; GRS_MASK_TILED:
+ * the bit mask in 32 x 32 pixel tiles, one 128-byte line per tile, so that
+ * neighbouring spots of nearby orientations share lines).  Results never
+ * depend on it.  No reference counterpart (the reference has one fixed
+ * launch shape). */
+#define GRS_MASK_BITS 0
+#define GRS_MASK_KEYS 1
+#define GRS_MASK_TILED 2
+#define GRS_TUNE_MAX 18
 grs_status grs_set_autotune(grs_ctx *ctx, int32_t enable);
-/* The chosen configuration of a kind (GRS_ERR_STATE before tuning) and the
- * measured milliseconds per evaluation of each of its n_configs candidates
- * (ms_per_eval holds 6 values; order: wpb 4, 2, 1 on the bit mask, then on
- * the keys). */
-grs_status grs_get_tuning(grs_ctx *ctx, int32_t kind, int32_t *waves_per_block, int32_t *hit_keys,
+/* The chosen configuration of a kind (GRS_ERR_STATE before tuning or
+ * pinning): waves per block, hit-test source (GRS_MASK_*), XCD mapping (0/1),
+ * and the measured milliseconds per evaluation of each of its n_configs
+ * candidates (ms_per_eval holds GRS_TUNE_MAX values; order: for mapping 0
+ * then 1, for each source in GRS_MASK_* order (kind 0: the bit mask only;
+ * the tiled mask only when it exists), waves per block 4, 2, 1). */
+grs_status grs_get_tuning(grs_ctx *ctx, int32_t kind, int32_t *waves_per_block, int32_t *mask_layout, int32_t *xcd_map,
                           int32_t *n_configs, double *ms_per_eval);
+/* Pins the launch configuration of a kind (no tuning for it; n_configs 0):
+ * waves per block 1, 2 or 4, a GRS_MASK_* source (GRS_MASK_BITS for kind 0),
+ * XCD mapping 0 or 1.  waves_per_block = 0 forgets the kind's configuration,
+ * so that the next large call tunes again.  No reference counterpart. */
+grs_status grs_set_launch_config(grs_ctx *ctx, int32_t kind, int32_t waves_per_block, int32_t mask_layout,
+                                 int32_t xcd_map);
 
 /* HIP graphs for grs_compete_csr and grs_index_seeds (default on): each
  * chunk's device pipeline (for the compete: key clearing, bounds, counts-only

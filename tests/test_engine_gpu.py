@@ -381,11 +381,11 @@ def test_autotuned_launches_give_identical_results(engine):
     engine.set_geometry(s.geo, s.hkl, s.rot)
     engine.set_distance_stack(s.proj)
     rng = np.random.default_rng(21)
-    q = np.vstack([s.q, scenes.random_quats(rng, 180000)])
+    q = np.vstack([s.q, scenes.random_quats(rng, 330000)])
     pos = s.voxels[1][13]
-    rows = np.arange(0, 481, 4)                    # 120 rows x 4 candidates x 729 samples
-    cands = np.array([s.q[c % 3] + np.concatenate([[0.0], rng.normal(scale=0.004, size=3)]) for c in range(480)])
-    cpos = np.array([s.voxels[r % 3][int(rng.integers(0, 27))] for r in range(120)])
+    rows = np.arange(0, 1921, 4)                   # 480 rows x 4 candidates x 729 samples
+    cands = np.array([s.q[c % 3] + np.concatenate([[0.0], rng.normal(scale=0.004, size=3)]) for c in range(1920)])
+    cpos = np.array([s.voxels[r % 3][int(rng.integers(0, 27))] for r in range(480)])
     engine.set_autotune(False)
     try:
         ref = engine.forward_comp(pos, q)
@@ -399,8 +399,59 @@ def test_autotuned_launches_give_identical_results(engine):
     for x, y in zip(out_c, ref_c):
         np.testing.assert_array_equal(x, y)
     t0, t1 = engine.tuning(0), engine.tuning(1)
-    assert t0 is not None and t0["waves_per_block"] in (1, 2, 4) and len(t0["ms_per_eval"]) == 3
-    assert t1 is not None and len(t1["ms_per_eval"]) == 6 and all(v > 0 for v in t1["ms_per_eval"])
+    # kind 0: waves per block x XCD mapping; kind 1 also x bit mask / keys / tiled bit mask
+    assert t0 is not None and t0["waves_per_block"] in (1, 2, 4) and len(t0["ms_per_eval"]) == 6
+    assert t0["mask_layout"] == "bits"
+    assert t1 is not None and len(t1["ms_per_eval"]) == 18 and all(v > 0 for v in t1["ms_per_eval"])
+    assert t1["mask_layout"] in ("bits", "keys", "bits_tiled32")
     # tuned launches afterwards, still identical
     for x, y in zip(engine.compete_csr(cpos, rows, cands, 8, 0.002), ref_c):
         np.testing.assert_array_equal(x, y)
+
+
+def test_every_pinned_launch_configuration_gives_identical_results(engine):
+    """Each launch configuration the autotuner can choose, pinned in turn:
+    waves per block x XCD mapping for the median kernel, and also x bit mask /
+    distance keys / tiled bit mask for the counts-only CSR and grid kernels;
+    every one equals the default configuration bit for bit (forward_comp,
+    compete_csr and index_seeds on a 256^2 x 19 scene, including a stack whose
+    sides are not multiples of the 32-pixel tile)."""
+    from grainrecon_amd import _lib as L
+
+    for det in (256, 250):
+        s = scenes.make_scene(seed=86, ngrains=3, spread_mm=0.03, nrot=19, det=det)
+        engine.set_geometry(s.geo, s.hkl, s.rot)
+        engine.set_distance_stack(s.proj)
+        rng = np.random.default_rng(22)
+        q = np.vstack([s.q, scenes.random_quats(rng, 3000)])
+        q = q[q[:, 0] >= 0.5][:2000]
+        pos = s.voxels[1][13]
+        rows = np.arange(0, 41, 4)
+        cands = np.array([s.q[c % 3] + np.concatenate([[0.0], rng.normal(scale=0.004, size=3)]) for c in range(40)])
+        cpos = np.array([s.voxels[r % 3][int(rng.integers(0, 27))] for r in range(10)])
+        engine.set_orientation_set(q)
+        seeds = np.array([s.centers[0], s.voxels[2][4]])
+        try:
+            for k in range(3):
+                engine.set_launch_config(k, 4, L.MASK_BITS, False)
+            ref = engine.forward_comp(pos, q)
+            ref_c = engine.compete_csr(cpos, rows, cands, 8, 0.002)
+            ref_i = engine.index_seeds(seeds, nr_seed=6)
+            print(f"det {det}: default configuration done", flush=True)
+            for xcd in (False, True):
+                for wpb in (4, 2, 1):
+                    engine.set_launch_config(0, wpb, L.MASK_BITS, xcd)
+                    for x, y in zip(engine.forward_comp(pos, q), ref):
+                        np.testing.assert_array_equal(x, y)
+                    for mask in (L.MASK_BITS, L.MASK_KEYS, L.MASK_TILED):
+                        engine.set_launch_config(1, wpb, mask, xcd)
+                        engine.set_launch_config(2, wpb, mask, xcd)
+                        for x, y in zip(engine.compete_csr(cpos, rows, cands, 8, 0.002), ref_c):
+                            np.testing.assert_array_equal(x, y)
+                        got = engine.index_seeds(seeds, nr_seed=6)
+                        for key in ("indexing", "candidate", "seed_idx", "seed_C"):
+                            np.testing.assert_array_equal(got[key], ref_i[key], err_msg=f"{key} {wpb} {mask} {xcd}")
+                    print(f"det {det}: wpb {wpb} xcd {xcd} equal", flush=True)
+        finally:
+            for k in range(3):
+                engine.set_launch_config(k, 0)
