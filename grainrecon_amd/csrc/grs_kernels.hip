@@ -41,11 +41,11 @@
 #define GRS_FAST_MIN_WAVES 3 /* resident waves per SIMD k_fast is compiled for */
 #endif
 #define GRS_QCAP 128   /* LDS ring slots per wave */
-#define GRS_GWT 8      /* doubles per hkl row in LDS */
+#define GRS_GWT 5      /* doubles per hkl row in LDS: G (3), 1/|G|, |G|^2 */
 #define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
 #define GRS_RANGE_DOUBLES(H) (4 * (int64_t)(H) + 1) /* 8H + 2 u32: omega-solve ranges */
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
-#define GRS_BT 9        /* doubles per omega of the beam table: SamposW (3), u (3), 1/u0, cos, sin */
+#define GRS_BT 14       /* doubles per omega of the beam table: the omega_terms (12), cos, sin */
 
 /* LDS doubles of one wave's region of k_fast */
 __host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
@@ -621,26 +621,58 @@ __device__ __forceinline__ bool res_round(double r, double *k) {
   return fr >= GRS_RES_TAU && fr <= 1.0 - GRS_RES_TAU;
 }
 
+/* the per (voxel, omega) terms of the spot construction (forward_comp.m:
+ * 36-54): the beam direction u, SamposW(2:3), Lsd - SamposW(1), center(2:3)
+ * = L u(2:3) / u0, L2 = (Lsd - SamposW(1)) / u0 (cos(alpha) = u0), the t
+ * numerator and the offsets of the tilted-detector intersection */
+struct om_terms {
+  double u0, u1, u2, sw1, sw2, k0, ctr1, ctr2, L2, tnum, ov1, ov2;
+};
+
+__device__ __forceinline__ void omega_terms(const grs_dgeo *g, const double *P, double c, double s, om_terms *w) {
+  double swv[3], uv[3];
+  beam_terms(g, P, c, s, swv, uv);
+  const double iu0 = fast_rcp(uv[0]);
+  w->u0 = uv[0];
+  w->u1 = uv[1];
+  w->u2 = uv[2];
+  w->k0 = g->Lsd - swv[0];
+  w->sw1 = swv[1];
+  w->sw2 = swv[2];
+  w->ctr1 = g->L * uv[1] * iu0;
+  w->ctr2 = g->L * uv[2] * iu0;
+  w->L2 = w->k0 * iu0;
+  const double *R = g->R;
+  w->tnum = __builtin_fma(R[0], w->k0, __builtin_fma(R[3], (g->dety00 - g->RAO) - swv[1], R[6] * (g->detz00 - swv[2])));
+  w->ov1 = (swv[1] - g->dety00) + g->RAO;
+  w->ov2 = swv[2] - g->detz00;
+}
+
 __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *P, const double *bt,
-                                           double gw0, double gw1, double gw2, b1_out *o) {
-  double swv[3], uv[3], iu0;
+                                           const double *gt, b1_out *o) {
+  om_terms w;
   if (bt) {
-    /* the beam table row of this omega: the same values as below */
-    swv[0] = bt[0];
-    swv[1] = bt[1];
-    swv[2] = bt[2];
-    uv[0] = bt[3];
-    uv[1] = bt[4];
-    uv[2] = bt[5];
-    iu0 = bt[6];
+    /* the beam table row of this omega: the same values as omega_terms */
+    w.u0 = bt[0];
+    w.u1 = bt[1];
+    w.u2 = bt[2];
+    w.sw1 = bt[3];
+    w.sw2 = bt[4];
+    w.k0 = bt[5];
+    w.ctr1 = bt[6];
+    w.ctr2 = bt[7];
+    w.L2 = bt[8];
+    w.tnum = bt[9];
+    w.ov1 = bt[10];
+    w.ov2 = bt[11];
   } else {
-    beam_terms(g, P, c, s, swv, uv);
-    iu0 = fast_rcp(uv[0]);
+    omega_terms(g, P, c, s, &w);
   }
-  const double u0 = uv[0], u1 = uv[1], u2 = uv[2];
+  const double gw0 = gt[0], gw1 = gt[1], gw2 = gt[2];
+  const double u0 = w.u0, u1 = w.u1, u2 = w.u2;
   const double G0 = c * gw0 - s * gw1, G1 = s * gw0 + c * gw1, G2 = gw2;
-  const double g2 = __builtin_fma(G0, G0, __builtin_fma(G1, G1, G2 * G2));
-  const double rg = fast_rsq(g2);
+  /* |G| is invariant under the rotation: 1/|G| and |G|^2 of the reflection */
+  const double rg = gt[3], g2 = gt[4];
   const double cb = __builtin_fma(u0, G0, __builtin_fma(u1, G1, u2 * G2)) * rg;
   const double lam = (-4.0 * GRS_PI) * cb * rg;
   const double lm = 1e-9 * g->lambda_max;
@@ -661,11 +693,11 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
    * grainpos * L / grainpos(1), so center/|center| = u, center(2:3) =
    * L u(2:3) / u0, L2 = (Lsd - SW1) / u0 (cos(alpha) = u0), and the t
    * numerator and offsets of forward_comp.m:51-54 */
-  const double k0 = g->Lsd - swv[0], sw1 = swv[1], sw2 = swv[2];
-  const double ctr1 = g->L * u1 * iu0, ctr2 = g->L * u2 * iu0, L2 = k0 * iu0;
+  const double k0 = w.k0, sw1 = w.sw1, sw2 = w.sw2;
+  const double ctr1 = w.ctr1, ctr2 = w.ctr2, L2 = w.L2;
   const double *R = g->R;
-  const double tnum = __builtin_fma(R[0], k0, __builtin_fma(R[3], (g->dety00 - g->RAO) - sw1, R[6] * (g->detz00 - sw2)));
-  const double ov1 = (sw1 - g->dety00) + g->RAO, ov2 = sw2 - g->detz00;
+  const double tnum = w.tnum;
+  const double ov1 = w.ov1, ov2 = w.ov2;
   const double inv = fast_rsq(nv2);
   const double ap = __builtin_fma(G1, u1, G2 * u2) * inv;
   if (!(__builtin_fabs(ap) <= 0.99)) return 2;
@@ -856,18 +888,23 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     double P[3];
     voxel_P(g, a.pos, P);
     for (int32_t om = (int32_t)threadIdx.x; om < g->nrot; om += (int32_t)blockDim.x) {
-      double sw[3], u[3];
-      beam_terms(g, P, a.cs[2 * om], a.cs[2 * om + 1], sw, u);
+      om_terms w;
+      omega_terms(g, P, a.cs[2 * om], a.cs[2 * om + 1], &w);
       double *r = bt + GRS_BT * om;
-      r[0] = sw[0];
-      r[1] = sw[1];
-      r[2] = sw[2];
-      r[3] = u[0];
-      r[4] = u[1];
-      r[5] = u[2];
-      r[6] = fast_rcp(u[0]);
-      r[7] = a.cs[2 * om];
-      r[8] = a.cs[2 * om + 1];
+      r[0] = w.u0;
+      r[1] = w.u1;
+      r[2] = w.u2;
+      r[3] = w.sw1;
+      r[4] = w.sw2;
+      r[5] = w.k0;
+      r[6] = w.ctr1;
+      r[7] = w.ctr2;
+      r[8] = w.L2;
+      r[9] = w.tnum;
+      r[10] = w.ov1;
+      r[11] = w.ov2;
+      r[12] = a.cs[2 * om];
+      r[13] = a.cs[2 * om + 1];
     }
     __syncthreads();
     btab = bt;
@@ -957,11 +994,9 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         t[0] = gw[0];
         t[1] = gw[1];
         t[2] = gw[2];
-        t[3] = dcb * gn;                              /* keep if dot <  tb0 */
-        t[4] = cbm * gn;                              /* keep if dot >  tb1 */
+        t[3] = ign;
+        t[4] = n2;
         double ml = lrel + (1e-12 * (4.0 * GRS_PI)) * ign; /* absolute lambda margin */
-        t[5] = (lmin - ml) * n2;                      /* keep if -4 pi dot > tl0 */
-        t[6] = (lmax + ml) * n2;                      /* keep if -4 pi dot < tl1 */
         if (solve) {
           /* cos(beta) window: beta and lambda bounds with the phase-A margins,
            * the detector cone and the detector rows.  cos(beta) = u . G/gn lies within rg W
@@ -1101,8 +1136,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         o.dy = o.dz = 0;
         o.win = 0;
         const double *brow = btab ? btab + GRS_BT * om : nullptr;
-        const double cc = brow ? brow[7] : a.cs[2 * om], ss = brow ? brow[8] : a.cs[2 * om + 1];
-        st = b1_classify(g, cc, ss, Pw, brow, t[0], t[1], t[2], &o);
+        const double cc = brow ? brow[12] : a.cs[2 * om], ss = brow ? brow[13] : a.cs[2 * om + 1];
+        st = b1_classify(g, cc, ss, Pw, brow, t, &o);
         if (st == 1)
           idx = tiled ? grs_tiled_bit((uint32_t)om, (uint32_t)o.dy, (uint32_t)o.dz, ntz, nty)
                       : (uint32_t)(o.dz - 1) + (uint32_t)(o.dy - 1) * nz32 + (uint32_t)om * nzny32;
@@ -1196,7 +1231,10 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         double Gt1 = s * t[0] + c * t[1];
         double dot = __builtin_fma(u2, t[2], __builtin_fma(u1, Gt1, u0 * Gt0));
         double l4 = m4pi * dot;
-        pass = (dot < t[3]) && (dot > t[4]) && (l4 > t[5]) && (l4 < t[6]);
+        /* the phase-A thresholds of the reflection (margins: see the header comment) */
+        const double gn = __builtin_sqrt(t[4]);
+        const double ml = lrel + (1e-12 * (4.0 * GRS_PI)) * t[3];
+        pass = (dot < dcb * gn) && (dot > cbm * gn) && (l4 > (lmin - ml) * t[4]) && (l4 < (lmax + ml) * t[4]);
       }
       uint64_t m = __ballot(pass);
       if (pass) ring1[(tail1 + lane_prefix(m)) & (GRS_QCAP - 1)] = ((uint32_t)om << 8) | (uint32_t)h;
