@@ -804,6 +804,51 @@ def grow_leg(eng, cfg, box_l, cpu, threads, seed=7007):
     return res
 
 
+def growth_all_leg(eng, cfg, n_ors, cpu, threads, seed=7107):
+    """Config 2's growth stage: an indexed orientation scored against every
+    voxel of the volume (cuda_grow_indexed_region over the 3.2M voxels,
+    grow_indexed_region_cuda.m:105-107 with the whole volume as the list) for
+    n_ors grains' orientations; kernel time by HIP events, 256 voxels of the
+    first orientation re-scored by the oracle."""
+    smp, geo = cfg.sample, cfg.geo
+    nx, ny, nz = smp.dims
+    M = smp.index.shape[0]
+    rows = np.column_stack([smp.index.astype(float), np.zeros(M), np.ones(M), np.full(M, 10.0)])
+    param1 = np.array([1, 1, 1, 0, smp.vox_mm, 0, 0, 0, nx, ny, nz, 1.0, 0.05, 10.0], float)
+    rng = np.random.default_rng(seed)
+    grains = rng.choice(smp.q.shape[0], size=n_ors, replace=False)
+    eng.grow_indexed_region(rows[:4096], smp.U[grains[0]], param1)  # warm-up
+    eng.synchronize()
+    eng.kernel_time(reset=True)
+    t = time.perf_counter()
+    first = None
+    for g in grains:
+        out = eng.grow_indexed_region(rows, smp.U[g], param1)
+        if first is None:
+            first = out
+    wall = time.perf_counter() - t
+    kms, kn = eng.kernel_time(reset=True)
+    rate = n_ors * M / (kms * 1e-3)
+    res = {"orientations": int(n_ors), "voxels": int(M), "evals": int(n_ors * M), "kernel_ms_per_orientation":
+           round(kms / max(kn, 1), 2), "value": round(rate, 1), "unit": "evals/s",
+           "wall_s_per_orientation": round(wall / n_ors, 3),
+           "note": "wall includes the 154 MB of voxel rows up and 180 MB of results down per call"}
+    if cpu:
+        import oracle
+
+        sel = rng.choice(M, size=256, replace=False)
+        stack = oracle.Stack(eng.get_distance_stack((int(geo.detzsize), int(geo.detysize), cfg.rot.size)))
+        ref = oracle.grow(geo, stack, rows[sel], smp.U[grains[0]], cfg.rot, cfg.hkl, param1, nthreads=threads)
+        got = first[sel]
+        ok = bool(np.array_equal(got[:, 3:5], ref[:, 3:5]) and np.array_equal(
+            got[:, 5].astype(np.float32).view(np.uint32), ref[:, 5].astype(np.float32).view(np.uint32))
+                  and np.array_equal(got[:, 6], ref[:, 6]))
+        res["parity"] = {"voxels_checked": 256, "counts_median_flag_equal": ok}
+    log(f"[bench] growth against all {M} voxels: {n_ors} orientations, {kms / max(kn, 1):.1f} ms of kernel each "
+        f"-> {rate:.4g} evals/s (wall {wall / n_ors:.3f} s per orientation incl. transfers)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -814,8 +859,10 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched-voxels", type=int, default=2048, help="voxels per batched compete step; 0 = off")
-    ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--batched-voxels", type=int, default=16384, help="voxels per batched compete step; 0 = off")
+    ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--growth-ors", type=int, default=4,
+                    help="orientations scored against every voxel in the growth leg; 0 = off")
     ap.add_argument("--al", type=int, default=1, help="run the FCC Al 2.5 M-voxel leg (1) or not (0)")
     ap.add_argument("--al-steps", type=int, default=2)
     ap.add_argument("--al-rows", type=int, default=16, help="Al voxels checked against the oracle")
@@ -983,6 +1030,23 @@ def main():
         result["gt_score"] = gt_leg(eng, cfg, args.gt_orientations, not args.no_cpu, threads)
     if args.grow_box > 0:
         result["grow_region"] = grow_leg(eng, cfg, args.grow_box, not args.no_cpu, threads)
+    if args.growth_ors > 0:
+        result["growth_all_voxels"] = growth_all_leg(eng, cfg, args.growth_ors, not args.no_cpu, threads)
+    # the whole jobs of configs 2 and 3 at the measured rates
+    est = {}
+    n_vox = int(smp.pos.shape[0])
+    est["config2_seeding_10k_seeds_s"] = round(1e4 * N / value, 1)
+    if "growth_all_voxels" in result:
+        est["config2_growth_300_ors_all_voxels_s"] = round(smp.q.shape[0] * n_vox / result["growth_all_voxels"]["value"], 1)
+    if "batched" in result:
+        b = result["batched"]
+        evals3 = n_vox * b["mean_candidates_per_voxel"] * b["samples_per_candidate"]
+        est["config3_evals"] = int(evals3)
+        est["config3_all_voxels_s"] = round(evals3 / b["value"], 1)
+    est["note"] = ("measured rates extrapolated to the full configurations: seeding 10k seed voxels x the "
+                   f"{N}-orientation set; growth every grain's orientation against all {n_vox} voxels; batched "
+                   "compete over every voxel's candidate list")
+    result["full_config_estimates"] = est
     if args.mapping_seeds > 0:
         result["mapping"] = mapping_leg(eng, cfg, q, args.mapping_seeds, args.index_nr_seed)
     if args.audit_evals > 0:

@@ -261,6 +261,26 @@ def test_errors_are_reported_not_raised_from_c(engine):
                                     _p1()[:11])
 
 
+def test_launch_configuration_errors_and_reset(engine):
+    """grs_set_launch_config rejects configurations outside the tuner's set
+    (waves per block other than 1, 2, 4; a mask source for kind 0; an unknown
+    source or mapping; an unknown kind) and, with 0 waves per block, forgets a
+    pinned configuration so that grs_get_tuning reports the kind untuned."""
+    from grainrecon_amd import _lib as L
+
+    s = scenes.make_scene(seed=62, ngrains=1, det=128, pixel_mm=6.72e-3 * 8)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    for args in ((0, 3, L.MASK_BITS, False), (0, 4, L.MASK_KEYS, False), (1, 4, 3, False), (3, 4, L.MASK_BITS, False)):
+        with pytest.raises(_lib.GrsError):
+            engine.set_launch_config(*args)
+    engine.set_launch_config(1, 2, L.MASK_TILED, True)
+    t = engine.tuning(1)
+    assert t == {"waves_per_block": 2, "mask_layout": "bits_tiled32", "xcd_map": True, "ms_per_eval": []}
+    engine.set_launch_config(1, 0)
+    assert engine.tuning(1) is None
+
+
 def _expand_local(qc, nr, g):
     """The engine's local-sampling grid (q1 slowest), same IEEE operations."""
     half = (g * nr) / 2.0

@@ -140,3 +140,25 @@ def test_device_normal_equations_equal_the_host_reduction(problem, delta):
         assert abs(dev[k] - host[k]) <= 1e-12 * abs(host[k]), k
     np.testing.assert_array_equal(dev["Agg"], dev["Agg"].T)
     np.testing.assert_array_equal(dev["Akk"], np.transpose(dev["Akk"], (0, 2, 1)))
+
+
+def test_normal_equations_argument_checks_and_empty_spot_set(engine, problem):
+    """grs_spot_normal rejects a negative or non-finite delta, and on a spot
+    set with a grain that has no spots returns zero blocks for that grain
+    (its LM step is then zero)."""
+    from grainrecon_amd import _lib
+
+    fit = problem["fit"]
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(_lib.GrsError):
+            fit.normal(problem["g7"], problem["gr6"], bad)
+    sp = problem["spots"]
+    keep = sp["grain"] != 2
+    sub = F.SpotFit(engine, sp["hkl"][keep], sp["omega"][keep], sp["meas"][keep], sp["grain"][keep], 4)
+    ne = sub.normal(problem["g7"], problem["gr6"], 0.0)
+    assert np.all(ne["Akk"][2] == 0) and np.all(ne["Agk"][2] == 0) and np.all(ne["bk"][2] == 0)
+    assert np.abs(ne["Akk"][0]).max() > 0
+    res = F.levenberg_marquardt(sub, problem["g7"], problem["gr6"] + 1e-4, fix_geometry=True, max_iter=20)
+    np.testing.assert_array_equal(res.gr6[2], problem["gr6"][2] + 1e-4)
+    # the resident spot set of the module's fixture again
+    F.SpotFit(engine, sp["hkl"], sp["omega"], sp["meas"], sp["grain"], 4)
