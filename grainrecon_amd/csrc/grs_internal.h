@@ -96,7 +96,7 @@ __host__ __device__ inline uint32_t grs_tiled_bit(uint32_t om, uint32_t dy, uint
 /* largest k of the device top-k (sortrows + first rows) */
 #define GRS_TOPK_MAX 4096
 /* doubles per voxel of the omega-solve bounds (k_voxel_bounds) */
-#define GRS_VB 12
+#define GRS_VB 14
 
 /* Launchers (return hipError_t). */
 /* d_vb: GRS_VB doubles per voxel, the omega-solve bounds */

@@ -49,7 +49,7 @@
 
 /* LDS doubles of one wave's region of k_fast */
 __host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
-  return (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4;
+  return (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4 + 9;
 }
 
 /* LDS doubles of the one-voxel beam table behind the wave regions: the
@@ -108,7 +108,7 @@ __device__ __forceinline__ void voxel_P(const grs_dgeo *g, const double *pos, do
 }
 
 /* SamposW and the unit beam direction u = grainpos / |grainpos| at
- * (cos omega, sin omega), to ~1e-16 relative (filters only). */
+ * (cos omega, sin omega), to a few 1e-15 relative (filters only). */
 __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, double c, double s, double *sw,
                                            double *u) {
   sw[0] = c * P[0] - s * P[1];
@@ -139,6 +139,9 @@ __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, d
  *   vb[7], vb[8]  Zt >= and Zb <= the detector rows term of every event
  *            (below); +-1e300 (no bound) where K is 1
  *   vb[9], vb[10], vb[11] sin(eta), cos(eta), tan(eta): eta >= asin |e . u|
+ *   vb[12] Ya >= |Yb|, |Yt|, the detector columns term of every event
+ *            (below); 1e300 (no bound) where K is 1
+ *   vb[13] delta >= |u - (1, 0, 0)|
  * All extremes are over the rotation angles the scorer visits, so they hold
  * at every event.  Corners are the detector corners 2 pixels outside the
  * pixel area (g->corner, frame of SamposW).
@@ -162,7 +165,10 @@ __device__ __forceinline__ void beam_terms(const grs_dgeo *g, const double *P, d
  * (L2 u0), and a spot on the detector has a slope between the smallest and
  * largest corner slope (all corners in front of the voxel).  So
  * (diffvec/L2) e_z lies in [s_min u0 - u2 - o_z/L2, s_max u0 - u2 - o_z/L2],
- * whose extremes over omega are Zb, Zt. */
+ * whose extremes over omega are Zb, Zt.
+ * Columns: likewise with the y/x slopes, (diffvec/L2) e_y lies in
+ * [t_min u0 - u1 - o_y/L2, t_max u0 - u1 - o_y/L2] (t the corner y/x slopes),
+ * whose extremes over omega are Yb, Yt. */
 __device__ __forceinline__ double wave_max_d(double v) {
   for (int off = 32; off >= 1; off >>= 1) {
     const double o = __shfl_xor(v, off, 64);
@@ -194,6 +200,7 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
   const double oy = g->RAO - g->P0y, oz = -g->P0z;
   const double on = __builtin_sqrt(oy * oy + oz * oz);
   double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0, ayz = 0.0, zt = -1e300, zb = 1e300;
+  double yt = -1e300, yb = 1e300;
   bool front = true, rows = true;
   for (int32_t om = tid; om < nrot; om += 256) {
     double sw[3], u[3];
@@ -210,16 +217,18 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
     u2hi = u2 > u2hi ? u2 : u2hi;
     const double ay = __builtin_sqrt(u1 * u1 + u2 * u2) * (1.0 + 1e-9) + 1e-12;
     ayz = ay > ayz ? ay : ayz;
-    double cmin = 1.0, smin = 1e300, smax = -1e300;
+    double cmin = 1.0, smin = 1e300, smax = -1e300, tmin = 1e300, tmax = -1e300;
     for (int j = 0; j < 4; j++) {
       const double dx = g->corner[3 * j] - sw0, dy = g->corner[3 * j + 1] - sw1, dz = g->corner[3 * j + 2] - sw2;
       const double cj = (u0 * dx + u1 * dy + u2 * dz) / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
       front = front && (cj > 1e-3);
       cmin = cj < cmin ? cj : cmin;
       rows = rows && (dx > 0.0);
-      const double sl = dz / dx;
+      const double sl = dz / dx, ty = dy / dx;
       smin = sl < smin ? sl : smin;
       smax = sl > smax ? sl : smax;
+      tmin = ty < tmin ? ty : tmin;
+      tmax = ty > tmax ? ty : tmax;
     }
     const double L2 = (g->Lsd - sw0) / u0;
     front = front && (L2 > 0.0) && (ay < 1.0);
@@ -237,9 +246,13 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
     const double t1 = (smax * u0 - u2) - ozl, t0 = (smin * u0 - u2) - ozl;
     zt = t1 > zt ? t1 : zt;
     zb = t0 < zb ? t0 : zb;
+    const double oyl = oy / L2;
+    const double y1 = (tmax * u0 - u1) - oyl, y0 = (tmin * u0 - u1) - oyl;
+    yt = y1 > yt ? y1 : yt;
+    yb = y0 < yb ? y0 : yb;
   }
   /* a NaN anywhere turns the bound off: the comparisons above skip NaNs */
-  const bool nan_seen = __ballot(!(d2max == d2max && kmax == kmax && zt == zt && zb == zb)) != 0;
+  const bool nan_seen = __ballot(!(d2max == d2max && kmax == kmax && zt == zt && zb == zb && yt == yt && yb == yb)) != 0;
   front = __ballot(!front) == 0;
   rows = __ballot(!rows) == 0;
   d2max = wave_max_d(d2max);
@@ -251,12 +264,14 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
   ayz = wave_max_d(ayz);
   zt = wave_max_d(zt);
   zb = wave_min_d(zb);
+  yt = wave_max_d(yt);
+  yb = wave_min_d(yb);
   /* the four waves' maxima, minima and flags (exact, so order-free) */
-  __shared__ double red[4][12];
+  __shared__ double red[4][14];
   if (lane == 0) {
     double *r = red[wv];
     r[0] = d2max, r[1] = kmax, r[2] = e0, r[3] = e1, r[4] = u2lo, r[5] = u2hi, r[6] = ayz, r[7] = zt, r[8] = zb;
-    r[9] = front ? 1.0 : 0.0, r[10] = rows ? 1.0 : 0.0, r[11] = nan_seen ? 1.0 : 0.0;
+    r[9] = front ? 1.0 : 0.0, r[10] = rows ? 1.0 : 0.0, r[11] = nan_seen ? 1.0 : 0.0, r[12] = yt, r[13] = yb;
   }
   __syncthreads();
   if (tid != 0) return;
@@ -272,6 +287,8 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
     ayz = r[6] > ayz ? r[6] : ayz;
     zt = r[7] > zt ? r[7] : zt;
     zb = r[8] < zb ? r[8] : zb;
+    yt = r[12] > yt ? r[12] : yt;
+    yb = r[13] < yb ? r[13] : yb;
     front = front && r[9] != 0.0;
     rows = rows && r[10] != 0.0;
     nan_any = nan_any || r[11] != 0.0;
@@ -299,30 +316,111 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
   o[9] = sn;
   o[10] = __builtin_sqrt(1.0 - sn * sn);
   o[11] = o[10] > 0.0 ? sn / o[10] : 1e300;
+  const double ya = __builtin_fabs(yt) > __builtin_fabs(yb) ? __builtin_fabs(yt) : __builtin_fabs(yb);
+  o[12] = use_rows ? ya * (1.0 + 1e-9) + 1e-12 : 1e300;
+  o[13] = delta;
 }
 
-/* Largest sin(theta) = -cos(beta) of an event that can reach the detector
- * rows, from the voxel bounds: with F = diffvec / L2 = sin(2 theta) /
- * sin(phiy) and e_z = Gz / konst, F e_z lies in [Zb, Zt].  For Gz > 0
- * (Gz < 0 likewise with -Zb): e_z >= Gz/|G|, and phiy = pi/2 - asin(a) -
- * 2 theta with |asin a| <= eta gives F >= sin(2 theta) / cos(2 theta - eta)
- * when 2 theta >= eta, so sin(2 theta) (1 - T sin eta) <= T cos eta
- * cos(2 theta) with T = Zt |G| / Gz, i.e. tan(2 theta) <= T' =
- * T cos(eta) / (1 - T sin(eta)), or 2 theta < eta.  Then
- * sin(theta)^2 = T'^2 / (2 q (q + 1)), q = sqrt(1 + T'^2), with T' raised to
- * tan(eta).  Returns 1 (no bound) where the bound does not apply. */
-__device__ __forceinline__ double rows_sin_theta(double gz, double zt, double zb, double sn, double cn, double te) {
-  if (!(gz != 0.0)) return 1.0;
+/* Refined hardware reciprocal / reciprocal square root (one Newton step,
+ * relative error below 5e-15): used only by the conservative filters, whose
+ * margins are 1e-12 or wider. */
+__device__ __forceinline__ double fast_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  return y * __builtin_fma(-x, y, 2.0);
+}
+__device__ __forceinline__ double fast_rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  return y * __builtin_fma(-0.5 * x, y * y, 1.5);
+}
+
+/* Largest sin(theta) = -cos(beta) of an event that can reach the detector,
+ * from the voxel bounds.  With F = diffvec / L2 = sin(2 theta) / sin(phiy)
+ * and the unit vector e = (0, Gt2, Gt3) / konst, F e_z lies in [Zb, Zt]
+ * (rows) and F e_y in [Yb, Yt] (columns), so F |e_y| <= Ya.
+ *   Rows: for Gz > 0 (Gz < 0 likewise with -Zb) e_z >= Gz/|G| = gz, as
+ *   konst <= |G|, so F <= T = Zt / gz.
+ *   Columns: Gt1/|G| differs from cos(beta) = u . G/|G| by at most
+ *   |u - (1, 0, 0)| <= delta, so |Gt1|/|G| <= s1 + delta for an event in the
+ *   window (s1 >= -cos(beta), the caller's bound before this one), konst^2
+ *   >= |G|^2 (1 - (s1 + delta)^2), e_z^2 <= gz^2 / (1 - (s1 + delta)^2) and
+ *   |e_y| = sqrt(1 - e_z^2) gives F <= T = Ya / sqrt(1 - that bound).
+ * With T the smaller of the two: phiy = pi/2 - asin(a) - 2 theta with
+ * |asin a| <= eta gives F >= sin(2 theta) / cos(2 theta - eta) when
+ * 2 theta >= eta, so sin(2 theta) (1 - T sin eta) <= T cos eta cos(2 theta),
+ * i.e. tan(2 theta) <= T' = T cos(eta) / (1 - T sin(eta)), or 2 theta < eta.
+ * Then sin(theta)^2 = T'^2 / (2 q (q + 1)), q = sqrt(1 + T'^2), with T'
+ * raised to tan(eta).  Returns 1 (no bound) where neither bound applies. */
+__device__ __forceinline__ double det_sin_theta(double gz, double s1, double zt, double zb, double ya, double dl,
+                                                double sn, double cn, double te) {
+  double T = 1e300;
   const double Z = gz > 0.0 ? zt : -zb;
-  if (!(Z > 0.0 && Z < 1e299)) return 1.0;
-  const double T = Z / __builtin_fabs(gz);
+  if (gz != 0.0 && Z > 0.0 && Z < 1e299) T = Z * fast_rcp(__builtin_fabs(gz));
+  if (ya < 1e299) {
+    const double gx = (s1 + 1e-12) + dl;
+    if (gx < 0.99) {
+      const double ez2 = (gz * gz) * fast_rcp(1.0 - gx * gx) * (1.0 + 1e-12);
+      if (ez2 < 0.98) {
+        const double Tc = ya * fast_rsq(1.0 - ez2);
+        T = Tc < T ? Tc : T;
+      }
+    }
+  }
+  if (!(T < 1e299)) return 1.0;
   const double den = 1.0 - T * sn;
   if (!(den > 0.0) || !(cn > 0.0)) return 1.0;
-  double Tp = T * cn / den;
+  double Tp = T * cn * fast_rcp(den);
   Tp = Tp > te ? Tp : te;
-  const double q = __builtin_sqrt(1.0 + Tp * Tp);
-  const double st = Tp / __builtin_sqrt((2.0 * q) * (q + 1.0));
+  const double q2 = 1.0 + Tp * Tp;
+  const double q = q2 * fast_rsq(q2);
+  const double st = Tp * fast_rsq((2.0 * q) * (q + 1.0));
   return st * (1.0 + 1e-9) + 1e-12;
+}
+
+/* grs_filt_acos and grs_filt_atan2 (grs_math.h: the same polynomials and
+ * reductions) with the square root and the quotient taken from the refined
+ * hardware reciprocals, which adds about 1e-14 to their error bounds (5e-12
+ * and 2e-12 rad); NaN outside [-1, 1] as there.  Out of line: inlined, they
+ * raise the register pressure of the whole kernel (spills in the candidate
+ * loop). */
+__device__ __noinline__ double filt_acos(double x) {
+  const double R[8] = GRS_FILT_ASIN_R;
+  const double ax = __builtin_fabs(x);
+  const bool small = ax <= 0.5;
+  const double zb = (1.0 - ax) * 0.5;
+  const double u = small ? x : (zb == 0.0 ? 0.0 : zb * fast_rsq(zb));
+  const double z = small ? x * x : zb;
+  double p = R[7];
+  p = __builtin_fma(p, z, R[6]);
+  p = __builtin_fma(p, z, R[5]);
+  p = __builtin_fma(p, z, R[4]);
+  p = __builtin_fma(p, z, R[3]);
+  p = __builtin_fma(p, z, R[2]);
+  p = __builtin_fma(p, z, R[1]);
+  p = __builtin_fma(p, z, R[0]);
+  const double as = __builtin_fma(u * z, p, u);
+  if (small) return GRS_PIO2_HI - as;
+  return x > 0.0 ? 2.0 * as : GRS_PI_HI - 2.0 * as;
+}
+__device__ __noinline__ double filt_atan2(double y, double x) {
+  const double S[7] = GRS_FILT_ATAN_S;
+  const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+  const double mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+  if (!(mx > 0.0)) return mx == 0.0 ? 0.0 : mx;
+  const bool big = mn > GRS_TAN_PI_8 * mx;
+  const double w = big ? (mn - mx) * fast_rcp(mn + mx) : mn * fast_rcp(mx);
+  const double z = w * w;
+  double p = S[6];
+  p = __builtin_fma(p, z, S[5]);
+  p = __builtin_fma(p, z, S[4]);
+  p = __builtin_fma(p, z, S[3]);
+  p = __builtin_fma(p, z, S[2]);
+  p = __builtin_fma(p, z, S[1]);
+  p = __builtin_fma(p, z, S[0]);
+  double a = __builtin_fma(w * z, p, w);
+  if (big) a = a + 0.5 * GRS_PIO2_HI;
+  if (ay > ax) a = GRS_PIO2_HI - a;
+  if (x < 0.0) a = GRS_PI_HI - a;
+  return y < 0.0 ? -a : a;
 }
 
 __global__ void k_sub_from_q(const grs_dgeo *g, const double *q, int64_t N, double *SUB, int32_t *valid) {
@@ -366,6 +464,10 @@ __global__ void k_math_eval(int fn, const double *x, double *y, int64_t n) {
     case 7: r = grs_asin(v); break;
     case 8: r = grs_filt_acos(v); break;
     case 9: r = grs_filt_atan2(grs_sin(v), grs_cos(v)); break;
+    case 10: r = filt_acos(v); break;
+    case 11: r = filt_atan2(grs_sin(v), grs_cos(v)); break;
+    case 12: r = fast_rcp(v); break;
+    case 13: r = fast_rsq(v); break;
     default: r = 1.0 / v; break;
   }
   y[i] = r;
@@ -577,24 +679,13 @@ __device__ __forceinline__ void grid_counts_out(const grs_eval_args &a, int64_t 
   }
 }
 
-/* Refined hardware reciprocal / reciprocal square root (one Newton step):
- * used only by the conservative position filter. */
-__device__ __forceinline__ double fast_rcp(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  return y * __builtin_fma(-x, y, 2.0);
-}
-__device__ __forceinline__ double fast_rsq(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  return y * __builtin_fma(-0.5 * x, y * y, 1.5);
-}
-
 /* Phase B1: resolve an event without the exact pipeline where that is safe.
  *
  * The spot position is recomputed from the same quantities through exact
  * identities -- sin(theta) = -cos(beta), cos(theta) = sqrt(1 - cos^2 beta),
  * lambda = -4 pi cos(beta) / |G|, sin(phix) = sqrt(1 - cos^2 phix),
  * sin(phix - 2 theta) by the difference formula -- with refined hardware
- * reciprocals (relative error about 1e-16).  Guards send ill-conditioned
+ * reciprocals (relative error below 5e-15).  Guards send ill-conditioned
  * events to the exact pipeline: |sin(phiy)| < 1e-3, |t denominator| < 1e-3,
  * |cos(phix)| > 0.99, G nearly along the beam.  Under the guards both this
  * value and the exact pipeline's are within about 1e-8 pixel of the true spot
@@ -757,18 +848,14 @@ __device__ __noinline__ int exact_event(const grs_dgeo *g, const double *pos, co
   return 1;
 }
 
-/* The filter-grade acos and atan2 out of line: inlined, they raise the
- * register pressure of the whole kernel (spills in the candidate loop). */
-__device__ __noinline__ double filt_acos(double x) { return grs_filt_acos(x); }
-__device__ __noinline__ double filt_atan2(double y, double x) { return grs_filt_atan2(y, x); }
-
 /* Omega solve of one reflection.  With (g0, g1) the in-plane part of a
  * vector that rotates with omega, |G| = gn and r = |(g0, g1)|, the term
  * r cos(omega + phi) / gn, phi = atan2(g1, g0), must lie in [lo_cb, hi_cb]
  * (the caller folds every other part of cos(beta), forward_comp.m:31, and its
  * bounds into the window), so omega + phi must lie in the arcs where
- * cos(psi) is in [lo_cb gn / r, hi_cb gn / r].  The arc ends come from the
- * filter-grade acos and atan2 of grs_math.h (errors below 5e-12 rad); the
+ * cos(psi) is in [lo_cb gn / r, hi_cb gn / r] (gn / r from a refined
+ * reciprocal square root).  The arc ends come from filt_acos and filt_atan2
+ * (errors below 5e-12 rad); the
  * arcs are widened by 1e-6 rad and the index bounds by 1e-6 steps; the
  * resulting ranges of rotation indices [i0, i1] are sorted
  * and disjoint.  Returns their number (at most 4) or -1 when more would be
@@ -776,7 +863,9 @@ __device__ __noinline__ double filt_atan2(double y, double x) { return grs_filt_
 __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1, double gn, double lo_cb,
                                          double hi_cb, int32_t nrot, double idom, int32_t *i0, int32_t *i1) {
   const double twopi = 2.0 * GRS_PI, eps = 1e-6;
-  const double r = __builtin_sqrt(g0 * g0 + g1 * g1);
+  const double r2 = g0 * g0 + g1 * g1;
+  const double ir = r2 > 0.0 ? fast_rsq(r2) : 0.0;
+  const double r = r2 > 0.0 ? r2 * ir : r2 * 0.0; /* NaN stays NaN */
   if (lo_cb != lo_cb || hi_cb != hi_cb || r != r || !(gn > 0.0)) return -1;
   if (!(hi_cb > lo_cb)) return 0;
   double lo[2], hi[2];
@@ -787,7 +876,7 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
     whole = lo_cb <= 1e-12 && hi_cb >= -1e-12;
     if (!whole) return 0;
   } else {
-    const double gr = gn / r;
+    const double gr = gn * ir;
     const double a = lo_cb * gr, b = hi_cb * gr;
     if (!(a < 1.0) || !(b > -1.0)) return 0;
     const double pa = a <= -1.0 ? GRS_PI : filt_acos(a);
@@ -870,6 +959,9 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
   uint32_t *rstart = ring3 + GRS_QCAP;                         /* omega-solve ranges: (i0 << 8 | hkl) */
   uint32_t *rcum = rstart + 4 * H;                             /* exclusive prefix of range lengths */
   double *Pw = gwt + (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H); /* S*pos of the voxel */
+  /* S*U*B of the evaluation: the omega solve reads it from LDS, which keeps
+   * its 18 registers free for the candidate loop */
+  double *sbuf = Pw + 4;
   /* the geometry constants live in LDS: the loop bodies read ~30 of them per
    * batch, and neither scalar registers (spilled) nor vector loads (the
    * compiler cannot prove them unchanged across the loop's stores) serve */
@@ -960,9 +1052,14 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     }
     /* per-reflection terms; with the omega solve also the candidate ranges */
     const bool solve = g->om_solve != 0;
-    if (lane == 0) voxel_P(g, a.pos + 3 * vi, Pw);
+    if (lane == 0) {
+      voxel_P(g, a.pos + 3 * vi, Pw);
+#pragma unroll
+      for (int k = 0; k < 9; k++) sbuf[k] = sub[k];
+    }
+    wave_sync();
     double vW = 0.0, vu2lo = 0.0, vu2hi = 0.0, sK = 1.0, val = 0.0, vbe = 0.0, vga = 0.0;
-    double vzt = 1e300, vzb = -1e300, vsn = 1.0, vcn = 0.0, vte = 0.0;
+    double vzt = 1e300, vzb = -1e300, vsn = 1.0, vcn = 0.0, vte = 0.0, vya = 1e300, vdl = 0.0;
     if (solve) {
       const double *b = a.vb + GRS_VB * vi;
       vW = b[0];
@@ -977,6 +1074,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       vsn = b[9];
       vcn = b[10];
       vte = b[11];
+      vya = b[12];
+      vdl = b[13];
     }
     uint32_t R = 0;
     bool overflow = false;
@@ -986,10 +1085,10 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       int32_t ri0[4], ri1[4];
       if (h < H) {
         double gw[3];
-        grs_Gw(sub, a.hkl + 3 * h, gw);
+        grs_Gw(sbuf, a.hkl + 3 * h, gw);
         double n2 = (gw[0] * gw[0] + gw[1] * gw[1]) + gw[2] * gw[2];
-        double gn = __builtin_sqrt(n2);
-        const double ign = 1.0 / gn;
+        const double ign = fast_rsq(n2);
+        const double gn = n2 * ign;
         double *t = gwt + h * GRS_GWT;
         t[0] = gw[0];
         t[1] = gw[1];
@@ -999,7 +1098,7 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         double ml = lrel + (1e-12 * (4.0 * GRS_PI)) * ign; /* absolute lambda margin */
         if (solve) {
           /* cos(beta) window: beta and lambda bounds with the phase-A margins,
-           * the detector cone and the detector rows.  cos(beta) = u . G/gn lies within rg W
+           * the detector cone and the detector rows and columns.  cos(beta) = u . G/gn lies within rg W
            * of r' cos(omega + phi') / gn + c0 + u2 g2 / gn, where
            * (g0', g1') = (g0 + ga g1, g1 - ga g0) has modulus r' and angle
            * phi' and u2 spans [vb[1], vb[2]] (the voxel bounds), so the
@@ -1009,12 +1108,13 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
           lo_cb = lw > lo_cb ? lw : lo_cb;
           lo_cb = -sK > lo_cb ? -sK : lo_cb;
           const double z = gw[2] * ign;
-          const double st = rows_sin_theta(z, vzt, vzb, vsn, vcn, vte);
+          const double st = det_sin_theta(z, -lo_cb, vzt, vzb, vya, vdl, vsn, vcn, vte);
           lo_cb = -st > lo_cb ? -st : lo_cb;
           double hi_cb = dcb;
           double hw = -(lmin - ml) * gn * inv4pi;
           hi_cb = hw < hi_cb ? hw : hi_cb;
-          const double rg = __builtin_sqrt(gw[0] * gw[0] + gw[1] * gw[1]) * ign;
+          const double q2 = gw[0] * gw[0] + gw[1] * gw[1];
+          const double rg = q2 > 0.0 ? q2 * fast_rsq(q2) * ign : 0.0;
           const double c0 = (val * gw[0] + vbe * gw[1]) * (0.5 * ign);
           const double sa = z * vu2lo, sb = z * vu2hi;
           const double slo = sa < sb ? sa : sb, shi = sa < sb ? sb : sa;

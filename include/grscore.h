@@ -570,8 +570,10 @@ grs_status grs_set_audit(grs_ctx *ctx, int32_t enable);
  * coordinates of the selected events. */
 grs_status grs_get_audit(grs_ctx *ctx, int64_t *counts, double *max_dev);
 /* Evaluates the engine's FP64 math on the device: fn 0 sin, 1 cos, 2 acos,
- * 3 atan, 4 round (half away from zero), 5 sqrt, 6 reciprocal, 7 asin.  Host
- * arrays. */
+ * 3 atan, 4 round (half away from zero), 5 sqrt, 6 reciprocal, 7 asin,
+ * 8 filter-grade acos, 9 filter-grade atan2(sin x, cos x); the omega solve's
+ * own forms: 10 acos, 11 atan2(sin x, cos x), 12 refined reciprocal,
+ * 13 refined reciprocal square root.  Host arrays. */
 grs_status grs_math_device(grs_ctx *ctx, int32_t fn, const double *x, double *y, int64_t n);
 /* Blocks until all work on ctx's stream is done. */
 grs_status grs_synchronize(grs_ctx *ctx);

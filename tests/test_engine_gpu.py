@@ -42,6 +42,32 @@ def test_math_host_equals_device(engine):
     assert np.array_equal(_bits(engine.math_device(6, x)), _bits(1.0 / x))
 
 
+def test_omega_solve_math_error_bounds(engine):
+    """The omega solve's own acos and atan2 (the filter-grade polynomials with
+    refined hardware reciprocals) keep the filter-grade bounds, 5e-12 and
+    3e-12 rad, and NaN outside [-1, 1]; the refined reciprocal and reciprocal
+    square root that the conservative filters use are within 5e-15 relative
+    (their margins are 1e-12 or wider)."""
+    rng = np.random.default_rng(21)
+    xa = np.concatenate([np.linspace(-1.0, 1.0, 200001), rng.uniform(-1, 1, 20000), 1 - rng.random(2000) ** 8,
+                         -1 + rng.random(2000) ** 8, [-1.0, -0.5, 0.0, 0.5, 1.0]])
+    xa = np.concatenate([xa, np.nextafter(0.5, 1.0) * np.array([1.0, -1.0]), np.nextafter(0.5, 0.0) * np.array([1.0, -1.0])])
+    ya = engine.math_device(10, xa)
+    assert np.max(np.abs(ya - np.arccos(xa))) < 5e-12
+    assert np.isnan(engine.math_device(10, np.array([1.0000000000000002, -1.5, np.nan]))).all()
+    special = np.arange(-8, 9) * (np.pi / 8)
+    xt = np.concatenate([np.linspace(-np.pi, np.pi, 200001), rng.uniform(-np.pi, np.pi, 20000), special,
+                         (special[:, None] + np.array([-1e-9, 1e-9, -1e-15, 1e-15])).ravel()])
+    xt = xt[(xt > -np.pi) & (xt <= np.pi)]
+    yt = engine.math_device(11, xt)
+    assert np.max(np.abs((yt - xt + np.pi) % (2 * np.pi) - np.pi)) < 3e-12
+    x = (1.0 + rng.random(100000)) * 10.0 ** rng.integers(-30, 30, 100000) * rng.choice([-1.0, 1.0], 100000)
+    assert np.max(np.abs(engine.math_device(12, x) * x - 1.0)) < 5e-15
+    xp = np.abs(x)
+    r = engine.math_device(13, xp)
+    assert np.max(np.abs(r * np.sqrt(xp) - 1.0)) < 5e-15
+
+
 def _variant_scene(name):
     kw = dict(seed=31, ngrains=3)
     if name == "tilted":
