@@ -45,7 +45,7 @@
 #define GRS_RING_DOUBLES (3 * GRS_QCAP * 4 / 8) /* three u32 rings per wave */
 #define GRS_RANGE_DOUBLES(H) (4 * (int64_t)(H) + 1) /* 8H + 2 u32: omega-solve ranges */
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
-#define GRS_BT 14       /* doubles per omega of the beam table: the omega_terms (12), cos, sin */
+#define GRS_BT 12       /* doubles per omega of the beam table: the omega_terms (10), cos, sin */
 
 /* LDS doubles of one wave's region of k_fast */
 __host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
@@ -379,10 +379,8 @@ __device__ __forceinline__ double det_sin_theta(double gz, double s1, double zt,
 /* grs_filt_acos and grs_filt_atan2 (grs_math.h: the same polynomials and
  * reductions) with the square root and the quotient taken from the refined
  * hardware reciprocals, which adds about 1e-14 to their error bounds (5e-12
- * and 2e-12 rad); NaN outside [-1, 1] as there.  Out of line: inlined, they
- * raise the register pressure of the whole kernel (spills in the candidate
- * loop). */
-__device__ __noinline__ double filt_acos(double x) {
+ * and 2e-12 rad); NaN outside [-1, 1] as there. */
+__device__ __forceinline__ double acos_core(double x) {
   const double R[8] = GRS_FILT_ASIN_R;
   const double ax = __builtin_fabs(x);
   const bool small = ax <= 0.5;
@@ -401,7 +399,7 @@ __device__ __noinline__ double filt_acos(double x) {
   if (small) return GRS_PIO2_HI - as;
   return x > 0.0 ? 2.0 * as : GRS_PI_HI - 2.0 * as;
 }
-__device__ __noinline__ double filt_atan2(double y, double x) {
+__device__ __forceinline__ double atan2_core(double y, double x) {
   const double S[7] = GRS_FILT_ATAN_S;
   const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
   const double mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
@@ -421,6 +419,27 @@ __device__ __noinline__ double filt_atan2(double y, double x) {
   if (ay > ax) a = GRS_PIO2_HI - a;
   if (x < 0.0) a = GRS_PI_HI - a;
   return y < 0.0 ? -a : a;
+}
+/* the device math test's entries (fn 10, 11) */
+__device__ __noinline__ double filt_acos(double x) { return acos_core(x); }
+__device__ __noinline__ double filt_atan2(double y, double x) { return atan2_core(y, x); }
+
+/* The omega solve's three angles in one out-of-line call: acos(a) (pi at
+ * a <= -1), acos(b) (0 at b >= 1) and atan2(y, x), the two polynomial
+ * chains of acos side by side.  Inlined, they raise the register pressure
+ * of the whole kernel (spills in the candidate loop); three calls cost 2%
+ * of the seeding launch more than one. */
+struct arc_angles {
+  double pa, pb, phi;
+};
+__device__ __noinline__ arc_angles arc_ends(double a, double b, double y, double x) {
+  arc_angles r;
+  r.pa = acos_core(a);
+  r.pb = acos_core(b);
+  r.phi = atan2_core(y, x);
+  if (a <= -1.0) r.pa = GRS_PI;
+  if (b >= 1.0) r.pb = 0.0;
+  return r;
 }
 
 __global__ void k_sub_from_q(const grs_dgeo *g, const double *q, int64_t N, double *SUB, int32_t *valid) {
@@ -713,11 +732,12 @@ __device__ __forceinline__ bool res_round(double r, double *k) {
 }
 
 /* the per (voxel, omega) terms of the spot construction (forward_comp.m:
- * 36-54): the beam direction u, SamposW(2:3), Lsd - SamposW(1), center(2:3)
- * = L u(2:3) / u0, L2 = (Lsd - SamposW(1)) / u0 (cos(alpha) = u0), the t
- * numerator and the offsets of the tilted-detector intersection */
+ * 36-54): the beam direction u, k0 = Lsd - SamposW(1), d = center(2:3) -
+ * SamposW(2:3) with center(2:3) = L u(2:3) / u0, L2 = (Lsd - SamposW(1)) /
+ * u0 (cos(alpha) = u0), the t numerator and the offsets of the
+ * tilted-detector intersection */
 struct om_terms {
-  double u0, u1, u2, sw1, sw2, k0, ctr1, ctr2, L2, tnum, ov1, ov2;
+  double u0, u1, u2, k0, d1, d2, L2, tnum, ov1, ov2;
 };
 
 __device__ __forceinline__ void omega_terms(const grs_dgeo *g, const double *P, double c, double s, om_terms *w) {
@@ -728,10 +748,8 @@ __device__ __forceinline__ void omega_terms(const grs_dgeo *g, const double *P, 
   w->u1 = uv[1];
   w->u2 = uv[2];
   w->k0 = g->Lsd - swv[0];
-  w->sw1 = swv[1];
-  w->sw2 = swv[2];
-  w->ctr1 = g->L * uv[1] * iu0;
-  w->ctr2 = g->L * uv[2] * iu0;
+  w->d1 = g->L * uv[1] * iu0 - swv[1];
+  w->d2 = g->L * uv[2] * iu0 - swv[2];
   w->L2 = w->k0 * iu0;
   const double *R = g->R;
   w->tnum = __builtin_fma(R[0], w->k0, __builtin_fma(R[3], (g->dety00 - g->RAO) - swv[1], R[6] * (g->detz00 - swv[2])));
@@ -740,22 +758,20 @@ __device__ __forceinline__ void omega_terms(const grs_dgeo *g, const double *P, 
 }
 
 __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s, const double *P, const double *bt,
-                                           const double *gt, b1_out *o) {
+                                           const double *gt, const double *thr, b1_out *o) {
   om_terms w;
   if (bt) {
     /* the beam table row of this omega: the same values as omega_terms */
     w.u0 = bt[0];
     w.u1 = bt[1];
     w.u2 = bt[2];
-    w.sw1 = bt[3];
-    w.sw2 = bt[4];
-    w.k0 = bt[5];
-    w.ctr1 = bt[6];
-    w.ctr2 = bt[7];
-    w.L2 = bt[8];
-    w.tnum = bt[9];
-    w.ov1 = bt[10];
-    w.ov2 = bt[11];
+    w.k0 = bt[3];
+    w.d1 = bt[4];
+    w.d2 = bt[5];
+    w.L2 = bt[6];
+    w.tnum = bt[7];
+    w.ov1 = bt[8];
+    w.ov2 = bt[9];
   } else {
     omega_terms(g, P, c, s, &w);
   }
@@ -766,13 +782,10 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
   const double rg = gt[3], g2 = gt[4];
   const double cb = __builtin_fma(u0, G0, __builtin_fma(u1, G1, u2 * G2)) * rg;
   const double lam = (-4.0 * GRS_PI) * cb * rg;
-  const double lm = 1e-9 * g->lambda_max;
   /* outside the window by more than the margins: not selected */
-  if (!(cb < 1e-12 && cb > g->cos_beta_max - 1e-12 && lam > g->lambda_min - lm && lam < g->lambda_max + lm))
-    return cb == cb && lam == lam ? 0 : 2;
+  if (!(cb < 1e-12 && cb > thr[0] && lam > thr[1] && lam < thr[2])) return cb == cb && lam == lam ? 0 : 2;
   /* inside by less than the margins: the exact pipeline decides */
-  if (!(cb < -1e-12 && cb > g->cos_beta_max + 1e-12 && lam > g->lambda_min + lm && lam < g->lambda_max - lm))
-    return 2;
+  if (!(cb < -1e-12 && cb > thr[3] && lam > thr[4] && lam < thr[5])) return 2;
   o->win = 1;
   const double omc = __builtin_fma(-cb, cb, 1.0);
   const double cth = omc * fast_rsq(omc);
@@ -782,10 +795,10 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
   if (!(nv2 >= 1e-12 * g2)) return 2;
   /* the remaining per (voxel, omega) terms from u and SamposW: center is
    * grainpos * L / grainpos(1), so center/|center| = u, center(2:3) =
-   * L u(2:3) / u0, L2 = (Lsd - SW1) / u0 (cos(alpha) = u0), and the t
-   * numerator and offsets of forward_comp.m:51-54 */
-  const double k0 = w.k0, sw1 = w.sw1, sw2 = w.sw2;
-  const double ctr1 = w.ctr1, ctr2 = w.ctr2, L2 = w.L2;
+   * L u(2:3) / u0, L2 = (Lsd - SW1) / u0 (cos(alpha) = u0); K_out(2:3) =
+   * (dety22, detz22) - SamposW(2:3) = d + diffvec e; and the t numerator
+   * and offsets of forward_comp.m:51-54 */
+  const double k0 = w.k0, d1 = w.d1, d2 = w.d2, L2 = w.L2;
   const double *R = g->R;
   const double tnum = w.tnum;
   const double ov1 = w.ov1, ov2 = w.ov2;
@@ -797,19 +810,19 @@ __device__ __forceinline__ int b1_classify(const grs_dgeo *g, double c, double s
   const double sphy = __builtin_fma(sph, c2t, -ap * s2t);
   if (!(__builtin_fabs(sphy) >= 1e-3)) return 2;
   const double diffvec = L2 * s2t * fast_rcp(sphy);
-  const double y22 = __builtin_fma(diffvec * G1, inv, ctr1);
-  const double z22 = __builtin_fma(diffvec * G2, inv, ctr2);
-  const double k1 = y22 - sw1, k2 = z22 - sw2;
-  const double ik = fast_rsq(__builtin_fma(k0, k0, __builtin_fma(k1, k1, k2 * k2)));
-  const double K0 = k0 * ik, K1 = k1 * ik, K2 = k2 * ik;
-  const double tden = __builtin_fma(R[0], K0, __builtin_fma(R[3], K1, R[6] * K2));
-  if (!(__builtin_fabs(tden) >= 1e-3)) return 2;
+  /* K_out = (Lsd, y22, z22) - SamposW unnormalised: t K_out is the same
+   * vector for K_out and K_out / |K_out|, and the guard |R(:,1) . K_out| >=
+   * 1e-3 |K_out| is taken squared */
+  const double k1 = __builtin_fma(diffvec * G1, inv, d1);
+  const double k2 = __builtin_fma(diffvec * G2, inv, d2);
+  const double kk = __builtin_fma(k0, k0, __builtin_fma(k1, k1, k2 * k2));
+  const double tden = __builtin_fma(R[0], k0, __builtin_fma(R[3], k1, R[6] * k2));
+  if (!(tden * tden >= 1e-6 * kk)) return 2;
   const double t = tnum * fast_rcp(tden);
-  const double w0 = __builtin_fma(t, K0, -k0), w1 = __builtin_fma(t, K1, ov1), w2 = __builtin_fma(t, K2, ov2);
-  const double Y = __builtin_fma(R[1], w0, __builtin_fma(R[4], w1, R[7] * w2));
-  const double Z = __builtin_fma(R[2], w0, __builtin_fma(R[5], w1, R[8] * w2));
-  const double ry = __builtin_fma(-Y, g->ipy, g->dety0);
-  const double rz = __builtin_fma(-Z, g->ipz, g->detz0);
+  const double w0 = __builtin_fma(t, k0, -k0), w1 = __builtin_fma(t, k1, ov1), w2 = __builtin_fma(t, k2, ov2);
+  /* thr[6..11]: -R(:,2) / pixelysize and -R(:,3) / pixelzsize */
+  const double ry = __builtin_fma(thr[6], w0, __builtin_fma(thr[7], w1, __builtin_fma(thr[8], w2, g->dety0)));
+  const double rz = __builtin_fma(thr[9], w0, __builtin_fma(thr[10], w1, __builtin_fma(thr[11], w2, g->detz0)));
   o->ry = ry;
   o->rz = rz;
   double ky, kz;
@@ -879,9 +892,8 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
     const double gr = gn * ir;
     const double a = lo_cb * gr, b = hi_cb * gr;
     if (!(a < 1.0) || !(b > -1.0)) return 0;
-    const double pa = a <= -1.0 ? GRS_PI : filt_acos(a);
-    const double pb = b >= 1.0 ? 0.0 : filt_acos(b);
-    const double phi = filt_atan2(g1, g0);
+    const arc_angles an = arc_ends(a, b, g1, g0);
+    const double pa = an.pa, pb = an.pb, phi = an.phi;
     if (pb <= eps && pa >= GRS_PI - eps) {
       whole = true;
     } else if (pb <= eps) {
@@ -907,7 +919,12 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
   }
   int n = 0;
   for (int k = 0; k < niv; k++) {
-    double m = __builtin_floor((g->om0 - hi[k]) / twopi);
+    /* the last period whose arc ends at or before om0.  The product rounds
+     * differently from the quotient only next to an integer, where m may be
+     * one less (that arc ends 2 pi below om0 and is skipped; the periods
+     * the scan can need, m..m+2 then, are still within the four steps) or
+     * one more (the period passed over ends about 2 pi below om0) */
+    double m = __builtin_floor((g->om0 - hi[k]) * (1.0 / twopi));
     for (int it = 0; it < 4; it++, m += 1.0) {
       const double L = lo[k] + twopi * m, U = hi[k] + twopi * m;
       if (L > g->om_last + 1e-9) break;
@@ -966,7 +983,23 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
    * batch, and neither scalar registers (spilled) nor vector loads (the
    * compiler cannot prove them unchanged across the loop's stores) serve */
   __shared__ grs_dgeo sg;
-  if (threadIdx.x == 0) sg = *a.g;
+  /* b1_classify's window thresholds (outer, then inner margins) and the
+   * detector rows scaled to pixels */
+  __shared__ double b1thr[12];
+  if (threadIdx.x == 0) {
+    sg = *a.g;
+    const double lm = 1e-9 * sg.lambda_max;
+    b1thr[0] = sg.cos_beta_max - 1e-12;
+    b1thr[1] = sg.lambda_min - lm;
+    b1thr[2] = sg.lambda_max + lm;
+    b1thr[3] = sg.cos_beta_max + 1e-12;
+    b1thr[4] = sg.lambda_min + lm;
+    b1thr[5] = sg.lambda_max - lm;
+    for (int k = 0; k < 3; k++) {
+      b1thr[6 + k] = -sg.R[3 * k + 1] * sg.ipy;
+      b1thr[9 + k] = -sg.R[3 * k + 2] * sg.ipz;
+    }
+  }
   __syncthreads();
   const grs_dgeo *g = &sg;
   const int wpb = (int)(blockDim.x >> 6);
@@ -986,17 +1019,15 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       r[0] = w.u0;
       r[1] = w.u1;
       r[2] = w.u2;
-      r[3] = w.sw1;
-      r[4] = w.sw2;
-      r[5] = w.k0;
-      r[6] = w.ctr1;
-      r[7] = w.ctr2;
-      r[8] = w.L2;
-      r[9] = w.tnum;
-      r[10] = w.ov1;
-      r[11] = w.ov2;
-      r[12] = a.cs[2 * om];
-      r[13] = a.cs[2 * om + 1];
+      r[3] = w.k0;
+      r[4] = w.d1;
+      r[5] = w.d2;
+      r[6] = w.L2;
+      r[7] = w.tnum;
+      r[8] = w.ov1;
+      r[9] = w.ov2;
+      r[10] = a.cs[2 * om];
+      r[11] = a.cs[2 * om + 1];
     }
     __syncthreads();
     btab = bt;
@@ -1236,8 +1267,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
         o.dy = o.dz = 0;
         o.win = 0;
         const double *brow = btab ? btab + GRS_BT * om : nullptr;
-        const double cc = brow ? brow[12] : a.cs[2 * om], ss = brow ? brow[13] : a.cs[2 * om + 1];
-        st = b1_classify(g, cc, ss, Pw, brow, t, &o);
+        const double cc = brow ? brow[10] : a.cs[2 * om], ss = brow ? brow[11] : a.cs[2 * om + 1];
+        st = b1_classify(g, cc, ss, Pw, brow, t, b1thr, &o);
         if (st == 1)
           idx = tiled ? grs_tiled_bit((uint32_t)om, (uint32_t)o.dy, (uint32_t)o.dz, ntz, nty)
                       : (uint32_t)(o.dz - 1) + (uint32_t)(o.dy - 1) * nz32 + (uint32_t)om * nzny32;
