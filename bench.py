@@ -55,8 +55,10 @@ def flop_per_eval(nrot: int, H: int) -> int:
 F_SETUP = 136    # per evaluation: q0, U(q), S*U*B (k_sub_from_q)
 F_SOLVE = 182    # per (evaluation, reflection): G, |G|, thresholds, the cos(beta) window with the detector
                  # rows and columns bound, om_ranges (a refined rcp/rsq counts its Newton step: 4 and 6)
-F_CAND = 27      # per candidate event: Gt, |Gt|, cos(beta), lambda and the window tests (b1_classify)
-F_SPOT = 90      # per candidate inside the windows: spot, tilted-detector intersection, pixel
+F_CAND = 20      # per candidate event: Gt, cos(beta), lambda and the window tests (b1_classify; the
+                 # thresholds come precomputed per block)
+F_SPOT = 79      # per candidate inside the windows: spot, tilted-detector intersection (K_out
+                 # unnormalised), pixel
 F_EXACT = 242    # per event sent to the exact pipeline: per-(voxel, omega) terms 124 + event 118
 
 

@@ -34,7 +34,7 @@ GT_MATCH_FUN = 1
 MASK_BITS = 0
 MASK_KEYS = 1
 MASK_TILED = 2
-TUNE_MAX = 18
+TUNE_MAX = 24
 
 _SIGS = {
     "grs_version": (ctypes.c_char_p, []),

@@ -38,7 +38,7 @@ struct tune_cfg {
 
 struct grs_tune_state {
   bool done = false;
-  tune_cfg cfg = {4, GRS_MASK_BITS, 0};
+  tune_cfg cfg = {8, GRS_MASK_BITS, 0};
   int32_t ncfg = 0;
   double ms_per_eval[GRS_TUNE_MAX] = {0};
 };
@@ -478,12 +478,12 @@ static grs_status check_ready(grs_ctx *c) {
 /* configurations: waves per block x block-to-evaluation mapping, and for
  * the counts-only kinds x hit-test source (bit mask, keys, tiled bit mask) */
 static int tune_configs(const grs_ctx *c, int kind, tune_cfg *out) {
-  static const int32_t wpbs[3] = {4, 2, 1};
+  static const int32_t wpbs[4] = {8, 4, 2, 1};
   int n = 0;
   for (int32_t x = 0; x < 2; x++)
     for (int32_t m = 0; m < (kind >= 1 ? 3 : 1); m++) {
       if (m == GRS_MASK_TILED && !c->tbits_ok) continue;
-      for (int w = 0; w < 3; w++) out[n++] = tune_cfg{wpbs[w], m, x};
+      for (int w = 0; w < 4; w++) out[n++] = tune_cfg{wpbs[w], m, x};
     }
   return n;
 }
@@ -494,8 +494,9 @@ static void apply_cfg(grs_eval_args *a, const tune_cfg &k) {
   a->xcd_map = k.xcd;
 }
 
-/* evaluations per tuning part: enough to keep every resident wave busy
- * (3 waves per SIMD, 4 SIMDs per CU) for four evaluations */
+/* evaluations per tuning part: enough to keep 12 resident waves per CU
+ * busy for four evaluations (the 17 parts of the median kernel then fit a
+ * 262,144-orientation seeding call) */
 static int64_t tune_part_min(const grs_ctx *c) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
@@ -531,7 +532,7 @@ static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t wa
     return kind >= 1 ? grs_launch_fast_counts(&x, c->H, waves, wpb, c->st) : grs_launch_fast(&x, c->H, waves, wpb, c->st);
   };
   if (!c->autotune || t.done || E < (int64_t)(2 * ncfg + 1) * tune_part_min(c)) {
-    tune_cfg k = t.done ? t.cfg : tune_cfg{4, GRS_MASK_BITS, 0};
+    tune_cfg k = t.done ? t.cfg : tune_cfg{8, GRS_MASK_BITS, 0};
     if (k.mask == GRS_MASK_TILED && !c->tbits_ok) k.mask = GRS_MASK_BITS;
     apply_cfg(&a, k);
     HIPCHK(c, launch(a, k.wpb));
@@ -751,7 +752,7 @@ grs_status grs_forward_comp_device_noalloc(grs_ctx *c, const double *pos, const 
   uint32_t *s_keys = reinterpret_cast<uint32_t *>(base + kNaKeys);
   /* the launch configuration the autotuner chose for this class, if any
    * (no tuning here: it records events) */
-  tune_cfg kc = {4, GRS_MASK_BITS, 0};
+  tune_cfg kc = {8, GRS_MASK_BITS, 0};
   auto it = c->tunes.find(tune_key(c, 0));
   if (it != c->tunes.end() && it->second.done) kc = it->second.cfg;
   HIPCHK(c, hipMemcpyAsync(s_pos, pos, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
@@ -2827,7 +2828,8 @@ grs_status grs_set_launch_config(grs_ctx *c, int32_t kind, int32_t waves_per_blo
     c->tunes.erase(key);
     return GRS_OK;
   }
-  if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4) || mask_layout < 0 || mask_layout > 2 ||
+  if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4 || waves_per_block == 8) ||
+      mask_layout < 0 || mask_layout > 2 ||
       (kind == 0 && mask_layout != GRS_MASK_BITS) || xcd_map < 0 || xcd_map > 1)
     return fail(c, GRS_ERR_ARG, "invalid launch configuration");
   grs_tune_state t;

@@ -36,9 +36,9 @@
 
 #include "grs_internal.h"
 
-#define GRS_WPB 4      /* largest number of waves per block of k_fast (the launch chooses 1, 2 or 4) */
+#define GRS_WPB 8      /* largest number of waves per block of k_fast (the launch chooses 1, 2, 4 or 8) */
 #ifndef GRS_FAST_MIN_WAVES
-#define GRS_FAST_MIN_WAVES 3 /* resident waves per SIMD k_fast is compiled for */
+#define GRS_FAST_MIN_WAVES 4 /* resident waves per SIMD k_fast is compiled for (128 VGPRs) */
 #endif
 #define GRS_QCAP 128   /* LDS ring slots per wave */
 #define GRS_GWT 5      /* doubles per hkl row in LDS: G (3), 1/|G|, |G|^2 */
@@ -53,11 +53,12 @@ __host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
 }
 
 /* LDS doubles of the one-voxel beam table behind the wave regions: the
- * table is kept when the block's dynamic LDS stays within 64 KB */
+ * table is kept when the block's dynamic LDS stays within 160 KB (one CU's
+ * LDS; with 8 waves per block two blocks, 16 waves, share a CU) */
 __host__ __device__ inline int64_t fast_tab_doubles(int32_t map, int32_t nrot, int32_t H, int wpb) {
   if (map != GRS_MAP_ONE_VOXEL || nrot <= 0) return 0;
   const int64_t t = (int64_t)GRS_BT * nrot;
-  return ((int64_t)wpb * fast_wave_doubles(H) + t) * (int64_t)sizeof(double) <= 65536 ? t : 0;
+  return ((int64_t)wpb * fast_wave_doubles(H) + t) * (int64_t)sizeof(double) <= 163840 ? t : 0;
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -1702,8 +1703,17 @@ static int device_cus() {
 /* Persistent launch: at most as many waves as can be resident at once. */
 template <bool MEDIAN, int MAP, bool AUDIT = false>
 static hipError_t launch_fast_impl(const grs_eval_args *a, int32_t H, int64_t n_waves, int wpb, hipStream_t st) {
-  if (wpb != 1 && wpb != 2 && wpb != 4) return hipErrorInvalidValue;
+  if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return hipErrorInvalidValue;
+  /* every wave of the launch needs a scratch slot: fewer slots than a block
+   * has waves take smaller blocks */
+  while (wpb > 1 && n_waves < wpb) wpb >>= 1;
   size_t lds = fast_lds(H, wpb, a);
+  if (lds > 65536) {
+    /* above the default dynamic LDS limit of a launch */
+    hipError_t e = hipFuncSetAttribute((const void *)k_fast<MEDIAN, MAP, AUDIT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   int64_t resident = (int64_t)device_cus() * fast_blocks_per_cu<MEDIAN, MAP, AUDIT>(lds, wpb);
   int64_t blocks = n_waves / wpb; /* n_waves is a multiple of 4: no wave beyond the scratch slots */
   if (blocks > resident) blocks = resident;
@@ -1794,8 +1804,9 @@ hipError_t grs_launch_check_keys(const uint32_t *d_keys, int64_t n, uint32_t *d_
 
 int64_t grs_fast_wave_slots(int device) {
   (void)device;
-  /* upper bound over the instantiations: the scratch is indexed by wave */
-  return (int64_t)device_cus() * 16 * GRS_WPB;
+  /* upper bound over the instantiations (at most 32 resident waves per
+   * CU): the scratch is indexed by wave */
+  return (int64_t)device_cus() * 64;
 }
 
 hipError_t grs_launch_fill_u64(unsigned long long *d_p, int64_t n, unsigned long long v, hipStream_t st) {

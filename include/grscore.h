@@ -519,7 +519,7 @@ grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, in
  * the counts-only grid kernel of grs_index_seeds -- for the current geometry
  * (reflection and rotation counts) that is large enough splits its
  * evaluations among the configurations, times them and keeps the fastest:
- * waves per block (4, 2, 1) x the block-to-evaluation mapping (blocks in
+ * waves per block (8, 4, 2, 1) x the block-to-evaluation mapping (blocks in
  * launch order, or each XCD given a contiguous run of evaluations), and for
  * kinds 1 and 2 x the hit-test source (GRS_MASK_BITS: the bit mask, 1 bit per
  * pixel in stack order; GRS_MASK_KEYS: the distance keys; GRS_MASK_TILED:
@@ -530,18 +530,18 @@ grs_status grs_kernel_time(grs_ctx *ctx, double *ms_total, int64_t *launches, in
 #define GRS_MASK_BITS 0
 #define GRS_MASK_KEYS 1
 #define GRS_MASK_TILED 2
-#define GRS_TUNE_MAX 18
+#define GRS_TUNE_MAX 24
 grs_status grs_set_autotune(grs_ctx *ctx, int32_t enable);
 /* The chosen configuration of a kind (GRS_ERR_STATE before tuning or
  * pinning): waves per block, hit-test source (GRS_MASK_*), XCD mapping (0/1),
  * and the measured milliseconds per evaluation of each of its n_configs
  * candidates (ms_per_eval holds GRS_TUNE_MAX values; order: for mapping 0
  * then 1, for each source in GRS_MASK_* order (kind 0: the bit mask only;
- * the tiled mask only when it exists), waves per block 4, 2, 1). */
+ * the tiled mask only when it exists), waves per block 8, 4, 2, 1). */
 grs_status grs_get_tuning(grs_ctx *ctx, int32_t kind, int32_t *waves_per_block, int32_t *mask_layout, int32_t *xcd_map,
                           int32_t *n_configs, double *ms_per_eval);
 /* Pins the launch configuration of a kind (no tuning for it; n_configs 0):
- * waves per block 1, 2 or 4, a GRS_MASK_* source (GRS_MASK_BITS for kind 0),
+ * waves per block 1, 2, 4 or 8, a GRS_MASK_* source (GRS_MASK_BITS for kind 0),
  * XCD mapping 0 or 1.  waves_per_block = 0 forgets the kind's configuration,
  * so that the next large call tunes again.  No reference counterpart. */
 grs_status grs_set_launch_config(grs_ctx *ctx, int32_t kind, int32_t waves_per_block, int32_t mask_layout,

@@ -289,7 +289,7 @@ def test_errors_are_reported_not_raised_from_c(engine):
 
 def test_launch_configuration_errors_and_reset(engine):
     """grs_set_launch_config rejects configurations outside the tuner's set
-    (waves per block other than 1, 2, 4; a mask source for kind 0; an unknown
+    (waves per block other than 1, 2, 4, 8; a mask source for kind 0; an unknown
     source or mapping; an unknown kind) and, with 0 waves per block, forgets a
     pinned configuration so that grs_get_tuning reports the kind untuned."""
     from grainrecon_amd import _lib as L
@@ -446,9 +446,9 @@ def test_autotuned_launches_give_identical_results(engine):
         np.testing.assert_array_equal(x, y)
     t0, t1 = engine.tuning(0), engine.tuning(1)
     # kind 0: waves per block x XCD mapping; kind 1 also x bit mask / keys / tiled bit mask
-    assert t0 is not None and t0["waves_per_block"] in (1, 2, 4) and len(t0["ms_per_eval"]) == 6
+    assert t0 is not None and t0["waves_per_block"] in (1, 2, 4, 8) and len(t0["ms_per_eval"]) == 8
     assert t0["mask_layout"] == "bits"
-    assert t1 is not None and len(t1["ms_per_eval"]) == 18 and all(v > 0 for v in t1["ms_per_eval"])
+    assert t1 is not None and len(t1["ms_per_eval"]) == 24 and all(v > 0 for v in t1["ms_per_eval"])
     assert t1["mask_layout"] in ("bits", "keys", "bits_tiled32")
     # tuned launches afterwards, still identical
     for x, y in zip(engine.compete_csr(cpos, rows, cands, 8, 0.002), ref_c):
@@ -485,7 +485,7 @@ def test_every_pinned_launch_configuration_gives_identical_results(engine):
             ref_i = engine.index_seeds(seeds, nr_seed=6)
             print(f"det {det}: default configuration done", flush=True)
             for xcd in (False, True):
-                for wpb in (4, 2, 1):
+                for wpb in (8, 4, 2, 1):
                     engine.set_launch_config(0, wpb, L.MASK_BITS, xcd)
                     for x, y in zip(engine.forward_comp(pos, q), ref):
                         np.testing.assert_array_equal(x, y)
