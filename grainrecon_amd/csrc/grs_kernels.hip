@@ -868,12 +868,13 @@ __device__ __noinline__ int exact_event(const grs_dgeo *g, const double *pos, co
  * (the caller folds every other part of cos(beta), forward_comp.m:31, and its
  * bounds into the window), so omega + phi must lie in the arcs where
  * cos(psi) is in [lo_cb gn / r, hi_cb gn / r] (gn / r from a refined
- * reciprocal square root).  The arc ends come from filt_acos and filt_atan2
- * (errors below 5e-12 rad); the
- * arcs are widened by 1e-6 rad and the index bounds by 1e-6 steps; the
- * resulting ranges of rotation indices [i0, i1] are sorted
- * and disjoint.  Returns their number (at most 4) or -1 when more would be
- * needed (the caller then scans every event). */
+ * reciprocal square root).  The arc angles come from one call of
+ * arc_ends (errors below 5e-12 rad); the arcs are widened by 1e-6 rad and
+ * the index bounds by 1e-6 steps.  The ranges of rotation indices [i0, i1]
+ * land in six fixed slots (two arcs x three periods) and are disjoint, in
+ * no particular order.  Returns the bit mask of the slots in use (at most
+ * four) or -1 when more would be needed (the caller then scans every
+ * event). */
 __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1, double gn, double lo_cb,
                                          double hi_cb, int32_t nrot, double idom, int32_t *i0, int32_t *i1) {
   const double twopi = 2.0 * GRS_PI, eps = 1e-6;
@@ -895,13 +896,19 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
     if (!(a < 1.0) || !(b > -1.0)) return 0;
     const arc_angles an = arc_ends(a, b, g1, g0);
     const double pa = an.pa, pb = an.pb, phi = an.phi;
-    if (pb <= eps && pa >= GRS_PI - eps) {
+    /* the gaps between the two arcs, around psi = 0 and psi = pi: one of at
+     * most 1e-5 rotation steps is closed (its events join the candidates),
+     * so that the index ranges of different arcs and periods never share
+     * an index */
+    const bool gap0 = (2.0 * (pb - eps)) * idom <= 1e-5;
+    const bool gap_pi = (twopi - 2.0 * (pa + eps)) * idom <= 1e-5;
+    if (gap0 && gap_pi) {
       whole = true;
-    } else if (pb <= eps) {
+    } else if (gap0) {
       lo[0] = (-pa - eps) - phi;
       hi[0] = (pa + eps) - phi;
       niv = 1;
-    } else if (pa >= GRS_PI - eps) {
+    } else if (gap_pi) {
       lo[0] = (pb - eps) - phi;
       hi[0] = (twopi - pb + eps) - phi;
       niv = 1;
@@ -918,50 +925,38 @@ __device__ __forceinline__ int om_ranges(const grs_dgeo *g, double g0, double g1
     i1[0] = nrot - 1;
     return 1;
   }
-  int n = 0;
-  for (int k = 0; k < niv; k++) {
+  /* slot 3 k + j: arc k in period m + j.  The arcs and their periods are
+   * more than 1e-5 steps apart, so the ranges are disjoint as they come;
+   * their order does not matter to the caller */
+  uint32_t mask = 0;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (k >= niv) break;
     /* the last period whose arc ends at or before om0.  The product rounds
      * differently from the quotient only next to an integer, where m may be
      * one less (that arc ends 2 pi below om0 and is skipped; the periods
      * the scan can need, m..m+2 then, are still within the four steps) or
      * one more (the period passed over ends about 2 pi below om0) */
-    double m = __builtin_floor((g->om0 - hi[k]) * (1.0 / twopi));
-    for (int it = 0; it < 4; it++, m += 1.0) {
-      const double L = lo[k] + twopi * m, U = hi[k] + twopi * m;
-      if (L > g->om_last + 1e-9) break;
-      if (U < g->om0 - 1e-9) continue;
+    const double m = __builtin_floor((g->om0 - hi[k]) * (1.0 / twopi));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double L = lo[k] + twopi * (m + (double)j), U = hi[k] + twopi * (m + (double)j);
       double f0 = __builtin_ceil((L - g->om0) * idom - 1e-6);
       double f1 = __builtin_floor((U - g->om0) * idom + 1e-6);
       f0 = f0 > 0.0 ? f0 : 0.0;
       f1 = f1 < (double)(nrot - 1) ? f1 : (double)(nrot - 1);
-      if (f0 <= f1) {
-        if (n == 4) return -1;
-        i0[n] = (int32_t)f0;
-        i1[n] = (int32_t)f1;
-        n++;
+      const bool ok = L <= g->om_last + 1e-9 && U >= g->om0 - 1e-9 && f0 <= f1;
+      if (j == 3) {
+        if (ok) return -1; /* a scan longer than the periods kept here */
+      } else {
+        i0[3 * k + j] = (int32_t)f0;
+        i1[3 * k + j] = (int32_t)f1;
+        mask |= (ok ? 1u : 0u) << (3 * k + j);
       }
     }
   }
-  /* sort by start, then make the ranges disjoint */
-  for (int a = 1; a < n; a++)
-    for (int b = a; b > 0 && i0[b] < i0[b - 1]; b--) {
-      int32_t t0 = i0[b], t1 = i1[b];
-      i0[b] = i0[b - 1];
-      i1[b] = i1[b - 1];
-      i0[b - 1] = t0;
-      i1[b - 1] = t1;
-    }
-  int m = 0;
-  int32_t last = -1;
-  for (int k = 0; k < n; k++) {
-    const int32_t a0 = i0[k] > last ? i0[k] : last + 1;
-    if (a0 > i1[k]) continue;
-    i0[m] = a0;
-    i1[m] = i1[k];
-    last = i1[k] > last ? i1[k] : last;
-    m++;
-  }
-  return m;
+  if (__popc(mask) > 4) return -1;
+  return (int)mask;
 }
 
 template <bool MEDIAN, int MAP, bool AUDIT>
@@ -1113,8 +1108,8 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
     bool overflow = false;
     for (int hb = 0; hb < H; hb += 64) {
       const int h = hb + lane;
-      int nr_h = 0;
-      int32_t ri0[4], ri1[4];
+      int nr_h = 0; /* bit mask of the valid slots of ri0/ri1, or -1 */
+      int32_t ri0[6], ri1[6];
       if (h < H) {
         double gw[3];
         grs_Gw(sbuf, a.hkl + 3 * h, gw);
@@ -1157,18 +1152,20 @@ __global__ __launch_bounds__(64 * GRS_WPB, GRS_FAST_MIN_WAVES) void k_fast(grs_e
       }
       if (solve) {
         overflow = overflow || (__ballot(nr_h < 0) != 0);
-        const int cnt = nr_h > 0 ? nr_h : 0;
+        const uint32_t msk = nr_h > 0 ? (uint32_t)nr_h : 0u;
+        const int cnt = __popc(msk);
         int incl = cnt;
         for (int off = 1; off < 64; off <<= 1) {
           int v = __shfl_up(incl, off, 64);
           if (lane >= off) incl += v;
         }
-        const uint32_t pos = R + (uint32_t)(incl - cnt);
+        uint32_t pos = R + (uint32_t)(incl - cnt);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          if (k < cnt) {
-            rstart[pos + k] = ((uint32_t)ri0[k] << 8) | (uint32_t)h;
-            rcum[pos + k] = (uint32_t)(ri1[k] - ri0[k] + 1);
+        for (int k = 0; k < 6; k++) {
+          if ((msk >> k) & 1u) {
+            rstart[pos] = ((uint32_t)ri0[k] << 8) | (uint32_t)h;
+            rcum[pos] = (uint32_t)(ri1[k] - ri0[k] + 1);
+            pos++;
           }
         }
         R += (uint32_t)__shfl(incl, 63, 64);
