@@ -501,3 +501,48 @@ def test_every_pinned_launch_configuration_gives_identical_results(engine):
         finally:
             for k in range(3):
                 engine.set_launch_config(k, 0)
+
+
+def test_small_launches_take_smaller_blocks(engine):
+    """With 8 waves per block pinned, launches of fewer evaluations than a
+    block has waves (1 to 9 orientations: 4 or 12 scratch slots) run in
+    smaller blocks; every result equals the oracle, for the one-voxel median
+    kernel, its allocation-free entry and the counts-only CSR kernel."""
+    from grainrecon_amd import _lib as L
+
+    s = scenes.make_scene(seed=87, ngrains=2, spread_mm=0.03, nrot=19, det=128, pixel_mm=6.72e-3 * 8)
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(23)
+    q_all = np.vstack([s.q, scenes.random_quats(rng, 16)])
+    pos = s.voxels[0][3]
+    try:
+        for k in range(3):
+            engine.set_launch_config(k, 8, L.MASK_BITS, False)
+        for n in (1, 3, 5, 8, 9):
+            q = q_all[:n]
+            print(f"[small] {n} orientations", flush=True)
+            ref = oracle.or_space_comp(s.geo, s.stack, pos, q, s.rot, s.hkl)
+            _assert_same(engine.forward_comp(pos, q), ref)
+            d_q = engine.to_device(q)
+            bufs = [engine.device_alloc(4 * n), engine.device_alloc(4 * n), engine.device_alloc(8 * n),
+                    engine.device_alloc(4 * n)]
+            try:
+                engine.forward_comp_device_noalloc(pos, d_q, n, *bufs)
+                got = (engine.to_host(bufs[0], (n,), np.int32), engine.to_host(bufs[1], (n,), np.int32),
+                       engine.to_host(bufs[2], (n,), np.float64), engine.to_host(bufs[3], (n,), np.float32))
+                _assert_same(got, ref)
+            finally:
+                for b in bufs + [d_q]:
+                    engine.device_free(b)
+        rows = np.array([0, 1, 3])
+        cands = np.array([s.q[c % 2] + np.concatenate([[0.0], rng.normal(scale=0.004, size=3)]) for c in range(3)])
+        cpos = np.array([s.voxels[0][2], s.voxels[1][4]])
+        got_c = engine.compete_csr(cpos, rows, cands, 2, 0.002)
+        for k in range(3):
+            engine.set_launch_config(k, 4, L.MASK_BITS, False)
+        for x, y in zip(got_c, engine.compete_csr(cpos, rows, cands, 2, 0.002)):
+            np.testing.assert_array_equal(x, y)
+    finally:
+        for k in range(3):
+            engine.set_launch_config(k, 0)
