@@ -186,6 +186,105 @@ __device__ __forceinline__ double wave_min_d(double v) {
   return v;
 }
 
+/* The running extremes of the voxel bounds over the rotation angles. */
+struct vb_acc {
+  double d2max, kmax, e0, e1, u2lo, u2hi, ayz, zt, zb, yt, yb;
+  bool front, rows;
+};
+
+__device__ __forceinline__ void vb_start(vb_acc &A) {
+  A.d2max = 0.0, A.kmax = 0.0, A.e0 = 0.0, A.e1 = 0.0, A.u2lo = 2.0, A.u2hi = -2.0, A.ayz = 0.0;
+  A.zt = -1e300, A.zb = 1e300, A.yt = -1e300, A.yb = 1e300;
+  A.front = true, A.rows = true;
+}
+
+/* One rotation angle's terms of the voxel bounds (P = S*pos of the voxel,
+ * al, be, ga, oy, oz, on as in the kernels below). */
+__device__ __forceinline__ void vb_omega(const grs_dgeo *g, const double *P, double c, double s, double al, double be,
+                                         double ga, double oy, double oz, double on, vb_acc &A) {
+  double sw[3], u[3];
+  beam_terms(g, P, c, s, sw, u);
+  const double u0 = u[0], u1 = u[1], u2 = u[2];
+  const double sw0 = sw[0], sw1 = sw[1], sw2 = sw[2];
+  const double d2 = (1.0 - u0) * (1.0 - u0) + u1 * u1 + u2 * u2;
+  A.d2max = d2 > A.d2max ? d2 : A.d2max;
+  const double a0 = __builtin_fabs(u0 - 1.0), a1 = __builtin_fabs(u1 - ((al * s + be * c) + ga));
+  A.e0 = a0 > A.e0 ? a0 : A.e0;
+  A.e1 = a1 > A.e1 ? a1 : A.e1;
+  A.u2lo = u2 < A.u2lo ? u2 : A.u2lo;
+  A.u2hi = u2 > A.u2hi ? u2 : A.u2hi;
+  const double ay = __builtin_sqrt(u1 * u1 + u2 * u2) * (1.0 + 1e-9) + 1e-12;
+  A.ayz = ay > A.ayz ? ay : A.ayz;
+  double cmin = 1.0, smin = 1e300, smax = -1e300, tmin = 1e300, tmax = -1e300;
+  for (int j = 0; j < 4; j++) {
+    const double dx = g->corner[3 * j] - sw0, dy = g->corner[3 * j + 1] - sw1, dz = g->corner[3 * j + 2] - sw2;
+    const double cj = (u0 * dx + u1 * dy + u2 * dz) / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    A.front = A.front && (cj > 1e-3);
+    cmin = cj < cmin ? cj : cmin;
+    A.rows = A.rows && (dx > 0.0);
+    const double sl = dz / dx, ty = dy / dx;
+    smin = sl < smin ? sl : smin;
+    smax = sl > smax ? sl : smax;
+    tmin = ty < tmin ? ty : tmin;
+    tmax = ty > tmax ? ty : tmax;
+  }
+  const double L2 = (g->Lsd - sw0) / u0;
+  A.front = A.front && (L2 > 0.0) && (ay < 1.0);
+  /* sin and cos of gamma + eps, gamma = acos(cmin) */
+  double se = on / (L2 * __builtin_sqrt(1.0 - (ay < 1.0 ? ay * ay : 0.0)));
+  se = se * (1.0 + 1e-9) + 1e-12;
+  A.front = A.front && (se < 1.0);
+  se = se < 1.0 ? se : 1.0;
+  const double ce = __builtin_sqrt(1.0 - se * se);
+  const double cm = cmin < 1.0 ? cmin : 1.0;
+  const double sg = __builtin_sqrt(1.0 - cm * cm);
+  const double k = (1.0 - (cm * ce - sg * se)) * 0.5;
+  A.kmax = k > A.kmax ? k : A.kmax;
+  const double ozl = oz / L2;
+  const double t1 = (smax * u0 - u2) - ozl, t0 = (smin * u0 - u2) - ozl;
+  A.zt = t1 > A.zt ? t1 : A.zt;
+  A.zb = t0 < A.zb ? t0 : A.zb;
+  const double oyl = oy / L2;
+  const double y1 = (tmax * u0 - u1) - oyl, y0 = (tmin * u0 - u1) - oyl;
+  A.yt = y1 > A.yt ? y1 : A.yt;
+  A.yb = y0 < A.yb ? y0 : A.yb;
+}
+
+/* a NaN anywhere turns the bound off: the comparisons above skip NaNs */
+__device__ __forceinline__ bool vb_nan(const vb_acc &A) {
+  return !(A.d2max == A.d2max && A.kmax == A.kmax && A.zt == A.zt && A.zb == A.zb && A.yt == A.yt && A.yb == A.yb);
+}
+
+/* The GRS_VB values of a voxel from the extremes over every rotation angle. */
+__device__ __forceinline__ void vb_write(const grs_dgeo *g, double al, double be, double ga, const vb_acc &A,
+                                         bool nan_any, double *o) {
+  const double delta = __builtin_sqrt(A.d2max) * (1.0 + 1e-9) + 1e-12;
+  const double alpha = 2.0 * grs_asin(delta * 0.5 < 1.0 ? delta * 0.5 : 1.0) + 1e-9;
+  double K = 1.0;
+  const bool ok = A.front && !nan_any && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
+  if (ok) {
+    K = A.kmax * (1.0 + 1e-6) + 1e-9;
+    K = K < 1.0 ? K : 1.0;
+  }
+  const bool use_rows = ok && A.rows && A.ayz < 1.0;
+  o[0] = ((__builtin_sqrt(al * al + be * be) * 0.5 + A.e0) + A.e1) * (1.0 + 1e-9) + 1e-12;
+  o[1] = A.u2lo - 1e-12;
+  o[2] = A.u2hi + 1e-12;
+  o[3] = K;
+  o[4] = al;
+  o[5] = be;
+  o[6] = ga;
+  o[7] = use_rows ? A.zt + (__builtin_fabs(A.zt) * 1e-9 + 1e-12) : 1e300;
+  o[8] = use_rows ? A.zb - (__builtin_fabs(A.zb) * 1e-9 + 1e-12) : -1e300;
+  const double sn = A.ayz < 1.0 ? A.ayz : 1.0;
+  o[9] = sn;
+  o[10] = __builtin_sqrt(1.0 - sn * sn);
+  o[11] = o[10] > 0.0 ? sn / o[10] : 1e300;
+  const double ya = __builtin_fabs(A.yt) > __builtin_fabs(A.yb) ? __builtin_fabs(A.yt) : __builtin_fabs(A.yb);
+  o[12] = use_rows ? ya * (1.0 + 1e-9) + 1e-12 : 1e300;
+  o[13] = delta;
+}
+
 /* One 256-thread block per voxel: thread t takes the rotation angles t,
  * t + 256, ... (one each for up to 256 angles, which keeps the latency of a
  * one-voxel call short), and the extremes are reduced across each wave and
@@ -200,126 +299,71 @@ __global__ __launch_bounds__(256) void k_voxel_bounds(const grs_dgeo *g, const d
   const double al = P[0] / g->Lss, be = P[1] / g->Lss, ga = (g->RAO - g->P0y) / g->Lss;
   const double oy = g->RAO - g->P0y, oz = -g->P0z;
   const double on = __builtin_sqrt(oy * oy + oz * oz);
-  double d2max = 0.0, kmax = 0.0, e0 = 0.0, e1 = 0.0, u2lo = 2.0, u2hi = -2.0, ayz = 0.0, zt = -1e300, zb = 1e300;
-  double yt = -1e300, yb = 1e300;
-  bool front = true, rows = true;
-  for (int32_t om = tid; om < nrot; om += 256) {
-    double sw[3], u[3];
-    const double c = cs[2 * om], s = cs[2 * om + 1];
-    beam_terms(g, P, c, s, sw, u);
-    const double u0 = u[0], u1 = u[1], u2 = u[2];
-    const double sw0 = sw[0], sw1 = sw[1], sw2 = sw[2];
-    const double d2 = (1.0 - u0) * (1.0 - u0) + u1 * u1 + u2 * u2;
-    d2max = d2 > d2max ? d2 : d2max;
-    const double a0 = __builtin_fabs(u0 - 1.0), a1 = __builtin_fabs(u1 - ((al * s + be * c) + ga));
-    e0 = a0 > e0 ? a0 : e0;
-    e1 = a1 > e1 ? a1 : e1;
-    u2lo = u2 < u2lo ? u2 : u2lo;
-    u2hi = u2 > u2hi ? u2 : u2hi;
-    const double ay = __builtin_sqrt(u1 * u1 + u2 * u2) * (1.0 + 1e-9) + 1e-12;
-    ayz = ay > ayz ? ay : ayz;
-    double cmin = 1.0, smin = 1e300, smax = -1e300, tmin = 1e300, tmax = -1e300;
-    for (int j = 0; j < 4; j++) {
-      const double dx = g->corner[3 * j] - sw0, dy = g->corner[3 * j + 1] - sw1, dz = g->corner[3 * j + 2] - sw2;
-      const double cj = (u0 * dx + u1 * dy + u2 * dz) / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-      front = front && (cj > 1e-3);
-      cmin = cj < cmin ? cj : cmin;
-      rows = rows && (dx > 0.0);
-      const double sl = dz / dx, ty = dy / dx;
-      smin = sl < smin ? sl : smin;
-      smax = sl > smax ? sl : smax;
-      tmin = ty < tmin ? ty : tmin;
-      tmax = ty > tmax ? ty : tmax;
-    }
-    const double L2 = (g->Lsd - sw0) / u0;
-    front = front && (L2 > 0.0) && (ay < 1.0);
-    /* sin and cos of gamma + eps, gamma = acos(cmin) */
-    double se = on / (L2 * __builtin_sqrt(1.0 - (ay < 1.0 ? ay * ay : 0.0)));
-    se = se * (1.0 + 1e-9) + 1e-12;
-    front = front && (se < 1.0);
-    se = se < 1.0 ? se : 1.0;
-    const double ce = __builtin_sqrt(1.0 - se * se);
-    const double cm = cmin < 1.0 ? cmin : 1.0;
-    const double sg = __builtin_sqrt(1.0 - cm * cm);
-    const double k = (1.0 - (cm * ce - sg * se)) * 0.5;
-    kmax = k > kmax ? k : kmax;
-    const double ozl = oz / L2;
-    const double t1 = (smax * u0 - u2) - ozl, t0 = (smin * u0 - u2) - ozl;
-    zt = t1 > zt ? t1 : zt;
-    zb = t0 < zb ? t0 : zb;
-    const double oyl = oy / L2;
-    const double y1 = (tmax * u0 - u1) - oyl, y0 = (tmin * u0 - u1) - oyl;
-    yt = y1 > yt ? y1 : yt;
-    yb = y0 < yb ? y0 : yb;
-  }
-  /* a NaN anywhere turns the bound off: the comparisons above skip NaNs */
-  const bool nan_seen = __ballot(!(d2max == d2max && kmax == kmax && zt == zt && zb == zb && yt == yt && yb == yb)) != 0;
-  front = __ballot(!front) == 0;
-  rows = __ballot(!rows) == 0;
-  d2max = wave_max_d(d2max);
-  kmax = wave_max_d(kmax);
-  e0 = wave_max_d(e0);
-  e1 = wave_max_d(e1);
-  u2lo = wave_min_d(u2lo);
-  u2hi = wave_max_d(u2hi);
-  ayz = wave_max_d(ayz);
-  zt = wave_max_d(zt);
-  zb = wave_min_d(zb);
-  yt = wave_max_d(yt);
-  yb = wave_min_d(yb);
+  vb_acc A;
+  vb_start(A);
+  for (int32_t om = tid; om < nrot; om += 256) vb_omega(g, P, cs[2 * om], cs[2 * om + 1], al, be, ga, oy, oz, on, A);
+  const bool nan_seen = __ballot(vb_nan(A)) != 0;
+  A.front = __ballot(!A.front) == 0;
+  A.rows = __ballot(!A.rows) == 0;
+  A.d2max = wave_max_d(A.d2max);
+  A.kmax = wave_max_d(A.kmax);
+  A.e0 = wave_max_d(A.e0);
+  A.e1 = wave_max_d(A.e1);
+  A.u2lo = wave_min_d(A.u2lo);
+  A.u2hi = wave_max_d(A.u2hi);
+  A.ayz = wave_max_d(A.ayz);
+  A.zt = wave_max_d(A.zt);
+  A.zb = wave_min_d(A.zb);
+  A.yt = wave_max_d(A.yt);
+  A.yb = wave_min_d(A.yb);
   /* the four waves' maxima, minima and flags (exact, so order-free) */
   __shared__ double red[4][14];
   if (lane == 0) {
     double *r = red[wv];
-    r[0] = d2max, r[1] = kmax, r[2] = e0, r[3] = e1, r[4] = u2lo, r[5] = u2hi, r[6] = ayz, r[7] = zt, r[8] = zb;
-    r[9] = front ? 1.0 : 0.0, r[10] = rows ? 1.0 : 0.0, r[11] = nan_seen ? 1.0 : 0.0, r[12] = yt, r[13] = yb;
+    r[0] = A.d2max, r[1] = A.kmax, r[2] = A.e0, r[3] = A.e1, r[4] = A.u2lo, r[5] = A.u2hi, r[6] = A.ayz;
+    r[7] = A.zt, r[8] = A.zb, r[9] = A.front ? 1.0 : 0.0, r[10] = A.rows ? 1.0 : 0.0, r[11] = nan_seen ? 1.0 : 0.0;
+    r[12] = A.yt, r[13] = A.yb;
   }
   __syncthreads();
   if (tid != 0) return;
   bool nan_any = false;
   for (int k = 0; k < 4; k++) {
     const double *r = red[k];
-    d2max = r[0] > d2max ? r[0] : d2max;
-    kmax = r[1] > kmax ? r[1] : kmax;
-    e0 = r[2] > e0 ? r[2] : e0;
-    e1 = r[3] > e1 ? r[3] : e1;
-    u2lo = r[4] < u2lo ? r[4] : u2lo;
-    u2hi = r[5] > u2hi ? r[5] : u2hi;
-    ayz = r[6] > ayz ? r[6] : ayz;
-    zt = r[7] > zt ? r[7] : zt;
-    zb = r[8] < zb ? r[8] : zb;
-    yt = r[12] > yt ? r[12] : yt;
-    yb = r[13] < yb ? r[13] : yb;
-    front = front && r[9] != 0.0;
-    rows = rows && r[10] != 0.0;
+    A.d2max = r[0] > A.d2max ? r[0] : A.d2max;
+    A.kmax = r[1] > A.kmax ? r[1] : A.kmax;
+    A.e0 = r[2] > A.e0 ? r[2] : A.e0;
+    A.e1 = r[3] > A.e1 ? r[3] : A.e1;
+    A.u2lo = r[4] < A.u2lo ? r[4] : A.u2lo;
+    A.u2hi = r[5] > A.u2hi ? r[5] : A.u2hi;
+    A.ayz = r[6] > A.ayz ? r[6] : A.ayz;
+    A.zt = r[7] > A.zt ? r[7] : A.zt;
+    A.zb = r[8] < A.zb ? r[8] : A.zb;
+    A.yt = r[12] > A.yt ? r[12] : A.yt;
+    A.yb = r[13] < A.yb ? r[13] : A.yb;
+    A.front = A.front && r[9] != 0.0;
+    A.rows = A.rows && r[10] != 0.0;
     nan_any = nan_any || r[11] != 0.0;
   }
-  const double delta = __builtin_sqrt(d2max) * (1.0 + 1e-9) + 1e-12;
-  const double alpha = 2.0 * grs_asin(delta * 0.5 < 1.0 ? delta * 0.5 : 1.0) + 1e-9;
-  double K = 1.0;
-  const bool ok = front && !nan_any && 2.0 * g->theta_win + alpha < g->pio2 - 1e-3;
-  if (ok) {
-    K = kmax * (1.0 + 1e-6) + 1e-9;
-    K = K < 1.0 ? K : 1.0;
-  }
-  const bool use_rows = ok && rows && ayz < 1.0;
-  double *o = vb + GRS_VB * v;
-  o[0] = ((__builtin_sqrt(al * al + be * be) * 0.5 + e0) + e1) * (1.0 + 1e-9) + 1e-12;
-  o[1] = u2lo - 1e-12;
-  o[2] = u2hi + 1e-12;
-  o[3] = K;
-  o[4] = al;
-  o[5] = be;
-  o[6] = ga;
-  o[7] = use_rows ? zt + (__builtin_fabs(zt) * 1e-9 + 1e-12) : 1e300;
-  o[8] = use_rows ? zb - (__builtin_fabs(zb) * 1e-9 + 1e-12) : -1e300;
-  const double sn = ayz < 1.0 ? ayz : 1.0;
-  o[9] = sn;
-  o[10] = __builtin_sqrt(1.0 - sn * sn);
-  o[11] = o[10] > 0.0 ? sn / o[10] : 1e300;
-  const double ya = __builtin_fabs(yt) > __builtin_fabs(yb) ? __builtin_fabs(yt) : __builtin_fabs(yb);
-  o[12] = use_rows ? ya * (1.0 + 1e-9) + 1e-12 : 1e300;
-  o[13] = delta;
+  vb_write(g, al, be, ga, A, nan_any, vb + GRS_VB * v);
+}
+
+/* The same bounds with one thread per voxel walking every rotation angle:
+ * for thousands of voxels, where a block per voxel spends most of its time
+ * in the reductions (3.2M voxels: about 34 ms as blocks).  Maxima, minima
+ * and flags are exact, so both kernels write the same values. */
+__global__ __launch_bounds__(256) void k_voxel_bounds_seq(const grs_dgeo *g, const double *pos, int64_t V,
+                                                          const double *cs, int32_t nrot, double *vb) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  double P[3];
+  voxel_P(g, pos + 3 * v, P);
+  const double al = P[0] / g->Lss, be = P[1] / g->Lss, ga = (g->RAO - g->P0y) / g->Lss;
+  const double oy = g->RAO - g->P0y, oz = -g->P0z;
+  const double on = __builtin_sqrt(oy * oy + oz * oz);
+  vb_acc A;
+  vb_start(A);
+  for (int32_t om = 0; om < nrot; om++) vb_omega(g, P, cs[2 * om], cs[2 * om + 1], al, be, ga, oy, oz, on, A);
+  vb_write(g, al, be, ga, A, vb_nan(A), vb + GRS_VB * v);
 }
 
 /* Refined hardware reciprocal / reciprocal square root (one Newton step,
@@ -1643,7 +1687,11 @@ hipError_t grs_launch_voxel_bounds(const grs_dgeo *d_g, const double *d_pos, int
                                    int32_t nrot, double *d_vb, hipStream_t st) {
   if (V <= 0) return hipSuccess;
   if (V > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_voxel_bounds, dim3((unsigned)V), dim3(256), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
+  if (V >= 4096)
+    hipLaunchKernelGGL(k_voxel_bounds_seq, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, d_g, d_pos, V, d_cs,
+                       nrot, d_vb);
+  else
+    hipLaunchKernelGGL(k_voxel_bounds, dim3((unsigned)V), dim3(256), 0, st, d_g, d_pos, V, d_cs, nrot, d_vb);
   return hipGetLastError();
 }
 

@@ -546,3 +546,30 @@ def test_small_launches_take_smaller_blocks(engine):
     finally:
         for k in range(3):
             engine.set_launch_config(k, 0)
+
+
+@pytest.mark.parametrize("variant", ["plain", "offsets"])
+def test_grow_many_voxels_equals_chunks_and_oracle(engine, variant):
+    """Above 4096 voxels the omega-solve bounds come from one thread per voxel
+    instead of one block per voxel: growth over 4,800 voxels equals the same
+    voxels scored in chunks of 1,000 bit for bit, and a sample of them the
+    oracle."""
+    s = scenes.make_scene(seed=41, ngrains=2, spread_mm=0.02)
+    if variant == "offsets":
+        s.geo.P0y, s.geo.P0z, s.geo.RotAxisOffset = 0.02, -0.015, 0.01
+    engine.set_geometry(s.geo, s.hkl, s.rot)
+    engine.set_distance_stack(s.proj)
+    rng = np.random.default_rng(8)
+    idx = np.stack(np.meshgrid(np.arange(10, 30), np.arange(10, 30), np.arange(14, 26), indexing="ij"), -1)
+    idx = idx.reshape(-1, 3).astype(float)
+    vox = np.hstack([idx, rng.uniform(0, 0.6, (idx.shape[0], 1)), (rng.random((idx.shape[0], 1)) < 0.9) * 1.0,
+                     rng.uniform(0, 30, (idx.shape[0], 1))])
+    assert vox.shape[0] == 4800
+    p1 = _p1()
+    got = engine.grow_indexed_region(vox, s.U[0], p1)
+    parts = np.vstack([engine.grow_indexed_region(vox[k:k + 1000], s.U[0], p1) for k in range(0, 4800, 1000)])
+    assert np.array_equal(_bits(got), _bits(parts))
+    assert got[:, 6].sum() > 0 and got[:, 3].sum() > 0
+    sel = np.concatenate([np.flatnonzero(got[:, 6] > 0)[:40], rng.choice(4800, size=120, replace=False)])
+    ref = oracle.grow(s.geo, s.stack, vox[sel], s.U[0], s.rot, s.hkl, p1)
+    assert np.array_equal(_bits(got[sel]), _bits(ref))
