@@ -156,6 +156,10 @@ def roofline_leg(eng, cfg, q, seeds, kernel_ms, N, config, n_audit):
         out["traffic"] = int(round(prof["hbm_bytes_per_eval"] * N))
         out["traffic_GB_per_s"] = round(out["traffic"] / t / 1e9, 1)
         out["traffic_over_gather"] = round(out["traffic"] / gather, 2)
+        if "valu_insts_per_eval" in prof:
+            # wave instructions x 4 cycles (64 lanes on a 16-wide SIMD) over
+            # the 1,024 SIMDs at 2.4 GHz during the launch
+            out["executed_valu_busy"] = round(N * prof["valu_insts_per_eval"] * 4 / (1024 * 2.4e9 * t), 3)
         out["counter_profile"] = "profiles/k_fast_seed_cfg2_counters.md (this build)"
     else:
         out["counter_profile"] = ("no counter profile of this build (profiles/k_fast_seed_cfg2_counters.md names "
