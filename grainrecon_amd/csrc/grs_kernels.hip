@@ -47,7 +47,8 @@
 #define GRS_REG_KEYS 16 /* keys per lane kept in registers for the median */
 #define GRS_BT 12       /* doubles per omega of the beam table: the omega_terms (10), cos, sin */
 
-/* LDS doubles of one wave's region of k_fast */
+/* LDS doubles of one wave's region of k_fast: the G table, the rings, the
+ * omega-solve ranges, S*pos of the voxel (4) and S*U*B (9) */
 __host__ __device__ inline int64_t fast_wave_doubles(int32_t H) {
   return (int64_t)H * GRS_GWT + GRS_RING_DOUBLES + GRS_RANGE_DOUBLES(H) + 4 + 9;
 }
