@@ -535,7 +535,10 @@ static grs_status launch_tuned(grs_ctx *c, grs_eval_args a, int kind, int64_t wa
     tune_cfg k = t.done ? t.cfg : tune_cfg{8, GRS_MASK_BITS, 0};
     if (k.mask == GRS_MASK_TILED && !c->tbits_ok) k.mask = GRS_MASK_BITS;
     apply_cfg(&a, k);
-    HIPCHK(c, launch(a, k.wpb));
+    /* fewer waves than CUs (the steps of the orientation fits): one wave
+     * per block, so that every evaluation has a CU to itself */
+    const int32_t wpb = waves < grs_fast_wave_slots(c->device) / 64 ? 1 : k.wpb;
+    HIPCHK(c, launch(a, wpb));
     return GRS_OK;
   }
   /* part 0 warms caches with the default configuration and is not timed;
@@ -1535,18 +1538,39 @@ static grs_status fit_batch(grs_ctx *c, const double *pos, const double *x0, int
   HIPCHK(c, grs_launch_voxel_bounds(P<grs_dgeo>(c->d_geo), P<double>(c->d_pos), M, P<double>(c->d_cs), c->nrot,
                                     P<double>(c->d_vb), c->st));
   /* each step: [q (4E doubles) | eval voxel (E) | eval orientation (E)]
-   * uploaded in one copy from pinned memory, [ns (E) | ni (E)] back in one */
+   * uploaded in one copy from pinned memory, [ns (E) | ni (E)] back in one.
+   * A launch of E evaluations takes about the time of one while every
+   * evaluation has a SIMD to itself; within that many, a fit's requested
+   * reflection goes out with the three points its value can lead to
+   * (grs_nm::follow_ups), and the one the method then requests is fed from
+   * that launch's value instead of a launch of its own.  The method's
+   * sequence of points, values and counts is unchanged. */
+  const int64_t spec_max = grs_fast_wave_slots(c->device) / 16;
   std::vector<int64_t> first;
+  std::vector<int32_t> npv, nspec;
   std::vector<double> fvals;
+  std::vector<double> su((size_t)M * 3 * GRS_NM_NMAX);
   for (;;) {
     first.assign((size_t)M + 1, 0);
-    int64_t E = 0;
+    npv.assign((size_t)M, 0);
+    nspec.assign((size_t)M, 0);
+    int64_t E = 0, nrefl = 0;
+    for (int64_t m = 0; m < M; m++) {
+      if (nm[m].done()) continue;
+      E += nm[m].npend;
+      nrefl += nm[m].st == grs_nm::REFLECT ? 1 : 0;
+    }
+    if (E == 0) break;
+    const bool spec = E + 3 * nrefl <= spec_max;
+    E = 0;
     for (int64_t m = 0; m < M; m++) {
       first[m] = E;
-      if (!nm[m].done()) E += nm[m].npend;
+      if (nm[m].done()) continue;
+      npv[m] = nm[m].npend;
+      if (spec) nspec[m] = nm[m].follow_ups(reinterpret_cast<double (*)[GRS_NM_NMAX]>(&su[(size_t)m * 3 * GRS_NM_NMAX]));
+      E += npv[m] + nspec[m];
     }
     first[M] = E;
-    if (E == 0) break;
     const size_t in_bytes = (sizeof(double) * 4 + sizeof(int32_t) * 2) * (size_t)E;
     const size_t cnt_bytes = sizeof(int32_t) * 2 * (size_t)E;
     if (c->h_pin_cap < in_bytes + cnt_bytes) {
@@ -1567,10 +1591,10 @@ static grs_status fit_batch(grs_ctx *c, const double *pos, const double *x0, int
     int32_t *hcnt = reinterpret_cast<int32_t *>(static_cast<char *>(c->h_pin) + in_bytes);
     for (int64_t m = 0; m < M; m++) {
       if (nm[m].done()) continue;
-      for (int p = 0; p < nm[m].npend; p++) {
+      for (int p = 0; p < npv[m] + nspec[m]; p++) {
         const int64_t e = first[m] + p;
         double x[3];
-        nm[m].xform(nm[m].pend[p], x);
+        nm[m].xform(p < npv[m] ? nm[m].pend[p] : &su[((size_t)m * 3 + (size_t)(p - npv[m])) * GRS_NM_NMAX], x);
         hq[4 * e] = 0.0; /* q0 is recomputed from the vector part (fit_OR_fun_local.m:9) */
         hq[4 * e + 1] = x[0];
         hq[4 * e + 2] = x[1];
@@ -1599,8 +1623,21 @@ static grs_status fit_batch(grs_ctx *c, const double *pos, const double *x0, int
     fvals.resize((size_t)E);
     for (int64_t e = 0; e < E; e++) /* fit_OR_fun_local.m:98; 0/0 (no event, or a skipped row) is NaN */
       fvals[e] = 1.0 - (double)hcnt[E + e] / (double)hcnt[e];
-    for (int64_t m = 0; m < M; m++)
-      if (first[m + 1] > first[m]) nm[m].feed(&fvals[first[m]]);
+    for (int64_t m = 0; m < M; m++) {
+      if (first[m + 1] == first[m]) continue;
+      nm[m].feed(&fvals[first[m]]);
+      /* an expansion or contraction found among the follow-ups (bit for bit) */
+      while (nspec[m] > 0 && !nm[m].done() && nm[m].npend == 1 &&
+             (nm[m].st == grs_nm::EXPAND || nm[m].st == grs_nm::CONTRACT_OUT || nm[m].st == grs_nm::CONTRACT_IN)) {
+        int hit = -1;
+        for (int s2 = 0; s2 < nspec[m] && hit < 0; s2++)
+          if (std::memcmp(nm[m].pend[0], &su[((size_t)m * 3 + (size_t)s2) * GRS_NM_NMAX],
+                          sizeof(double) * (size_t)nm[m].n) == 0)
+            hit = s2;
+        if (hit < 0) break;
+        nm[m].feed(&fvals[first[m] + npv[m] + hit]);
+      }
+    }
   }
   for (int64_t m = 0; m < M; m++) {
     nm[m].result(x_out + 3 * m, fval + m);

@@ -154,6 +154,22 @@ struct grs_nm {
     st = REFLECT;
   }
 
+  /* While a reflection is requested: the three points its value can lead to
+   * next (expansion, outside and inside contraction), computed with the
+   * statements feed() uses, so that a caller can evaluate them in the same
+   * batch; returns how many (0 in any other state).  The method itself is
+   * unchanged: feed() still requests each of them when the method needs it. */
+  GRS_NM_FN int follow_ups(double (*u)[GRS_NM_NMAX]) const {
+    GRS_NO_CONTRACT
+    if (st != REFLECT) return 0;
+    for (int k = 0; k < n; k++) {
+      u[0][k] = (1.0 + 2.0) * xbar[k] - 2.0 * v[n][k];
+      u[1][k] = (1.0 + 0.5) * xbar[k] - 0.5 * v[n][k];
+      u[2][k] = (1.0 - 0.5) * xbar[k] + 0.5 * v[n][k];
+    }
+    return 3;
+  }
+
   GRS_NM_FN void end_iteration() {
     sort_simplex();
     it += 1;

@@ -295,7 +295,10 @@ grs_status grs_device_copy(grs_ctx *ctx, void *d_dst, const void *d_src, int64_t
  * 0.02), by fminsearchbnd with MaxFunEvals, MaxIter, TolFun, TolX (the
  * reference passes 2000, 2000, 1e-10 and the default 1e-4).  Each fit follows
  * the sequential method exactly; the evaluations of all active fits are
- * scored together in one device launch per step.  A point with no selected
+ * scored together in one device launch per step (while the batch is small,
+ * with each requested reflection's three possible follow-ups, so that an
+ * expansion or contraction needs no launch of its own; the counts returned
+ * are the method's own).  A point with no selected
  * event or |q_vec| > 1 has the value NaN.  Outputs: x_out (3 per fit), fval,
  * the number of evaluations and of iterations. */
 grs_status grs_fit_or_local(grs_ctx *ctx, const double *pos, const double *x0, int64_t M, double half_width,

@@ -88,6 +88,8 @@ def harness(tmp_path_factory):
     L.OBJ = ctypes.CFUNCTYPE(ctypes.c_double, dp, ctypes.c_int)
     L.nm_run_simplex.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                  L.OBJ, dp, dp, ip, ip]
+    L.nm_run_follow_ups.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    L.OBJ, dp, dp, ip, ip, dp, ctypes.c_int, ip, ip]
     L.nm_start.argtypes = [ctypes.c_int, dp, dp, dp, dp]
     L.nm_fminsearchbnd.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, L.OBJ, dp, dp, ip, ip]
@@ -154,6 +156,69 @@ def test_stopping_test_and_budget():
     visited = []
     _, _, nf, it, _, _ = nelder_mead(_table_objective(visited), START, max_iter=2, tol_fun=2.0, tol_x=3.99)
     assert (nf, it) == (4, 2)
+
+
+def _run_follow_ups(harness, f, sim0, max_it, tol_f, tol_x, cap=4096):
+    n = sim0.shape[1]
+
+    @harness.OBJ
+    def cb(u, nn):
+        return f([u[k] for k in range(nn)])
+
+    sim = np.zeros((n + 1) * n)
+    fv = np.zeros(n + 1)
+    fed = np.zeros(cap * n)
+    nf, it, n_fed = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    hits = (ctypes.c_int * 3)()
+    assert harness.nm_run_follow_ups(n, _dp(sim0), 2000, max_it, tol_f, tol_x, cb, _dp(sim), _dp(fv),
+                                     ctypes.byref(nf), ctypes.byref(it), _dp(fed), cap, ctypes.byref(n_fed),
+                                     hits) == 0
+    assert n_fed.value <= cap
+    pts = [tuple(fed[n * j:n * j + n]) for j in range(n_fed.value)]
+    return sim.reshape(n + 1, n), fv, nf.value, it.value, pts, list(hits)
+
+
+def test_follow_ups_feed_the_derived_sequence(harness):
+    """The engine's fits evaluate each reflection's three follow-ups
+    (grs_nm::follow_ups) in the same launch and feed the expansion or
+    contraction the method requests from them.  On the derived sequence
+    (off-table points are NaN, which only a wrongly fed follow-up would
+    reach) the points fed, the counts and the final simplex are the known
+    answer, and every follow-up kind is used: the expansions of it 2 and 6,
+    the outside contractions of it 3 and 7, the inside ones of it 4 and 5."""
+    f = lambda u: TABLE.get(tuple(float(c) for c in u), math.nan)  # noqa: E731
+    sim, fv, nf, it, fed, hits = _run_follow_ups(harness, f, np.array(START, dtype=float), 8, 0.0, 0.0)
+    assert fed == SEQUENCE
+    assert (nf, it) == (NF, IT)
+    assert [(tuple(sim[j]), fv[j]) for j in range(3)] == FINAL
+    assert hits == [2, 2, 2]
+
+
+def test_follow_ups_equal_the_plain_drive_bit_for_bit(harness):
+    """On a three-dimensional Rosenbrock run to convergence, driving the
+    state machine with follow-ups gives the plain drive's points, values and
+    counts bit for bit."""
+    def rosen(u):
+        return 100.0 * (u[1] - u[0] * u[0]) ** 2 + (1.0 - u[0]) ** 2 + 0.5 * (u[2] - 0.25) ** 2
+
+    sim0 = np.array([[-1.2, 1.0, 0.9], [-1.26, 1.0, 0.9], [-1.2, 1.05, 0.9], [-1.2, 1.0, 0.945]])
+    visited = []
+
+    @harness.OBJ
+    def cb(u, n):
+        p = [u[k] for k in range(n)]
+        visited.append(tuple(p))
+        return rosen(p)
+
+    sim = np.zeros(12)
+    fv = np.zeros(4)
+    nf, it = ctypes.c_int(), ctypes.c_int()
+    assert harness.nm_run_simplex(3, _dp(sim0), 2000, 500, 1e-12, 1e-9, cb, _dp(sim), _dp(fv), ctypes.byref(nf),
+                                  ctypes.byref(it)) == 0
+    sim2, fv2, nf2, it2, fed, hits = _run_follow_ups(harness, rosen, sim0, 500, 1e-12, 1e-9)
+    assert fed == visited
+    assert np.array_equal(sim2.reshape(-1), sim) and np.array_equal(fv2, fv) and (nf2, it2) == (nf.value, it.value)
+    assert min(hits) > 0 and nf.value > 200
 
 
 def test_nan_vertex_sorts_last(harness):
