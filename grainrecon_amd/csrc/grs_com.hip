@@ -39,7 +39,7 @@ __device__ static double com_pair_dist(const grs_dgeo *g, const double *r, const
   return grs_sqrt(ey * ey + ez * ez);
 }
 
-#define COM_THREADS 256
+#define COM_THREADS 512
 
 /* One block per fit m: thread 0 steps the Nelder-Mead state machine
  * (started on the host) in LDS; for the requested points the first threads

@@ -46,8 +46,16 @@ GRS_NM_FN static inline double grs_nm_eps(double x) {
   return ldexp(1.0, e - 53);
 }
 
+/* The loops of the per-iteration steps run over the fixed bounds with the
+ * live entries selected inside, so that on the device they unroll into
+ * independent loads and stores (a kernel steps the method from one lane);
+ * the entries and the order of the arithmetic are those of plain loops. */
+#define GRS_NM_UNROLL _Pragma("unroll")
+
 GRS_NM_FN static inline void grs_nm_copy(double *d, const double *s, int n) {
-  for (int k = 0; k < n; k++) d[k] = s[k];
+  GRS_NM_UNROLL
+  for (int k = 0; k < GRS_NM_NMAX; k++)
+    if (k < n) d[k] = s[k];
 }
 
 /* MATLAB max of two values: NaN ignored unless both are NaN */
@@ -134,21 +142,34 @@ struct grs_nm {
       return;
     }
     double dfv = __builtin_nan(""), dx = __builtin_nan(""), vmax = __builtin_nan("");
-    for (int j = 1; j <= n; j++) dfv = grs_nm_max(dfv, fabs(fv[0] - fv[j]));
-    for (int j = 1; j <= n; j++)
-      for (int k = 0; k < n; k++) dx = grs_nm_max(dx, fabs(v[j][k] - v[0][k]));
-    for (int k = 0; k < n; k++) vmax = grs_nm_max(vmax, v[0][k]);
+    GRS_NM_UNROLL
+    for (int j = 1; j <= GRS_NM_NMAX; j++)
+      if (j <= n) dfv = grs_nm_max(dfv, fabs(fv[0] - fv[j]));
+    GRS_NM_UNROLL
+    for (int j = 1; j <= GRS_NM_NMAX; j++) {
+      GRS_NM_UNROLL
+      for (int k = 0; k < GRS_NM_NMAX; k++)
+        if (j <= n && k < n) dx = grs_nm_max(dx, fabs(v[j][k] - v[0][k]));
+    }
+    GRS_NM_UNROLL
+    for (int k = 0; k < GRS_NM_NMAX; k++)
+      if (k < n) vmax = grs_nm_max(vmax, v[0][k]);
     if (dfv <= grs_nm_max(tol_f, 10.0 * grs_nm_eps(fv[0])) && dx <= grs_nm_max(tol_x, 10.0 * grs_nm_eps(vmax))) {
       st = DONE;
       npend = 0;
       return;
     }
-    for (int k = 0; k < n; k++) {
+    GRS_NM_UNROLL
+    for (int k = 0; k < GRS_NM_NMAX; k++) {
       double acc = 0.0;
-      for (int j = 0; j < n; j++) acc += v[j][k];
-      xbar[k] = acc / (double)n;
+      GRS_NM_UNROLL
+      for (int j = 0; j < GRS_NM_NMAX; j++)
+        if (j < n) acc += v[j][k];
+      if (k < n) xbar[k] = acc / (double)n;
     }
-    for (int k = 0; k < n; k++) xr[k] = (1.0 + 1.0) * xbar[k] - 1.0 * v[n][k];
+    GRS_NM_UNROLL
+    for (int k = 0; k < GRS_NM_NMAX; k++)
+      if (k < n) xr[k] = (1.0 + 1.0) * xbar[k] - 1.0 * v[n][k];
     grs_nm_copy(pend[0], xr, n);
     npend = 1;
     st = REFLECT;
@@ -204,7 +225,9 @@ struct grs_nm {
         fxr = f[0];
         nf += 1;
         if (fxr < fv[0]) {
-          for (int k = 0; k < n; k++) pend[0][k] = (1.0 + 2.0) * xbar[k] - 2.0 * v[n][k];
+          GRS_NM_UNROLL
+          for (int k = 0; k < GRS_NM_NMAX; k++)
+            if (k < n) pend[0][k] = (1.0 + 2.0) * xbar[k] - 2.0 * v[n][k];
           npend = 1;
           st = EXPAND;
           return;
@@ -216,11 +239,15 @@ struct grs_nm {
           return;
         }
         if (fxr < fv[n]) {
-          for (int k = 0; k < n; k++) pend[0][k] = (1.0 + 0.5) * xbar[k] - 0.5 * v[n][k];
+          GRS_NM_UNROLL
+          for (int k = 0; k < GRS_NM_NMAX; k++)
+            if (k < n) pend[0][k] = (1.0 + 0.5) * xbar[k] - 0.5 * v[n][k];
           npend = 1;
           st = CONTRACT_OUT;
         } else {
-          for (int k = 0; k < n; k++) pend[0][k] = (1.0 - 0.5) * xbar[k] + 0.5 * v[n][k];
+          GRS_NM_UNROLL
+          for (int k = 0; k < GRS_NM_NMAX; k++)
+            if (k < n) pend[0][k] = (1.0 - 0.5) * xbar[k] + 0.5 * v[n][k];
           npend = 1;
           st = CONTRACT_IN;
         }
