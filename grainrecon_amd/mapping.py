@@ -313,7 +313,9 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
     cand_id = np.zeros(P)
     fa_track = np.zeros(P)
     gid_entry = ds.GrainId.reshape(-1, order="F") > 0
-    n_gid_prev = int(np.count_nonzero(gid_entry))
+    # GrainId > 0 count, kept up to date at each assignment (a recount of the
+    # whole volume per seed costs milliseconds)
+    n_gid = n_gid_prev = int(np.count_nonzero(gid_entry))
     p1 = np.array([rvp_first[0], rvp_first[1], rvp_first[2], 1.0 if simap else 0.0, vox_size, center[0], center[1],
                    center[2], tomo_dim[0], tomo_dim[1], tomo_dim[2]], dtype=float)
     eng.set_orientation_set(OR_q)
@@ -438,7 +440,10 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                 idx = g["indexed"] - 1
                 nr_vox[i] = idx.shape[0]
                 xi, yi, zi = idx[:, 0], idx[:, 1], idx[:, 2]
+                lin_u = np.unique(np.ravel_multi_index((xi, yi, zi), dims))
+                was = np.count_nonzero(ds.GrainId[np.unravel_index(lin_u, dims)] > 0)
                 ds.GrainId[xi, yi, zi] = gid
+                n_gid += (lin_u.size if gid > 0 else 0) - int(was)
                 eu = temp[8:11]
                 ds.Rodrigues[:, xi, yi, zi] = temp[11:14][:, None]
                 ds.IPF001[:, xi, yi, zi] = (eu / math.sqrt((eu[0] * eu[0] + eu[1] * eu[1]) + eu[2] * eu[2]))[:, None]
@@ -469,7 +474,6 @@ def grain_mapping_engine(eng, pos_list, OR_q, ds: MappingDS, *, trust_comp, min_
                 pos_new[i] = pn
                 pos_ix = pn.copy()
         # :293-299: the seed's record; column 13 = voxels that gained a grain id
-        n_gid = int(np.count_nonzero(ds.GrainId > 0))
         seed_ix[i] = np.concatenate([ind_final[i, :11], [nr_vox[i], n_gid - n_gid_prev, cand_id[i], fa_track[i]]])
         n_gid_prev = n_gid
         k = i + 1  # MATLAB's seed number
