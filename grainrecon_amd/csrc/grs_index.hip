@@ -169,27 +169,38 @@ __global__ void k_qall_owner(const int64_t *ls_n, int64_t P, int64_t K, int64_t 
 __global__ void k_qall_build(const double *ls_q, const int64_t *ls_n, const int32_t *owner,
                              const unsigned long long *Rmax, int64_t P, int64_t K, int64_t cap, int64_t Rcap,
                              double *q_rows, uint32_t *gcount) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P * Rcap) return;
-  const int64_t p = t / Rcap, row = t - p * Rcap;
-  double *q = q_rows + 4 * t;
-  const int32_t j = owner[t];
-  if (row >= (int64_t)Rmax[p]) {
-    const double nan = __builtin_nan("");
-    q[0] = q[1] = q[2] = q[3] = nan;
-    return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t grp = -1; /* the group row t is counted in, or -1 */
+  if (t < P * Rcap) {
+    const int64_t p = t / Rcap, row = t - p * Rcap;
+    double *q = q_rows + 4 * t;
+    const int32_t j = owner[t];
+    if (row >= (int64_t)Rmax[p]) {
+      const double nan = __builtin_nan("");
+      q[0] = q[1] = q[2] = q[3] = nan;
+    } else if (j == 0) {
+      q[0] = q[1] = q[2] = q[3] = 0.0;
+    } else {
+      const int64_t len = ls_n[p * K + j - 1];
+      const double *src = ls_q + ((p * K + j - 1) * cap + (row - (int64_t)(j - 1) * len)) * 4;
+      q[0] = src[0];
+      q[1] = src[1];
+      q[2] = src[2];
+      q[3] = src[3];
+      grp = p * K + j - 1;
+    }
   }
-  if (j == 0) {
-    q[0] = q[1] = q[2] = q[3] = 0.0;
-    return;
+  /* the row counts: one atomic per distinct group among the wave's rows
+   * (a group's rows are contiguous, so a wave holds one or two) */
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t todo = __ballot(grp >= 0);
+  while (todo) {
+    const int lead = __ffsll((unsigned long long)todo) - 1;
+    const int64_t g = __shfl(grp, lead, 64);
+    const uint64_t same = __ballot(grp == g);
+    if (lane == lead) atomicAdd(gcount + g, (uint32_t)__popcll(same));
+    todo &= ~same;
   }
-  const int64_t len = ls_n[p * K + j - 1];
-  const double *src = ls_q + ((p * K + j - 1) * cap + (row - (int64_t)(j - 1) * len)) * 4;
-  q[0] = src[0];
-  q[1] = src[1];
-  q[2] = src[2];
-  q[3] = src[3];
-  atomicAdd(gcount + p * K + j - 1, 1u);
 }
 
 /* one wavefront per (seed position, group) */
