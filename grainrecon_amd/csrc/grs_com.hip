@@ -8,8 +8,8 @@
  * (grs_event_pixel), its pixel rounded, and the distance to the measured
  * centroid (columns 19:20 of the pair row) taken; ErrMean is the mean.
  * k_com_nm runs fminsearchbnd's Nelder-Mead for each grain inside the
- * kernel (one block per grain), so a fit costs no host round trip per
- * simplex step.
+ * kernel (one block per grain, its first wave stepping the method with one
+ * lane per coordinate), so a fit costs no host round trip per simplex step.
  */
 #include <hip/hip_runtime.h>
 
@@ -39,10 +39,189 @@ __device__ static double com_pair_dist(const grs_dgeo *g, const double *r, const
   return grs_sqrt(ey * ey + ez * ez);
 }
 
+/* grs_nm::feed stepped by the 64 lanes of one wave.  Every lane takes the
+ * method's decisions on the shared values (fv, fxr, counts, state; all lanes
+ * write the same values) and lane k alone updates coordinate k of the
+ * simplex, the requested points, xbar and xr, with grs_nm's statements for
+ * that coordinate; the stopping test's largest vertex distance and largest
+ * coordinate are reduced across the lanes (a maximum is exact).  The state
+ * afterwards is grs_nm::feed's, bit for bit: the sequential form on one lane
+ * spent most of a fit's time on these loops. */
+struct nm_lanes {
+  grs_nm *s;
+  int k;
+  bool on;
+
+  __device__ static double wave_max(double x) {
+    for (int off = 32; off > 0; off >>= 1) x = grs_nm_max(x, __shfl_xor(x, off, 64));
+    return x;
+  }
+
+  __device__ void sort() {
+    const int n = s->n;
+    for (int i = 1; i <= n; i++) {
+      const double f = s->fv[i];
+      const double row = on ? s->v[i][k] : 0.0;
+      int j = i - 1;
+      while (j >= 0 && (f == f) && (s->fv[j] != s->fv[j] || s->fv[j] > f)) {
+        s->fv[j + 1] = s->fv[j];
+        if (on) s->v[j + 1][k] = s->v[j][k];
+        j--;
+      }
+      s->fv[j + 1] = f;
+      if (on) s->v[j + 1][k] = row;
+    }
+  }
+
+  __device__ void begin() {
+    GRS_NO_CONTRACT
+    const int n = s->n;
+    if (!(s->nf < s->max_fe && s->it < s->max_it)) {
+      s->st = grs_nm::DONE;
+      s->npend = 0;
+      return;
+    }
+    double dfv = __builtin_nan(""), dx = __builtin_nan(""), vmax = __builtin_nan("");
+    for (int j = 1; j <= n; j++) dfv = grs_nm_max(dfv, fabs(s->fv[0] - s->fv[j]));
+    if (on) {
+      const double v0 = s->v[0][k];
+      for (int j = 1; j <= n; j++) dx = grs_nm_max(dx, fabs(s->v[j][k] - v0));
+      vmax = v0;
+    }
+    dx = wave_max(dx);
+    vmax = wave_max(vmax);
+    if (dfv <= grs_nm_max(s->tol_f, 10.0 * grs_nm_eps(s->fv[0])) &&
+        dx <= grs_nm_max(s->tol_x, 10.0 * grs_nm_eps(vmax))) {
+      s->st = grs_nm::DONE;
+      s->npend = 0;
+      return;
+    }
+    if (on) {
+      double acc = 0.0;
+      for (int j = 0; j < n; j++) acc += s->v[j][k];
+      const double xb = acc / (double)n;
+      s->xbar[k] = xb;
+      const double xr = (1.0 + 1.0) * xb - 1.0 * s->v[n][k];
+      s->xr[k] = xr;
+      s->pend[0][k] = xr;
+    }
+    s->npend = 1;
+    s->st = grs_nm::REFLECT;
+  }
+
+  __device__ void end_iter() {
+    sort();
+    s->it = s->it + 1;
+    begin();
+  }
+
+  __device__ void shrink() {
+    GRS_NO_CONTRACT
+    const int n = s->n;
+    if (on) {
+      const double v0 = s->v[0][k];
+      for (int j = 1; j <= n; j++) {
+        const double w = v0 + 0.5 * (s->v[j][k] - v0);
+        s->v[j][k] = w;
+        s->pend[j - 1][k] = w;
+      }
+    }
+    s->npend = n;
+    s->st = grs_nm::SHRINK;
+  }
+
+  /* coordinate k of the next request: a xbar - b v_n */
+  __device__ void request(double a, double b, grs_nm::State st) {
+    GRS_NO_CONTRACT
+    if (on) s->pend[0][k] = a * s->xbar[k] - b * s->v[s->n][k];
+    s->npend = 1;
+    s->st = st;
+  }
+
+  __device__ void feed(const double *f) {
+    GRS_NO_CONTRACT
+    const int n = s->n;
+    switch (s->st) {
+      case grs_nm::INIT:
+        for (int j = 0; j <= n; j++) {
+          if (on) s->v[j][k] = s->pend[j][k];
+          s->fv[j] = f[j];
+        }
+        s->nf = n + 1;
+        sort();
+        s->it = 1;
+        begin();
+        return;
+      case grs_nm::REFLECT: {
+        const double fxr = f[0];
+        s->fxr = fxr;
+        s->nf = s->nf + 1;
+        if (fxr < s->fv[0]) {
+          request(1.0 + 2.0, 2.0, grs_nm::EXPAND);
+          return;
+        }
+        if (fxr < s->fv[n - 1]) {
+          if (on) s->v[n][k] = s->xr[k];
+          s->fv[n] = fxr;
+          end_iter();
+          return;
+        }
+        if (fxr < s->fv[n]) {
+          request(1.0 + 0.5, 0.5, grs_nm::CONTRACT_OUT);
+        } else {
+          /* (1 - 0.5) xbar + 0.5 v_n */
+          if (on) s->pend[0][k] = (1.0 - 0.5) * s->xbar[k] + 0.5 * s->v[n][k];
+          s->npend = 1;
+          s->st = grs_nm::CONTRACT_IN;
+        }
+        return;
+      }
+      case grs_nm::EXPAND:
+        s->nf = s->nf + 1;
+        if (f[0] < s->fxr) {
+          if (on) s->v[n][k] = s->pend[0][k];
+          s->fv[n] = f[0];
+        } else {
+          if (on) s->v[n][k] = s->xr[k];
+          s->fv[n] = s->fxr;
+        }
+        end_iter();
+        return;
+      case grs_nm::CONTRACT_OUT:
+        s->nf = s->nf + 1;
+        if (f[0] <= s->fxr) {
+          if (on) s->v[n][k] = s->pend[0][k];
+          s->fv[n] = f[0];
+          end_iter();
+        } else {
+          shrink();
+        }
+        return;
+      case grs_nm::CONTRACT_IN:
+        s->nf = s->nf + 1;
+        if (f[0] < s->fv[n]) {
+          if (on) s->v[n][k] = s->pend[0][k];
+          s->fv[n] = f[0];
+          end_iter();
+        } else {
+          shrink();
+        }
+        return;
+      case grs_nm::SHRINK:
+        for (int j = 1; j <= n; j++) s->fv[j] = f[j - 1];
+        s->nf = s->nf + n;
+        end_iter();
+        return;
+      case grs_nm::DONE:
+        return;
+    }
+  }
+};
+
 #define COM_THREADS 512
 
-/* One block per fit m: thread 0 steps the Nelder-Mead state machine
- * (started on the host) in LDS; for the requested points the first threads
+/* One block per fit m: the first wave steps the Nelder-Mead state machine
+ * (started on the host) in LDS (nm_lanes); for the requested points the first threads
  * build each point's U and SUB, all threads compute the (point, pair)
  * distances into dis (9 slots of the grain's pair count per grain), and
  * thread p adds point p's distances in pair order, so ErrMean is the
@@ -107,7 +286,10 @@ __global__ __launch_bounds__(COM_THREADS) void k_com_nm(const grs_dgeo *g, const
       fs[t] = acc / (double)n;
     }
     __syncthreads();
-    if (t == 0) nm->feed(fs);
+    if (t < 64) { /* the first wave steps the method */
+      nm_lanes L{nm, t, t < nm->n};
+      L.feed(fs);
+    }
     __syncthreads();
   }
   if (t == 0) nms[m] = *nm;
