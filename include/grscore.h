@@ -546,7 +546,9 @@ grs_status grs_get_tuning(grs_ctx *ctx, int32_t kind, int32_t *waves_per_block, 
 /* Pins the launch configuration of a kind (no tuning for it; n_configs 0):
  * waves per block 1, 2, 4 or 8, a GRS_MASK_* source (GRS_MASK_BITS for kind 0),
  * XCD mapping 0 or 1.  waves_per_block = 0 forgets the kind's configuration,
- * so that the next large call tunes again.  No reference counterpart. */
+ * so that the next large call tunes again.  Whatever the configuration, a
+ * launch of fewer waves than the device has CUs (the orientation fits'
+ * steps) runs one wave per block.  No reference counterpart. */
 grs_status grs_set_launch_config(grs_ctx *ctx, int32_t kind, int32_t waves_per_block, int32_t mask_layout,
                                  int32_t xcd_map);
 
